@@ -1,0 +1,208 @@
+"""FVM step of jax-cfd restated on the CPU (oracle).
+
+TEST INFRASTRUCTURE ONLY (see oracle/__init__.py).
+
+``FVMStep`` mirrors ``semi_implicit_navier_stokes`` (jax_cfd/base/
+equations.py:120-151) with ``time_stepper=forward_euler``
+(time_stepping.py:59-118), an ``advect_van_leer`` or
+``advect_van_leer_using_limiters`` convection, explicit diffusion, optional
+Kolmogorov + linear forcing, and ``projection`` (pressure.py:181-198) with
+either ``solve_fast_diag`` (pressure.py:115-157 via fast_diagonalization.py:
+199-266) or ``solve_cg`` (pressure.py:67-112).
+
+Pinning (tests/test_oracle_fvm.py): the 4x4 periodic Poisson KAT
+``x = b - mean(b)`` (pressure_test.py:197-213); circulant pseudo-inverse vs a
+dense ``numpy.linalg.lstsq`` brute force (fast_diagonalization_test.py:
+58-130); projection divergence (pressure_test.py:91-110); divergence and
+momentum invariants over many steps (equations_test.py:84-163); the
+Taylor-Green closed form (validation_test.py:34-169).  ``solve_cg`` follows the
+published conjugate-gradient algorithm of jax.scipy.sparse.linalg.cg (no
+preconditioner, stop when |r|^2 <= max(tol^2 |b|^2, atol^2)); it is checked
+against the fast-diagonalisation solve, but its individual iterates are
+"parity unpinned" (no JAX here to compare against).
+"""
+from __future__ import annotations
+
+import dataclasses
+import functools
+from typing import Callable, Optional, Sequence, Tuple
+
+import numpy as np
+
+from oracle import grid_ops as go
+from oracle.grid_ops import A, Grid
+
+
+# ------------------------------------------------------------ pressure solve
+def laplacian_eigenvalues(grid: Grid):
+  """Eigenvalues of the periodic 1-D Laplacians, fp64.
+
+  array_utils.py:168-173 builds the circulant with first column
+  [-2, 1, 0, ..., 0, 1] / h^2; fast_diagonalization.py:211-214 takes
+  ``fft`` of it for every axis but the last and ``rfft`` for the last.
+  """
+  out = []
+  for axis, (n, h) in enumerate(zip(grid.shape, grid.step)):
+    column = np.zeros(n)
+    column[0] = -2 / h ** 2
+    column[1] = column[-1] = 1 / h ** 2
+    if axis == grid.ndim - 1:
+      out.append(np.fft.rfft(column))
+    else:
+      out.append(np.fft.fft(column))
+  return out
+
+
+def poisson_pinv_diagonal(grid: Grid, dtype=np.float32):
+  """Lambda^+ of fast_diagonalization.pseudoinverse (fast_diagonalization.py:228-266).
+
+  cutoff = 10 * eps(dtype); values are formed in fp64/complex128 and cast to
+  the complex type of `dtype` (jnp.asarray with x64 disabled).
+  """
+  eig = laplacian_eigenvalues(grid)
+  summed = functools.reduce(np.add.outer, eig)
+  cutoff = 10 * np.finfo(dtype).eps
+  with np.errstate(divide='ignore', invalid='ignore'):
+    diag = np.where(abs(summed) > cutoff, 1 / summed, 0)
+  ctype = np.complex64 if np.dtype(dtype) == np.float32 else np.complex128
+  return diag.astype(ctype)
+
+
+def solve_fast_diag(v: Sequence[A], grid: Grid) -> A:
+  """pressure.py:115-157 (periodic): q = irfftn(Lambda^+ * rfftn(div v))."""
+  rhs = go.divergence(v, grid)
+  dtype = rhs.data.dtype.type
+  diag = poisson_pinv_diagonal(grid, dtype)
+  q = np.fft.irfftn(diag * np.fft.rfftn(rhs.data), s=rhs.data.shape,
+                    axes=tuple(range(rhs.data.ndim)))
+  return A(q.real.astype(dtype), rhs.offset)
+
+
+def _periodic_laplacian(x: np.ndarray, grid: Grid) -> np.ndarray:
+  return go.laplacian(A(x, grid.cell_center), grid).data
+
+
+def solve_cg(v: Sequence[A], grid: Grid, rtol: float = 1e-6,
+             atol: float = 1e-6, maxiter: Optional[int] = None,
+             return_iterations: bool = False):
+  """pressure.py:67-112: unpreconditioned CG on the periodic Laplacian, x0 = 0.
+
+  Algorithm of jax.scipy.sparse.linalg.cg (the public CG recurrence with the
+  SciPy-style stopping rule |r|^2 > max(tol^2 |b|^2, atol^2) as loop
+  condition, maxiter default 10 * size).
+  """
+  rhs = go.divergence(v, grid)
+  b = rhs.data
+  dtype = b.dtype.type
+  if maxiter is None:
+    maxiter = 10 * b.size
+  bs = np.vdot(b, b).real.astype(dtype)
+  atol2 = max(np.square(dtype(rtol)) * bs, np.square(dtype(atol)))
+  x = np.zeros_like(b)
+  r = b - _periodic_laplacian(x, grid)
+  p = r
+  gamma = np.vdot(r, r).real.astype(dtype)
+  k = 0
+  while gamma > atol2 and k < maxiter:
+    ap = _periodic_laplacian(p, grid)
+    alpha = gamma / np.vdot(p, ap).real.astype(dtype)
+    x = x + alpha * p
+    r = r - alpha * ap
+    gamma_new = np.vdot(r, r).real.astype(dtype)
+    beta = gamma_new / gamma
+    p = r + beta * p
+    gamma = gamma_new
+    k += 1
+  out = A(x.astype(dtype), rhs.offset)
+  return (out, k) if return_iterations else out
+
+
+def projection(v: Sequence[A], grid: Grid,
+               solve: Callable = solve_fast_diag) -> Tuple[A, ...]:
+  """pressure.py:181-198: v - forward_difference(q)."""
+  q = solve(v, grid)
+  out = []
+  for axis, u in enumerate(v):
+    qg = go.forward_difference(q, axis, grid.step[axis])
+    out.append(A(u.data - qg.data, u.offset))
+  return tuple(out)
+
+
+# --------------------------------------------------------------- time step
+def stable_time_step(max_velocity: float, max_courant_number: float,
+                     viscosity: float, grid: Grid,
+                     implicit_diffusion: bool = False) -> float:
+  """equations.py:45-59."""
+  dt = go.stable_time_step_advection(max_velocity, max_courant_number, grid)
+  if not implicit_diffusion:
+    diffusion_dt = go.stable_time_step_diffusion(viscosity, grid)
+    if diffusion_dt < dt:
+      raise ValueError('stable time step for diffusion is smaller than '
+                       f'the chosen timestep: {diffusion_dt} vs {dt}')
+  return dt
+
+
+@dataclasses.dataclass
+class FVMStep:
+  """semi_implicit_navier_stokes + forward_euler, restated.
+
+  convect: 'van_leer' (advect_van_leer for every component, as the
+  channel-flow notebook wires it) or 'van_leer_limiters' (the factory default,
+  advect_van_leer_using_limiters).  pressure: 'fast_diag' or 'cg'.
+  """
+  grid: Grid
+  dt: float
+  density: float = 1.0
+  viscosity: Optional[float] = 1e-3
+  forcing: Optional[go.Forcing] = None
+  convect: str = 'van_leer'
+  pressure: str = 'fast_diag'
+  cg_rtol: float = 1e-6
+  cg_atol: float = 1e-6
+
+  def convection(self, v: Sequence[A]):
+    if self.convect == 'van_leer':
+      fn = go.advect_van_leer
+    elif self.convect == 'van_leer_limiters':
+      fn = go.advect_van_leer_using_limiters
+    else:
+      raise ValueError(self.convect)
+    return tuple(fn(u, v, self.dt, self.grid).data for u in v)
+
+  def explicit_terms(self, v: Sequence[A]):
+    """equations.py:102-114: (convection + diffusion) + forcing / density."""
+    dv = list(self.convection(v))
+    if self.viscosity is not None:
+      nu = self.viscosity / self.density
+      dv = [d + go.diffuse(u, nu, self.grid).data for d, u in zip(dv, v)]
+    if self.forcing is not None:
+      f = self.forcing(v, self.grid)
+      dv = [d + fi / self.density for d, fi in zip(dv, f)]
+    return tuple(A(d, u.offset) for d, u in zip(dv, v))
+
+  def solve(self, v, grid):
+    if self.pressure == 'fast_diag':
+      return solve_fast_diag(v, grid)
+    if self.pressure == 'cg':
+      return solve_cg(v, grid, self.cg_rtol, self.cg_atol)
+    raise ValueError(self.pressure)
+
+  def intermediate(self, v: Sequence[A]):
+    """u* = u0 + dt * (0 + 1 * k0)  (time_stepping.py:88-101, b = [1])."""
+    k0 = self.explicit_terms(v)
+    return tuple(A(u.data + self.dt * (0 + 1 * k.data), u.offset)
+                 for u, k in zip(v, k0))
+
+  def __call__(self, v: Sequence[A]) -> Tuple[A, ...]:
+    return projection(self.intermediate(v), self.grid, self.solve)
+
+  def rollout(self, v: Sequence[A], steps: int) -> Tuple[A, ...]:
+    for _ in range(steps):
+      v = self(v)
+    return v
+
+
+def velocity(u: np.ndarray, v: np.ndarray, grid: Grid) -> Tuple[A, A]:
+  """Wrap two arrays at the staggered cell-face offsets (grids.py:567-577)."""
+  f = grid.cell_faces
+  return (A(u, f[0]), A(v, f[1]))
