@@ -1,0 +1,145 @@
+/* jcfd.h -- C ABI of libjcfd_amd, the MI355X (gfx950) time-step hot path of
+ * jax-cfd.
+ *
+ * The reference (google/jax-cfd) is pure Python on JAX; its "interface" for
+ * this path is the step-function contract
+ *     step_fn(state) -> state
+ * built by jax_cfd/base/equations.py:120-151 (semi_implicit_navier_stokes,
+ * FVM state = tuple of two staggered GridVariables) and by
+ * jax_cfd/spectral/time_stepping.py:186-197 over
+ * jax_cfd/spectral/equations.py:129-220 (crank_nicolson_rk4 of
+ * NavierStokes2D, state = complex64 rfft2 of the vorticity), consumed by
+ * jax_cfd/base/funcutils.py:82-126 (repeated / trajectory).  Every entry
+ * point below replaces one of those Python-level operations; the citation
+ * on each names the reference function whose arithmetic it performs.
+ *
+ * Conventions
+ *   - Arrays are passed as jcfd_tensor, laid out exactly like DLPack's
+ *     DLTensor (data, device, ndim, dtype, shape, strides, byte_offset), so a
+ *     DLPack producer's DLTensor* can be passed directly.  Arrays must be
+ *     C-contiguous (strides NULL or row-major), on a ROCm device
+ *     (device_type 10).  FVM fields: float32 (n, n), axis 1 contiguous.
+ *     Spectral fields: complex64 (n, n/2 + 1).
+ *   - All work is enqueued on the given HIP stream (hipStream_t passed as
+ *     void*; NULL = default stream) and is asynchronous.
+ *   - Functions return JCFD_OK (0) or a negative error code; the message of
+ *     the last error on the calling thread is returned by jcfd_last_error().
+ *     Nothing throws across the ABI.
+ *   - A plan owns its device constants and scratch; one plan must not be
+ *     used from two streams at once.  Distinct plans are independent.
+ */
+#ifndef JCFD_H_
+#define JCFD_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------ errors */
+#define JCFD_OK 0
+#define JCFD_ERR_INVALID_ARGUMENT (-1) /* bad shape / dtype / stride / value */
+#define JCFD_ERR_UNSUPPORTED (-2)      /* valid for the reference, not built here */
+#define JCFD_ERR_DEVICE (-3)           /* HIP runtime error */
+#define JCFD_ERR_OUT_OF_MEMORY (-4)
+#define JCFD_ERR_BUDGET (-5)           /* would exceed a caller-set byte budget */
+
+/* ------------------------------------------------ DLPack-layout tensor */
+typedef struct {
+  int32_t device_type; /* 1 = CPU, 10 = ROCm */
+  int32_t device_id;
+} jcfd_device;
+
+typedef struct {
+  uint8_t code; /* 2 = float, 5 = complex */
+  uint8_t bits; /* 32 (float32) or 64 (complex64) */
+  uint16_t lanes;
+} jcfd_dtype;
+
+typedef struct {
+  void* data;
+  jcfd_device device;
+  int32_t ndim;
+  jcfd_dtype dtype;
+  int64_t* shape;
+  int64_t* strides; /* in elements; NULL = C-contiguous */
+  uint64_t byte_offset;
+} jcfd_tensor;
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* jcfd_last_error(void);
+/* Library version string. */
+const char* jcfd_version(void);
+
+/* ------------------------------------------------------------ FVM path */
+#define JCFD_CONVECT_VAN_LEER 0          /* advection.py:210-342 */
+#define JCFD_CONVECT_VAN_LEER_LIMITERS 1 /* advection.py:387-395 */
+#define JCFD_PRESSURE_FAST_DIAG 0        /* pressure.py:115-157 */
+#define JCFD_PRESSURE_CG 1               /* pressure.py:67-112 */
+
+typedef struct {
+  int32_t n;               /* cells per side; square periodic grid */
+  double dx, dy;           /* grid steps (grids.py:551-555) */
+  double dt;               /* time step */
+  double density;          /* rho */
+  double viscosity;        /* nu; ignored unless has_viscosity */
+  int32_t has_viscosity;   /* 0 mirrors viscosity=None */
+  int32_t convect;         /* JCFD_CONVECT_* */
+  int32_t pressure;        /* JCFD_PRESSURE_* */
+  double cg_rtol, cg_atol; /* solve_cg tolerances */
+  int32_t cg_maxiter;      /* <= 0: 10 * n * n, as jax.scipy.sparse.linalg.cg */
+  int32_t has_forcing;     /* 0 mirrors forcing=None */
+  double linear_coefficient;     /* linear_forcing coefficient (forcings.py:107-113) */
+  const float* forcing_x;  /* host array, n floats: constant x-forcing f_x(y_j) of
+                              kolmogorov_forcing (forcings.py:63-104); may be NULL
+                              (= zero) */
+} jcfd_fvm_config;
+
+typedef struct jcfd_fvm_plan jcfd_fvm_plan;
+
+/* Build a plan: validates the config, computes Lambda^+ of
+ * fast_diagonalization.pseudoinverse (fast_diagonalization.py:199-266, cutoff
+ * 10 * eps(float32)), FFT twiddles, and allocates scratch on the current
+ * device.  Replaces the factory semi_implicit_navier_stokes
+ * (equations.py:120-151) with time_stepper=forward_euler. */
+int jcfd_fvm_plan_create(const jcfd_fvm_config* config, jcfd_fvm_plan** plan);
+int jcfd_fvm_plan_destroy(jcfd_fvm_plan* plan);
+/* Device bytes held by the plan (constants + scratch). */
+int64_t jcfd_fvm_plan_device_bytes(const jcfd_fvm_plan* plan);
+
+/* nsteps applications of step_fn (time_stepping.py:88-104 forward Euler +
+ * pressure.py:181-198 projection), i.e. funcutils.repeated(step_fn, nsteps)
+ * (funcutils.py:82-88).  u_out/v_out may alias u/v. */
+int jcfd_fvm_step(jcfd_fvm_plan* plan, const jcfd_tensor* u, const jcfd_tensor* v,
+                  jcfd_tensor* u_out, jcfd_tensor* v_out, int64_t nsteps, void* stream);
+
+/* Explicit terms dv/dt = convection + nu/rho * laplacian + forcing / rho
+ * (equations.py:77-116, the explicit_terms_with_same_bcs seam). */
+int jcfd_fvm_explicit_terms(jcfd_fvm_plan* plan, const jcfd_tensor* u, const jcfd_tensor* v,
+                            jcfd_tensor* du, jcfd_tensor* dv, void* stream);
+
+/* pressure.projection(v, solve_fast_diag) (pressure.py:181-198). */
+int jcfd_fvm_projection(jcfd_fvm_plan* plan, const jcfd_tensor* u, const jcfd_tensor* v,
+                        jcfd_tensor* u_out, jcfd_tensor* v_out, void* stream);
+
+/* fd.divergence of a staggered velocity (finite_differences.py:136-143). */
+int jcfd_fvm_divergence(jcfd_fvm_plan* plan, const jcfd_tensor* u, const jcfd_tensor* v,
+                        jcfd_tensor* out, void* stream);
+
+/* fd.curl_2d of a staggered velocity (finite_differences.py:186-193),
+ * offset (1, 1). */
+int jcfd_fvm_curl(jcfd_fvm_plan* plan, const jcfd_tensor* u, const jcfd_tensor* v,
+                  jcfd_tensor* out, void* stream);
+
+/* out = irfft2(multiplier * rfft2(x)), multiplier float32 (n, n/2 + 1) on the
+ * device: the spectral filter of filter_utils.filter (filter_utils.py:32-42)
+ * for a multiplier that is even in k (then ifftn(fftn(x) m).real equals it). */
+int jcfd_fvm_spectral_filter(jcfd_fvm_plan* plan, const jcfd_tensor* x,
+                             const jcfd_tensor* multiplier, jcfd_tensor* out, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* JCFD_H_ */

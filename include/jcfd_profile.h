@@ -1,0 +1,36 @@
+/* jcfd_profile.h -- measurement hooks of libjcfd_amd.
+ *
+ * No counterpart in the reference (jax-cfd has no kernels of its own to
+ * time).  For benchmarks: each function launches one kernel class of a plan
+ * `reps` times back to back on `stream`, brackets the launches with HIP
+ * events recorded on that same stream, and reports the mean duration per
+ * launch in milliseconds.  Inputs are read, outputs go to the plan's scratch
+ * (the caller's arrays are not modified).
+ */
+#ifndef JCFD_PROFILE_H_
+#define JCFD_PROFILE_H_
+
+#include "jcfd.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Kernel classes of one FVM step, in launch order. */
+#define JCFD_FVM_KERNEL_ROWS_FORWARD 0 /* stencil + Euler + div + row FFT */
+#define JCFD_FVM_KERNEL_POISSON_COLS 1 /* column FFT, Lambda^+, inverse */
+#define JCFD_FVM_KERNEL_ROWS_PROJECT 2 /* inverse row FFT + correction */
+#define JCFD_FVM_KERNEL_COUNT 3
+
+/* ms_per_launch[k] = mean duration of kernel class k over `reps` launches. */
+int jcfd_fvm_time_kernels(jcfd_fvm_plan* plan, const jcfd_tensor* u, const jcfd_tensor* v,
+                          int32_t reps, double* ms_per_launch, void* stream);
+
+/* Algorithmic HBM bytes one launch of each kernel class moves (distinct
+ * logical inputs read once + outputs written once). */
+int jcfd_fvm_kernel_bytes(const jcfd_fvm_plan* plan, int64_t* bytes_per_launch);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* JCFD_PROFILE_H_ */

@@ -1,0 +1,381 @@
+// C ABI of libjcfd_amd: argument validation, plans, and the step loops.
+// Kernels live in fvm.hip / spectral.hip; this file only orchestrates.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "jcfd.h"
+#include "jcfd_internal.h"
+
+namespace jcfd {
+
+namespace {
+thread_local std::string g_last_error;
+}
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+int fail(int code, const std::string& msg) {
+  set_error(msg);
+  return code;
+}
+
+int check_hip(hipError_t e, const char* what) {
+  if (e == hipSuccess) return JCFD_OK;
+  return fail(JCFD_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+namespace {
+
+constexpr int kDLFloat = 2;
+constexpr int kDLComplex = 5;
+constexpr int kDLROCM = 10;
+
+// Validate a C-contiguous 2-D tensor of the given dtype and shape on `device`.
+int check_tensor(const jcfd_tensor* t, const char* name, int code, int bits, int64_t d0, int64_t d1,
+                 int device, void** ptr) {
+  if (t == nullptr) return fail(JCFD_ERR_INVALID_ARGUMENT, std::string(name) + ": null tensor");
+  if (t->device.device_type != kDLROCM)
+    return fail(JCFD_ERR_INVALID_ARGUMENT,
+                std::string(name) + ": expected a ROCm device tensor (device_type 10), got " +
+                    std::to_string(t->device.device_type));
+  if (device >= 0 && t->device.device_id != device)
+    return fail(JCFD_ERR_INVALID_ARGUMENT, std::string(name) + ": tensor on device " +
+                                               std::to_string(t->device.device_id) +
+                                               ", plan on device " + std::to_string(device));
+  if (t->dtype.code != code || t->dtype.bits != bits || t->dtype.lanes != 1)
+    return fail(JCFD_ERR_INVALID_ARGUMENT,
+                std::string(name) + ": wrong dtype (expected " +
+                    (code == kDLFloat ? "float32" : "complex64") + ")");
+  if (t->ndim != 2 || t->shape == nullptr || t->shape[0] != d0 || t->shape[1] != d1)
+    return fail(JCFD_ERR_INVALID_ARGUMENT, std::string(name) + ": expected shape (" +
+                                               std::to_string(d0) + ", " + std::to_string(d1) + ")");
+  if (t->strides != nullptr && !(t->strides[1] == 1 && t->strides[0] == d1))
+    return fail(JCFD_ERR_INVALID_ARGUMENT, std::string(name) + ": must be C-contiguous");
+  if (t->data == nullptr) return fail(JCFD_ERR_INVALID_ARGUMENT, std::string(name) + ": null data");
+  *ptr = static_cast<char*>(t->data) + t->byte_offset;
+  return JCFD_OK;
+}
+
+template <typename T>
+int device_alloc(T** p, size_t count, int64_t* total) {
+  hipError_t e = hipMalloc(reinterpret_cast<void**>(p), count * sizeof(T));
+  if (e != hipSuccess) {
+    *p = nullptr;
+    (void)hipGetLastError();
+    return fail(JCFD_ERR_OUT_OF_MEMORY, "hipMalloc of " + std::to_string(count * sizeof(T)) +
+                                            " bytes failed: " + hipGetErrorString(e));
+  }
+  *total += static_cast<int64_t>(count * sizeof(T));
+  return JCFD_OK;
+}
+
+// exp(-2 pi i k / n), k in [0, n), rounded from fp64.
+std::vector<float2> twiddles(int n) {
+  std::vector<float2> tw(n);
+  for (int k = 0; k < n; ++k) {
+    const double a = -2.0 * M_PI * static_cast<double>(k) / static_cast<double>(n);
+    tw[k] = make_float2(static_cast<float>(std::cos(a)), static_cast<float>(std::sin(a)));
+  }
+  return tw;
+}
+
+// Lambda^+ / n^2 for the periodic 5-point Laplacian on an n x n grid
+// (fast_diagonalization.py:199-266 with array_utils.py:168-173): eigenvalues
+// (2 cos(2 pi p / n) - 2) / h^2 per axis in fp64, rfft layout on the last axis,
+// zero where |lambda| <= 10 * eps(float32); the irfftn 1/n^2 is folded in.
+std::vector<float> poisson_multiplier(int n, double dx, double dy) {
+  const int h1 = n / 2 + 1;
+  const double cutoff = static_cast<double>(10.0f * 1.1920928955078125e-07f);
+  std::vector<float> lam(static_cast<size_t>(n) * h1);
+  for (int p = 0; p < n; ++p) {
+    const double lx = (-2.0 / (dx * dx)) + 2.0 * std::cos(2.0 * M_PI * p / n) / (dx * dx);
+    for (int c = 0; c < h1; ++c) {
+      const double ly = (-2.0 / (dy * dy)) + 2.0 * std::cos(2.0 * M_PI * c / n) / (dy * dy);
+      const double s = lx + ly;
+      const double inv = (std::fabs(s) > cutoff) ? 1.0 / s : 0.0;
+      lam[static_cast<size_t>(p) * h1 + c] = static_cast<float>(inv / (static_cast<double>(n) * n));
+    }
+  }
+  return lam;
+}
+
+}  // namespace
+}  // namespace jcfd
+
+using namespace jcfd;
+
+struct jcfd_fvm_plan {
+  jcfd_fvm_config cfg;
+  FvmParams params;
+  int device = 0;
+  float* us = nullptr;
+  float* vs = nullptr;
+  float2* spec = nullptr;
+  float* lam = nullptr;
+  float2* tw = nullptr;
+  float* fx = nullptr;
+  int64_t bytes = 0;
+};
+
+extern "C" {
+
+const char* jcfd_last_error(void) { return g_last_error.c_str(); }
+
+const char* jcfd_version(void) { return "jcfd_amd 0.1 (gfx950)"; }
+
+int jcfd_fvm_plan_create(const jcfd_fvm_config* config, jcfd_fvm_plan** plan) {
+  if (config == nullptr || plan == nullptr) return fail(JCFD_ERR_INVALID_ARGUMENT, "null argument");
+  *plan = nullptr;
+  const jcfd_fvm_config& c = *config;
+  if (!fvm_size_supported(c.n))
+    return fail(JCFD_ERR_UNSUPPORTED,
+                "FVM grid size " + std::to_string(c.n) + " not built (32..1024, powers of two)");
+  if (!(c.dx > 0) || !(c.dy > 0) || !std::isfinite(c.dx) || !std::isfinite(c.dy))
+    return fail(JCFD_ERR_INVALID_ARGUMENT, "grid steps must be positive and finite");
+  if (!std::isfinite(c.dt) || !std::isfinite(c.density) || c.density == 0)
+    return fail(JCFD_ERR_INVALID_ARGUMENT, "dt and density must be finite, density non-zero");
+  if (c.convect != JCFD_CONVECT_VAN_LEER)
+    return fail(JCFD_ERR_UNSUPPORTED, "only advect_van_leer convection is built");
+  if (c.pressure != JCFD_PRESSURE_FAST_DIAG)
+    return fail(JCFD_ERR_UNSUPPORTED, "only the solve_fast_diag pressure solve is built");
+
+  auto* p = new (std::nothrow) jcfd_fvm_plan();
+  if (p == nullptr) return fail(JCFD_ERR_OUT_OF_MEMORY, "host allocation failed");
+  p->cfg = c;
+  p->cfg.forcing_x = nullptr;
+  if (hipGetDevice(&p->device) != hipSuccess) {
+    delete p;
+    return fail(JCFD_ERR_DEVICE, "hipGetDevice failed (no GPU?)");
+  }
+  FvmParams& q = p->params;
+  const int n = c.n;
+  q.n = n;
+  q.dx = static_cast<float>(c.dx);
+  q.dy = static_cast<float>(c.dy);
+  // finite_differences.laplacian: scales = square(1 / array(step, float32)), summed in fp32
+  const float ix = 1.0f / static_cast<float>(c.dx), iy = 1.0f / static_cast<float>(c.dy);
+  q.inv_dx_sq = ix * ix;
+  q.inv_dy_sq = iy * iy;
+  q.lap_sum = q.inv_dx_sq + q.inv_dy_sq;
+  q.dt = static_cast<float>(c.dt);
+  q.dt_over_dx = static_cast<float>(c.dt / c.dx);
+  q.dt_over_dy = static_cast<float>(c.dt / c.dy);
+  q.has_diffusion = c.has_viscosity ? 1 : 0;
+  q.nu = static_cast<float>(c.has_viscosity ? c.viscosity / c.density : 0.0);
+  q.rho = static_cast<float>(c.density);
+  q.has_forcing = c.has_forcing ? 1 : 0;
+  q.lin = static_cast<float>(c.linear_coefficient);
+  q.convect = c.convect;
+
+  const size_t nn = static_cast<size_t>(n) * n;
+  int rc = JCFD_OK;
+  if ((rc = device_alloc(&p->us, nn, &p->bytes)) || (rc = device_alloc(&p->vs, nn, &p->bytes)) ||
+      (rc = device_alloc(&p->spec, nn / 2, &p->bytes)) ||
+      (rc = device_alloc(&p->lam, static_cast<size_t>(n) * (n / 2 + 1), &p->bytes)) ||
+      (rc = device_alloc(&p->tw, static_cast<size_t>(n), &p->bytes)) ||
+      (rc = device_alloc(&p->fx, static_cast<size_t>(n), &p->bytes))) {
+    jcfd_fvm_plan_destroy(p);
+    return rc;
+  }
+  const std::vector<float> lam = poisson_multiplier(n, c.dx, c.dy);
+  const std::vector<float2> tw = twiddles(n);
+  std::vector<float> fx(n, 0.0f);
+  if (c.has_forcing && c.forcing_x != nullptr) std::memcpy(fx.data(), c.forcing_x, n * sizeof(float));
+  if ((rc = check_hip(hipMemcpy(p->lam, lam.data(), lam.size() * sizeof(float), hipMemcpyHostToDevice),
+                      "copy Lambda")) ||
+      (rc = check_hip(hipMemcpy(p->tw, tw.data(), tw.size() * sizeof(float2), hipMemcpyHostToDevice),
+                      "copy twiddles")) ||
+      (rc = check_hip(hipMemcpy(p->fx, fx.data(), fx.size() * sizeof(float), hipMemcpyHostToDevice),
+                      "copy forcing"))) {
+    jcfd_fvm_plan_destroy(p);
+    return rc;
+  }
+  *plan = p;
+  return JCFD_OK;
+}
+
+int jcfd_fvm_plan_destroy(jcfd_fvm_plan* p) {
+  if (p == nullptr) return JCFD_OK;
+  for (void* ptr : {static_cast<void*>(p->us), static_cast<void*>(p->vs), static_cast<void*>(p->spec),
+                    static_cast<void*>(p->lam), static_cast<void*>(p->tw), static_cast<void*>(p->fx)})
+    if (ptr != nullptr) (void)hipFree(ptr);
+  delete p;
+  return JCFD_OK;
+}
+
+int64_t jcfd_fvm_plan_device_bytes(const jcfd_fvm_plan* p) { return p ? p->bytes : 0; }
+
+int jcfd_fvm_step(jcfd_fvm_plan* p, const jcfd_tensor* u, const jcfd_tensor* v, jcfd_tensor* u_out,
+                  jcfd_tensor* v_out, int64_t nsteps, void* stream) {
+  if (p == nullptr) return fail(JCFD_ERR_INVALID_ARGUMENT, "null plan");
+  if (nsteps < 0) return fail(JCFD_ERR_INVALID_ARGUMENT, "nsteps must be >= 0");
+  const int n = p->params.n;
+  void *pu, *pv, *pu2, *pv2;
+  int rc;
+  if ((rc = check_tensor(u, "u", kDLFloat, 32, n, n, p->device, &pu)) ||
+      (rc = check_tensor(v, "v", kDLFloat, 32, n, n, p->device, &pv)) ||
+      (rc = check_tensor(u_out, "u_out", kDLFloat, 32, n, n, p->device, &pu2)) ||
+      (rc = check_tensor(v_out, "v_out", kDLFloat, 32, n, n, p->device, &pv2)))
+    return rc;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (nsteps == 0) {
+    if (pu2 != pu && (rc = check_hip(hipMemcpyAsync(pu2, pu, sizeof(float) * n * n, hipMemcpyDeviceToDevice, s), "copy")))
+      return rc;
+    if (pv2 != pv && (rc = check_hip(hipMemcpyAsync(pv2, pv, sizeof(float) * n * n, hipMemcpyDeviceToDevice, s), "copy")))
+      return rc;
+    return JCFD_OK;
+  }
+  FvmBuffers b{};
+  b.us = p->us;
+  b.vs = p->vs;
+  b.spec = p->spec;
+  b.u_out = static_cast<float*>(pu2);
+  b.v_out = static_cast<float*>(pv2);
+  b.fx = p->fx;
+  b.lam = p->lam;
+  b.tw = p->tw;
+  for (int64_t step = 0; step < nsteps; ++step) {
+    b.u = step == 0 ? static_cast<const float*>(pu) : b.u_out;
+    b.v = step == 0 ? static_cast<const float*>(pv) : b.v_out;
+    if ((rc = fvm_launch_step(p->params, b, s))) return rc;
+  }
+  return JCFD_OK;
+}
+
+int jcfd_fvm_explicit_terms(jcfd_fvm_plan* p, const jcfd_tensor* u, const jcfd_tensor* v,
+                            jcfd_tensor* du, jcfd_tensor* dv, void* stream) {
+  if (p == nullptr) return fail(JCFD_ERR_INVALID_ARGUMENT, "null plan");
+  const int n = p->params.n;
+  void *pu, *pv, *pdu, *pdv;
+  int rc;
+  if ((rc = check_tensor(u, "u", kDLFloat, 32, n, n, p->device, &pu)) ||
+      (rc = check_tensor(v, "v", kDLFloat, 32, n, n, p->device, &pv)) ||
+      (rc = check_tensor(du, "du", kDLFloat, 32, n, n, p->device, &pdu)) ||
+      (rc = check_tensor(dv, "dv", kDLFloat, 32, n, n, p->device, &pdv)))
+    return rc;
+  if (pdu == pu || pdu == pv || pdv == pu || pdv == pv)
+    return fail(JCFD_ERR_INVALID_ARGUMENT, "explicit terms: outputs must not alias inputs");
+  return fvm_launch_explicit_terms(p->params, static_cast<const float*>(pu),
+                                   static_cast<const float*>(pv), static_cast<float*>(pdu),
+                                   static_cast<float*>(pdv), p->fx, static_cast<hipStream_t>(stream));
+}
+
+int jcfd_fvm_projection(jcfd_fvm_plan* p, const jcfd_tensor* u, const jcfd_tensor* v,
+                        jcfd_tensor* u_out, jcfd_tensor* v_out, void* stream) {
+  if (p == nullptr) return fail(JCFD_ERR_INVALID_ARGUMENT, "null plan");
+  const int n = p->params.n;
+  void *pu, *pv, *pu2, *pv2;
+  int rc;
+  if ((rc = check_tensor(u, "u", kDLFloat, 32, n, n, p->device, &pu)) ||
+      (rc = check_tensor(v, "v", kDLFloat, 32, n, n, p->device, &pv)) ||
+      (rc = check_tensor(u_out, "u_out", kDLFloat, 32, n, n, p->device, &pu2)) ||
+      (rc = check_tensor(v_out, "v_out", kDLFloat, 32, n, n, p->device, &pv2)))
+    return rc;
+  return fvm_launch_projection(p->params, static_cast<const float*>(pu), static_cast<const float*>(pv),
+                               static_cast<float*>(pu2), static_cast<float*>(pv2), p->spec, p->lam,
+                               p->tw, static_cast<hipStream_t>(stream));
+}
+
+int jcfd_fvm_divergence(jcfd_fvm_plan* p, const jcfd_tensor* u, const jcfd_tensor* v,
+                        jcfd_tensor* out, void* stream) {
+  if (p == nullptr) return fail(JCFD_ERR_INVALID_ARGUMENT, "null plan");
+  const int n = p->params.n;
+  void *pu, *pv, *po;
+  int rc;
+  if ((rc = check_tensor(u, "u", kDLFloat, 32, n, n, p->device, &pu)) ||
+      (rc = check_tensor(v, "v", kDLFloat, 32, n, n, p->device, &pv)) ||
+      (rc = check_tensor(out, "out", kDLFloat, 32, n, n, p->device, &po)))
+    return rc;
+  return fvm_launch_divergence(p->params, static_cast<const float*>(pu), static_cast<const float*>(pv),
+                               static_cast<float*>(po), static_cast<hipStream_t>(stream));
+}
+
+int jcfd_fvm_time_kernels(jcfd_fvm_plan* p, const jcfd_tensor* u, const jcfd_tensor* v,
+                          int32_t reps, double* ms_per_launch, void* stream) {
+  if (p == nullptr || ms_per_launch == nullptr)
+    return fail(JCFD_ERR_INVALID_ARGUMENT, "null argument");
+  if (reps <= 0) return fail(JCFD_ERR_INVALID_ARGUMENT, "reps must be positive");
+  const int n = p->params.n;
+  void *pu, *pv;
+  int rc;
+  if ((rc = check_tensor(u, "u", kDLFloat, 32, n, n, p->device, &pu)) ||
+      (rc = check_tensor(v, "v", kDLFloat, 32, n, n, p->device, &pv)))
+    return rc;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  FvmBuffers b{};
+  b.u = static_cast<const float*>(pu);
+  b.v = static_cast<const float*>(pv);
+  b.us = p->us;
+  b.vs = p->vs;
+  b.spec = p->spec;
+  b.u_out = p->us;  // rows-project reads and writes the same element per thread
+  b.v_out = p->vs;
+  b.fx = p->fx;
+  b.lam = p->lam;
+  b.tw = p->tw;
+  hipEvent_t e0, e1;
+  if ((rc = check_hip(hipEventCreate(&e0), "event")) ) return rc;
+  if ((rc = check_hip(hipEventCreate(&e1), "event"))) {
+    (void)hipEventDestroy(e0);
+    return rc;
+  }
+  // one full step first so every scratch buffer holds valid data
+  rc = fvm_launch_step(p->params, b, s);
+  for (int k = 0; k < JCFD_FVM_KERNEL_COUNT && rc == JCFD_OK; ++k) {
+    rc = check_hip(hipEventRecord(e0, s), "event record");
+    for (int r = 0; r < reps && rc == JCFD_OK; ++r) rc = fvm_launch_kernel(p->params, b, k, s);
+    if (rc == JCFD_OK) rc = check_hip(hipEventRecord(e1, s), "event record");
+    if (rc == JCFD_OK) rc = check_hip(hipEventSynchronize(e1), "event sync");
+    float ms = 0.f;
+    if (rc == JCFD_OK) rc = check_hip(hipEventElapsedTime(&ms, e0, e1), "elapsed");
+    ms_per_launch[k] = static_cast<double>(ms) / reps;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return rc;
+}
+
+int jcfd_fvm_kernel_bytes(const jcfd_fvm_plan* p, int64_t* bytes) {
+  if (p == nullptr || bytes == nullptr) return fail(JCFD_ERR_INVALID_ARGUMENT, "null argument");
+  const int64_t cells = static_cast<int64_t>(p->params.n) * p->params.n;
+  bytes[JCFD_FVM_KERNEL_ROWS_FORWARD] = 20 * cells;  // read u,v; write u*,v*; write packed spectrum
+  bytes[JCFD_FVM_KERNEL_POISSON_COLS] = 8 * cells;   // read + write packed spectrum
+  bytes[JCFD_FVM_KERNEL_ROWS_PROJECT] = 20 * cells;  // read spectrum, u*, v*; write u, v
+  return JCFD_OK;
+}
+
+int jcfd_fvm_curl(jcfd_fvm_plan* p, const jcfd_tensor* u, const jcfd_tensor* v, jcfd_tensor* out,
+                  void* stream) {
+  if (p == nullptr) return fail(JCFD_ERR_INVALID_ARGUMENT, "null plan");
+  const int n = p->params.n;
+  void *pu, *pv, *po;
+  int rc;
+  if ((rc = check_tensor(u, "u", kDLFloat, 32, n, n, p->device, &pu)) ||
+      (rc = check_tensor(v, "v", kDLFloat, 32, n, n, p->device, &pv)) ||
+      (rc = check_tensor(out, "out", kDLFloat, 32, n, n, p->device, &po)))
+    return rc;
+  return fvm_launch_curl(p->params, static_cast<const float*>(pu), static_cast<const float*>(pv),
+                         static_cast<float*>(po), static_cast<hipStream_t>(stream));
+}
+
+int jcfd_fvm_spectral_filter(jcfd_fvm_plan* p, const jcfd_tensor* x, const jcfd_tensor* multiplier,
+                             jcfd_tensor* out, void* stream) {
+  if (p == nullptr) return fail(JCFD_ERR_INVALID_ARGUMENT, "null plan");
+  const int n = p->params.n;
+  void *px, *pm, *po;
+  int rc;
+  if ((rc = check_tensor(x, "x", kDLFloat, 32, n, n, p->device, &px)) ||
+      (rc = check_tensor(multiplier, "multiplier", kDLFloat, 32, n, n / 2 + 1, p->device, &pm)) ||
+      (rc = check_tensor(out, "out", kDLFloat, 32, n, n, p->device, &po)))
+    return rc;
+  return fvm_launch_filter(p->params, static_cast<const float*>(px), static_cast<const float*>(pm),
+                           static_cast<float*>(po), p->spec, p->tw, static_cast<hipStream_t>(stream));
+}
+
+}  // extern "C"

@@ -1,0 +1,504 @@
+// FVM time step on the staggered MAC grid: hand-written kernels for gfx950.
+//
+// One forward-Euler step of semi_implicit_navier_stokes
+// (jax_cfd/base/equations.py:120-151, time_stepping.py:59-118) with
+// advect_van_leer convection (advection.py:210-342), explicit diffusion
+// (diffusion.py:35-37), Kolmogorov + linear forcing (forcings.py:63-178) and
+// the periodic solve_fast_diag projection (pressure.py:115-198) runs as three
+// kernels over one instance:
+//
+//   A  fvm_rows_forward   rows [i0, i0+RB): stage u, v rows i0-3..i0+RB+1 in
+//                         LDS, evaluate the explicit terms and the Euler
+//                         update u* = u + dt k (rows i0-1.. for the backward
+//                         difference), write u*, v*, form div u* and take the
+//                         packed real FFT of each row -> spectrum rows.
+//   B  fvm_poisson_cols   tiles of W spectrum columns: forward column FFT,
+//                         multiply by Lambda^+ / n^2 (the DC/Nyquist pair in
+//                         word 0 is unpacked, scaled separately and repacked),
+//                         inverse column FFT.
+//   C  fvm_rows_project   rows [i0, i0+RB]: inverse packed real FFT -> q, then
+//                         u = u* - (q[i+1,j] - q[i,j]) / dx,
+//                         v = v* - (q[i,j+1] - q[i,j]) / dy.
+//
+// Layout: u, v are (n, n) fp32, C-contiguous, axis 1 (y) fastest, offsets
+// (1, .5) and (.5, 1).  Threads run along y so every HBM access is
+// coalesced.  The arithmetic keeps the reference's operation order (no FMA
+// contraction in this file) so fp32 results track jax-cfd's to round-off.
+#include <hip/hip_runtime.h>
+
+#include "jcfd_fft.h"
+#include "jcfd_internal.h"
+
+#pragma clang fp contract(off)
+
+namespace jcfd {
+namespace {
+
+constexpr int kThreads = 256;
+
+__device__ __forceinline__ float sgnf(float x) { return (x > 0.f) ? 1.f : ((x < 0.f) ? -1.f : x); }
+
+// Van Leer flux correction (advection.py:284-334 for one face):
+// pre = 0.5 (1 - (dt/h)|U|)|U|;  phi = num (sgn den + sgn num) den / D,
+// D = |den| + |num| if |den| > 0 else 1.
+__device__ __forceinline__ float vl_correction(float U, float num_pos, float num_neg, float den,
+                                               float dt_over_h) {
+  const float num = (U > 0.f) ? num_pos : num_neg;
+  const float aden = fabsf(den);
+  const float dnn = (aden > 0.f) ? (aden + fabsf(num)) : 1.0f;
+  const float inv = den / dnn;
+  const float phi = (num * (sgnf(den) + sgnf(num))) * inv;
+  const float av = fabsf(U);
+  const float pre = (0.5f * (1.f - dt_over_h * av)) * av;
+  return pre * phi;
+}
+
+// Upwind flux + correction for a face with face velocity U, upwind values
+// c_pos (U > 0) / c_neg (U <= 0) and limiter stencil (cm2, cm1, c0, cp1) around
+// the face: num_pos = cm1 - cm2, num_neg = cp1 - c0, den = c0 - cm1.
+__device__ __forceinline__ float vl_flux(float U, float c_pos, float c_neg, float cm2, float cm1,
+                                         float c0, float cp1, float dt_over_h) {
+  const float up = (U > 0.f) ? U * c_pos : U * c_neg;
+  return up + vl_correction(U, cm1 - cm2, cp1 - c0, c0 - cm1, dt_over_h);
+}
+
+// Rows of the LDS staging window: local row lr holds global row i0 - 3 + lr.
+template <int N>
+struct Window {
+  const float* su;  // u rows
+  const float* sv;  // v rows
+  __device__ __forceinline__ float U(int lr, int j) const { return su[lr * N + (j & (N - 1))]; }
+  __device__ __forceinline__ float V(int lr, int j) const { return sv[lr * N + (j & (N - 1))]; }
+};
+
+// Explicit terms k_u, k_v at (local row lr, column j); see advection.py:210-342
+// for the face fluxes (the limiter stencil of the cross-direction faces is
+// read one cell over along the component's staggered axis, as the reference's
+// loop variable at advection.py:291 makes it), equations.py:102-110 for the
+// sum, finite_differences.py:127-133 for the Laplacian.
+template <int N>
+__device__ __forceinline__ void explicit_terms_at(const Window<N>& w, int lr, int j,
+                                                  const FvmParams& p, float fx_j, float& ku,
+                                                  float& kv) {
+  const float hx = p.dt_over_dx, hy = p.dt_over_dy;
+  // ---- u component (offset (1, .5)) ----
+  // x faces at a = i and a = i + 1 (local rows lr, lr + 1)
+  auto fx_u = [&](int a) {
+    const float Uf = 0.5f * w.U(a - 1, j) + 0.5f * w.U(a, j);
+    return vl_flux(Uf, w.U(a - 1, j), w.U(a, j), w.U(a - 2, j), w.U(a - 1, j), w.U(a, j),
+                   w.U(a + 1, j), hx);
+  };
+  // y faces at (a, b) = (i + 1, j) and (i + 1, j - 1)
+  auto fy_u = [&](int a, int b) {
+    const float Vf = 0.5f * w.V(a - 1, b) + 0.5f * w.V(a, b);
+    return vl_flux(Vf, w.U(a - 1, b), w.U(a - 1, b + 1), w.U(a, b - 1), w.U(a, b), w.U(a, b + 1),
+                   w.U(a, b + 2), hy);
+  };
+  const float dfx_u = (fx_u(lr + 1) - fx_u(lr)) / p.dx;
+  const float dfy_u = (fy_u(lr + 1, j) - fy_u(lr + 1, j - 1)) / p.dy;
+  const float adv_u = -(dfx_u + dfy_u);
+
+  // ---- v component (offset (.5, 1)) ----
+  // x faces at (a, b) = (i, j + 1) and (i - 1, j + 1)
+  auto fx_v = [&](int a, int b) {
+    const float Uf = 0.5f * w.U(a, b - 1) + 0.5f * w.U(a, b);
+    return vl_flux(Uf, w.V(a, b - 1), w.V(a + 1, b - 1), w.V(a - 1, b), w.V(a, b), w.V(a + 1, b),
+                   w.V(a + 2, b), hx);
+  };
+  // y faces at b = j + 1 and b = j (row i)
+  auto fy_v = [&](int b) {
+    const float Vf = 0.5f * w.V(lr, b - 1) + 0.5f * w.V(lr, b);
+    return vl_flux(Vf, w.V(lr, b - 1), w.V(lr, b), w.V(lr, b - 2), w.V(lr, b - 1), w.V(lr, b),
+                   w.V(lr, b + 1), hy);
+  };
+  const float dfx_v = (fx_v(lr, j + 1) - fx_v(lr - 1, j + 1)) / p.dx;
+  const float dfy_v = (fy_v(j + 1) - fy_v(j)) / p.dy;
+  const float adv_v = -(dfx_v + dfy_v);
+
+  ku = adv_u;
+  kv = adv_v;
+  if (p.has_diffusion) {
+    const float uc = w.U(lr, j), vc = w.V(lr, j);
+    const float lap_u = ((-2.f * uc) * p.lap_sum + (w.U(lr - 1, j) + w.U(lr + 1, j)) * p.inv_dx_sq) +
+                        (w.U(lr, j - 1) + w.U(lr, j + 1)) * p.inv_dy_sq;
+    const float lap_v = ((-2.f * vc) * p.lap_sum + (w.V(lr - 1, j) + w.V(lr + 1, j)) * p.inv_dx_sq) +
+                        (w.V(lr, j - 1) + w.V(lr, j + 1)) * p.inv_dy_sq;
+    ku = ku + p.nu * lap_u;
+    kv = kv + p.nu * lap_v;
+  }
+  if (p.has_forcing) {
+    const float fu = p.lin * w.U(lr, j) + fx_j;
+    const float fv = p.lin * w.V(lr, j) + 0.f;
+    ku = ku + fu / p.rho;
+    kv = kv + fv / p.rho;
+  }
+}
+
+template <int N, int ROWS>
+__device__ __forceinline__ void stage_rows(float* su, float* sv, const float* __restrict__ u,
+                                           const float* __restrict__ v, int row0, int tid) {
+  for (int e = tid; e < ROWS * N; e += kThreads) {
+    const int lr = e / N, j = e % N;
+    const int gi = (row0 + lr) & (N - 1);
+    su[e] = u[gi * N + j];
+    sv[e] = v[gi * N + j];
+  }
+}
+
+// MODE 0: full step (write u*, v*, spectrum of div u*).
+// MODE 1: projection only (spectrum of div u for the given u, v).
+// MODE 2: explicit terms only (write k_u, k_v to us, vs).
+// MODE 3: spectrum of the field b.u itself (spectral filtering).
+template <int N, int RB, int MODE>
+__global__ __launch_bounds__(kThreads) void fvm_rows_forward(FvmParams p, FvmBuffers b) {
+  constexpr int H = N / 2;
+  constexpr int WR = RB + 5;  // staged rows i0-3 .. i0+RB+1
+  __shared__ float su[WR * N];
+  __shared__ float sv[WR * N];
+  __shared__ float sus[(RB + 1) * N];  // u* rows i0-1 .. i0+RB-1
+  __shared__ float svs[RB * N];        // v* rows i0 .. i0+RB-1
+  __shared__ float2 zb[RB * H];        // div rows, then their FFTs
+  __shared__ float2 twn[N];
+  __shared__ float2 twh[H];
+  const int tid = threadIdx.x;
+  const int i0 = blockIdx.x * RB;
+
+  if constexpr (MODE == 3) {
+    float* zf3 = reinterpret_cast<float*>(zb);
+    for (int e = tid; e < RB * N; e += kThreads) zf3[e] = b.u[(i0 + e / N) * N + e % N];
+  } else if constexpr (MODE == 1) {
+    // u* = u: stage rows i0-1 .. i0+RB-1 of u and rows i0 .. of v directly.
+    for (int e = tid; e < (RB + 1) * N; e += kThreads) {
+      const int lr = e / N, j = e % N;
+      sus[e] = b.u[((i0 - 1 + lr) & (N - 1)) * N + j];
+      if (lr < RB) svs[e] = b.v[((i0 + lr) & (N - 1)) * N + j];
+    }
+  } else {
+    stage_rows<N, WR>(su, sv, b.u, b.v, i0 - 3, tid);
+  }
+  if constexpr (MODE != 2) {
+    load_twiddles(twn, b.tw, N, 1, tid, kThreads);
+    load_twiddles(twh, b.tw, H, 2, tid, kThreads);
+  }
+  __syncthreads();
+
+  if constexpr (MODE == 0 || MODE == 2) {
+    Window<N> w{su, sv};
+    // rows i0-1 .. i0+RB-1 -> local rows 2 .. RB+2 of the window
+    const int first = (MODE == 2) ? 1 : 0;
+    for (int e = tid + first * N; e < (RB + 1) * N; e += kThreads) {
+      const int r = e / N, j = e % N;  // r = 0 -> row i0 - 1
+      const int lr = r + 2;
+      const float fxj = p.has_forcing ? b.fx[j] : 0.f;
+      float ku, kv;
+      explicit_terms_at<N>(w, lr, j, p, fxj, ku, kv);
+      const int gi = (i0 - 1 + r) & (N - 1);
+      if constexpr (MODE == 2) {
+        b.us[gi * N + j] = ku;
+        b.vs[gi * N + j] = kv;
+      } else {
+        const float ust = w.U(lr, j) + p.dt * ku;
+        sus[e] = ust;
+        if (r >= 1) {
+          const float vst = w.V(lr, j) + p.dt * kv;
+          svs[e - N] = vst;
+          b.us[gi * N + j] = ust;
+          b.vs[gi * N + j] = vst;
+        }
+      }
+    }
+    if constexpr (MODE == 2) return;
+    __syncthreads();
+  }
+
+  if constexpr (MODE != 3) {
+    // div u* (finite_differences.py:136-143): (0 + (u*[i] - u*[i-1]) / dx) + (v*[j] - v*[j-1]) / dy
+    float* zf = reinterpret_cast<float*>(zb);
+    for (int e = tid; e < RB * N; e += kThreads) {
+      const int r = e / N, j = e % N;
+      const float ddx = (sus[(r + 1) * N + j] - sus[r * N + j]) / p.dx;
+      const float ddy = (svs[r * N + j] - svs[r * N + ((j - 1) & (N - 1))]) / p.dy;
+      zf[e] = ddx + ddy;
+    }
+    __syncthreads();
+  }
+  lds_fft<H, RB, kThreads, 1, H, false, false>(zb, twh, tid);
+  for (int e = tid; e < RB * H; e += kThreads) {
+    const int r = e / H, k = e % H;
+    b.spec[(i0 + r) * H + k] = r2c_combine(zb + r * H, k, H, twn[k]);
+  }
+}
+
+// Column pass of the Poisson solve on a tile of W packed-spectrum columns.
+template <int N, int W>
+__global__ __launch_bounds__(kThreads) void fvm_poisson_cols(FvmBuffers b, float scale) {
+  constexpr int H = N / 2;
+  constexpr int PER = (N * W + kThreads - 1) / kThreads;
+  __shared__ float2 buf[N * W];
+  __shared__ float2 twn[N];
+  const int tid = threadIdx.x;
+  const int c0 = blockIdx.x * W;
+  for (int e = tid; e < N * W; e += kThreads) {
+    const int pp = e / W, c = e % W;
+    buf[e] = b.spec[pp * H + c0 + c];
+  }
+  load_twiddles(twn, b.tw, N, 1, tid, kThreads);
+  __syncthreads();
+  lds_fft<N, W, kThreads, W, 1, true, false>(buf, twn, tid);
+  float2 nv[PER];
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    const int e = tid + q * kThreads;
+    if (e < N * W) {
+      const int pp = e / W, c = e % W, cg = c0 + c;
+      const float2 y = buf[e];
+      if (cg == 0) {
+        // word 0 packs the DC (real) and Nyquist (imaginary) columns.
+        const float2 ym = cconj(buf[((N - pp) & (N - 1)) * W + c]);
+        const float2 f0 = make_float2(0.5f * (y.x + ym.x), 0.5f * (y.y + ym.y));
+        const float2 d = csub(y, ym);
+        const float2 fh = make_float2(0.5f * d.y, -0.5f * d.x);
+        const float l0 = b.lam[pp * (H + 1)] * scale;
+        const float lh = b.lam[pp * (H + 1) + H] * scale;
+        const float2 g0 = cscale(f0, l0), gh = cscale(fh, lh);
+        nv[q] = make_float2(g0.x - gh.y, g0.y + gh.x);  // g0 + i gh
+      } else {
+        nv[q] = cscale(y, b.lam[pp * (H + 1) + cg] * scale);
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    const int e = tid + q * kThreads;
+    if (e < N * W) buf[e] = nv[q];
+  }
+  __syncthreads();
+  lds_fft<N, W, kThreads, W, 1, true, true>(buf, twn, tid);
+  for (int e = tid; e < N * W; e += kThreads) {
+    const int pp = e / W, c = e % W;
+    b.spec[pp * H + c0 + c] = buf[e];
+  }
+}
+
+// Inverse row transforms of q and the velocity correction.
+// MODE 0: velocity correction; MODE 1: write q itself to u_out.
+template <int N, int RB, int MODE>
+__global__ __launch_bounds__(kThreads) void fvm_rows_project(FvmParams p, FvmBuffers b) {
+  constexpr int H = N / 2;
+  constexpr int T = RB + 1;
+  __shared__ float2 qb[T * H];
+  __shared__ float2 twn[N];
+  __shared__ float2 twh[H];
+  const int tid = threadIdx.x;
+  const int i0 = blockIdx.x * RB;
+  load_twiddles(twn, b.tw, N, 1, tid, kThreads);
+  load_twiddles(twh, b.tw, H, 2, tid, kThreads);
+  __syncthreads();
+  for (int e = tid; e < T * H; e += kThreads) {
+    const int r = e / H, k = e % H;
+    const int gi = (i0 + r) & (N - 1);
+    const float2 pk = b.spec[gi * H + k];
+    const float2 pmk = b.spec[gi * H + ((H - k) & (H - 1))];
+    qb[e] = c2r_combine(pk, pmk, k, twn[k]);
+  }
+  __syncthreads();
+  lds_fft<H, T, kThreads, 1, H, false, true>(qb, twh, tid);
+  const float* q = reinterpret_cast<const float*>(qb);
+  if constexpr (MODE == 1) {
+    for (int e = tid; e < RB * N; e += kThreads) b.u_out[i0 * N + e] = q[e];
+    return;
+  }
+  for (int e = tid; e < RB * N; e += kThreads) {
+    const int r = e / N, j = e % N;
+    const int gi = i0 + r;
+    const float qc = q[r * N + j];
+    const float gx = (q[(r + 1) * N + j] + (-qc)) / p.dx;
+    const float gy = (q[r * N + ((j + 1) & (N - 1))] + (-qc)) / p.dy;
+    b.u_out[gi * N + j] = b.us[gi * N + j] - gx;
+    b.v_out[gi * N + j] = b.vs[gi * N + j] - gy;
+  }
+}
+
+// curl_2d (finite_differences.py:186-193): forward differences, offset (1, 1).
+__global__ void fvm_curl_kernel(FvmParams p, const float* __restrict__ u,
+                                const float* __restrict__ v, float* __restrict__ out) {
+  const int n = p.n;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n * n) return;
+  const int i = e / n, j = e % n;
+  const int ip = (i + 1) % n, jp = (j + 1) % n;
+  const float dvdx = (v[ip * n + j] + (-v[e])) / p.dx;
+  const float dudy = (u[i * n + jp] + (-u[e])) / p.dy;
+  out[e] = dvdx - dudy;
+}
+
+__global__ void fvm_divergence_kernel(FvmParams p, const float* __restrict__ u,
+                                      const float* __restrict__ v, float* __restrict__ out) {
+  const int n = p.n;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n * n) return;
+  const int i = e / n, j = e % n;
+  const int im = (i - 1 + n) % n, jm = (j - 1 + n) % n;
+  const float ddx = (u[e] - u[im * n + j]) / p.dx;
+  const float ddy = (v[e] - v[i * n + jm]) / p.dy;
+  out[e] = ddx + ddy;
+}
+
+// Per-size tile shapes: RB rows per row block, W columns per column tile.
+template <int N> struct Tiles;
+template <> struct Tiles<32> { static constexpr int RB = 4, W = 8; };
+template <> struct Tiles<64> { static constexpr int RB = 4, W = 8; };
+template <> struct Tiles<128> { static constexpr int RB = 4, W = 8; };
+template <> struct Tiles<256> { static constexpr int RB = 4, W = 8; };
+template <> struct Tiles<512> { static constexpr int RB = 2, W = 8; };
+template <> struct Tiles<1024> { static constexpr int RB = 1, W = 4; };
+
+template <int N>
+int launch_step(const FvmParams& p, const FvmBuffers& b, hipStream_t s) {
+  constexpr int RB = Tiles<N>::RB, W = Tiles<N>::W;
+  hipLaunchKernelGGL((fvm_rows_forward<N, RB, 0>), dim3(N / RB), dim3(kThreads), 0, s, p, b);
+  hipLaunchKernelGGL((fvm_poisson_cols<N, W>), dim3(N / 2 / W), dim3(kThreads), 0, s, b, 1.0f);
+  hipLaunchKernelGGL((fvm_rows_project<N, RB, 0>), dim3(N / RB), dim3(kThreads), 0, s, p, b);
+  return check_hip(hipGetLastError(), "fvm step launch");
+}
+
+template <int N>
+int launch_one(const FvmParams& p, const FvmBuffers& b, int which, hipStream_t s) {
+  constexpr int RB = Tiles<N>::RB, W = Tiles<N>::W;
+  switch (which) {
+    case 0:
+      hipLaunchKernelGGL((fvm_rows_forward<N, RB, 0>), dim3(N / RB), dim3(kThreads), 0, s, p, b);
+      break;
+    case 1:
+      hipLaunchKernelGGL((fvm_poisson_cols<N, W>), dim3(N / 2 / W), dim3(kThreads), 0, s, b, 1.0f);
+      break;
+    default:
+      hipLaunchKernelGGL((fvm_rows_project<N, RB, 0>), dim3(N / RB), dim3(kThreads), 0, s, p, b);
+  }
+  return check_hip(hipGetLastError(), "fvm kernel launch");
+}
+
+template <int N>
+int launch_explicit(const FvmParams& p, const FvmBuffers& b, hipStream_t s) {
+  constexpr int RB = Tiles<N>::RB;
+  hipLaunchKernelGGL((fvm_rows_forward<N, RB, 2>), dim3(N / RB), dim3(kThreads), 0, s, p, b);
+  return check_hip(hipGetLastError(), "fvm explicit-terms launch");
+}
+
+template <int N>
+int launch_projection(const FvmParams& p, const FvmBuffers& b, hipStream_t s) {
+  constexpr int RB = Tiles<N>::RB, W = Tiles<N>::W;
+  hipLaunchKernelGGL((fvm_rows_forward<N, RB, 1>), dim3(N / RB), dim3(kThreads), 0, s, p, b);
+  hipLaunchKernelGGL((fvm_poisson_cols<N, W>), dim3(N / 2 / W), dim3(kThreads), 0, s, b, 1.0f);
+  hipLaunchKernelGGL((fvm_rows_project<N, RB, 0>), dim3(N / RB), dim3(kThreads), 0, s, p, b);
+  return check_hip(hipGetLastError(), "fvm projection launch");
+}
+
+template <int N>
+int launch_filter(const FvmParams& p, const FvmBuffers& b, hipStream_t s) {
+  constexpr int RB = Tiles<N>::RB, W = Tiles<N>::W;
+  const float scale = 1.0f / (static_cast<float>(N) * static_cast<float>(N));
+  hipLaunchKernelGGL((fvm_rows_forward<N, RB, 3>), dim3(N / RB), dim3(kThreads), 0, s, p, b);
+  hipLaunchKernelGGL((fvm_poisson_cols<N, W>), dim3(N / 2 / W), dim3(kThreads), 0, s, b, scale);
+  hipLaunchKernelGGL((fvm_rows_project<N, RB, 1>), dim3(N / RB), dim3(kThreads), 0, s, p, b);
+  return check_hip(hipGetLastError(), "spectral filter launch");
+}
+
+template <template <int> class F, typename... Args>
+int dispatch_n(int n, Args&&... args) {
+  switch (n) {
+    case 32: return F<32>::run(args...);
+    case 64: return F<64>::run(args...);
+    case 128: return F<128>::run(args...);
+    case 256: return F<256>::run(args...);
+    case 512: return F<512>::run(args...);
+    case 1024: return F<1024>::run(args...);
+    default: return fail(JCFD_ERR_UNSUPPORTED, "grid size not supported: " + std::to_string(n));
+  }
+}
+
+template <int N> struct StepOp {
+  static int run(const FvmParams& p, const FvmBuffers& b, hipStream_t s) { return launch_step<N>(p, b, s); }
+};
+template <int N> struct ExplicitOp {
+  static int run(const FvmParams& p, const FvmBuffers& b, hipStream_t s) { return launch_explicit<N>(p, b, s); }
+};
+template <int N> struct OneOp {
+  static int run(const FvmParams& p, const FvmBuffers& b, int which, hipStream_t s) {
+    return launch_one<N>(p, b, which, s);
+  }
+};
+template <int N> struct FilterOp {
+  static int run(const FvmParams& p, const FvmBuffers& b, hipStream_t s) { return launch_filter<N>(p, b, s); }
+};
+template <int N> struct ProjOp {
+  static int run(const FvmParams& p, const FvmBuffers& b, hipStream_t s) { return launch_projection<N>(p, b, s); }
+};
+
+}  // namespace
+
+bool fvm_size_supported(int n) {
+  return n == 32 || n == 64 || n == 128 || n == 256 || n == 512 || n == 1024;
+}
+
+int fvm_launch_step(const FvmParams& p, const FvmBuffers& b, hipStream_t s) {
+  return dispatch_n<StepOp>(p.n, p, b, s);
+}
+
+int fvm_launch_kernel(const FvmParams& p, const FvmBuffers& b, int which, hipStream_t s) {
+  return dispatch_n<OneOp>(p.n, p, b, which, s);
+}
+
+int fvm_launch_explicit_terms(const FvmParams& p, const float* u, const float* v, float* du,
+                              float* dv, const float* fx, hipStream_t s) {
+  FvmBuffers b{};
+  b.u = u;
+  b.v = v;
+  b.us = du;
+  b.vs = dv;
+  b.fx = fx;
+  return dispatch_n<ExplicitOp>(p.n, p, b, s);
+}
+
+int fvm_launch_projection(const FvmParams& p, const float* u, const float* v, float* u_out,
+                          float* v_out, float2* spec, const float* lam, const float2* tw,
+                          hipStream_t s) {
+  FvmBuffers b{};
+  b.u = u;
+  b.v = v;
+  b.us = const_cast<float*>(u);
+  b.vs = const_cast<float*>(v);
+  b.u_out = u_out;
+  b.v_out = v_out;
+  b.spec = spec;
+  b.lam = lam;
+  b.tw = tw;
+  return dispatch_n<ProjOp>(p.n, p, b, s);
+}
+
+int fvm_launch_filter(const FvmParams& p, const float* x, const float* multiplier, float* out,
+                      float2* spec, const float2* tw, hipStream_t s) {
+  FvmBuffers b{};
+  b.u = x;
+  b.u_out = out;
+  b.spec = spec;
+  b.lam = multiplier;
+  b.tw = tw;
+  return dispatch_n<FilterOp>(p.n, p, b, s);
+}
+
+int fvm_launch_curl(const FvmParams& p, const float* u, const float* v, float* out, hipStream_t s) {
+  const int total = p.n * p.n;
+  hipLaunchKernelGGL(fvm_curl_kernel, dim3((total + 255) / 256), dim3(256), 0, s, p, u, v, out);
+  return check_hip(hipGetLastError(), "curl launch");
+}
+
+int fvm_launch_divergence(const FvmParams& p, const float* u, const float* v, float* out,
+                          hipStream_t s) {
+  const int total = p.n * p.n;
+  hipLaunchKernelGGL(fvm_divergence_kernel, dim3((total + 255) / 256), dim3(256), 0, s, p, u, v, out);
+  return check_hip(hipGetLastError(), "divergence launch");
+}
+
+}  // namespace jcfd

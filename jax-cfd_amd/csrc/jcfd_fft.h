@@ -1,0 +1,198 @@
+// Workgroup-level FFT building blocks for gfx950 (device code only).
+//
+// Transforms live in LDS and are done in place with Stockham auto-sort passes
+// (radix 8, then 4, then 2): each pass loads R values per butterfly into
+// registers, waits on a barrier, applies twiddles and an R-point DFT in
+// registers, and writes the results back in auto-sorted order.  A workgroup
+// runs T independent transforms of length N at once; element e of transform t
+// is at buf[e * ES + t * TS], so the same code serves
+//   * row transforms      (transform-major: ES = 1, TS = row pitch), and
+//   * column-tile transforms (element-major: ES = tile width, TS = 1).
+// Butterflies are enumerated so that neighbouring lanes touch neighbouring
+// LDS words in both layouts (TFAST picks transform-index-fastest).
+//
+// Real transforms of length n use the half-length complex transform of
+// z[m] = x[2m] + i x[2m+1] (a real row in LDS reinterpreted as float2 is
+// already z) plus one combine step.  The "packed" half spectrum stores the
+// n/2 + 1 bins of a real row in n/2 complex words: word 0 holds
+// (X[0], X[n/2]) (both real), words 1..n/2-1 hold X[1..n/2-1].
+//
+// Conventions: forward = exp(-2 pi i jk/N), unnormalised; inverse =
+// exp(+2 pi i jk/N), unnormalised (callers fold 1/N into their multipliers).
+// Twiddle tables hold exp(-2 pi i k/N), k in [0, N), rounded from fp64.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace jcfd {
+
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
+__device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+// multiply by -i (forward) or +i (inverse)
+template <bool INV>
+__device__ __forceinline__ float2 mul_mi(float2 a) {
+  return INV ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
+}
+
+template <bool INV>
+__device__ __forceinline__ void dft2(float2& a, float2& b) {
+  float2 t = a;
+  a = cadd(t, b);
+  b = csub(t, b);
+}
+
+template <bool INV>
+__device__ __forceinline__ void dft4(float2& v0, float2& v1, float2& v2, float2& v3) {
+  float2 t0 = cadd(v0, v2), t1 = csub(v0, v2);
+  float2 t2 = cadd(v1, v3), t3 = mul_mi<INV>(csub(v1, v3));
+  v0 = cadd(t0, t2);
+  v2 = csub(t0, t2);
+  v1 = cadd(t1, t3);
+  v3 = csub(t1, t3);
+}
+
+template <bool INV>
+__device__ __forceinline__ void dft8(float2 (&v)[8]) {
+  constexpr float h = 0.70710678118654752440f;
+  dft4<INV>(v[0], v[2], v[4], v[6]);  // even half -> E0..E3 in v0,v2,v4,v6
+  dft4<INV>(v[1], v[3], v[5], v[7]);  // odd half  -> O0..O3 in v1,v3,v5,v7
+  // W8^s, s = 1..3 (forward: exp(-i pi s / 4))
+  float2 o1 = v[3], o2 = v[5], o3 = v[7];
+  o1 = INV ? make_float2(h * (o1.x - o1.y), h * (o1.x + o1.y))
+           : make_float2(h * (o1.x + o1.y), h * (o1.y - o1.x));
+  o2 = mul_mi<INV>(o2);
+  o3 = INV ? make_float2(-h * (o3.x + o3.y), h * (o3.x - o3.y))
+           : make_float2(h * (o3.y - o3.x), -h * (o3.x + o3.y));
+  float2 e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6], o0 = v[1];
+  v[0] = cadd(e0, o0);
+  v[4] = csub(e0, o0);
+  v[1] = cadd(e1, o1);
+  v[5] = csub(e1, o1);
+  v[2] = cadd(e2, o2);
+  v[6] = csub(e2, o2);
+  v[3] = cadd(e3, o3);
+  v[7] = csub(e3, o3);
+}
+
+template <int R, bool INV>
+__device__ __forceinline__ void dft_radix(float2 (&v)[R]) {
+  if constexpr (R == 2) {
+    dft2<INV>(v[0], v[1]);
+  } else if constexpr (R == 4) {
+    dft4<INV>(v[0], v[1], v[2], v[3]);
+  } else {
+    static_assert(R == 8, "radix");
+    dft8<INV>(v);
+  }
+}
+
+// One in-place Stockham pass of radix R with NS = product of earlier radices.
+template <int N, int R, int NS, int T, int NT, int ES, int TS, bool TFAST, bool INV>
+__device__ __forceinline__ void stockham_pass(float2* buf, const float2* tw, int tid) {
+  constexpr int NB = N / R;
+  constexpr int TOT = T * NB;
+  constexpr int BPT = (TOT + NT - 1) / NT;
+  constexpr int TWS = N / (NS * R);
+  float2 v[BPT][R];
+#pragma unroll
+  for (int b = 0; b < BPT; ++b) {
+    const int g = tid + b * NT;
+    if (TOT % NT == 0 || g < TOT) {
+      const int t = TFAST ? g % T : g / NB;
+      const int j = TFAST ? g / T : g % NB;
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[b][r] = buf[(j + r * NB) * ES + t * TS];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int b = 0; b < BPT; ++b) {
+    const int g = tid + b * NT;
+    if (TOT % NT == 0 || g < TOT) {
+      const int t = TFAST ? g % T : g / NB;
+      const int j = TFAST ? g / T : g % NB;
+      const int k = j % NS;
+      if constexpr (NS > 1) {
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          float2 w = tw[r * k * TWS];
+          if (INV) w.y = -w.y;
+          v[b][r] = cmul(v[b][r], w);
+        }
+      }
+      float2 x[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) x[r] = v[b][r];
+      dft_radix<R, INV>(x);
+      const int base = (j / NS) * NS * R + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) buf[(base + r * NS) * ES + t * TS] = x[r];
+    }
+  }
+  __syncthreads();
+}
+
+template <int N, int NS, int T, int NT, int ES, int TS, bool TFAST, bool INV>
+__device__ __forceinline__ void fft_passes(float2* buf, const float2* tw, int tid) {
+  if constexpr (NS < N) {
+    constexpr int REM = N / NS;
+    constexpr int R = (REM % 8 == 0) ? 8 : ((REM % 4 == 0) ? 4 : 2);
+    stockham_pass<N, R, NS, T, NT, ES, TS, TFAST, INV>(buf, tw, tid);
+    fft_passes<N, NS * R, T, NT, ES, TS, TFAST, INV>(buf, tw, tid);
+  }
+}
+
+// T in-place complex transforms of length N in LDS.  Callers must have a
+// barrier between writing `buf` and calling this; it ends with a barrier.
+template <int N, int T, int NT, int ES, int TS, bool TFAST, bool INV>
+__device__ __forceinline__ void lds_fft(float2* buf, const float2* tw, int tid) {
+  static_assert((N & (N - 1)) == 0 && N >= 2, "power-of-two length");
+  fft_passes<N, 1, T, NT, ES, TS, TFAST, INV>(buf, tw, tid);
+}
+
+// Copy a length-N twiddle table to LDS (stride 1 in the table of length
+// `table_n`; N must divide table_n).  No barrier.
+__device__ __forceinline__ void load_twiddles(float2* dst, const float2* __restrict__ src, int n,
+                                              int table_stride, int tid, int nt) {
+  for (int k = tid; k < n; k += nt) dst[k] = src[k * table_stride];
+}
+
+// After an in-place length-n/2 forward transform of z (row in LDS), the
+// packed half-spectrum word k of the real row: needs Z[k] and Z[n/2 - k].
+// twn = exp(-2 pi i k / n) for this k.
+__device__ __forceinline__ float2 r2c_combine(const float2* Z, int k, int half, float2 twn) {
+  if (k == 0) {
+    float2 z0 = Z[0];
+    return make_float2(z0.x + z0.y, z0.x - z0.y);
+  }
+  const float2 a = Z[k];
+  const float2 b = cconj(Z[half - k]);
+  // Ze = (a + b)/2, Zo = (a - b)/(2i); X = Ze + W^k Zo
+  const float2 ze = make_float2(0.5f * (a.x + b.x), 0.5f * (a.y + b.y));
+  const float2 d = csub(a, b);
+  const float2 zo = make_float2(0.5f * d.y, -0.5f * d.x);
+  return cadd(ze, cmul(twn, zo));
+}
+
+// Inverse combine: from packed half-spectrum words P[k], P[n/2 - k] build
+// Z[k] whose length-n/2 inverse transform interleaves the (unnormalised)
+// real output.  twn = exp(-2 pi i k / n).
+__device__ __forceinline__ float2 c2r_combine(float2 pk, float2 pmk, int k, float2 twn) {
+  float2 a, b;
+  if (k == 0) {
+    a = make_float2(pk.x, 0.f);  // X[0], imaginary part dropped
+    b = make_float2(pk.y, 0.f);  // conj(X[n/2]), real
+  } else {
+    a = pk;
+    b = cconj(pmk);
+  }
+  const float2 e = cadd(a, b);
+  const float2 o = cmul(csub(a, b), cconj(twn));
+  return make_float2(e.x - o.y, e.y + o.x);  // E + i O
+}
+
+}  // namespace jcfd

@@ -1,0 +1,170 @@
+"""Loader for libjcfd_amd.so and the DLPack hand-off to its C ABI.
+
+The library is the only compute path: if it is missing, or a call fails, an
+exception is raised -- there is no CPU fallback.  torch is imported first so
+the library binds to the HIP runtime torch already loaded (same soname).
+"""
+from __future__ import annotations
+
+import contextlib
+import ctypes
+import os
+import threading
+
+import torch  # noqa: F401  (must load before the HIP library; see docstring)
+import torch.utils.dlpack
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)),
+                        "libjcfd_amd.so")
+
+JCFD_OK = 0
+JCFD_ERR_INVALID_ARGUMENT = -1
+JCFD_ERR_UNSUPPORTED = -2
+JCFD_ERR_DEVICE = -3
+JCFD_ERR_OUT_OF_MEMORY = -4
+JCFD_ERR_BUDGET = -5
+
+CONVECT_VAN_LEER = 0
+CONVECT_VAN_LEER_LIMITERS = 1
+PRESSURE_FAST_DIAG = 0
+PRESSURE_CG = 1
+
+
+class BudgetExceededError(RuntimeError):
+  """A request would exceed the configured device-memory budget."""
+
+
+class JcfdDeviceError(RuntimeError):
+  """The HIP runtime reported an error."""
+
+
+class FvmConfig(ctypes.Structure):
+  _fields_ = [
+      ("n", ctypes.c_int32),
+      ("dx", ctypes.c_double), ("dy", ctypes.c_double),
+      ("dt", ctypes.c_double),
+      ("density", ctypes.c_double),
+      ("viscosity", ctypes.c_double),
+      ("has_viscosity", ctypes.c_int32),
+      ("convect", ctypes.c_int32),
+      ("pressure", ctypes.c_int32),
+      ("cg_rtol", ctypes.c_double), ("cg_atol", ctypes.c_double),
+      ("cg_maxiter", ctypes.c_int32),
+      ("has_forcing", ctypes.c_int32),
+      ("linear_coefficient", ctypes.c_double),
+      ("forcing_x", ctypes.POINTER(ctypes.c_float)),
+  ]
+
+
+_P = ctypes.c_void_p
+_SIGNATURES = {
+    "jcfd_last_error": (ctypes.c_char_p, []),
+    "jcfd_version": (ctypes.c_char_p, []),
+    "jcfd_fvm_plan_create": (ctypes.c_int, [ctypes.POINTER(FvmConfig), ctypes.POINTER(_P)]),
+    "jcfd_fvm_plan_destroy": (ctypes.c_int, [_P]),
+    "jcfd_fvm_plan_device_bytes": (ctypes.c_int64, [_P]),
+    "jcfd_fvm_step": (ctypes.c_int, [_P, _P, _P, _P, _P, ctypes.c_int64, _P]),
+    "jcfd_fvm_explicit_terms": (ctypes.c_int, [_P, _P, _P, _P, _P, _P]),
+    "jcfd_fvm_projection": (ctypes.c_int, [_P, _P, _P, _P, _P, _P]),
+    "jcfd_fvm_divergence": (ctypes.c_int, [_P, _P, _P, _P, _P]),
+    "jcfd_fvm_curl": (ctypes.c_int, [_P, _P, _P, _P, _P]),
+    "jcfd_fvm_spectral_filter": (ctypes.c_int, [_P, _P, _P, _P, _P]),
+}
+
+_lib = None
+_lock = threading.Lock()
+
+
+def exported_symbols():
+  return sorted(_SIGNATURES)
+
+
+def load():
+  """Load the library once; raise if it has not been built."""
+  global _lib
+  if _lib is not None:
+    return _lib
+  with _lock:
+    if _lib is None:
+      if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is missing: build it with `python jax-cfd_amd/build.py`"
+            " (there is no non-HIP fallback)")
+      lib = ctypes.CDLL(LIB_PATH)
+      for name, (res, args) in _SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+      _lib = lib
+  return _lib
+
+
+def last_error() -> str:
+  return load().jcfd_last_error().decode()
+
+
+def check(rc: int, what: str) -> None:
+  if rc == JCFD_OK:
+    return
+  msg = f"{what}: {last_error()}"
+  if rc == JCFD_ERR_INVALID_ARGUMENT:
+    raise ValueError(msg)
+  if rc == JCFD_ERR_UNSUPPORTED:
+    raise NotImplementedError(msg)
+  if rc == JCFD_ERR_OUT_OF_MEMORY:
+    raise MemoryError(msg)
+  if rc == JCFD_ERR_BUDGET:
+    raise BudgetExceededError(msg)
+  raise JcfdDeviceError(msg)
+
+
+# -------------------------------------------------------------- DLPack
+_PyCapsule_GetPointer = ctypes.pythonapi.PyCapsule_GetPointer
+_PyCapsule_GetPointer.restype = ctypes.c_void_p
+_PyCapsule_GetPointer.argtypes = [ctypes.py_object, ctypes.c_char_p]
+
+
+class _DLHandles:
+  """Keeps DLPack capsules alive while the C call uses their DLTensor*."""
+
+  def __init__(self):
+    self._caps = []
+
+  def __call__(self, t: torch.Tensor) -> int:
+    cap = torch.utils.dlpack.to_dlpack(t)
+    self._caps.append(cap)
+    # DLManagedTensor starts with its DLTensor, whose layout jcfd_tensor shares.
+    return _PyCapsule_GetPointer(cap, b"dltensor")
+
+
+@contextlib.contextmanager
+def dlpack_args():
+  h = _DLHandles()
+  try:
+    yield h
+  finally:
+    h._caps.clear()
+
+
+def stream_handle(device: torch.device) -> int:
+  return torch.cuda.current_stream(device).cuda_stream
+
+
+def as_device_tensor(x, dtype: torch.dtype, device=None) -> torch.Tensor:
+  """Zero-copy view of a DLPack-capable array, or a copy onto the GPU.
+
+  Device placement and dtype conversion are data movement, not compute.
+  """
+  if not isinstance(x, torch.Tensor):
+    if hasattr(x, "__dlpack__"):
+      try:
+        x = torch.from_dlpack(x)
+      except Exception:  # e.g. numpy arrays of unsupported dtype
+        x = torch.as_tensor(x)
+    else:
+      x = torch.as_tensor(x)
+  if device is None:
+    device = x.device if x.is_cuda else torch.device("cuda", torch.cuda.current_device())
+  if x.device != device or x.dtype != dtype:
+    x = x.to(device=device, dtype=dtype)
+  return x.contiguous()

@@ -1,0 +1,161 @@
+"""FVM plans: the Python side of the jcfd_fvm_* C ABI."""
+from __future__ import annotations
+
+import ctypes
+import dataclasses
+import functools
+from typing import Optional, Tuple
+
+import numpy as np
+import torch
+
+from jax_cfd_amd import _lib
+
+
+@dataclasses.dataclass(frozen=True)
+class FvmSpec:
+  """Everything a plan depends on (hashable, so plans can be shared)."""
+  n: int
+  dx: float
+  dy: float
+  dt: float = 0.0
+  density: float = 1.0
+  viscosity: Optional[float] = None
+  convect: int = _lib.CONVECT_VAN_LEER
+  pressure: int = _lib.PRESSURE_FAST_DIAG
+  cg_rtol: float = 1e-6
+  cg_atol: float = 1e-6
+  cg_maxiter: int = 0
+  linear_coefficient: Optional[float] = None  # None: no forcing at all
+  forcing_x: Optional[Tuple[float, ...]] = None  # f_x(y_j), n values
+
+
+def spec_from_grid(grid, **kw) -> FvmSpec:
+  if grid.ndim != 2 or grid.shape[0] != grid.shape[1]:
+    raise NotImplementedError(
+        f'the FVM kernels are built for square 2-D grids, got shape {grid.shape}')
+  return FvmSpec(n=grid.shape[0], dx=grid.step[0], dy=grid.step[1], **kw)
+
+
+class FvmPlan:
+  """Owns one jcfd_fvm_plan (device constants + scratch) on one GPU."""
+
+  def __init__(self, spec: FvmSpec, device: Optional[torch.device] = None):
+    lib = _lib.load()
+    self.spec = spec
+    self.device = device if device is not None else torch.device(
+        'cuda', torch.cuda.current_device())
+    cfg = _lib.FvmConfig()
+    cfg.n = spec.n
+    cfg.dx, cfg.dy = spec.dx, spec.dy
+    cfg.dt = spec.dt
+    cfg.density = spec.density
+    cfg.has_viscosity = int(spec.viscosity is not None)
+    cfg.viscosity = spec.viscosity if spec.viscosity is not None else 0.0
+    cfg.convect = spec.convect
+    cfg.pressure = spec.pressure
+    cfg.cg_rtol, cfg.cg_atol = spec.cg_rtol, spec.cg_atol
+    cfg.cg_maxiter = spec.cg_maxiter
+    cfg.has_forcing = int(spec.linear_coefficient is not None)
+    cfg.linear_coefficient = spec.linear_coefficient or 0.0
+    self._fx = None
+    if spec.forcing_x is not None:
+      self._fx = np.ascontiguousarray(np.asarray(spec.forcing_x, np.float32))
+      if self._fx.shape != (spec.n,):
+        raise ValueError('forcing_x must hold n values')
+      cfg.forcing_x = self._fx.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+    handle = ctypes.c_void_p()
+    with torch.cuda.device(self.device):
+      _lib.check(lib.jcfd_fvm_plan_create(ctypes.byref(cfg), ctypes.byref(handle)),
+                 'jcfd_fvm_plan_create')
+    self._handle = handle
+    self._lib = lib
+
+  @property
+  def device_bytes(self) -> int:
+    return int(self._lib.jcfd_fvm_plan_device_bytes(self._handle))
+
+  def close(self):
+    if getattr(self, '_handle', None) is not None and self._handle.value:
+      self._lib.jcfd_fvm_plan_destroy(self._handle)
+      self._handle = None
+
+  def __del__(self):
+    try:
+      self.close()
+    except Exception:  # interpreter shutdown
+      pass
+
+  def _field(self, x) -> torch.Tensor:
+    return _lib.as_device_tensor(x, torch.float32, self.device)
+
+  def _empty(self):
+    n = self.spec.n
+    return torch.empty((n, n), dtype=torch.float32, device=self.device)
+
+  def step(self, u, v, nsteps: int = 1, out=None):
+    """nsteps forward-Euler + projection steps; returns (u, v) tensors."""
+    u, v = self._field(u), self._field(v)
+    uo, vo = out if out is not None else (self._empty(), self._empty())
+    with _lib.dlpack_args() as dl:
+      rc = self._lib.jcfd_fvm_step(self._handle, dl(u), dl(v), dl(uo), dl(vo),
+                                   int(nsteps), _lib.stream_handle(self.device))
+    _lib.check(rc, 'jcfd_fvm_step')
+    return uo, vo
+
+  def explicit_terms(self, u, v):
+    u, v = self._field(u), self._field(v)
+    du, dv = self._empty(), self._empty()
+    with _lib.dlpack_args() as dl:
+      rc = self._lib.jcfd_fvm_explicit_terms(self._handle, dl(u), dl(v), dl(du),
+                                             dl(dv), _lib.stream_handle(self.device))
+    _lib.check(rc, 'jcfd_fvm_explicit_terms')
+    return du, dv
+
+  def projection(self, u, v):
+    u, v = self._field(u), self._field(v)
+    uo, vo = self._empty(), self._empty()
+    with _lib.dlpack_args() as dl:
+      rc = self._lib.jcfd_fvm_projection(self._handle, dl(u), dl(v), dl(uo), dl(vo),
+                                         _lib.stream_handle(self.device))
+    _lib.check(rc, 'jcfd_fvm_projection')
+    return uo, vo
+
+  def divergence(self, u, v):
+    u, v = self._field(u), self._field(v)
+    out = self._empty()
+    with _lib.dlpack_args() as dl:
+      rc = self._lib.jcfd_fvm_divergence(self._handle, dl(u), dl(v), dl(out),
+                                         _lib.stream_handle(self.device))
+    _lib.check(rc, 'jcfd_fvm_divergence')
+    return out
+
+  def curl(self, u, v):
+    u, v = self._field(u), self._field(v)
+    out = self._empty()
+    with _lib.dlpack_args() as dl:
+      rc = self._lib.jcfd_fvm_curl(self._handle, dl(u), dl(v), dl(out),
+                                   _lib.stream_handle(self.device))
+    _lib.check(rc, 'jcfd_fvm_curl')
+    return out
+
+  def spectral_filter(self, x, multiplier):
+    x = self._field(x)
+    m = _lib.as_device_tensor(multiplier, torch.float32, self.device)
+    out = self._empty()
+    with _lib.dlpack_args() as dl:
+      rc = self._lib.jcfd_fvm_spectral_filter(self._handle, dl(x), dl(m), dl(out),
+                                              _lib.stream_handle(self.device))
+    _lib.check(rc, 'jcfd_fvm_spectral_filter')
+    return out
+
+
+@functools.lru_cache(maxsize=64)
+def _cached_plan(spec: FvmSpec, device_index: int) -> FvmPlan:
+  return FvmPlan(spec, torch.device('cuda', device_index))
+
+
+def plan_for(spec: FvmSpec, device: Optional[torch.device] = None) -> FvmPlan:
+  if device is None:
+    device = torch.device('cuda', torch.cuda.current_device())
+  return _cached_plan(spec, device.index if device.index is not None else 0)

@@ -1,0 +1,66 @@
+"""Advection operators (jax_cfd/base/advection.py).
+
+``advect_van_leer`` is the scheme the FVM kernels implement.  Inside a step
+factory it is read symbolically; called directly on a velocity component it
+runs the explicit-terms kernel with diffusion and forcing switched off.
+"""
+from __future__ import annotations
+
+from jax_cfd_amd import _symbolic
+from jax_cfd_amd.base import _fvm, boundaries, grids
+
+
+def _component_index(c, v):
+  for i, u in enumerate(v):
+    if c is u or (c.data is u.data and c.offset == u.offset):
+      return i
+  return None
+
+
+def _probe_term(kind, c, v, dt):
+  if not isinstance(c.data, _symbolic.Affine) or c.data.const is not None \
+      or c.data.coef != 1.0:
+    raise NotImplementedError(f'{kind}: only the velocity components themselves '
+                              'can be advected by the fused kernels')
+  if not all(isinstance(u.data, _symbolic.Affine) for u in v):
+    raise NotImplementedError(f'{kind}: advecting velocity must be the state')
+  term = _symbolic.Term(kind, comp=c.data.comp, dt=dt)
+  return grids.GridArray(term, c.offset, c.grid)
+
+
+def advect_van_leer(c: grids.GridVariable, v: grids.GridVariableVector, dt: float,
+                    mode: str = boundaries.Padding.MIRROR) -> grids.GridArray:
+  """Van Leer flux-limited advection of velocity component `c` by `v`
+  (advection.py:210-342, including its limiter-stencil indexing)."""
+  del mode  # periodic only
+  if _symbolic.is_probe(c):
+    return _probe_term('advect_van_leer', c, v, dt)
+  idx = _component_index(c, v)
+  if idx is None or len(v) != 2:
+    raise NotImplementedError(
+        'advect_van_leer on the GPU advects the components of a 2-D staggered '
+        'velocity field by that field')
+  grid = grids.consistent_grid(*[u.array for u in v])
+  if not boundaries.has_all_periodic_boundary_conditions(*v):
+    raise NotImplementedError('non-periodic boundary conditions')
+  spec = _fvm.spec_from_grid(grid, dt=float(dt))
+  plan = _fvm.plan_for(spec, v[0].data.device)
+  du, dv = plan.explicit_terms(v[0].data, v[1].data)
+  return grids.GridArray((du, dv)[idx], c.offset, grid)
+
+
+def advect_van_leer_using_limiters(c, v, dt):
+  """Van Leer via TVD-limited Lax-Wendroff (advection.py:387-395): the
+  default `convect` of semi_implicit_navier_stokes.  Recognised by the step
+  factory, which reports that this kernel variant is not built."""
+  if _symbolic.is_probe(c):
+    return _probe_term('advect_van_leer_using_limiters', c, v, dt)
+  raise NotImplementedError(
+      'advect_van_leer_using_limiters has no GPU kernel yet; use advect_van_leer')
+
+
+def stable_time_step(max_velocity: float, max_courant_number: float,
+                     grid: grids.Grid) -> float:
+  """advection.py:398-416."""
+  dx = min(grid.step)
+  return max_courant_number * dx / max_velocity

@@ -1,0 +1,93 @@
+"""The step factories read the reference-style operator callables on the
+host (no GPU needed): supported compositions map to a kernel spec, others
+raise NotImplementedError (never a silent fallback)."""
+import functools
+
+import numpy as np
+import pytest
+
+import jax_cfd_amd.base as cfd
+from jax_cfd_amd import _lib
+
+TWO_PI = 2 * np.pi
+
+
+def _grid(n=64):
+  return cfd.grids.Grid((n, n), domain=((0, TWO_PI), (0, TWO_PI)))
+
+
+def _vl(dt):
+  return lambda v: tuple(cfd.advection.advect_van_leer(u, v, dt) for u in v)
+
+
+def test_forced_configuration_spec():
+  grid = _grid(256)
+  dt = cfd.equations.stable_time_step(7.0, 0.5, 1e-3, grid)
+  assert dt == pytest.approx(0.0017531209004407328, rel=1e-15)
+  forcing = cfd.forcings.simple_turbulence_forcing(
+      grid, constant_magnitude=1.0, constant_wavenumber=4,
+      linear_coefficient=-0.1, forcing_type='kolmogorov')
+  step = cfd.equations.semi_implicit_navier_stokes(
+      density=1.0, viscosity=1e-3, dt=dt, grid=grid, convect=_vl(dt),
+      forcing=forcing)
+  s = step.spec
+  assert s.n == 256 and s.viscosity == 1e-3 and s.linear_coefficient == -0.1
+  y = (np.arange(256, dtype=np.float32) + np.float32(0.5)) * np.float32(grid.step[1])
+  np.testing.assert_array_equal(np.asarray(s.forcing_x, np.float32),
+                                np.sin(4 * y).astype(np.float32))
+  assert s.convect == _lib.CONVECT_VAN_LEER
+
+
+def test_unforced_configuration_spec():
+  grid = _grid(64)
+  step = cfd.equations.semi_implicit_navier_stokes(1.0, 1e-3, 0.02, grid,
+                                                   convect=_vl(0.02))
+  assert step.spec.linear_coefficient is None and step.spec.forcing_x is None
+
+
+def test_kolmogorov_only_and_linear_only():
+  grid = _grid(64)
+  f1 = cfd.forcings.kolmogorov_forcing(grid, scale=2.0, k=3)
+  s1 = cfd.equations.semi_implicit_navier_stokes(1.0, 1e-3, 0.02, grid,
+                                                 convect=_vl(0.02), forcing=f1).spec
+  assert s1.linear_coefficient == 0.0 and max(s1.forcing_x) > 1.9
+  f2 = cfd.forcings.linear_forcing(grid, 0.3)
+  s2 = cfd.equations.semi_implicit_navier_stokes(1.0, None, 0.02, grid,
+                                                 convect=_vl(0.02), forcing=f2).spec
+  assert s2.linear_coefficient == 0.3 and s2.viscosity is None
+  assert not any(s2.forcing_x)
+
+
+@pytest.mark.parametrize("kw", [
+    dict(convect=None),  # reference default: advect_van_leer_using_limiters
+    dict(convect=lambda v: tuple(cfd.advection.advect_van_leer(v[0], v, 0.02) for _ in v)),
+    dict(convect=lambda v: tuple(cfd.advection.advect_van_leer(u, v, 0.01) for u in v)),
+    dict(convect=_vl(0.02), forcing=cfd.forcings.kolmogorov_forcing(_grid(), swap_xy=True)),
+    dict(convect=_vl(0.02), time_stepper=cfd.time_stepping.classic_rk4),
+    dict(convect=_vl(0.02), diffuse=lambda c, nu: c.array),
+])
+def test_unsupported_compositions_raise(kw):
+  with pytest.raises(NotImplementedError):
+    cfd.equations.semi_implicit_navier_stokes(1.0, 1e-3, 0.02, _grid(), **kw)
+
+
+def test_non_square_grid_raises():
+  grid = cfd.grids.Grid((64, 32), domain=((0, TWO_PI), (0, TWO_PI)))
+  with pytest.raises(NotImplementedError):
+    cfd.equations.semi_implicit_navier_stokes(1.0, 1e-3, 0.02, grid,
+                                              convect=_vl(0.02))
+
+
+def test_grid_matches_reference_geometry():
+  """grids.py:499-658: steps, faces and fp32 meshes."""
+  grid = cfd.grids.Grid((8, 8), domain=((0, TWO_PI), (0, TWO_PI)))
+  assert grid.step == (TWO_PI / 8, TWO_PI / 8)
+  assert grid.cell_faces == ((1.0, 0.5), (0.5, 1.0))
+  x, y = grid.mesh((1.0, 0.5))
+  assert x.dtype == np.float32
+  np.testing.assert_allclose(x[:, 0], (np.arange(8) + 1) * TWO_PI / 8, rtol=1e-6)
+  np.testing.assert_allclose(y[0], (np.arange(8) + .5) * TWO_PI / 8, rtol=1e-6)
+  kx, ky = grid.rfft_mesh()
+  assert kx.shape == (8, 5)
+  np.testing.assert_allclose(2 * np.pi * ky[0], np.arange(5), atol=1e-5)
+  np.testing.assert_allclose(2 * np.pi * kx[:, 0], [0, 1, 2, 3, -4, -3, -2, -1], atol=1e-5)
