@@ -1,0 +1,147 @@
+#!/usr/bin/env python3
+"""Throughput of jax-cfd's FVM time step on one MI355X (gfx950).
+
+Workload: the forced 256x256 periodic MAC-grid run -- fp32, viscosity 1e-3,
+density 1, Kolmogorov forcing sin(4y) scale 1 with linear drag 0.1,
+advect_van_leer, explicit diffusion, fast-diagonalisation projection,
+dt = stable_time_step(v_max=7, CFL=0.5), initial velocity
+filtered_velocity_field(seed 1, peak wavenumber 4, v_max 7).  A bench
+"step" is one FVM time step of that instance; the default K = 2048 steps is
+one full rollout.  `value` is cell-steps/s over the K timed steps with the
+state already resident in HBM.
+
+Also reported: per-kernel HIP-event durations and the HBM roofline fraction
+of the dominant kernel (algorithmic bytes / mean launch time / 8 TB/s), and
+the CPU oracle (NumPy restatement of the reference, one core) timed on a
+bounded sample of the same workload on this host.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for _p in (ROOT, os.path.join(ROOT, "jax-cfd_amd")):
+  if _p not in sys.path:
+    sys.path.insert(0, _p)
+
+METRIC = ("served vs plain cell-steps/s at rel-L2≤1e-3; p50/p99 latency; "
+          "HBM %peak")
+HBM_PEAK_GBS = 8000.0
+N, SEED, VMAX, PEAK, NU, CFL = 256, 1, 7.0, 4, 1e-3, 0.5
+
+
+def _args():
+  ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+  ap.add_argument("--gpus", type=int, default=1)
+  ap.add_argument("--steps", type=int, default=2048)
+  ap.add_argument("--warmup", type=int, default=64)
+  ap.add_argument("--kernel-reps", type=int, default=200)
+  ap.add_argument("--cpu-seconds", type=float, default=15.0,
+                  help="wall-clock budget of the CPU oracle sample (0 = skip)")
+  return ap.parse_args()
+
+
+def _traffic(kernel, workload):
+  """HBM bytes per launch from the committed rocprofv3 counter summary."""
+  path = os.path.join(ROOT, "profiles", "traffic.json")
+  try:
+    with open(path) as f:
+      return json.load(f).get(workload, {}).get(kernel)
+  except (OSError, ValueError):
+    return None
+
+
+def _cpu_baseline(seconds):
+  import numpy as np
+  from oracle import fvm as ofvm
+  from oracle import grid_ops as go
+  from oracle import initial_conditions as oic
+  grid = go.Grid.from_domain((N, N), ((0, 2 * np.pi), (0, 2 * np.pi)))
+  v = oic.filtered_velocity_field(SEED, grid, VMAX, PEAK)
+  dt = ofvm.stable_time_step(VMAX, CFL, NU, grid)
+  step = ofvm.FVMStep(grid, dt=dt, viscosity=NU,
+                      forcing=go.Forcing(1.0, 4, -0.1))
+  v = step(v)  # warm-up
+  steps, t0 = 0, time.perf_counter()
+  while time.perf_counter() - t0 < seconds or steps < 3:
+    v = step(v)
+    steps += 1
+  wall = time.perf_counter() - t0
+  return {"value": N * N * steps / wall, "unit": "cell-steps/s", "cores": 1,
+          "kind": "port",
+          "sample": f"{steps} steps of the {N}x{N} forced FVM workload, NumPy "
+                    f"fp32 restatement of the reference step, single thread, "
+                    f"{wall:.1f} s"}
+
+
+def main():
+  args = _args()
+  if args.gpus != 1:
+    print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported here "
+          "(independent instances shard across GPUs with no communication; "
+          "see the multi-GPU section of DESIGN.md)", file=sys.stderr)
+    sys.exit(2)
+  import numpy as np
+  import torch
+  import jax_cfd_amd.base as cfd
+
+  dev = torch.device("cuda", 0)
+  torch.cuda.set_device(dev)
+  grid = cfd.grids.Grid((N, N), domain=((0, 2 * np.pi), (0, 2 * np.pi)))
+  dt = cfd.equations.stable_time_step(VMAX, CFL, NU, grid)
+  forcing = cfd.forcings.simple_turbulence_forcing(
+      grid, constant_magnitude=1.0, constant_wavenumber=4,
+      linear_coefficient=-0.1, forcing_type="kolmogorov")
+  step = cfd.equations.semi_implicit_navier_stokes(
+      density=1.0, viscosity=NU, dt=dt, grid=grid,
+      convect=lambda v: tuple(cfd.advection.advect_van_leer(u, v, dt) for u in v),
+      forcing=forcing)
+  v0 = cfd.initial_conditions.filtered_velocity_field(SEED, grid, VMAX, PEAK)
+  plan = step.plan(dev)
+  u, v = v0[0].data.clone(), v0[1].data.clone()
+  print(f"[bench] workload fvm {N}x{N} forced, dt={dt:.6g}; warmup {args.warmup}",
+        file=sys.stderr, flush=True)
+  plan.step(u, v, args.warmup, out=(u, v))
+  torch.cuda.synchronize()
+  t0 = time.perf_counter()
+  plan.step(u, v, args.steps, out=(u, v))
+  torch.cuda.synchronize()
+  wall = time.perf_counter() - t0
+  assert torch.isfinite(u).all() and torch.isfinite(v).all()
+  value = N * N * args.steps / wall
+  print(f"[bench] {args.steps} steps in {wall:.4f} s -> {value:.4g} cell-steps/s",
+        file=sys.stderr, flush=True)
+
+  kms = plan.time_kernels(u, v, args.kernel_reps)
+  kbytes = plan.kernel_bytes()
+  dominant = max(kms, key=kms.get)
+  achieved = kbytes[dominant] / (kms[dominant] * 1e-3) / 1e9
+  workload = f"fvm{N}"
+  roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+              "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+              "traffic": _traffic(dominant, workload), "kernel": dominant,
+              "bytes_per_launch": kbytes[dominant],
+              "ms_per_launch": round(kms[dominant], 6)}
+  kernels = {k: {"ms_per_launch": round(kms[k], 6), "algorithmic_bytes": kbytes[k],
+                 "GBps": round(kbytes[k] / (kms[k] * 1e-3) / 1e9, 2)} for k in kms}
+  print(f"[bench] kernels {json.dumps(kernels)}", file=sys.stderr, flush=True)
+
+  cpu = _cpu_baseline(args.cpu_seconds) if args.cpu_seconds > 0 else None
+  out = {
+      "metric": METRIC, "value": value, "unit": "cell-steps/s", "n_gpus": 1,
+      "steps": args.steps, "warmup": args.warmup,
+      "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True,
+      "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+      "data": "synthetic (seeded filtered_velocity_field)",
+      "config": {"workload": "fvm-256x256-kolmogorov-forced-vanleer-fastdiag",
+                 "n": N, "dt": dt, "viscosity": NU, "seed": SEED, "v_max": VMAX,
+                 "path": "plain (one instance, 3 kernels per step)"},
+      "roofline": roofline, "cpu_baseline": cpu, "kernels": kernels,
+  }
+  print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+  main()

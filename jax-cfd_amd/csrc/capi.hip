@@ -11,6 +11,7 @@
 
 #include "jcfd.h"
 #include "jcfd_internal.h"
+#include "jcfd_profile.h"
 
 namespace jcfd {
 

@@ -130,6 +130,23 @@ class FvmPlan:
     _lib.check(rc, 'jcfd_fvm_divergence')
     return out
 
+  KERNELS = ('fvm_rows_forward', 'fvm_poisson_cols', 'fvm_rows_project')
+
+  def time_kernels(self, u, v, reps: int = 200):
+    """Mean ms per launch of each step kernel (HIP events on this stream)."""
+    u, v = self._field(u), self._field(v)
+    ms = (ctypes.c_double * 3)()
+    with _lib.dlpack_args() as dl:
+      rc = self._lib.jcfd_fvm_time_kernels(self._handle, dl(u), dl(v), int(reps), ms,
+                                           _lib.stream_handle(self.device))
+    _lib.check(rc, 'jcfd_fvm_time_kernels')
+    return dict(zip(self.KERNELS, list(ms)))
+
+  def kernel_bytes(self):
+    b = (ctypes.c_int64 * 3)()
+    _lib.check(self._lib.jcfd_fvm_kernel_bytes(self._handle, b), 'jcfd_fvm_kernel_bytes')
+    return dict(zip(self.KERNELS, [int(x) for x in b]))
+
   def curl(self, u, v):
     u, v = self._field(u), self._field(v)
     out = self._empty()
