@@ -139,6 +139,48 @@ int jcfd_fvm_curl(jcfd_fvm_plan* plan, const jcfd_tensor* u, const jcfd_tensor* 
 int jcfd_fvm_spectral_filter(jcfd_fvm_plan* plan, const jcfd_tensor* x,
                              const jcfd_tensor* multiplier, jcfd_tensor* out, void* stream);
 
+/* ------------------------------------------------------- spectral path */
+typedef struct {
+  int32_t n;               /* cells per side; square periodic grid */
+  double dx, dy;           /* grid steps */
+  double viscosity, drag;  /* NavierStokes2D(viscosity, drag) */
+  int32_t smooth;          /* 1: apply the 2/3 filter (the reference applies it
+                              whenever smooth is not None) */
+  double dt;               /* crank_nicolson_rk4 time step */
+  int32_t batch;           /* instances advanced per call (same n), >= 1 */
+  int32_t n_forcing;       /* non-zero entries of the constant forcing curl */
+  const int32_t* forcing_index; /* host, 2 * n_forcing ints: (p, c), 0 <= c <= n/2 */
+  const float* forcing_value;   /* host, 2 * n_forcing floats: (re, im) */
+} jcfd_ns2d_config;
+
+typedef struct jcfd_ns2d_plan jcfd_ns2d_plan;
+
+/* Plan for crank_nicolson_rk4(NavierStokes2D(...), dt)
+ * (spectral/time_stepping.py:186-197, spectral/equations.py:129-220): the
+ * wavenumber tables, 2/3 filter bounds and the five stage scalars. */
+int jcfd_ns2d_plan_create(const jcfd_ns2d_config* config, jcfd_ns2d_plan** plan);
+int jcfd_ns2d_plan_destroy(jcfd_ns2d_plan* plan);
+int64_t jcfd_ns2d_plan_device_bytes(const jcfd_ns2d_plan* plan);
+
+/* nsteps CN-RK4 steps of every instance; w_hat/w_out complex64 of shape
+ * (batch, n, n/2 + 1), or (n, n/2 + 1) when batch == 1.  w_out may alias
+ * w_hat.  funcutils.repeated(step_fn, nsteps) (funcutils.py:82-88). */
+int jcfd_ns2d_step(jcfd_ns2d_plan* plan, const jcfd_tensor* w_hat, jcfd_tensor* w_out,
+                   int64_t nsteps, void* stream);
+
+/* NavierStokes2D.explicit_terms (spectral/equations.py:160-184). */
+int jcfd_ns2d_explicit_terms(jcfd_ns2d_plan* plan, const jcfd_tensor* w_hat, jcfd_tensor* out,
+                             void* stream);
+
+/* jnp.fft.rfftn / irfftn over the last two axes of a batch (the transforms
+ * spectral/equations.py:163-180 applies; also how the reference's spectral
+ * runs build their initial state, spectral/equations_test.py:216-219).
+ * x: float32 (batch, n, n) or (n, n); x_hat: complex64 (batch, n, n/2 + 1).
+ * irfft2 ignores the imaginary parts of the DC and Nyquist columns after the
+ * axis-0 inverse, as pocketfft's c2r does. */
+int jcfd_ns2d_rfft2(jcfd_ns2d_plan* plan, const jcfd_tensor* x, jcfd_tensor* x_hat, void* stream);
+int jcfd_ns2d_irfft2(jcfd_ns2d_plan* plan, const jcfd_tensor* x_hat, jcfd_tensor* x, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

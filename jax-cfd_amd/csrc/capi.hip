@@ -63,6 +63,35 @@ int check_tensor(const jcfd_tensor* t, const char* name, int code, int bits, int
   return JCFD_OK;
 }
 
+// A batch of `batch` (d1, d2) fields: shape (batch, d1, d2), or (d1, d2) when
+// batch == 1.  C-contiguous.
+int check_batched(const jcfd_tensor* t, const char* name, int code, int bits, int batch, int64_t d1,
+                  int64_t d2, int device, void** ptr) {
+  if (t == nullptr) return fail(JCFD_ERR_INVALID_ARGUMENT, std::string(name) + ": null tensor");
+  if (t->ndim == 2 && batch == 1) return check_tensor(t, name, code, bits, d1, d2, device, ptr);
+  if (t->device.device_type != kDLROCM)
+    return fail(JCFD_ERR_INVALID_ARGUMENT, std::string(name) + ": expected a ROCm device tensor");
+  if (device >= 0 && t->device.device_id != device)
+    return fail(JCFD_ERR_INVALID_ARGUMENT, std::string(name) + ": tensor on another device");
+  if (t->dtype.code != code || t->dtype.bits != bits || t->dtype.lanes != 1)
+    return fail(JCFD_ERR_INVALID_ARGUMENT, std::string(name) + ": wrong dtype (expected " +
+                                               (code == kDLFloat ? "float32" : "complex64") + ")");
+  if (t->ndim != 3 || t->shape == nullptr || t->shape[0] != batch || t->shape[1] != d1 ||
+      t->shape[2] != d2)
+    return fail(JCFD_ERR_INVALID_ARGUMENT, std::string(name) + ": expected shape (" +
+                                               std::to_string(batch) + ", " + std::to_string(d1) +
+                                               ", " + std::to_string(d2) + ")");
+  if (t->strides != nullptr && !(t->strides[2] == 1 && t->strides[1] == d2 && t->strides[0] == d1 * d2))
+    return fail(JCFD_ERR_INVALID_ARGUMENT, std::string(name) + ": must be C-contiguous");
+  if (t->data == nullptr) return fail(JCFD_ERR_INVALID_ARGUMENT, std::string(name) + ": null data");
+  *ptr = static_cast<char*>(t->data) + t->byte_offset;
+  return JCFD_OK;
+}
+
+int check_spectral(const jcfd_tensor* t, const char* name, int batch, int n, int device, void** ptr) {
+  return check_batched(t, name, kDLComplex, 64, batch, n, n / 2 + 1, device, ptr);
+}
+
 template <typename T>
 int device_alloc(T** p, size_t count, int64_t* total) {
   hipError_t e = hipMalloc(reinterpret_cast<void**>(p), count * sizeof(T));
@@ -123,6 +152,30 @@ struct jcfd_fvm_plan {
   float* fx = nullptr;
   int64_t bytes = 0;
 };
+
+struct jcfd_ns2d_plan {
+  jcfd_ns2d_config cfg;
+  int device = 0;
+  int n = 0, batch = 1;
+  NsParams stages[5];
+  float* kx = nullptr;
+  float* ky = nullptr;
+  float2* tw = nullptr;
+  int* force_idx = nullptr;
+  float2* force_val = nullptr;
+  float2* h = nullptr;
+  float2* g = nullptr;
+  float2* pbuf = nullptr;
+  int64_t bytes = 0;
+};
+
+namespace {
+// Carpenter-Kennedy CN-RK4 coefficients (spectral/time_stepping.py:192-194).
+const double kCnAlpha[6] = {0, 0.1496590219993, 0.3704009573644, 0.6222557631345, 0.9582821306748, 1};
+const double kCnBeta[5] = {0, -0.4178904745, -1.192151694643, -1.697784692471, -1.514183444257};
+const double kCnGamma[5] = {0.1496590219993, 0.3792103129999, 0.8229550293869, 0.6994504559488,
+                            0.1530572479681};
+}  // namespace
 
 extern "C" {
 
@@ -377,6 +430,180 @@ int jcfd_fvm_spectral_filter(jcfd_fvm_plan* p, const jcfd_tensor* x, const jcfd_
     return rc;
   return fvm_launch_filter(p->params, static_cast<const float*>(px), static_cast<const float*>(pm),
                            static_cast<float*>(po), p->spec, p->tw, static_cast<hipStream_t>(stream));
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------ spectral path
+extern "C" {
+
+int jcfd_ns2d_plan_destroy(jcfd_ns2d_plan* p) {
+  if (p == nullptr) return JCFD_OK;
+  for (void* ptr : {static_cast<void*>(p->kx), static_cast<void*>(p->ky), static_cast<void*>(p->tw),
+                    static_cast<void*>(p->force_idx), static_cast<void*>(p->force_val),
+                    static_cast<void*>(p->h), static_cast<void*>(p->g), static_cast<void*>(p->pbuf)})
+    if (ptr != nullptr) (void)hipFree(ptr);
+  delete p;
+  return JCFD_OK;
+}
+
+int jcfd_ns2d_plan_create(const jcfd_ns2d_config* config, jcfd_ns2d_plan** plan) {
+  if (config == nullptr || plan == nullptr) return fail(JCFD_ERR_INVALID_ARGUMENT, "null argument");
+  *plan = nullptr;
+  const jcfd_ns2d_config& c = *config;
+  if (!ns_size_supported(c.n))
+    return fail(JCFD_ERR_UNSUPPORTED,
+                "spectral grid size " + std::to_string(c.n) + " not built (32..512, powers of two)");
+  if (!(c.dx > 0) || !(c.dy > 0) || !std::isfinite(c.dx) || !std::isfinite(c.dy))
+    return fail(JCFD_ERR_INVALID_ARGUMENT, "grid steps must be positive and finite");
+  if (!std::isfinite(c.dt) || !std::isfinite(c.viscosity) || !std::isfinite(c.drag))
+    return fail(JCFD_ERR_INVALID_ARGUMENT, "dt, viscosity and drag must be finite");
+  if (c.batch < 1) return fail(JCFD_ERR_INVALID_ARGUMENT, "batch must be >= 1");
+  if (c.n_forcing < 0 || (c.n_forcing > 0 && (c.forcing_index == nullptr || c.forcing_value == nullptr)))
+    return fail(JCFD_ERR_INVALID_ARGUMENT, "forcing entries missing");
+  const int n = c.n, h1 = n / 2 + 1;
+  for (int t = 0; t < c.n_forcing; ++t) {
+    const int fp = c.forcing_index[2 * t], fc = c.forcing_index[2 * t + 1];
+    if (fp < 0 || fp >= n || fc < 0 || fc >= h1)
+      return fail(JCFD_ERR_INVALID_ARGUMENT, "forcing index out of range");
+  }
+  auto* p = new (std::nothrow) jcfd_ns2d_plan();
+  if (p == nullptr) return fail(JCFD_ERR_OUT_OF_MEMORY, "host allocation failed");
+  p->cfg = c;
+  p->cfg.forcing_index = nullptr;
+  p->cfg.forcing_value = nullptr;
+  p->n = n;
+  p->batch = c.batch;
+  if (hipGetDevice(&p->device) != hipSuccess) {
+    delete p;
+    return fail(JCFD_ERR_DEVICE, "hipGetDevice failed (no GPU?)");
+  }
+  // fftfreq / rfftfreq in fp32: ordinal / fp32(d * n) (grids.py:604-658)
+  std::vector<float> kx(n), ky(h1);
+  const float dnx = static_cast<float>(c.dx * n), dny = static_cast<float>(c.dy * n);
+  for (int i = 0; i < n; ++i) kx[i] = static_cast<float>(i < (n + 1) / 2 ? i : i - n) / dnx;
+  for (int i = 0; i < h1; ++i) ky[i] = static_cast<float>(i) / dny;
+  const std::vector<float2> tw = twiddles(n);
+  const size_t ext = static_cast<size_t>(n) * h1, half = static_cast<size_t>(n) * (n / 2);
+  const size_t B = static_cast<size_t>(c.batch);
+  const int nf = std::max(c.n_forcing, 1);
+  int rc = JCFD_OK;
+  if ((rc = device_alloc(&p->kx, n, &p->bytes)) || (rc = device_alloc(&p->ky, h1, &p->bytes)) ||
+      (rc = device_alloc(&p->tw, n, &p->bytes)) || (rc = device_alloc(&p->force_idx, 2 * nf, &p->bytes)) ||
+      (rc = device_alloc(&p->force_val, nf, &p->bytes)) || (rc = device_alloc(&p->h, B * ext, &p->bytes)) ||
+      (rc = device_alloc(&p->g, B * 4 * half, &p->bytes)) || (rc = device_alloc(&p->pbuf, B * half, &p->bytes))) {
+    jcfd_ns2d_plan_destroy(p);
+    return rc;
+  }
+  std::vector<int> fidx(2 * nf, 0);
+  std::vector<float2> fval(nf, make_float2(0.f, 0.f));
+  for (int t = 0; t < c.n_forcing; ++t) {
+    fidx[2 * t] = c.forcing_index[2 * t];
+    fidx[2 * t + 1] = c.forcing_index[2 * t + 1];
+    fval[t] = make_float2(c.forcing_value[2 * t], c.forcing_value[2 * t + 1]);
+  }
+  if ((rc = check_hip(hipMemcpy(p->kx, kx.data(), n * sizeof(float), hipMemcpyHostToDevice), "copy kx")) ||
+      (rc = check_hip(hipMemcpy(p->ky, ky.data(), h1 * sizeof(float), hipMemcpyHostToDevice), "copy ky")) ||
+      (rc = check_hip(hipMemcpy(p->tw, tw.data(), n * sizeof(float2), hipMemcpyHostToDevice), "copy tw")) ||
+      (rc = check_hip(hipMemcpy(p->force_idx, fidx.data(), fidx.size() * sizeof(int), hipMemcpyHostToDevice),
+                      "copy forcing")) ||
+      (rc = check_hip(hipMemcpy(p->force_val, fval.data(), fval.size() * sizeof(float2), hipMemcpyHostToDevice),
+                      "copy forcing"))) {
+    jcfd_ns2d_plan_destroy(p);
+    return rc;
+  }
+  // 2/3 rule bounds with Python int / floor-division semantics (spectral/utils.py:128-134)
+  const int a = static_cast<int>(2.0 / 3.0 * n);
+  const int cols = static_cast<int>(2.0 / 3.0 * h1);
+  for (int k = 0; k < 5; ++k) {
+    NsParams& q = p->stages[k];
+    q.kx = p->kx;
+    q.ky = p->ky;
+    q.tw = p->tw;
+    q.force_idx = p->force_idx;
+    q.force_val = p->force_val;
+    q.n_force = c.n_forcing;
+    q.two_pi = static_cast<float>(2.0 * M_PI);
+    q.neg_four_pi_sq = static_cast<float>(-(2.0 * M_PI) * (2.0 * M_PI));
+    q.inv_nn = 1.0f / (static_cast<float>(n) * static_cast<float>(n));
+    q.nu = static_cast<float>(c.viscosity);
+    q.drag = static_cast<float>(c.drag);
+    q.smooth = c.smooth ? 1 : 0;
+    q.filt_row_lo = a / 2;
+    q.filt_row_hi = n - (a + 1) / 2;
+    q.filt_cols = cols;
+    q.beta = static_cast<float>(kCnBeta[k]);
+    q.g = static_cast<float>(kCnGamma[k] * c.dt);
+    q.mu = static_cast<float>(0.5 * c.dt * (kCnAlpha[k + 1] - kCnAlpha[k]));
+    q.read_h = k > 0 ? 1 : 0;
+  }
+  *plan = p;
+  return JCFD_OK;
+}
+
+int64_t jcfd_ns2d_plan_device_bytes(const jcfd_ns2d_plan* p) { return p ? p->bytes : 0; }
+
+int jcfd_ns2d_step(jcfd_ns2d_plan* p, const jcfd_tensor* w_hat, jcfd_tensor* w_out, int64_t nsteps,
+                   void* stream) {
+  if (p == nullptr) return fail(JCFD_ERR_INVALID_ARGUMENT, "null plan");
+  if (nsteps < 0) return fail(JCFD_ERR_INVALID_ARGUMENT, "nsteps must be >= 0");
+  void *pin, *pout;
+  int rc;
+  if ((rc = check_spectral(w_hat, "w_hat", p->batch, p->n, p->device, &pin)) ||
+      (rc = check_spectral(w_out, "w_out", p->batch, p->n, p->device, &pout)))
+    return rc;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const size_t bytes = sizeof(float2) * p->batch * static_cast<size_t>(p->n) * (p->n / 2 + 1);
+  if (nsteps == 0) {
+    if (pin != pout) return check_hip(hipMemcpyAsync(pout, pin, bytes, hipMemcpyDeviceToDevice, s), "copy");
+    return JCFD_OK;
+  }
+  NsBuffers b{};
+  b.u_out = static_cast<float2*>(pout);
+  b.h = p->h;
+  b.g = p->g;
+  b.pbuf = p->pbuf;
+  for (int64_t step = 0; step < nsteps; ++step) {
+    b.u_in = step == 0 ? static_cast<const float2*>(pin) : b.u_out;
+    if ((rc = ns_launch_step(p->n, p->stages, b, p->batch, s))) return rc;
+  }
+  return JCFD_OK;
+}
+
+int jcfd_ns2d_explicit_terms(jcfd_ns2d_plan* p, const jcfd_tensor* w_hat, jcfd_tensor* out,
+                             void* stream) {
+  if (p == nullptr) return fail(JCFD_ERR_INVALID_ARGUMENT, "null plan");
+  void *pin, *pout;
+  int rc;
+  if ((rc = check_spectral(w_hat, "w_hat", p->batch, p->n, p->device, &pin)) ||
+      (rc = check_spectral(out, "out", p->batch, p->n, p->device, &pout)))
+    return rc;
+  if (pin == pout) return fail(JCFD_ERR_INVALID_ARGUMENT, "explicit terms: out must not alias w_hat");
+  return ns_launch_explicit_terms(p->n, p->stages[0], static_cast<const float2*>(pin),
+                                  static_cast<float2*>(pout), p->g, p->pbuf, p->batch,
+                                  static_cast<hipStream_t>(stream));
+}
+
+int jcfd_ns2d_rfft2(jcfd_ns2d_plan* p, const jcfd_tensor* x, jcfd_tensor* out, void* stream) {
+  if (p == nullptr) return fail(JCFD_ERR_INVALID_ARGUMENT, "null plan");
+  void *pin, *pout;
+  int rc;
+  if ((rc = check_batched(x, "x", kDLFloat, 32, p->batch, p->n, p->n, p->device, &pin)) ||
+      (rc = check_spectral(out, "out", p->batch, p->n, p->device, &pout)))
+    return rc;
+  return ns_launch_rfft2(p->n, p->stages[0], static_cast<const float*>(pin), static_cast<float2*>(pout),
+                         p->pbuf, p->batch, static_cast<hipStream_t>(stream));
+}
+
+int jcfd_ns2d_irfft2(jcfd_ns2d_plan* p, const jcfd_tensor* x_hat, jcfd_tensor* out, void* stream) {
+  if (p == nullptr) return fail(JCFD_ERR_INVALID_ARGUMENT, "null plan");
+  void *pin, *pout;
+  int rc;
+  if ((rc = check_spectral(x_hat, "x_hat", p->batch, p->n, p->device, &pin)) ||
+      (rc = check_batched(out, "out", kDLFloat, 32, p->batch, p->n, p->n, p->device, &pout)))
+    return rc;
+  return ns_launch_irfft2(p->n, p->stages[0], static_cast<const float2*>(pin), static_cast<float*>(pout),
+                          p->pbuf, p->batch, static_cast<hipStream_t>(stream));
 }
 
 }  // extern "C"

@@ -60,4 +60,41 @@ int fvm_launch_filter(const FvmParams& p, const float* x, const float* multiplie
                       float2* spec, const float2* tw, hipStream_t s);
 bool fvm_size_supported(int n);
 
+// Device-side parameters of a pseudospectral stage (spectral.hip).  The
+// wavenumber tables and multipliers follow the reference's fp32 values;
+// beta, g = gamma dt and mu are the stage's CN-RK4 scalars.
+struct NsParams {
+  const float* kx;        // fftfreq(n, dx), n values
+  const float* ky;        // rfftfreq(n, dy), n/2+1 values
+  const float2* tw;       // twiddles exp(-2 pi i k / n)
+  const int* force_idx;   // (p, c) pairs of the forcing-curl entries
+  const float2* force_val;
+  int n_force;
+  float two_pi;           // fp32(2 pi)
+  float neg_four_pi_sq;   // fp32((2 pi i)^2)
+  float inv_nn;           // 1 / n^2 (exact)
+  float nu, drag;
+  int smooth;
+  int filt_row_lo, filt_row_hi, filt_cols;
+  float beta, g, mu;
+  int read_h;
+};
+
+struct NsBuffers {
+  const float2* u_in;  // state in, (batch, n, n/2+1)
+  float2* u_out;       // state out
+  float2* h;           // low-storage RK register, external layout
+  float2* g;           // 4 packed spectra per instance, (batch, 4, n, n/2)
+  float2* pbuf;        // packed product spectrum, (batch, n, n/2)
+};
+
+bool ns_size_supported(int n);
+int ns_launch_step(int n, const NsParams* stages, const NsBuffers& b, int batch, hipStream_t s);
+int ns_launch_explicit_terms(int n, const NsParams& q, const float2* w, float2* out, float2* g,
+                             float2* pbuf, int batch, hipStream_t s);
+int ns_launch_rfft2(int n, const NsParams& q, const float* x, float2* out, float2* pbuf, int batch,
+                    hipStream_t s);
+int ns_launch_irfft2(int n, const NsParams& q, const float2* in, float* x, float2* pbuf, int batch,
+                     hipStream_t s);
+
 }  // namespace jcfd

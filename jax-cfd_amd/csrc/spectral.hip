@@ -1,0 +1,491 @@
+// Pseudospectral vorticity step (NavierStokes2D + crank_nicolson_rk4) for gfx950.
+//
+// State: the complex64 rfft2 of the vorticity, (n, n/2+1) per instance
+// (jax_cfd/spectral/equations.py:129-190).  One CN-RK4 step
+// (spectral/time_stepping.py:117-170, 186-197) is five stages of
+//   F = explicit_terms(u);  h = F + beta_k h;
+//   u = (u + gamma_k dt h + mu_k L u) / (1 - mu_k L),  L = nu lap - drag,
+// where explicit_terms = filter * rfftn(-(d_x w * v_x + d_y w * v_y)) + forcing
+// curl, with v from vorticity_to_velocity (spectral/utils.py:154-189).
+//
+// Kernels (all launched with blockIdx.y = instance of a same-size batch):
+//   ns_cols_inverse   column tile of W packed columns: build the four spectra
+//                     d_x w, d_y w, v_x, v_y (scaled by 1/n^2), inverse column
+//                     FFT each, write them packed -> G[4].
+//   ns_rows           RB rows: four inverse packed real row FFTs, the
+//                     product -(gx vx + gy vy), forward packed real row FFT -> P.
+//   ns_cols_forward   column tile: forward column FFT of P, unpack the DC/
+//                     Nyquist word, 2/3 filter, forcing curl, then either write
+//                     F (explicit-terms seam) or the RK/CN update of (u, h),
+//                     optionally followed directly by the next stage's
+//                     ns_cols_inverse work on the same tile.
+//
+// Packed half spectra: word 0 of a row holds (X[0], X[n/2]) of a real row;
+// the column pass of word 0 therefore carries two Hermitian columns, which
+// are separated with Y(p) = A(p) + i B(p), A = (Y(p) + conj Y(-p))/2,
+// B = (Y(p) - conj Y(-p))/(2i).  On the way in, (A, B) are symmetrised the
+// same way, which reproduces irfftn's dropping of the imaginary parts of the
+// DC and Nyquist columns after the axis-0 inverse transform.
+#include <hip/hip_runtime.h>
+
+#include "jcfd_fft.h"
+#include "jcfd_internal.h"
+
+#pragma clang fp contract(off)
+
+namespace jcfd {
+namespace {
+
+constexpr int kThreads = 256;
+
+// Per-element multipliers of the four derivative spectra, following the
+// reference's fp32/complex64 operation order.
+struct NsMult {
+  float ax;   // 2 pi kx  (d_x multiplier is i ax)
+  float ay;   // 2 pi ky
+  float psi;  // -1 / laplace'  (laplace' = (2 pi i)^2 (kx^2 + ky^2), [0,0] := 1)
+};
+
+__device__ __forceinline__ NsMult ns_mult(const NsParams& q, int p, int c) {
+  const float kx = q.kx[p], ky = q.ky[c];
+  NsMult m;
+  m.ax = q.two_pi * kx;
+  m.ay = q.two_pi * ky;
+  const float s = kx * kx + ky * ky;
+  const float lap = (p == 0 && c == 0) ? 1.0f : q.neg_four_pi_sq * s;
+  m.psi = -1.0f / lap;
+  return m;
+}
+
+// i a * z
+__device__ __forceinline__ float2 mul_i(float a, float2 z) { return make_float2(-a * z.y, a * z.x); }
+
+// Spectrum m in {0: d_x w, 1: d_y w, 2: v_x, 3: v_y, 4: w itself} at (p, c).
+__device__ __forceinline__ float2 ns_spectrum(int m, const NsMult& k, float2 w) {
+  switch (m) {
+    case 0: return mul_i(k.ax, w);
+    case 1: return mul_i(k.ay, w);
+    case 2: return mul_i(k.ay, make_float2(k.psi * w.x, k.psi * w.y));
+    case 3: return mul_i(-k.ax, make_float2(k.psi * w.x, k.psi * w.y));
+    default: return w;
+  }
+}
+
+template <int N, int W>
+struct ColShared {
+  float2 wt[N * W];  // vorticity tile [p][c]
+  float2 wn[N];      // vorticity column N/2 (tile 0 only)
+  float2 buf[N * W]; // work tile
+  float2 tw[N];
+};
+
+// Build spectrum m of the tile into sm.buf (scaled by 1/n^2), symmetrising and
+// packing the DC/Nyquist pair into local column 0 of tile 0.
+template <int N, int W>
+__device__ __forceinline__ void build_spectrum(ColShared<N, W>& sm, const NsParams& q, int m,
+                                               int c0, int tid) {
+  constexpr int H = N / 2;
+  for (int e = tid; e < N * W; e += kThreads) {
+    const int p = e / W, c = e % W, cg = c0 + c;
+    float2 out;
+    if (cg != 0) {
+      out = ns_spectrum(m, ns_mult(q, p, cg), sm.wt[e]);
+    } else {
+      const int pm = (N - p) & (N - 1);
+      const float2 a0 = ns_spectrum(m, ns_mult(q, p, 0), sm.wt[p * W]);
+      const float2 b0 = ns_spectrum(m, ns_mult(q, pm, 0), sm.wt[pm * W]);
+      const float2 ah = ns_spectrum(m, ns_mult(q, p, H), sm.wn[p]);
+      const float2 bh = ns_spectrum(m, ns_mult(q, pm, H), sm.wn[pm]);
+      const float2 s0 = make_float2(0.5f * (a0.x + b0.x), 0.5f * (a0.y - b0.y));
+      const float2 sh = make_float2(0.5f * (ah.x + bh.x), 0.5f * (ah.y - bh.y));
+      out = make_float2(s0.x - sh.y, s0.y + sh.x);  // s0 + i sh
+    }
+    sm.buf[e] = make_float2(out.x * q.inv_nn, out.y * q.inv_nn);
+  }
+}
+
+// Inverse column transforms of the tile in sm.wt/sm.wn -> G[0..NSPEC): the
+// four derivative spectra (NSPEC = 4) or the field itself (NSPEC = 1).
+template <int N, int W, int NSPEC = 4>
+__device__ __forceinline__ void cols_inverse_body(ColShared<N, W>& sm, const NsParams& q,
+                                                  float2* __restrict__ g, int c0, int tid) {
+  constexpr int H = N / 2;
+  const size_t gstride = static_cast<size_t>(N) * H;
+  for (int m = 0; m < NSPEC; ++m) {
+    build_spectrum<N, W>(sm, q, NSPEC == 1 ? 4 : m, c0, tid);
+    __syncthreads();
+    lds_fft<N, W, kThreads, W, 1, true, true>(sm.buf, sm.tw, tid);
+    float2* gm = g + m * gstride;
+    for (int e = tid; e < N * W; e += kThreads) {
+      const int p = e / W, c = e % W;
+      gm[p * H + c0 + c] = sm.buf[e];
+    }
+    __syncthreads();
+  }
+}
+
+template <int N, int W>
+__device__ __forceinline__ void load_w_tile(ColShared<N, W>& sm, const float2* __restrict__ w,
+                                            int c0, int tid) {
+  constexpr int H1 = N / 2 + 1;
+  for (int e = tid; e < N * W; e += kThreads) {
+    const int p = e / W, c = e % W;
+    sm.wt[e] = w[p * H1 + c0 + c];
+  }
+  if (c0 == 0)
+    for (int p = tid; p < N; p += kThreads) sm.wn[p] = w[p * H1 + N / 2];
+}
+
+template <int N, int W, int NSPEC>
+__global__ __launch_bounds__(kThreads) void ns_cols_inverse(NsParams q, const float2* __restrict__ w,
+                                                            float2* __restrict__ g) {
+  __shared__ ColShared<N, W> sm;
+  const int tid = threadIdx.x;
+  const int c0 = blockIdx.x * W;
+  const size_t inst = blockIdx.y;
+  w += inst * static_cast<size_t>(N) * (N / 2 + 1);
+  g += inst * NSPEC * static_cast<size_t>(N) * (N / 2);
+  load_w_tile<N, W>(sm, w, c0, tid);
+  load_twiddles(sm.tw, q.tw, N, 1, tid, kThreads);
+  __syncthreads();
+  cols_inverse_body<N, W, NSPEC>(sm, q, g, c0, tid);
+}
+
+// Packed real row transforms of a batch of real (N, N) fields.
+template <int N, int RB>
+__global__ __launch_bounds__(kThreads) void rows_r2c(NsParams q, const float* __restrict__ x,
+                                                     float2* __restrict__ pout) {
+  constexpr int H = N / 2;
+  __shared__ float2 zb[RB * H];
+  __shared__ float2 twn[N];
+  __shared__ float2 twh[H];
+  const int tid = threadIdx.x;
+  const int i0 = blockIdx.x * RB;
+  const size_t inst = blockIdx.y;
+  x += inst * static_cast<size_t>(N) * N;
+  pout += inst * static_cast<size_t>(N) * H;
+  load_twiddles(twn, q.tw, N, 1, tid, kThreads);
+  load_twiddles(twh, q.tw, H, 2, tid, kThreads);
+  float* zf = reinterpret_cast<float*>(zb);
+  for (int e = tid; e < RB * N; e += kThreads) zf[e] = x[static_cast<size_t>(i0) * N + e];
+  __syncthreads();
+  lds_fft<H, RB, kThreads, 1, H, false, false>(zb, twh, tid);
+  for (int e = tid; e < RB * H; e += kThreads) {
+    const int r = e / H, k = e % H;
+    pout[static_cast<size_t>(i0 + r) * H + k] = r2c_combine(zb + r * H, k, H, twn[k]);
+  }
+}
+
+template <int N, int RB>
+__global__ __launch_bounds__(kThreads) void rows_c2r(NsParams q, const float2* __restrict__ pin,
+                                                     float* __restrict__ x) {
+  constexpr int H = N / 2;
+  __shared__ float2 zb[RB * H];
+  __shared__ float2 twn[N];
+  __shared__ float2 twh[H];
+  const int tid = threadIdx.x;
+  const int i0 = blockIdx.x * RB;
+  const size_t inst = blockIdx.y;
+  x += inst * static_cast<size_t>(N) * N;
+  pin += inst * static_cast<size_t>(N) * H;
+  load_twiddles(twn, q.tw, N, 1, tid, kThreads);
+  load_twiddles(twh, q.tw, H, 2, tid, kThreads);
+  __syncthreads();
+  for (int e = tid; e < RB * H; e += kThreads) {
+    const int r = e / H, k = e % H;
+    const float2* row = pin + static_cast<size_t>(i0 + r) * H;
+    zb[e] = c2r_combine(row[k], row[(H - k) & (H - 1)], k, twn[k]);
+  }
+  __syncthreads();
+  lds_fft<H, RB, kThreads, 1, H, false, true>(zb, twh, tid);
+  const float* zf = reinterpret_cast<const float*>(zb);
+  for (int e = tid; e < RB * N; e += kThreads) x[static_cast<size_t>(i0) * N + e] = zf[e];
+}
+
+template <int N, int RB>
+__global__ __launch_bounds__(kThreads) void ns_rows(NsParams q, const float2* __restrict__ g,
+                                                    float2* __restrict__ pout) {
+  constexpr int H = N / 2;
+  __shared__ float2 rows[4 * RB * H];  // transform t = m * RB + r
+  __shared__ float2 prod[RB * H];
+  __shared__ float2 twn[N];
+  __shared__ float2 twh[H];
+  const int tid = threadIdx.x;
+  const int i0 = blockIdx.x * RB;
+  const size_t inst = blockIdx.y;
+  const size_t gstride = static_cast<size_t>(N) * H;
+  g += inst * 4 * gstride;
+  pout += inst * gstride;
+  load_twiddles(twn, q.tw, N, 1, tid, kThreads);
+  load_twiddles(twh, q.tw, H, 2, tid, kThreads);
+  __syncthreads();
+  for (int e = tid; e < 4 * RB * H; e += kThreads) {
+    const int t = e / H, k = e % H;
+    const int m = t / RB, r = t % RB;
+    const float2* row = g + m * gstride + static_cast<size_t>(i0 + r) * H;
+    rows[e] = c2r_combine(row[k], row[(H - k) & (H - 1)], k, twn[k]);
+  }
+  __syncthreads();
+  lds_fft<H, 4 * RB, kThreads, 1, H, false, true>(rows, twh, tid);
+  // advection = -(grad_x * vx + grad_y * vy)  (spectral/equations.py:170)
+  const float* gx = reinterpret_cast<const float*>(rows);
+  const float* gy = gx + RB * N;
+  const float* vx = gy + RB * N;
+  const float* vy = vx + RB * N;
+  float* pr = reinterpret_cast<float*>(prod);
+  for (int e = tid; e < RB * N; e += kThreads) pr[e] = -(gx[e] * vx[e] + gy[e] * vy[e]);
+  __syncthreads();
+  lds_fft<H, RB, kThreads, 1, H, false, false>(prod, twh, tid);
+  for (int e = tid; e < RB * H; e += kThreads) {
+    const int r = e / H, k = e % H;
+    pout[static_cast<size_t>(i0 + r) * H + k] = r2c_combine(prod + r * H, k, H, twn[k]);
+  }
+}
+
+// 2/3 rule mask (spectral/utils.py:128-134).
+__device__ __forceinline__ float ns_filter(const NsParams& q, int p, int c) {
+  if (!q.smooth) return 1.0f;
+  const bool row = (p < q.filt_row_lo) || (p >= q.filt_row_hi);
+  return (row && c < q.filt_cols) ? 1.0f : 0.0f;
+}
+
+// MODE 0: write F (explicit terms) to u_out.  MODE 1: RK/CN update of (u, h).
+// MODE 2: update, then the next stage's inverse column work on the new tile.
+template <int N, int W, int MODE>
+__global__ __launch_bounds__(kThreads) void ns_cols_forward(NsParams q, const float2* __restrict__ pin,
+                                                            const float2* u_in, float2* u_out,
+                                                            float2* h, float2* __restrict__ g) {
+  constexpr int H = N / 2;
+  constexpr int H1 = H + 1;
+  constexpr int PER = (N * W + kThreads - 1) / kThreads;
+  __shared__ ColShared<N, W> sm;
+  const int tid = threadIdx.x;
+  const int c0 = blockIdx.x * W;
+  const size_t inst = blockIdx.y;
+  const size_t ext = static_cast<size_t>(N) * H1;
+  pin += inst * static_cast<size_t>(N) * H;
+  if (u_in) u_in += inst * ext;
+  u_out += inst * ext;
+  if (h) h += inst * ext;
+  if (g) g += inst * 4 * static_cast<size_t>(N) * H;
+
+  for (int e = tid; e < N * W; e += kThreads) {
+    const int p = e / W, c = e % W;
+    sm.buf[e] = pin[p * H + c0 + c];
+  }
+  load_twiddles(sm.tw, q.tw, N, 1, tid, kThreads);
+  __syncthreads();
+  lds_fft<N, W, kThreads, W, 1, true, false>(sm.buf, sm.tw, tid);
+
+  // F at (p, cg) for this thread's elements; tile 0 also produces column N/2.
+  float2 f[PER], fh[PER];
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const int e = tid + k * kThreads;
+    if (e < N * W) {
+      const int p = e / W, c = e % W, cg = c0 + c;
+      const float2 y = sm.buf[e];
+      if (cg == 0) {
+        const float2 ym = cconj(sm.buf[((N - p) & (N - 1)) * W]);
+        const float2 a = make_float2(0.5f * (y.x + ym.x), 0.5f * (y.y + ym.y));
+        const float2 d = csub(y, ym);
+        const float2 b = make_float2(0.5f * d.y, -0.5f * d.x);
+        const float m0 = ns_filter(q, p, 0), mh = ns_filter(q, p, H);
+        f[k] = make_float2(a.x * m0, a.y * m0);
+        fh[k] = make_float2(b.x * mh, b.y * mh);
+      } else {
+        const float m = ns_filter(q, p, cg);
+        f[k] = make_float2(y.x * m, y.y * m);
+      }
+    }
+  }
+  __syncthreads();  // sm.buf free from here on
+
+  // Forcing curl entries (sparse; constant in time) are added by their owners.
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const int e = tid + k * kThreads;
+    if (e < N * W) {
+      const int p = e / W, c = e % W, cg = c0 + c;
+      for (int t = 0; t < q.n_force; ++t) {
+        const int fp = q.force_idx[2 * t], fc = q.force_idx[2 * t + 1];
+        if (fp == p && fc == cg) f[k] = cadd(f[k], q.force_val[t]);
+        if (cg == 0 && fp == p && fc == H) fh[k] = cadd(fh[k], q.force_val[t]);
+      }
+    }
+  }
+
+  if constexpr (MODE == 0) {
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int e = tid + k * kThreads;
+      if (e < N * W) {
+        const int p = e / W, c = e % W, cg = c0 + c;
+        u_out[p * H1 + cg] = f[k];
+        if (cg == 0) u_out[p * H1 + H] = fh[k];
+      }
+    }
+    return;
+  } else {
+    // h = F + beta h;  u = (u + g h + mu L u) / (1 - mu L)
+    auto update = [&](int p, int cg, float2 fv) -> float2 {
+      const size_t idx = static_cast<size_t>(p) * H1 + cg;
+      float2 hv = fv;
+      if (q.read_h) {
+        const float2 ho = h[idx];
+        hv = make_float2(fv.x + q.beta * ho.x, fv.y + q.beta * ho.y);
+      }
+      h[idx] = hv;
+      const float2 uo = u_in[idx];
+      const float kx = q.kx[p], ky = q.ky[cg];
+      const float lap = q.neg_four_pi_sq * (kx * kx + ky * ky);
+      const float L = q.nu * lap - q.drag;
+      const float2 lu = make_float2(L * uo.x, L * uo.y);
+      const float2 x = make_float2((uo.x + q.g * hv.x) + q.mu * lu.x, (uo.y + q.g * hv.y) + q.mu * lu.y);
+      const float inv = 1.0f / (1.0f - q.mu * L);
+      const float2 un = make_float2(inv * x.x, inv * x.y);
+      u_out[idx] = un;
+      return un;
+    };
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int e = tid + k * kThreads;
+      if (e < N * W) {
+        const int p = e / W, c = e % W, cg = c0 + c;
+        const float2 un = update(p, cg, f[k]);
+        if constexpr (MODE == 2) {
+          sm.wt[e] = un;
+          if (cg == 0) sm.wn[p] = update(p, H, fh[k]);
+        } else {
+          if (cg == 0) update(p, H, fh[k]);
+        }
+      }
+    }
+    if constexpr (MODE == 2) {
+      __syncthreads();
+      cols_inverse_body<N, W>(sm, q, g, c0, tid);
+    }
+  }
+}
+
+template <int N> struct NsTiles;
+template <> struct NsTiles<32> { static constexpr int W = 8, RB = 2; };
+template <> struct NsTiles<64> { static constexpr int W = 8, RB = 2; };
+template <> struct NsTiles<128> { static constexpr int W = 8, RB = 2; };
+template <> struct NsTiles<256> { static constexpr int W = 8, RB = 2; };
+template <> struct NsTiles<512> { static constexpr int W = 8, RB = 1; };
+
+template <int N>
+struct NsOps {
+  static constexpr int W = NsTiles<N>::W, RB = NsTiles<N>::RB;
+  static int cols_inverse(const NsParams& q, const float2* w, float2* g, int batch, hipStream_t s) {
+    hipLaunchKernelGGL((ns_cols_inverse<N, W, 4>), dim3(N / 2 / W, batch), dim3(kThreads), 0, s, q, w, g);
+    return check_hip(hipGetLastError(), "ns_cols_inverse");
+  }
+  static int rfft2(const NsParams& q, const float* x, float2* out, float2* pbuf, int batch,
+                   hipStream_t s) {
+    NsParams plain = q;
+    plain.smooth = 0;
+    plain.n_force = 0;
+    hipLaunchKernelGGL((rows_r2c<N, RB>), dim3(N / RB, batch), dim3(kThreads), 0, s, q, x, pbuf);
+    hipLaunchKernelGGL((ns_cols_forward<N, W, 0>), dim3(N / 2 / W, batch), dim3(kThreads), 0, s, plain,
+                       pbuf, nullptr, out, nullptr, nullptr);
+    return check_hip(hipGetLastError(), "rfft2");
+  }
+  static int irfft2(const NsParams& q, const float2* in, float* x, float2* pbuf, int batch,
+                    hipStream_t s) {
+    hipLaunchKernelGGL((ns_cols_inverse<N, W, 1>), dim3(N / 2 / W, batch), dim3(kThreads), 0, s, q, in, pbuf);
+    hipLaunchKernelGGL((rows_c2r<N, RB>), dim3(N / RB, batch), dim3(kThreads), 0, s, q, pbuf, x);
+    return check_hip(hipGetLastError(), "irfft2");
+  }
+  static int rows(const NsParams& q, const float2* g, float2* pbuf, int batch, hipStream_t s) {
+    hipLaunchKernelGGL((ns_rows<N, RB>), dim3(N / RB, batch), dim3(kThreads), 0, s, q, g, pbuf);
+    return check_hip(hipGetLastError(), "ns_rows");
+  }
+  static int cols_forward(int mode, const NsParams& q, const float2* pbuf, const float2* u_in,
+                          float2* u_out, float2* h, float2* g, int batch, hipStream_t s) {
+    const dim3 grid(N / 2 / W, batch);
+    if (mode == 0)
+      hipLaunchKernelGGL((ns_cols_forward<N, W, 0>), grid, dim3(kThreads), 0, s, q, pbuf, u_in, u_out, h, g);
+    else if (mode == 1)
+      hipLaunchKernelGGL((ns_cols_forward<N, W, 1>), grid, dim3(kThreads), 0, s, q, pbuf, u_in, u_out, h, g);
+    else
+      hipLaunchKernelGGL((ns_cols_forward<N, W, 2>), grid, dim3(kThreads), 0, s, q, pbuf, u_in, u_out, h, g);
+    return check_hip(hipGetLastError(), "ns_cols_forward");
+  }
+};
+
+template <int N>
+int ns_step_impl(const NsParams* stages, const NsBuffers& b, int batch, hipStream_t s) {
+  int rc = NsOps<N>::cols_inverse(stages[0], b.u_in, b.g, batch, s);
+  for (int k = 0; k < 5 && rc == JCFD_OK; ++k) {
+    rc = NsOps<N>::rows(stages[k], b.g, b.pbuf, batch, s);
+    if (rc) break;
+    const float2* u_src = (k == 0) ? b.u_in : b.u_out;
+    rc = NsOps<N>::cols_forward(k < 4 ? 2 : 1, stages[k], b.pbuf, u_src, b.u_out, b.h, b.g,
+                                batch, s);
+  }
+  return rc;
+}
+
+template <int N>
+int ns_explicit_impl(const NsParams& q, const float2* w, float2* out, float2* g, float2* pbuf,
+                     int batch, hipStream_t s) {
+  int rc = NsOps<N>::cols_inverse(q, w, g, batch, s);
+  if (!rc) rc = NsOps<N>::rows(q, g, pbuf, batch, s);
+  if (!rc) rc = NsOps<N>::cols_forward(0, q, pbuf, nullptr, out, nullptr, nullptr, batch, s);
+  return rc;
+}
+
+}  // namespace
+
+bool ns_size_supported(int n) { return n == 32 || n == 64 || n == 128 || n == 256 || n == 512; }
+
+int ns_launch_step(int n, const NsParams* stages, const NsBuffers& b, int batch, hipStream_t s) {
+  switch (n) {
+    case 32: return ns_step_impl<32>(stages, b, batch, s);
+    case 64: return ns_step_impl<64>(stages, b, batch, s);
+    case 128: return ns_step_impl<128>(stages, b, batch, s);
+    case 256: return ns_step_impl<256>(stages, b, batch, s);
+    case 512: return ns_step_impl<512>(stages, b, batch, s);
+    default: return fail(JCFD_ERR_UNSUPPORTED, "spectral grid size not built");
+  }
+}
+
+int ns_launch_rfft2(int n, const NsParams& q, const float* x, float2* out, float2* pbuf, int batch,
+                    hipStream_t s) {
+  switch (n) {
+    case 32: return NsOps<32>::rfft2(q, x, out, pbuf, batch, s);
+    case 64: return NsOps<64>::rfft2(q, x, out, pbuf, batch, s);
+    case 128: return NsOps<128>::rfft2(q, x, out, pbuf, batch, s);
+    case 256: return NsOps<256>::rfft2(q, x, out, pbuf, batch, s);
+    case 512: return NsOps<512>::rfft2(q, x, out, pbuf, batch, s);
+    default: return fail(JCFD_ERR_UNSUPPORTED, "spectral grid size not built");
+  }
+}
+
+int ns_launch_irfft2(int n, const NsParams& q, const float2* in, float* x, float2* pbuf, int batch,
+                     hipStream_t s) {
+  switch (n) {
+    case 32: return NsOps<32>::irfft2(q, in, x, pbuf, batch, s);
+    case 64: return NsOps<64>::irfft2(q, in, x, pbuf, batch, s);
+    case 128: return NsOps<128>::irfft2(q, in, x, pbuf, batch, s);
+    case 256: return NsOps<256>::irfft2(q, in, x, pbuf, batch, s);
+    case 512: return NsOps<512>::irfft2(q, in, x, pbuf, batch, s);
+    default: return fail(JCFD_ERR_UNSUPPORTED, "spectral grid size not built");
+  }
+}
+
+int ns_launch_explicit_terms(int n, const NsParams& q, const float2* w, float2* out, float2* g,
+                             float2* pbuf, int batch, hipStream_t s) {
+  switch (n) {
+    case 32: return ns_explicit_impl<32>(q, w, out, g, pbuf, batch, s);
+    case 64: return ns_explicit_impl<64>(q, w, out, g, pbuf, batch, s);
+    case 128: return ns_explicit_impl<128>(q, w, out, g, pbuf, batch, s);
+    case 256: return ns_explicit_impl<256>(q, w, out, g, pbuf, batch, s);
+    case 512: return ns_explicit_impl<512>(q, w, out, g, pbuf, batch, s);
+    default: return fail(JCFD_ERR_UNSUPPORTED, "spectral grid size not built");
+  }
+}
+
+}  // namespace jcfd

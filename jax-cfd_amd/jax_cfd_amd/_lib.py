@@ -56,8 +56,29 @@ class FvmConfig(ctypes.Structure):
   ]
 
 
+class Ns2dConfig(ctypes.Structure):
+  _fields_ = [
+      ("n", ctypes.c_int32),
+      ("dx", ctypes.c_double), ("dy", ctypes.c_double),
+      ("viscosity", ctypes.c_double), ("drag", ctypes.c_double),
+      ("smooth", ctypes.c_int32),
+      ("dt", ctypes.c_double),
+      ("batch", ctypes.c_int32),
+      ("n_forcing", ctypes.c_int32),
+      ("forcing_index", ctypes.POINTER(ctypes.c_int32)),
+      ("forcing_value", ctypes.POINTER(ctypes.c_float)),
+  ]
+
+
 _P = ctypes.c_void_p
 _SIGNATURES = {
+    "jcfd_ns2d_plan_create": (ctypes.c_int, [ctypes.POINTER(Ns2dConfig), ctypes.POINTER(_P)]),
+    "jcfd_ns2d_plan_destroy": (ctypes.c_int, [_P]),
+    "jcfd_ns2d_plan_device_bytes": (ctypes.c_int64, [_P]),
+    "jcfd_ns2d_step": (ctypes.c_int, [_P, _P, _P, ctypes.c_int64, _P]),
+    "jcfd_ns2d_explicit_terms": (ctypes.c_int, [_P, _P, _P, _P]),
+    "jcfd_ns2d_rfft2": (ctypes.c_int, [_P, _P, _P, _P]),
+    "jcfd_ns2d_irfft2": (ctypes.c_int, [_P, _P, _P, _P]),
     "jcfd_last_error": (ctypes.c_char_p, []),
     "jcfd_version": (ctypes.c_char_p, []),
     "jcfd_fvm_plan_create": (ctypes.c_int, [ctypes.POINTER(FvmConfig), ctypes.POINTER(_P)]),

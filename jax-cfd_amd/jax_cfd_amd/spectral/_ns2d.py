@@ -1,0 +1,124 @@
+"""Spectral plans: the Python side of the jcfd_ns2d_* C ABI."""
+from __future__ import annotations
+
+import ctypes
+import dataclasses
+from typing import Optional, Tuple
+
+import numpy as np
+import torch
+
+from jax_cfd_amd import _lib
+
+
+@dataclasses.dataclass(frozen=True)
+class Ns2dSpec:
+  n: int
+  dx: float
+  dy: float
+  viscosity: float
+  drag: float
+  smooth: bool
+  dt: float
+  # sparse constant forcing curl: ((p, c, re, im), ...)
+  forcing: Tuple[Tuple[int, int, float, float], ...] = ()
+
+
+class Ns2dPlan:
+  """Owns one jcfd_ns2d_plan for `batch` same-size instances on one GPU."""
+
+  def __init__(self, spec: Ns2dSpec, batch: int = 1,
+               device: Optional[torch.device] = None):
+    lib = _lib.load()
+    self.spec = spec
+    self.batch = int(batch)
+    self.device = device if device is not None else torch.device(
+        'cuda', torch.cuda.current_device())
+    cfg = _lib.Ns2dConfig()
+    cfg.n = spec.n
+    cfg.dx, cfg.dy = spec.dx, spec.dy
+    cfg.viscosity, cfg.drag = spec.viscosity, spec.drag
+    cfg.smooth = int(bool(spec.smooth))
+    cfg.dt = spec.dt
+    cfg.batch = self.batch
+    cfg.n_forcing = len(spec.forcing)
+    self._fi = np.ascontiguousarray(
+        np.array([(p, c) for p, c, _, _ in spec.forcing] or [(0, 0)], np.int32))
+    self._fv = np.ascontiguousarray(
+        np.array([(re, im) for _, _, re, im in spec.forcing] or [(0, 0)], np.float32))
+    cfg.forcing_index = self._fi.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+    cfg.forcing_value = self._fv.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+    handle = ctypes.c_void_p()
+    with torch.cuda.device(self.device):
+      _lib.check(lib.jcfd_ns2d_plan_create(ctypes.byref(cfg), ctypes.byref(handle)),
+                 'jcfd_ns2d_plan_create')
+    self._handle = handle
+    self._lib = lib
+
+  @property
+  def shape(self):
+    n = self.spec.n
+    return (n, n // 2 + 1) if self.batch == 1 else (self.batch, n, n // 2 + 1)
+
+  @property
+  def device_bytes(self) -> int:
+    return int(self._lib.jcfd_ns2d_plan_device_bytes(self._handle))
+
+  def close(self):
+    if getattr(self, '_handle', None) is not None and self._handle.value:
+      self._lib.jcfd_ns2d_plan_destroy(self._handle)
+      self._handle = None
+
+  def __del__(self):
+    try:
+      self.close()
+    except Exception:
+      pass
+
+  def _field(self, x):
+    t = _lib.as_device_tensor(x, torch.complex64, self.device)
+    if tuple(t.shape) != self.shape:
+      t = t.reshape(self.shape)
+    return t
+
+  def step(self, w, nsteps: int = 1, out=None):
+    w = self._field(w)
+    o = out if out is not None else torch.empty(self.shape, dtype=torch.complex64,
+                                                device=self.device)
+    with _lib.dlpack_args() as dl:
+      rc = self._lib.jcfd_ns2d_step(self._handle, dl(w), dl(o), int(nsteps),
+                                    _lib.stream_handle(self.device))
+    _lib.check(rc, 'jcfd_ns2d_step')
+    return o
+
+  @property
+  def real_shape(self):
+    n = self.spec.n
+    return (n, n) if self.batch == 1 else (self.batch, n, n)
+
+  def rfft2(self, x):
+    x = _lib.as_device_tensor(x, torch.float32, self.device).reshape(self.real_shape)
+    o = torch.empty(self.shape, dtype=torch.complex64, device=self.device)
+    with _lib.dlpack_args() as dl:
+      rc = self._lib.jcfd_ns2d_rfft2(self._handle, dl(x), dl(o),
+                                     _lib.stream_handle(self.device))
+    _lib.check(rc, 'jcfd_ns2d_rfft2')
+    return o
+
+  def irfft2(self, x_hat):
+    x_hat = self._field(x_hat)
+    o = torch.empty(self.real_shape, dtype=torch.float32, device=self.device)
+    with _lib.dlpack_args() as dl:
+      rc = self._lib.jcfd_ns2d_irfft2(self._handle, dl(x_hat), dl(o),
+                                      _lib.stream_handle(self.device))
+    _lib.check(rc, 'jcfd_ns2d_irfft2')
+    return o
+
+  def explicit_terms(self, w):
+    w = self._field(w)
+    o = torch.empty(self.shape, dtype=torch.complex64, device=self.device)
+    with _lib.dlpack_args() as dl:
+      rc = self._lib.jcfd_ns2d_explicit_terms(self._handle, dl(w), dl(o),
+                                              _lib.stream_handle(self.device))
+    _lib.check(rc, 'jcfd_ns2d_explicit_terms')
+    return o
