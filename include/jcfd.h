@@ -95,6 +95,8 @@ typedef struct {
   const float* forcing_x;  /* host array, n floats: constant x-forcing f_x(y_j) of
                               kolmogorov_forcing (forcings.py:63-104); may be NULL
                               (= zero) */
+  int32_t batch;           /* instances the plan holds scratch for (<= 1: one);
+                              batched fields are (batch, n, n) */
 } jcfd_fvm_config;
 
 typedef struct jcfd_fvm_plan jcfd_fvm_plan;
@@ -108,12 +110,24 @@ int jcfd_fvm_plan_create(const jcfd_fvm_config* config, jcfd_fvm_plan** plan);
 int jcfd_fvm_plan_destroy(jcfd_fvm_plan* plan);
 /* Device bytes held by the plan (constants + scratch). */
 int64_t jcfd_fvm_plan_device_bytes(const jcfd_fvm_plan* plan);
+/* Multi-step calls replay steps 2..nsteps from a captured hipGraph of 32
+ * steps when enabled (default: off for a single FVM instance, on for batched
+ * spectral plans); 0 launches every kernel individually.  Same results. */
+int jcfd_fvm_plan_set_graphs(jcfd_fvm_plan* plan, int32_t enabled);
 
 /* nsteps applications of step_fn (time_stepping.py:88-104 forward Euler +
  * pressure.py:181-198 projection), i.e. funcutils.repeated(step_fn, nsteps)
  * (funcutils.py:82-88).  u_out/v_out may alias u/v. */
 int jcfd_fvm_step(jcfd_fvm_plan* plan, const jcfd_tensor* u, const jcfd_tensor* v,
                   jcfd_tensor* u_out, jcfd_tensor* v_out, int64_t nsteps, void* stream);
+
+/* Continuous-batching form of jcfd_fvm_step: advances, in place, only the
+ * instances listed in `slots` (int32, 1-D, on the device, distinct values in
+ * [0, batch)) of the (batch, n, n) fields u and v, by nsteps steps.  Other
+ * slots are neither read nor written.  Every listed instance gets exactly
+ * the arithmetic of jcfd_fvm_step on that instance alone. */
+int jcfd_fvm_step_slots(jcfd_fvm_plan* plan, jcfd_tensor* u, jcfd_tensor* v,
+                        const jcfd_tensor* slots, int64_t nsteps, void* stream);
 
 /* Explicit terms dv/dt = convection + nu/rho * laplacian + forcing / rho
  * (equations.py:77-116, the explicit_terms_with_same_bcs seam). */
@@ -161,12 +175,19 @@ typedef struct jcfd_ns2d_plan jcfd_ns2d_plan;
 int jcfd_ns2d_plan_create(const jcfd_ns2d_config* config, jcfd_ns2d_plan** plan);
 int jcfd_ns2d_plan_destroy(jcfd_ns2d_plan* plan);
 int64_t jcfd_ns2d_plan_device_bytes(const jcfd_ns2d_plan* plan);
+/* As jcfd_fvm_plan_set_graphs, for the spectral plan. */
+int jcfd_ns2d_plan_set_graphs(jcfd_ns2d_plan* plan, int32_t enabled);
 
 /* nsteps CN-RK4 steps of every instance; w_hat/w_out complex64 of shape
  * (batch, n, n/2 + 1), or (n, n/2 + 1) when batch == 1.  w_out may alias
  * w_hat.  funcutils.repeated(step_fn, nsteps) (funcutils.py:82-88). */
 int jcfd_ns2d_step(jcfd_ns2d_plan* plan, const jcfd_tensor* w_hat, jcfd_tensor* w_out,
                    int64_t nsteps, void* stream);
+
+/* Continuous-batching form: advances in place only the listed slots of a
+ * (batch, n, n/2 + 1) state (see jcfd_fvm_step_slots). */
+int jcfd_ns2d_step_slots(jcfd_ns2d_plan* plan, jcfd_tensor* w_hat, const jcfd_tensor* slots,
+                         int64_t nsteps, void* stream);
 
 /* NavierStokes2D.explicit_terms (spectral/equations.py:160-184). */
 int jcfd_ns2d_explicit_terms(jcfd_ns2d_plan* plan, const jcfd_tensor* w_hat, jcfd_tensor* out,

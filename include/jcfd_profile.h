@@ -30,6 +30,17 @@ int jcfd_fvm_time_kernels(jcfd_fvm_plan* plan, const jcfd_tensor* u, const jcfd_
  * logical inputs read once + outputs written once). */
 int jcfd_fvm_kernel_bytes(const jcfd_fvm_plan* plan, int64_t* bytes_per_launch);
 
+/* Kernel classes of one CN-RK4 step of the spectral plan. */
+#define JCFD_NS2D_KERNEL_COLS_INVERSE 0       /* stage-0 inverse column FFTs */
+#define JCFD_NS2D_KERNEL_ROWS 1               /* row FFTs + nonlinear product */
+#define JCFD_NS2D_KERNEL_COLS_UPDATE_INVERSE 2 /* forward cols + update + next inverse */
+#define JCFD_NS2D_KERNEL_COLS_UPDATE 3        /* forward cols + final update */
+#define JCFD_NS2D_KERNEL_COUNT 4
+
+int jcfd_ns2d_time_kernels(jcfd_ns2d_plan* plan, const jcfd_tensor* w_hat, int32_t reps,
+                           double* ms_per_launch, void* stream);
+int jcfd_ns2d_kernel_bytes(const jcfd_ns2d_plan* plan, int64_t* bytes_per_launch);
+
 #ifdef __cplusplus
 }
 #endif

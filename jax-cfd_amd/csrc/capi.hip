@@ -135,6 +135,84 @@ std::vector<float> poisson_multiplier(int n, double dx, double dy) {
   return lam;
 }
 
+// Replays `chunk` steps captured once into a hipGraph.  Capture happens on a
+// plan-owned non-blocking stream (default priority); events order it after
+// the caller's prior work and the caller's later work after it.  The graph
+// bakes in its buffer addresses, so it is rebuilt when `key` (the output
+// state pointer) or the chunk length changes.
+struct GraphCache {
+  bool enabled = true;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev_in = nullptr, ev_out = nullptr;
+  hipGraphExec_t exec = nullptr;
+  const void* key = nullptr;
+  const void* key2 = nullptr;
+  int key3 = 0;
+  const void* key4 = nullptr;
+  int chunk = 0;
+
+  void release() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (ev_in) (void)hipEventDestroy(ev_in);
+    if (ev_out) (void)hipEventDestroy(ev_out);
+    if (stream) (void)hipStreamDestroy(stream);
+    exec = nullptr;
+    ev_in = ev_out = nullptr;
+    stream = nullptr;
+  }
+
+  // launch(s) enqueues one chunk of steps on stream s.
+  template <typename F>
+  int run(const void* k, const void* k2, const void* k4, int k3, int chunk_len, int64_t reps,
+          hipStream_t caller, F&& launch) {
+    int rc = JCFD_OK;
+    if (stream == nullptr) {
+      if ((rc = check_hip(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "graph stream")) ||
+          (rc = check_hip(hipEventCreateWithFlags(&ev_in, hipEventDisableTiming), "graph event")) ||
+          (rc = check_hip(hipEventCreateWithFlags(&ev_out, hipEventDisableTiming), "graph event")))
+        return rc;
+    }
+    if (exec == nullptr || key != k || key2 != k2 || key3 != k3 || key4 != k4 || chunk != chunk_len) {
+      if (exec) {
+        (void)hipGraphExecDestroy(exec);
+        exec = nullptr;
+      }
+      hipGraph_t graph = nullptr;
+      if ((rc = check_hip(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "begin capture")))
+        return rc;
+      const int lrc = launch(stream);
+      const hipError_t ec = hipStreamEndCapture(stream, &graph);
+      if (lrc) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return lrc;
+      }
+      if ((rc = check_hip(ec, "end capture"))) return rc;
+      rc = check_hip(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "graph instantiate");
+      (void)hipGraphDestroy(graph);
+      if (rc) {
+        exec = nullptr;
+        return rc;
+      }
+      key = k;
+      key2 = k2;
+      key3 = k3;
+      key4 = k4;
+      chunk = chunk_len;
+    }
+    if ((rc = check_hip(hipEventRecord(ev_in, caller), "event record")) ||
+        (rc = check_hip(hipStreamWaitEvent(stream, ev_in, 0), "stream wait")))
+      return rc;
+    for (int64_t r = 0; r < reps; ++r)
+      if ((rc = check_hip(hipGraphLaunch(exec, stream), "graph launch"))) return rc;
+    if ((rc = check_hip(hipEventRecord(ev_out, stream), "event record")) ||
+        (rc = check_hip(hipStreamWaitEvent(caller, ev_out, 0), "stream wait")))
+      return rc;
+    return JCFD_OK;
+  }
+};
+
+constexpr int kGraphChunk = 32;  // steps per captured graph
+
 }  // namespace
 }  // namespace jcfd
 
@@ -144,6 +222,7 @@ struct jcfd_fvm_plan {
   jcfd_fvm_config cfg;
   FvmParams params;
   int device = 0;
+  int batch = 1;
   float* us = nullptr;
   float* vs = nullptr;
   float2* spec = nullptr;
@@ -151,6 +230,7 @@ struct jcfd_fvm_plan {
   float2* tw = nullptr;
   float* fx = nullptr;
   int64_t bytes = 0;
+  GraphCache graphs;
 };
 
 struct jcfd_ns2d_plan {
@@ -167,6 +247,7 @@ struct jcfd_ns2d_plan {
   float2* g = nullptr;
   float2* pbuf = nullptr;
   int64_t bytes = 0;
+  GraphCache graphs;
 };
 
 namespace {
@@ -175,6 +256,63 @@ const double kCnAlpha[6] = {0, 0.1496590219993, 0.3704009573644, 0.6222557631345
 const double kCnBeta[5] = {0, -0.4178904745, -1.192151694643, -1.697784692471, -1.514183444257};
 const double kCnGamma[5] = {0.1496590219993, 0.3792103129999, 0.8229550293869, 0.6994504559488,
                             0.1530572479681};
+
+constexpr int kDLInt = 0;
+
+// A 1-D int32 device tensor of distinct slot indices in [0, batch); the
+// values are the caller's contract (the kernels index with them directly),
+// only the shape, dtype and device are checked here.
+int check_slots(const jcfd_tensor* t, int batch, int device, void** ptr, int* count) {
+  if (t == nullptr) return fail(JCFD_ERR_INVALID_ARGUMENT, "slots: null tensor");
+  if (t->device.device_type != kDLROCM || (device >= 0 && t->device.device_id != device))
+    return fail(JCFD_ERR_INVALID_ARGUMENT, "slots: must be on the plan's ROCm device");
+  if (t->dtype.code != kDLInt || t->dtype.bits != 32 || t->dtype.lanes != 1)
+    return fail(JCFD_ERR_INVALID_ARGUMENT, "slots: dtype must be int32");
+  if (t->ndim != 1 || t->shape == nullptr || t->shape[0] < 0 || t->shape[0] > batch)
+    return fail(JCFD_ERR_INVALID_ARGUMENT, "slots: must be 1-D with at most `batch` entries");
+  if (t->strides != nullptr && t->shape[0] > 1 && t->strides[0] != 1)
+    return fail(JCFD_ERR_INVALID_ARGUMENT, "slots: must be contiguous");
+  *count = static_cast<int>(t->shape[0]);
+  *ptr = static_cast<char*>(t->data) + t->byte_offset;
+  return JCFD_OK;
+}
+
+FvmBuffers fvm_buffers(const jcfd_fvm_plan* p) {
+  FvmBuffers b{};
+  b.us = p->us;
+  b.vs = p->vs;
+  b.spec = p->spec;
+  b.fx = p->fx;
+  b.lam = p->lam;
+  b.tw = p->tw;
+  return b;
+}
+
+// nsteps steps: the first reads b.u/b.v, the rest run in place on
+// b.u_out/b.v_out (replayed from a graph when enabled).
+int fvm_run(jcfd_fvm_plan* p, FvmBuffers b, int64_t nsteps, hipStream_t s) {
+  int rc;
+  if ((rc = fvm_launch_step(p->params, b, s))) return rc;
+  b.u = b.u_out;
+  b.v = b.v_out;
+  int64_t remaining = nsteps - 1;
+  if (p->graphs.enabled && remaining >= kGraphChunk) {
+    auto launch = [&](hipStream_t gs) {
+      for (int i = 0; i < kGraphChunk; ++i) {
+        const int r = fvm_launch_step(p->params, b, gs);
+        if (r) return r;
+      }
+      return static_cast<int>(JCFD_OK);
+    };
+    if ((rc = p->graphs.run(b.u_out, b.v_out, b.slots, b.count, kGraphChunk,
+                            remaining / kGraphChunk, s, launch)))
+      return rc;
+    remaining %= kGraphChunk;
+  }
+  for (int64_t step = 0; step < remaining; ++step)
+    if ((rc = fvm_launch_step(p->params, b, s))) return rc;
+  return JCFD_OK;
+}
 }  // namespace
 
 extern "C" {
@@ -203,6 +341,7 @@ int jcfd_fvm_plan_create(const jcfd_fvm_config* config, jcfd_fvm_plan** plan) {
   if (p == nullptr) return fail(JCFD_ERR_OUT_OF_MEMORY, "host allocation failed");
   p->cfg = c;
   p->cfg.forcing_x = nullptr;
+  p->graphs.enabled = false;  // one instance: kernels already run back to back
   if (hipGetDevice(&p->device) != hipSuccess) {
     delete p;
     return fail(JCFD_ERR_DEVICE, "hipGetDevice failed (no GPU?)");
@@ -227,7 +366,9 @@ int jcfd_fvm_plan_create(const jcfd_fvm_config* config, jcfd_fvm_plan** plan) {
   q.lin = static_cast<float>(c.linear_coefficient);
   q.convect = c.convect;
 
-  const size_t nn = static_cast<size_t>(n) * n;
+  p->batch = c.batch > 1 ? c.batch : 1;
+  p->graphs.enabled = p->batch > 1;
+  const size_t nn = static_cast<size_t>(n) * n * p->batch;
   int rc = JCFD_OK;
   if ((rc = device_alloc(&p->us, nn, &p->bytes)) || (rc = device_alloc(&p->vs, nn, &p->bytes)) ||
       (rc = device_alloc(&p->spec, nn / 2, &p->bytes)) ||
@@ -259,7 +400,14 @@ int jcfd_fvm_plan_destroy(jcfd_fvm_plan* p) {
   for (void* ptr : {static_cast<void*>(p->us), static_cast<void*>(p->vs), static_cast<void*>(p->spec),
                     static_cast<void*>(p->lam), static_cast<void*>(p->tw), static_cast<void*>(p->fx)})
     if (ptr != nullptr) (void)hipFree(ptr);
+  p->graphs.release();
   delete p;
+  return JCFD_OK;
+}
+
+int jcfd_fvm_plan_set_graphs(jcfd_fvm_plan* p, int32_t enabled) {
+  if (p == nullptr) return fail(JCFD_ERR_INVALID_ARGUMENT, "null plan");
+  p->graphs.enabled = enabled != 0;
   return JCFD_OK;
 }
 
@@ -272,34 +420,50 @@ int jcfd_fvm_step(jcfd_fvm_plan* p, const jcfd_tensor* u, const jcfd_tensor* v, 
   const int n = p->params.n;
   void *pu, *pv, *pu2, *pv2;
   int rc;
-  if ((rc = check_tensor(u, "u", kDLFloat, 32, n, n, p->device, &pu)) ||
-      (rc = check_tensor(v, "v", kDLFloat, 32, n, n, p->device, &pv)) ||
-      (rc = check_tensor(u_out, "u_out", kDLFloat, 32, n, n, p->device, &pu2)) ||
-      (rc = check_tensor(v_out, "v_out", kDLFloat, 32, n, n, p->device, &pv2)))
+  if ((rc = check_batched(u, "u", kDLFloat, 32, p->batch, n, n, p->device, &pu)) ||
+      (rc = check_batched(v, "v", kDLFloat, 32, p->batch, n, n, p->device, &pv)) ||
+      (rc = check_batched(u_out, "u_out", kDLFloat, 32, p->batch, n, n, p->device, &pu2)) ||
+      (rc = check_batched(v_out, "v_out", kDLFloat, 32, p->batch, n, n, p->device, &pv2)))
     return rc;
   hipStream_t s = static_cast<hipStream_t>(stream);
+  const size_t bytes = sizeof(float) * p->batch * static_cast<size_t>(n) * n;
   if (nsteps == 0) {
-    if (pu2 != pu && (rc = check_hip(hipMemcpyAsync(pu2, pu, sizeof(float) * n * n, hipMemcpyDeviceToDevice, s), "copy")))
+    if (pu2 != pu && (rc = check_hip(hipMemcpyAsync(pu2, pu, bytes, hipMemcpyDeviceToDevice, s), "copy")))
       return rc;
-    if (pv2 != pv && (rc = check_hip(hipMemcpyAsync(pv2, pv, sizeof(float) * n * n, hipMemcpyDeviceToDevice, s), "copy")))
+    if (pv2 != pv && (rc = check_hip(hipMemcpyAsync(pv2, pv, bytes, hipMemcpyDeviceToDevice, s), "copy")))
       return rc;
     return JCFD_OK;
   }
-  FvmBuffers b{};
-  b.us = p->us;
-  b.vs = p->vs;
-  b.spec = p->spec;
+  FvmBuffers b = fvm_buffers(p);
+  b.u = static_cast<const float*>(pu);
+  b.v = static_cast<const float*>(pv);
   b.u_out = static_cast<float*>(pu2);
   b.v_out = static_cast<float*>(pv2);
-  b.fx = p->fx;
-  b.lam = p->lam;
-  b.tw = p->tw;
-  for (int64_t step = 0; step < nsteps; ++step) {
-    b.u = step == 0 ? static_cast<const float*>(pu) : b.u_out;
-    b.v = step == 0 ? static_cast<const float*>(pv) : b.v_out;
-    if ((rc = fvm_launch_step(p->params, b, s))) return rc;
-  }
-  return JCFD_OK;
+  b.count = p->batch;
+  return fvm_run(p, b, nsteps, s);
+}
+
+int jcfd_fvm_step_slots(jcfd_fvm_plan* p, jcfd_tensor* u, jcfd_tensor* v, const jcfd_tensor* slots,
+                        int64_t nsteps, void* stream) {
+  if (p == nullptr) return fail(JCFD_ERR_INVALID_ARGUMENT, "null plan");
+  if (nsteps < 0) return fail(JCFD_ERR_INVALID_ARGUMENT, "nsteps must be >= 0");
+  const int n = p->params.n;
+  void *pu, *pv, *ps;
+  int count = 0;
+  int rc;
+  if ((rc = check_batched(u, "u", kDLFloat, 32, p->batch, n, n, p->device, &pu)) ||
+      (rc = check_batched(v, "v", kDLFloat, 32, p->batch, n, n, p->device, &pv)) ||
+      (rc = check_slots(slots, p->batch, p->device, &ps, &count)))
+    return rc;
+  if (nsteps == 0 || count == 0) return JCFD_OK;
+  FvmBuffers b = fvm_buffers(p);
+  b.u = static_cast<const float*>(pu);
+  b.v = static_cast<const float*>(pv);
+  b.u_out = static_cast<float*>(pu);
+  b.v_out = static_cast<float*>(pv);
+  b.slots = static_cast<const int*>(ps);
+  b.count = count;
+  return fvm_run(p, b, nsteps, static_cast<hipStream_t>(stream));
 }
 
 int jcfd_fvm_explicit_terms(jcfd_fvm_plan* p, const jcfd_tensor* u, const jcfd_tensor* v,
@@ -435,6 +599,42 @@ int jcfd_fvm_spectral_filter(jcfd_fvm_plan* p, const jcfd_tensor* x, const jcfd_
 }  // extern "C"
 
 // ------------------------------------------------------------ spectral path
+namespace {
+// nsteps CN-RK4 steps of `count` instances (slots, or 0..count-1): the first
+// reads b.u_in, the rest run in place on b.u_out (graph replay when enabled).
+int ns_run(jcfd_ns2d_plan* p, NsBuffers b, const int* slots, int count, int64_t nsteps,
+           hipStream_t s) {
+  NsParams stages[5];
+  for (int k = 0; k < 5; ++k) {
+    stages[k] = p->stages[k];
+    stages[k].slots = slots;
+  }
+  b.h = p->h;
+  b.g = p->g;
+  b.pbuf = p->pbuf;
+  int rc;
+  if ((rc = ns_launch_step(p->n, stages, b, count, s))) return rc;
+  b.u_in = b.u_out;
+  int64_t remaining = nsteps - 1;
+  if (p->graphs.enabled && remaining >= kGraphChunk) {
+    auto launch = [&](hipStream_t gs) {
+      for (int i = 0; i < kGraphChunk; ++i) {
+        const int r = ns_launch_step(p->n, stages, b, count, gs);
+        if (r) return r;
+      }
+      return static_cast<int>(JCFD_OK);
+    };
+    if ((rc = p->graphs.run(b.u_out, nullptr, slots, count, kGraphChunk, remaining / kGraphChunk, s,
+                            launch)))
+      return rc;
+    remaining %= kGraphChunk;
+  }
+  for (int64_t step = 0; step < remaining; ++step)
+    if ((rc = ns_launch_step(p->n, stages, b, count, s))) return rc;
+  return JCFD_OK;
+}
+}  // namespace
+
 extern "C" {
 
 int jcfd_ns2d_plan_destroy(jcfd_ns2d_plan* p) {
@@ -443,7 +643,14 @@ int jcfd_ns2d_plan_destroy(jcfd_ns2d_plan* p) {
                     static_cast<void*>(p->force_idx), static_cast<void*>(p->force_val),
                     static_cast<void*>(p->h), static_cast<void*>(p->g), static_cast<void*>(p->pbuf)})
     if (ptr != nullptr) (void)hipFree(ptr);
+  p->graphs.release();
   delete p;
+  return JCFD_OK;
+}
+
+int jcfd_ns2d_plan_set_graphs(jcfd_ns2d_plan* p, int32_t enabled) {
+  if (p == nullptr) return fail(JCFD_ERR_INVALID_ARGUMENT, "null plan");
+  p->graphs.enabled = enabled != 0;
   return JCFD_OK;
 }
 
@@ -474,6 +681,7 @@ int jcfd_ns2d_plan_create(const jcfd_ns2d_config* config, jcfd_ns2d_plan** plan)
   p->cfg.forcing_value = nullptr;
   p->n = n;
   p->batch = c.batch;
+  p->graphs.enabled = c.batch > 1;  // replay pays off only once launches are long
   if (hipGetDevice(&p->device) != hipSuccess) {
     delete p;
     return fail(JCFD_ERR_DEVICE, "hipGetDevice failed (no GPU?)");
@@ -536,6 +744,7 @@ int jcfd_ns2d_plan_create(const jcfd_ns2d_config* config, jcfd_ns2d_plan** plan)
     q.g = static_cast<float>(kCnGamma[k] * c.dt);
     q.mu = static_cast<float>(0.5 * c.dt * (kCnAlpha[k + 1] - kCnAlpha[k]));
     q.read_h = k > 0 ? 1 : 0;
+    q.slots = nullptr;
   }
   *plan = p;
   return JCFD_OK;
@@ -559,15 +768,25 @@ int jcfd_ns2d_step(jcfd_ns2d_plan* p, const jcfd_tensor* w_hat, jcfd_tensor* w_o
     return JCFD_OK;
   }
   NsBuffers b{};
+  b.u_in = static_cast<const float2*>(pin);
   b.u_out = static_cast<float2*>(pout);
-  b.h = p->h;
-  b.g = p->g;
-  b.pbuf = p->pbuf;
-  for (int64_t step = 0; step < nsteps; ++step) {
-    b.u_in = step == 0 ? static_cast<const float2*>(pin) : b.u_out;
-    if ((rc = ns_launch_step(p->n, p->stages, b, p->batch, s))) return rc;
-  }
-  return JCFD_OK;
+  return ns_run(p, b, nullptr, p->batch, nsteps, s);
+}
+
+int jcfd_ns2d_step_slots(jcfd_ns2d_plan* p, jcfd_tensor* w_hat, const jcfd_tensor* slots,
+                         int64_t nsteps, void* stream) {
+  if (p == nullptr) return fail(JCFD_ERR_INVALID_ARGUMENT, "null plan");
+  if (nsteps < 0) return fail(JCFD_ERR_INVALID_ARGUMENT, "nsteps must be >= 0");
+  void *pw, *ps;
+  int count = 0, rc;
+  if ((rc = check_spectral(w_hat, "w_hat", p->batch, p->n, p->device, &pw)) ||
+      (rc = check_slots(slots, p->batch, p->device, &ps, &count)))
+    return rc;
+  if (nsteps == 0 || count == 0) return JCFD_OK;
+  NsBuffers b{};
+  b.u_in = static_cast<const float2*>(pw);
+  b.u_out = static_cast<float2*>(pw);
+  return ns_run(p, b, static_cast<const int*>(ps), count, nsteps, static_cast<hipStream_t>(stream));
 }
 
 int jcfd_ns2d_explicit_terms(jcfd_ns2d_plan* p, const jcfd_tensor* w_hat, jcfd_tensor* out,
@@ -582,6 +801,56 @@ int jcfd_ns2d_explicit_terms(jcfd_ns2d_plan* p, const jcfd_tensor* w_hat, jcfd_t
   return ns_launch_explicit_terms(p->n, p->stages[0], static_cast<const float2*>(pin),
                                   static_cast<float2*>(pout), p->g, p->pbuf, p->batch,
                                   static_cast<hipStream_t>(stream));
+}
+
+int jcfd_ns2d_time_kernels(jcfd_ns2d_plan* p, const jcfd_tensor* w_hat, int32_t reps,
+                           double* ms_per_launch, void* stream) {
+  if (p == nullptr || ms_per_launch == nullptr) return fail(JCFD_ERR_INVALID_ARGUMENT, "null argument");
+  if (reps <= 0) return fail(JCFD_ERR_INVALID_ARGUMENT, "reps must be positive");
+  void* pin;
+  int rc;
+  if ((rc = check_spectral(w_hat, "w_hat", p->batch, p->n, p->device, &pin))) return rc;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const size_t count = static_cast<size_t>(p->batch) * p->n * (p->n / 2 + 1);
+  float2* tmp = nullptr;
+  int64_t unused = 0;
+  if ((rc = device_alloc(&tmp, count, &unused))) return rc;
+  NsBuffers b{};
+  b.u_in = static_cast<const float2*>(pin);
+  b.u_out = tmp;
+  b.h = p->h;
+  b.g = p->g;
+  b.pbuf = p->pbuf;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  rc = check_hip(hipEventCreate(&e0), "event");
+  if (!rc) rc = check_hip(hipEventCreate(&e1), "event");
+  if (!rc) rc = ns_launch_step(p->n, p->stages, b, p->batch, s);  // fills every scratch buffer
+  for (int k = 0; k < JCFD_NS2D_KERNEL_COUNT && rc == JCFD_OK; ++k) {
+    rc = check_hip(hipEventRecord(e0, s), "event record");
+    for (int r = 0; r < reps && rc == JCFD_OK; ++r) rc = ns_launch_kernel(p->n, k, p->stages, b, p->batch, s);
+    if (!rc) rc = check_hip(hipEventRecord(e1, s), "event record");
+    if (!rc) rc = check_hip(hipEventSynchronize(e1), "event sync");
+    float ms = 0.f;
+    if (!rc) rc = check_hip(hipEventElapsedTime(&ms, e0, e1), "elapsed");
+    ms_per_launch[k] = static_cast<double>(ms) / reps;
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  (void)hipStreamSynchronize(s);
+  (void)hipFree(tmp);
+  return rc;
+}
+
+int jcfd_ns2d_kernel_bytes(const jcfd_ns2d_plan* p, int64_t* bytes) {
+  if (p == nullptr || bytes == nullptr) return fail(JCFD_ERR_INVALID_ARGUMENT, "null argument");
+  const int64_t n = p->n, B = p->batch;
+  const int64_t ext = 8 * n * (n / 2 + 1);  // one complex64 half spectrum
+  const int64_t packed = 8 * n * (n / 2);   // one packed half spectrum
+  bytes[JCFD_NS2D_KERNEL_COLS_INVERSE] = B * (ext + 4 * packed);
+  bytes[JCFD_NS2D_KERNEL_ROWS] = B * (4 * packed + packed);
+  bytes[JCFD_NS2D_KERNEL_COLS_UPDATE_INVERSE] = B * (packed + 2 * ext + 2 * ext + 4 * packed);
+  bytes[JCFD_NS2D_KERNEL_COLS_UPDATE] = B * (packed + 2 * ext + 2 * ext);
+  return JCFD_OK;
 }
 
 int jcfd_ns2d_rfft2(jcfd_ns2d_plan* p, const jcfd_tensor* x, jcfd_tensor* out, void* stream) {

@@ -161,6 +161,7 @@ __global__ __launch_bounds__(kThreads) void fvm_rows_forward(FvmParams p, FvmBuf
   __shared__ float2 twn[N];
   __shared__ float2 twh[H];
   const int tid = threadIdx.x;
+  b = fvm_instance(b, N);
   const int i0 = blockIdx.x * RB;
 
   if constexpr (MODE == 3) {
@@ -237,6 +238,7 @@ __global__ __launch_bounds__(kThreads) void fvm_poisson_cols(FvmBuffers b, float
   __shared__ float2 buf[N * W];
   __shared__ float2 twn[N];
   const int tid = threadIdx.x;
+  b = fvm_instance(b, N);
   const int c0 = blockIdx.x * W;
   for (int e = tid; e < N * W; e += kThreads) {
     const int pp = e / W, c = e % W;
@@ -291,6 +293,7 @@ __global__ __launch_bounds__(kThreads) void fvm_rows_project(FvmParams p, FvmBuf
   __shared__ float2 twn[N];
   __shared__ float2 twh[H];
   const int tid = threadIdx.x;
+  b = fvm_instance(b, N);
   const int i0 = blockIdx.x * RB;
   load_twiddles(twn, b.tw, N, 1, tid, kThreads);
   load_twiddles(twh, b.tw, H, 2, tid, kThreads);
@@ -357,9 +360,9 @@ template <> struct Tiles<1024> { static constexpr int RB = 1, W = 4; };
 template <int N>
 int launch_step(const FvmParams& p, const FvmBuffers& b, hipStream_t s) {
   constexpr int RB = Tiles<N>::RB, W = Tiles<N>::W;
-  hipLaunchKernelGGL((fvm_rows_forward<N, RB, 0>), dim3(N / RB), dim3(kThreads), 0, s, p, b);
-  hipLaunchKernelGGL((fvm_poisson_cols<N, W>), dim3(N / 2 / W), dim3(kThreads), 0, s, b, 1.0f);
-  hipLaunchKernelGGL((fvm_rows_project<N, RB, 0>), dim3(N / RB), dim3(kThreads), 0, s, p, b);
+  hipLaunchKernelGGL((fvm_rows_forward<N, RB, 0>), dim3(N / RB, nb(b)), dim3(kThreads), 0, s, p, b);
+  hipLaunchKernelGGL((fvm_poisson_cols<N, W>), dim3(N / 2 / W, nb(b)), dim3(kThreads), 0, s, b, 1.0f);
+  hipLaunchKernelGGL((fvm_rows_project<N, RB, 0>), dim3(N / RB, nb(b)), dim3(kThreads), 0, s, p, b);
   return check_hip(hipGetLastError(), "fvm step launch");
 }
 
@@ -368,13 +371,13 @@ int launch_one(const FvmParams& p, const FvmBuffers& b, int which, hipStream_t s
   constexpr int RB = Tiles<N>::RB, W = Tiles<N>::W;
   switch (which) {
     case 0:
-      hipLaunchKernelGGL((fvm_rows_forward<N, RB, 0>), dim3(N / RB), dim3(kThreads), 0, s, p, b);
+      hipLaunchKernelGGL((fvm_rows_forward<N, RB, 0>), dim3(N / RB, nb(b)), dim3(kThreads), 0, s, p, b);
       break;
     case 1:
-      hipLaunchKernelGGL((fvm_poisson_cols<N, W>), dim3(N / 2 / W), dim3(kThreads), 0, s, b, 1.0f);
+      hipLaunchKernelGGL((fvm_poisson_cols<N, W>), dim3(N / 2 / W, nb(b)), dim3(kThreads), 0, s, b, 1.0f);
       break;
     default:
-      hipLaunchKernelGGL((fvm_rows_project<N, RB, 0>), dim3(N / RB), dim3(kThreads), 0, s, p, b);
+      hipLaunchKernelGGL((fvm_rows_project<N, RB, 0>), dim3(N / RB, nb(b)), dim3(kThreads), 0, s, p, b);
   }
   return check_hip(hipGetLastError(), "fvm kernel launch");
 }
@@ -382,16 +385,16 @@ int launch_one(const FvmParams& p, const FvmBuffers& b, int which, hipStream_t s
 template <int N>
 int launch_explicit(const FvmParams& p, const FvmBuffers& b, hipStream_t s) {
   constexpr int RB = Tiles<N>::RB;
-  hipLaunchKernelGGL((fvm_rows_forward<N, RB, 2>), dim3(N / RB), dim3(kThreads), 0, s, p, b);
+  hipLaunchKernelGGL((fvm_rows_forward<N, RB, 2>), dim3(N / RB, nb(b)), dim3(kThreads), 0, s, p, b);
   return check_hip(hipGetLastError(), "fvm explicit-terms launch");
 }
 
 template <int N>
 int launch_projection(const FvmParams& p, const FvmBuffers& b, hipStream_t s) {
   constexpr int RB = Tiles<N>::RB, W = Tiles<N>::W;
-  hipLaunchKernelGGL((fvm_rows_forward<N, RB, 1>), dim3(N / RB), dim3(kThreads), 0, s, p, b);
-  hipLaunchKernelGGL((fvm_poisson_cols<N, W>), dim3(N / 2 / W), dim3(kThreads), 0, s, b, 1.0f);
-  hipLaunchKernelGGL((fvm_rows_project<N, RB, 0>), dim3(N / RB), dim3(kThreads), 0, s, p, b);
+  hipLaunchKernelGGL((fvm_rows_forward<N, RB, 1>), dim3(N / RB, nb(b)), dim3(kThreads), 0, s, p, b);
+  hipLaunchKernelGGL((fvm_poisson_cols<N, W>), dim3(N / 2 / W, nb(b)), dim3(kThreads), 0, s, b, 1.0f);
+  hipLaunchKernelGGL((fvm_rows_project<N, RB, 0>), dim3(N / RB, nb(b)), dim3(kThreads), 0, s, p, b);
   return check_hip(hipGetLastError(), "fvm projection launch");
 }
 
@@ -399,9 +402,9 @@ template <int N>
 int launch_filter(const FvmParams& p, const FvmBuffers& b, hipStream_t s) {
   constexpr int RB = Tiles<N>::RB, W = Tiles<N>::W;
   const float scale = 1.0f / (static_cast<float>(N) * static_cast<float>(N));
-  hipLaunchKernelGGL((fvm_rows_forward<N, RB, 3>), dim3(N / RB), dim3(kThreads), 0, s, p, b);
-  hipLaunchKernelGGL((fvm_poisson_cols<N, W>), dim3(N / 2 / W), dim3(kThreads), 0, s, b, scale);
-  hipLaunchKernelGGL((fvm_rows_project<N, RB, 1>), dim3(N / RB), dim3(kThreads), 0, s, p, b);
+  hipLaunchKernelGGL((fvm_rows_forward<N, RB, 3>), dim3(N / RB, nb(b)), dim3(kThreads), 0, s, p, b);
+  hipLaunchKernelGGL((fvm_poisson_cols<N, W>), dim3(N / 2 / W, nb(b)), dim3(kThreads), 0, s, b, scale);
+  hipLaunchKernelGGL((fvm_rows_project<N, RB, 1>), dim3(N / RB, nb(b)), dim3(kThreads), 0, s, p, b);
   return check_hip(hipGetLastError(), "spectral filter launch");
 }
 

@@ -42,7 +42,28 @@ struct FvmBuffers {
   const float* fx;       // forcing column f_x(y_j), n floats (may be null)
   const float* lam;      // Lambda^+ / n^2, (n, n/2 + 1) floats
   const float2* tw;      // twiddles exp(-2 pi i k / n), n entries
+  // Batches: instance k of a launch (blockIdx.y = k) is slots[k], or k when
+  // slots is null; fields of instance s start at s * n * n (spectrum:
+  // s * n * n / 2).
+  const int* slots;
+  int count;  // instances in this launch (gridDim.y); <= 0 means 1
 };
+
+inline int nb(const FvmBuffers& b) { return b.count > 0 ? b.count : 1; }
+
+// Per-instance view of FvmBuffers for the instance this block works on.
+__device__ __forceinline__ FvmBuffers fvm_instance(FvmBuffers b, int n) {
+  const size_t s = b.slots ? static_cast<size_t>(b.slots[blockIdx.y]) : blockIdx.y;
+  const size_t st = s * n * n;
+  if (b.u) b.u += st;
+  if (b.v) b.v += st;
+  if (b.us) b.us += st;
+  if (b.vs) b.vs += st;
+  if (b.u_out) b.u_out += st;
+  if (b.v_out) b.v_out += st;
+  if (b.spec) b.spec += st / 2;
+  return b;
+}
 
 // Kernel launchers (fvm.hip).  All are stream-ordered and asynchronous.
 int fvm_launch_step(const FvmParams& p, const FvmBuffers& b, hipStream_t s);
@@ -78,6 +99,7 @@ struct NsParams {
   int filt_row_lo, filt_row_hi, filt_cols;
   float beta, g, mu;
   int read_h;
+  const int* slots;       // instance of launch row blockIdx.y, or null = blockIdx.y
 };
 
 struct NsBuffers {
@@ -94,6 +116,10 @@ int ns_launch_explicit_terms(int n, const NsParams& q, const float2* w, float2* 
                              float2* pbuf, int batch, hipStream_t s);
 int ns_launch_rfft2(int n, const NsParams& q, const float* x, float2* out, float2* pbuf, int batch,
                     hipStream_t s);
+// One kernel class of the step: 0 cols-inverse, 1 rows, 2 cols-forward with
+// update + next inverse, 3 cols-forward with the final update.
+int ns_launch_kernel(int n, int which, const NsParams* stages, const NsBuffers& b, int batch,
+                     hipStream_t s);
 int ns_launch_irfft2(int n, const NsParams& q, const float2* in, float* x, float2* pbuf, int batch,
                      hipStream_t s);
 

@@ -142,7 +142,7 @@ __global__ __launch_bounds__(kThreads) void ns_cols_inverse(NsParams q, const fl
   __shared__ ColShared<N, W> sm;
   const int tid = threadIdx.x;
   const int c0 = blockIdx.x * W;
-  const size_t inst = blockIdx.y;
+  const size_t inst = q.slots ? static_cast<size_t>(q.slots[blockIdx.y]) : blockIdx.y;
   w += inst * static_cast<size_t>(N) * (N / 2 + 1);
   g += inst * NSPEC * static_cast<size_t>(N) * (N / 2);
   load_w_tile<N, W>(sm, w, c0, tid);
@@ -161,7 +161,7 @@ __global__ __launch_bounds__(kThreads) void rows_r2c(NsParams q, const float* __
   __shared__ float2 twh[H];
   const int tid = threadIdx.x;
   const int i0 = blockIdx.x * RB;
-  const size_t inst = blockIdx.y;
+  const size_t inst = q.slots ? static_cast<size_t>(q.slots[blockIdx.y]) : blockIdx.y;
   x += inst * static_cast<size_t>(N) * N;
   pout += inst * static_cast<size_t>(N) * H;
   load_twiddles(twn, q.tw, N, 1, tid, kThreads);
@@ -185,7 +185,7 @@ __global__ __launch_bounds__(kThreads) void rows_c2r(NsParams q, const float2* _
   __shared__ float2 twh[H];
   const int tid = threadIdx.x;
   const int i0 = blockIdx.x * RB;
-  const size_t inst = blockIdx.y;
+  const size_t inst = q.slots ? static_cast<size_t>(q.slots[blockIdx.y]) : blockIdx.y;
   x += inst * static_cast<size_t>(N) * N;
   pin += inst * static_cast<size_t>(N) * H;
   load_twiddles(twn, q.tw, N, 1, tid, kThreads);
@@ -212,7 +212,7 @@ __global__ __launch_bounds__(kThreads) void ns_rows(NsParams q, const float2* __
   __shared__ float2 twh[H];
   const int tid = threadIdx.x;
   const int i0 = blockIdx.x * RB;
-  const size_t inst = blockIdx.y;
+  const size_t inst = q.slots ? static_cast<size_t>(q.slots[blockIdx.y]) : blockIdx.y;
   const size_t gstride = static_cast<size_t>(N) * H;
   g += inst * 4 * gstride;
   pout += inst * gstride;
@@ -261,7 +261,7 @@ __global__ __launch_bounds__(kThreads) void ns_cols_forward(NsParams q, const fl
   __shared__ ColShared<N, W> sm;
   const int tid = threadIdx.x;
   const int c0 = blockIdx.x * W;
-  const size_t inst = blockIdx.y;
+  const size_t inst = q.slots ? static_cast<size_t>(q.slots[blockIdx.y]) : blockIdx.y;
   const size_t ext = static_cast<size_t>(N) * H1;
   pin += inst * static_cast<size_t>(N) * H;
   if (u_in) u_in += inst * ext;
@@ -440,6 +440,30 @@ int ns_explicit_impl(const NsParams& q, const float2* w, float2* out, float2* g,
 }  // namespace
 
 bool ns_size_supported(int n) { return n == 32 || n == 64 || n == 128 || n == 256 || n == 512; }
+
+namespace {
+template <int N>
+int ns_one_impl(int which, const NsParams* stages, const NsBuffers& b, int batch, hipStream_t s) {
+  switch (which) {
+    case 0: return NsOps<N>::cols_inverse(stages[0], b.u_in, b.g, batch, s);
+    case 1: return NsOps<N>::rows(stages[0], b.g, b.pbuf, batch, s);
+    case 2: return NsOps<N>::cols_forward(2, stages[1], b.pbuf, b.u_out, b.u_out, b.h, b.g, batch, s);
+    default: return NsOps<N>::cols_forward(1, stages[4], b.pbuf, b.u_out, b.u_out, b.h, b.g, batch, s);
+  }
+}
+}  // namespace
+
+int ns_launch_kernel(int n, int which, const NsParams* stages, const NsBuffers& b, int batch,
+                     hipStream_t s) {
+  switch (n) {
+    case 32: return ns_one_impl<32>(which, stages, b, batch, s);
+    case 64: return ns_one_impl<64>(which, stages, b, batch, s);
+    case 128: return ns_one_impl<128>(which, stages, b, batch, s);
+    case 256: return ns_one_impl<256>(which, stages, b, batch, s);
+    case 512: return ns_one_impl<512>(which, stages, b, batch, s);
+    default: return fail(JCFD_ERR_UNSUPPORTED, "spectral grid size not built");
+  }
+}
 
 int ns_launch_step(int n, const NsParams* stages, const NsBuffers& b, int batch, hipStream_t s) {
   switch (n) {

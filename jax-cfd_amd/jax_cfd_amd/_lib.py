@@ -53,6 +53,7 @@ class FvmConfig(ctypes.Structure):
       ("has_forcing", ctypes.c_int32),
       ("linear_coefficient", ctypes.c_double),
       ("forcing_x", ctypes.POINTER(ctypes.c_float)),
+      ("batch", ctypes.c_int32),
   ]
 
 
@@ -75,8 +76,13 @@ _SIGNATURES = {
     "jcfd_ns2d_plan_create": (ctypes.c_int, [ctypes.POINTER(Ns2dConfig), ctypes.POINTER(_P)]),
     "jcfd_ns2d_plan_destroy": (ctypes.c_int, [_P]),
     "jcfd_ns2d_plan_device_bytes": (ctypes.c_int64, [_P]),
+    "jcfd_ns2d_plan_set_graphs": (ctypes.c_int, [_P, ctypes.c_int32]),
+    "jcfd_fvm_plan_set_graphs": (ctypes.c_int, [_P, ctypes.c_int32]),
     "jcfd_ns2d_step": (ctypes.c_int, [_P, _P, _P, ctypes.c_int64, _P]),
     "jcfd_ns2d_explicit_terms": (ctypes.c_int, [_P, _P, _P, _P]),
+    "jcfd_ns2d_time_kernels": (ctypes.c_int, [_P, _P, ctypes.c_int32,
+                                              ctypes.POINTER(ctypes.c_double), _P]),
+    "jcfd_ns2d_kernel_bytes": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int64)]),
     "jcfd_ns2d_rfft2": (ctypes.c_int, [_P, _P, _P, _P]),
     "jcfd_ns2d_irfft2": (ctypes.c_int, [_P, _P, _P, _P]),
     "jcfd_last_error": (ctypes.c_char_p, []),
@@ -85,6 +91,8 @@ _SIGNATURES = {
     "jcfd_fvm_plan_destroy": (ctypes.c_int, [_P]),
     "jcfd_fvm_plan_device_bytes": (ctypes.c_int64, [_P]),
     "jcfd_fvm_step": (ctypes.c_int, [_P, _P, _P, _P, _P, ctypes.c_int64, _P]),
+    "jcfd_fvm_step_slots": (ctypes.c_int, [_P, _P, _P, _P, ctypes.c_int64, _P]),
+    "jcfd_ns2d_step_slots": (ctypes.c_int, [_P, _P, _P, ctypes.c_int64, _P]),
     "jcfd_fvm_explicit_terms": (ctypes.c_int, [_P, _P, _P, _P, _P, _P]),
     "jcfd_fvm_projection": (ctypes.c_int, [_P, _P, _P, _P, _P, _P]),
     "jcfd_fvm_divergence": (ctypes.c_int, [_P, _P, _P, _P, _P]),

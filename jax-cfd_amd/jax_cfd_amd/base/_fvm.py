@@ -40,9 +40,11 @@ def spec_from_grid(grid, **kw) -> FvmSpec:
 class FvmPlan:
   """Owns one jcfd_fvm_plan (device constants + scratch) on one GPU."""
 
-  def __init__(self, spec: FvmSpec, device: Optional[torch.device] = None):
+  def __init__(self, spec: FvmSpec, device: Optional[torch.device] = None,
+               batch: int = 1):
     lib = _lib.load()
     self.spec = spec
+    self.batch = int(batch)
     self.device = device if device is not None else torch.device(
         'cuda', torch.cuda.current_device())
     cfg = _lib.FvmConfig()
@@ -64,6 +66,7 @@ class FvmPlan:
       if self._fx.shape != (spec.n,):
         raise ValueError('forcing_x must hold n values')
       cfg.forcing_x = self._fx.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+    cfg.batch = self.batch
     handle = ctypes.c_void_p()
     with torch.cuda.device(self.device):
       _lib.check(lib.jcfd_fvm_plan_create(ctypes.byref(cfg), ctypes.byref(handle)),
@@ -74,6 +77,10 @@ class FvmPlan:
   @property
   def device_bytes(self) -> int:
     return int(self._lib.jcfd_fvm_plan_device_bytes(self._handle))
+
+  def set_graphs(self, enabled: bool):
+    _lib.check(self._lib.jcfd_fvm_plan_set_graphs(self._handle, int(bool(enabled))),
+               'jcfd_fvm_plan_set_graphs')
 
   def close(self):
     if getattr(self, '_handle', None) is not None and self._handle.value:
@@ -89,14 +96,28 @@ class FvmPlan:
   def _field(self, x) -> torch.Tensor:
     return _lib.as_device_tensor(x, torch.float32, self.device)
 
-  def _empty(self):
+  @property
+  def shape(self):
     n = self.spec.n
-    return torch.empty((n, n), dtype=torch.float32, device=self.device)
+    return (n, n) if self.batch == 1 else (self.batch, n, n)
+
+  def _empty(self, batched: bool = False):
+    n = self.spec.n
+    return torch.empty(self.shape if batched else (n, n), dtype=torch.float32,
+                       device=self.device)
+
+  def step_slots(self, u: torch.Tensor, v: torch.Tensor, slots: torch.Tensor,
+                 nsteps: int):
+    """Advance only `slots` (int32 device tensor) of (batch, n, n) u, v in place."""
+    with _lib.dlpack_args() as dl:
+      rc = self._lib.jcfd_fvm_step_slots(self._handle, dl(u), dl(v), dl(slots),
+                                         int(nsteps), _lib.stream_handle(self.device))
+    _lib.check(rc, 'jcfd_fvm_step_slots')
 
   def step(self, u, v, nsteps: int = 1, out=None):
     """nsteps forward-Euler + projection steps; returns (u, v) tensors."""
     u, v = self._field(u), self._field(v)
-    uo, vo = out if out is not None else (self._empty(), self._empty())
+    uo, vo = out if out is not None else (self._empty(True), self._empty(True))
     with _lib.dlpack_args() as dl:
       rc = self._lib.jcfd_fvm_step(self._handle, dl(u), dl(v), dl(uo), dl(vo),
                                    int(nsteps), _lib.stream_handle(self.device))

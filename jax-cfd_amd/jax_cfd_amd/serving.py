@@ -1,0 +1,375 @@
+"""Continuous-batching rollout server for mixed FVM / spectral requests.
+
+Each request asks for `steps` steps of one solver on one grid size, from the
+seeded initial field of that solver (FVM: filtered_velocity_field; spectral:
+rfft2 of its curl).  The server keeps every admitted request's state resident
+on the GPU in a slot of a *pool* -- one pool per (solver, n), holding one
+batched plan and a (capacity, ...) state tensor -- and advances all active
+slots of a pool together with one slot-indexed launch sequence
+(jcfd_fvm_step_slots / jcfd_ns2d_step_slots).  Pools run on their own HIP
+streams, so their kernels overlap on the device; the host loop polls each
+pool's completion event and, as soon as a pool's chunk is done, retires the
+requests that reached their step count, admits waiting requests into the
+freed slots and relaunches it.  A chunk advances every active slot by the
+smallest remaining step count (capped), so no request overshoots.
+
+Admission is bounded by a device-memory budget: a pool's bytes (plan scratch
++ states) are reserved when it is created; a pool that does not fit waits
+until other pools drain, and a request that can never fit is rejected.
+
+Every slot gets exactly the arithmetic of a lone instance (the kernels never
+mix instances), so a request's result is bit-identical alone or in a mix.
+"""
+from __future__ import annotations
+
+import collections
+import dataclasses
+import time
+from typing import Deque, Dict, List, Optional, Tuple
+
+import numpy as np
+import torch
+
+from jax_cfd_amd import _lib
+from jax_cfd_amd.base import _fvm, equations, grids, initial_conditions
+from jax_cfd_amd.base import finite_differences
+from jax_cfd_amd.spectral import _ns2d
+from jax_cfd_amd.spectral import equations as spectral_equations
+
+TWO_PI = 2 * np.pi
+
+
+@dataclasses.dataclass
+class RolloutRequest:
+  request_id: int
+  solver: str          # 'fvm' or 'spectral'
+  n: int
+  steps: int
+  seed: int
+  arrival: float = 0.0  # seconds after the server starts
+
+
+@dataclasses.dataclass
+class RolloutResult:
+  request_id: int
+  solver: str
+  n: int
+  steps: int
+  status: str                       # 'done' or 'rejected'
+  state: Optional[Tuple[np.ndarray, ...]] = None
+  arrival: float = 0.0
+  admitted: float = 0.0
+  finished: float = 0.0
+
+  @property
+  def latency(self) -> float:
+    return self.finished - self.arrival
+
+
+# --------------------------------------------------------------- solvers
+class FvmFamily:
+  """The forced FVM configuration: viscosity 1e-3, density 1, Kolmogorov
+  sin(4y) forcing with linear drag 0.1, Van Leer, fast-diagonalisation
+  projection, dt = stable_time_step(7, 0.5); IC filtered_velocity_field(seed,
+  v_max 7, peak wavenumber 4)."""
+  name = 'fvm'
+  V_MAX, PEAK, NU, CFL = 7.0, 4, 1e-3, 0.5
+
+  def __init__(self, n: int, device: torch.device):
+    self.n = n
+    self.device = device
+    self.grid = grids.Grid((n, n), domain=((0, TWO_PI), (0, TWO_PI)))
+    dt = equations.stable_time_step(self.V_MAX, self.CFL, self.NU, self.grid)
+    from jax_cfd_amd.base import advection, forcings
+    step = equations.semi_implicit_navier_stokes(
+        density=1.0, viscosity=self.NU, dt=dt, grid=self.grid,
+        convect=lambda v: tuple(advection.advect_van_leer(u, v, dt) for u in v),
+        forcing=forcings.simple_turbulence_forcing(
+            self.grid, constant_magnitude=1.0, constant_wavenumber=4,
+            linear_coefficient=-0.1, forcing_type='kolmogorov'))
+    self.spec = step.spec
+    self.dt = dt
+
+  def state_bytes(self, capacity: int) -> int:
+    return 2 * capacity * self.n * self.n * 4
+
+  def make_plan(self, capacity: int):
+    return _fvm.FvmPlan(self.spec, self.device, batch=capacity)
+
+  def alloc_state(self, capacity: int):
+    n = self.n
+    return (torch.zeros((capacity, n, n), dtype=torch.float32, device=self.device),
+            torch.zeros((capacity, n, n), dtype=torch.float32, device=self.device))
+
+  def initial_state(self, seed: int):
+    v = initial_conditions.filtered_velocity_field(seed, self.grid, self.V_MAX,
+                                                   self.PEAK, device=self.device)
+    return (v[0].data, v[1].data)
+
+  def write_slot(self, state, slot, init):
+    state[0][slot].copy_(init[0])
+    state[1][slot].copy_(init[1])
+
+  def advance(self, plan, state, slots, nsteps):
+    plan.step_slots(state[0], state[1], slots, nsteps)
+
+  def read_slot(self, state, slot):
+    return (state[0][slot].clone(), state[1][slot].clone())
+
+
+class SpectralFamily:
+  """ForcedNavierStokes2D(viscosity 1e-3, smooth) with crank_nicolson_rk4,
+  dt 1e-3; IC rfft2(curl_2d(filtered_velocity_field(seed, v_max 1, peak 4)))
+  (v_max 1 is the reference's default for filtered_velocity_field)."""
+  name = 'spectral'
+  V_MAX, PEAK, NU, DT = 1.0, 4, 1e-3, 1e-3
+
+  def __init__(self, n: int, device: torch.device):
+    self.n = n
+    self.device = device
+    self.grid = grids.Grid((n, n), domain=((0, TWO_PI), (0, TWO_PI)))
+    self.equation = spectral_equations.ForcedNavierStokes2D(self.NU, self.grid, True)
+    self.spec = self.equation.spec(self.DT)
+    self._fft = None
+
+  def state_bytes(self, capacity: int) -> int:
+    return capacity * self.n * (self.n // 2 + 1) * 8
+
+  def make_plan(self, capacity: int):
+    return _ns2d.Ns2dPlan(self.spec, capacity, self.device)
+
+  def alloc_state(self, capacity: int):
+    n = self.n
+    return (torch.zeros((capacity, n, n // 2 + 1), dtype=torch.complex64,
+                        device=self.device),)
+
+  def initial_state(self, seed: int):
+    v = initial_conditions.filtered_velocity_field(seed, self.grid, self.V_MAX,
+                                                   self.PEAK, device=self.device)
+    w = finite_differences.curl_2d(v).data
+    if self._fft is None:
+      self._fft = _ns2d.Ns2dPlan(self.spec, 1, self.device)
+    return (self._fft.rfft2(w),)
+
+  def write_slot(self, state, slot, init):
+    state[0][slot].copy_(init[0])
+
+  def advance(self, plan, state, slots, nsteps):
+    plan.step_slots(state[0], slots, nsteps)
+
+  def read_slot(self, state, slot):
+    return (state[0][slot].clone(),)
+
+
+FAMILIES = {'fvm': FvmFamily, 'spectral': SpectralFamily}
+
+
+# ------------------------------------------------------------------ pools
+class Pool:
+  """Device-resident slots of one (solver, n) with one batched plan."""
+
+  def __init__(self, family, capacity: int):
+    self.family = family
+    self.capacity = capacity
+    self.plan = family.make_plan(capacity)
+    self.state = family.alloc_state(capacity)
+    self.stream = torch.cuda.Stream(device=family.device)  # default priority
+    # active slot list; a fixed buffer so captured graphs stay valid
+    self.slot_buf = torch.zeros(capacity, dtype=torch.int32, device=family.device)
+    self.free: List[int] = list(range(capacity))[::-1]
+    self.active: Dict[int, List] = {}  # slot -> [request, remaining, admitted]
+    self.event: Optional[torch.cuda.Event] = None
+    self.chunk_done: List[int] = []
+    self.bytes = (self.plan.device_bytes + family.state_bytes(capacity)
+                  + 4 * capacity)
+
+  @property
+  def busy(self) -> bool:
+    return self.event is not None and not self.event.query()
+
+  def admit(self, req: RolloutRequest, now: float, ic_stream: torch.cuda.Stream):
+    """Build the request's initial field into a free slot.  All initial
+    fields are built on one server-wide stream (their helper plans are
+    shared); the pool stream waits for it before its next launch."""
+    slot = self.free.pop()
+    with torch.cuda.stream(ic_stream):
+      init = self.family.initial_state(req.seed)
+      self.family.write_slot(self.state, slot, init)
+      ready = torch.cuda.Event()
+      ready.record(ic_stream)
+    self.stream.wait_event(ready)
+    self.active[slot] = [req, req.steps, now]
+    return slot
+
+  def launch(self, max_chunk: int):
+    """Advance every active slot by the smallest remaining count (capped)."""
+    if not self.active:
+      return
+    chunk = min(max_chunk, min(a[1] for a in self.active.values()))
+    slots = sorted(self.active)
+    with torch.cuda.stream(self.stream):
+      idx = self.slot_buf[:len(slots)]
+      idx.copy_(torch.tensor(slots, dtype=torch.int32))
+      self.family.advance(self.plan, self.state, idx, chunk)
+      self.event = torch.cuda.Event()
+      self.event.record(self.stream)
+    for s in slots:
+      self.active[s][1] -= chunk
+    self.chunk_done = [s for s in slots if self.active[s][1] == 0]
+
+  def harvest(self, now_fn) -> List[RolloutResult]:
+    out = []
+    for slot in self.chunk_done:
+      req, _, admitted = self.active.pop(slot)
+      with torch.cuda.stream(self.stream):
+        data = self.family.read_slot(self.state, slot)
+        host = tuple(d.cpu() for d in data)  # synchronises the pool stream
+      out.append(RolloutResult(req.request_id, req.solver, req.n, req.steps, 'done',
+                               tuple(h.numpy() for h in host), req.arrival, admitted,
+                               now_fn()))
+      self.free.append(slot)
+    self.chunk_done = []
+    return out
+
+
+DEFAULT_CAPACITY = {64: 256, 128: 128, 256: 64, 512: 32}
+
+
+class RolloutServer:
+  """Continuous-batching server (see the module docstring)."""
+
+  def __init__(self, hbm_budget_bytes: float = 200e9,
+               capacity: Optional[Dict[int, int]] = None, max_chunk: int = 128,
+               device: Optional[torch.device] = None):
+    self.device = device if device is not None else torch.device(
+        'cuda', torch.cuda.current_device())
+    self.budget = float(hbm_budget_bytes)
+    self.capacity = dict(DEFAULT_CAPACITY if capacity is None else capacity)
+    self.max_chunk = int(max_chunk)
+    self.pools: Dict[Tuple[str, int], Pool] = {}
+    self.families: Dict[Tuple[str, int], object] = {}
+    self.reserved = 0
+    self.ic_stream = torch.cuda.Stream(device=self.device)
+
+  def _family(self, solver, n):
+    key = (solver, n)
+    if key not in self.families:
+      if solver not in FAMILIES:
+        raise ValueError(f'unknown solver {solver!r}')
+      self.families[key] = FAMILIES[solver](n, self.device)
+    return self.families[key]
+
+  def _pool(self, solver, n) -> Optional[Pool]:
+    """The pool for (solver, n), created if the budget allows; None if it
+    must wait.  Raises BudgetExceededError if it can never fit."""
+    key = (solver, n)
+    if key in self.pools:
+      return self.pools[key]
+    fam = self._family(solver, n)
+    cap = self.capacity.get(n, 16)
+    # shrink the pool until it fits the remaining budget (never below 1)
+    per_slot = fam.state_bytes(1) + self._scratch_per_slot(fam)
+    if per_slot > self.budget:
+      raise _lib.BudgetExceededError(
+          f'one {solver} {n}x{n} slot needs {per_slot} bytes > budget {self.budget:.0f}')
+    cap = int(max(1, min(cap, (self.budget - self.reserved) // per_slot)))
+    if self.reserved + cap * per_slot > self.budget:
+      return None
+    pool = Pool(fam, cap)
+    self.pools[key] = pool
+    self.reserved += pool.bytes
+    return pool
+
+  @staticmethod
+  def _scratch_per_slot(fam) -> int:
+    n = fam.n
+    if fam.name == 'fvm':
+      return (2 * n * n + n * n // 2 * 2) * 4  # u*, v*, packed spectrum
+    return (n * (n // 2 + 1) + 5 * n * (n // 2)) * 8  # h, 4 spectra, product
+
+  def _release_idle_pools(self, waiting_keys):
+    for key in list(self.pools):
+      pool = self.pools[key]
+      if not pool.active and not pool.busy and key not in waiting_keys:
+        self.reserved -= pool.bytes
+        pool.plan.close()
+        del self.pools[key]
+
+  def run(self, requests: List[RolloutRequest], realtime: bool = True) -> List[RolloutResult]:
+    """Serve `requests` (arrival times relative to now); returns results in
+    completion order.  realtime=False admits everything at t = 0."""
+    pending: Deque[RolloutRequest] = collections.deque(
+        sorted(requests, key=lambda r: (r.arrival, r.request_id)))
+    waiting: Deque[RolloutRequest] = collections.deque()
+    results: List[RolloutResult] = []
+    t0 = time.perf_counter()
+    now = lambda: time.perf_counter() - t0  # noqa: E731
+    while pending or waiting or any(p.active for p in self.pools.values()):
+      t = now()
+      while pending and (not realtime or pending[0].arrival <= t):
+        waiting.append(pending.popleft())
+      progressed = False
+      # 1) retire finished chunks, admit into free slots, relaunch
+      for pool in list(self.pools.values()):
+        if pool.busy:
+          continue
+        if pool.chunk_done:
+          results.extend(pool.harvest(now))
+          progressed = True
+      still: Deque[RolloutRequest] = collections.deque()
+      for req in waiting:
+        try:
+          pool = self._pool(req.solver, req.n)
+        except _lib.BudgetExceededError:
+          results.append(RolloutResult(req.request_id, req.solver, req.n, req.steps,
+                                       'rejected', None, req.arrival, now(), now()))
+          progressed = True
+          continue
+        if pool is None or pool.busy or not pool.free:
+          still.append(req)
+          continue
+        pool.admit(req, now(), self.ic_stream)
+        progressed = True
+      waiting = still
+      for pool in self.pools.values():
+        if not pool.busy and pool.active and not pool.chunk_done:
+          pool.launch(self.max_chunk)
+          progressed = True
+      if waiting and not progressed:
+        self._release_idle_pools({(r.solver, r.n) for r in waiting})
+      if not progressed:
+        if pending and realtime and not waiting and not any(
+            p.active for p in self.pools.values()):
+          time.sleep(max(0.0, min(pending[0].arrival - now(), 0.01)))
+        else:
+          time.sleep(50e-6)
+    torch.cuda.synchronize(self.device)
+    return results
+
+
+def poisson_requests(count: int, rate: float = 200.0, seed: int = 0,
+                     sizes=(64, 128, 256, 512), steps=(100, 2000),
+                     solvers=('fvm', 'spectral')) -> List[RolloutRequest]:
+  """Seeded Poisson arrivals with uniformly drawn solver, size and steps."""
+  rng = np.random.default_rng(seed)
+  gaps = rng.exponential(1.0 / rate, count)
+  arrivals = np.cumsum(gaps) - gaps[0]
+  out = []
+  for i in range(count):
+    out.append(RolloutRequest(
+        request_id=i, solver=str(rng.choice(solvers)), n=int(rng.choice(sizes)),
+        steps=int(rng.integers(steps[0], steps[1] + 1)),
+        seed=int(rng.integers(0, 2 ** 31 - 1)), arrival=float(arrivals[i])))
+  return out
+
+
+def summarize(results: List[RolloutResult], wall: float) -> dict:
+  done = [r for r in results if r.status == 'done']
+  lat = np.array([r.latency for r in done]) if done else np.zeros(1)
+  cell_steps = float(sum(r.n * r.n * r.steps for r in done))
+  return {'requests': len(results), 'done': len(done),
+          'rejected': sum(r.status == 'rejected' for r in results),
+          'wall_s': wall, 'cell_steps': cell_steps,
+          'cell_steps_per_s': cell_steps / wall if wall > 0 else 0.0,
+          'p50_latency_s': float(np.percentile(lat, 50)),
+          'p99_latency_s': float(np.percentile(lat, 99))}

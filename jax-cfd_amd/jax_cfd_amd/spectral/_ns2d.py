@@ -64,6 +64,10 @@ class Ns2dPlan:
   def device_bytes(self) -> int:
     return int(self._lib.jcfd_ns2d_plan_device_bytes(self._handle))
 
+  def set_graphs(self, enabled: bool):
+    _lib.check(self._lib.jcfd_ns2d_plan_set_graphs(self._handle, int(bool(enabled))),
+               'jcfd_ns2d_plan_set_graphs')
+
   def close(self):
     if getattr(self, '_handle', None) is not None and self._handle.value:
       self._lib.jcfd_ns2d_plan_destroy(self._handle)
@@ -91,6 +95,24 @@ class Ns2dPlan:
     _lib.check(rc, 'jcfd_ns2d_step')
     return o
 
+  KERNELS = ('ns_cols_inverse', 'ns_rows', 'ns_cols_update_inverse', 'ns_cols_update')
+  LAUNCHES_PER_STEP = {'ns_cols_inverse': 1, 'ns_rows': 5, 'ns_cols_update_inverse': 4,
+                       'ns_cols_update': 1}
+
+  def time_kernels(self, w, reps: int = 50):
+    w = self._field(w)
+    ms = (ctypes.c_double * 4)()
+    with _lib.dlpack_args() as dl:
+      rc = self._lib.jcfd_ns2d_time_kernels(self._handle, dl(w), int(reps), ms,
+                                            _lib.stream_handle(self.device))
+    _lib.check(rc, 'jcfd_ns2d_time_kernels')
+    return dict(zip(self.KERNELS, list(ms)))
+
+  def kernel_bytes(self):
+    b = (ctypes.c_int64 * 4)()
+    _lib.check(self._lib.jcfd_ns2d_kernel_bytes(self._handle, b), 'jcfd_ns2d_kernel_bytes')
+    return dict(zip(self.KERNELS, [int(x) for x in b]))
+
   @property
   def real_shape(self):
     n = self.spec.n
@@ -113,6 +135,13 @@ class Ns2dPlan:
                                       _lib.stream_handle(self.device))
     _lib.check(rc, 'jcfd_ns2d_irfft2')
     return o
+
+  def step_slots(self, w: torch.Tensor, slots: torch.Tensor, nsteps: int):
+    """Advance only `slots` (int32 device tensor) of the (batch, n, n/2+1) w in place."""
+    with _lib.dlpack_args() as dl:
+      rc = self._lib.jcfd_ns2d_step_slots(self._handle, dl(w), dl(slots), int(nsteps),
+                                          _lib.stream_handle(self.device))
+    _lib.check(rc, 'jcfd_ns2d_step_slots')
 
   def explicit_terms(self, w):
     w = self._field(w)

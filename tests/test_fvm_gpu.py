@@ -181,6 +181,23 @@ def test_repeated_equals_single_steps_and_trajectory():
   assert torch.equal(frames0[0].data[0], s[0].data)
 
 
+def test_graph_replay_matches_individual_launches():
+  """Steps replayed from the captured hipGraph equal eagerly launched steps,
+  across replays separated by other eager work."""
+  n = 64
+  ogrid, v0, dt, ostep, step = _config(*C0)
+  s = _state(v0, n)
+  plan = step.plan()
+  plan.set_graphs(False)
+  ref_u, ref_v = plan.step(s[0].data, s[1].data, 70)
+  plan.set_graphs(True)
+  for _ in range(3):
+    u, v = plan.step(s[0].data, s[1].data, 70)
+    torch.cuda.synchronize()
+    assert torch.equal(u, ref_u) and torch.equal(v, ref_v)
+    plan.projection(u, v)  # unrelated eager work between replays
+
+
 def test_output_may_alias_input():
   n = 64
   ogrid, v0, dt, ostep, step = _config(*C0)

@@ -1,0 +1,91 @@
+"""Continuous-batching server: results identical alone and in a mix, equal
+to the plain one-instance path, admission bounded by the memory budget."""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+
+def _srv(**kw):
+  from jax_cfd_amd import serving
+  return serving.RolloutServer(**kw)
+
+
+def _req(i, solver, n, steps, seed, arrival=0.0):
+  from jax_cfd_amd import serving
+  return serving.RolloutRequest(i, solver, n, steps, seed, arrival)
+
+
+def _by_id(results):
+  return {r.request_id: r for r in results}
+
+
+def test_alone_equals_mixed():
+  target = [_req(0, 'fvm', 64, 150, 11), _req(1, 'spectral', 64, 120, 12)]
+  alone = _by_id(_srv().run(target, realtime=False))
+  mix = target + [_req(2, 'fvm', 64, 90, 3), _req(3, 'spectral', 64, 300, 4),
+                  _req(4, 'fvm', 128, 200, 5), _req(5, 'spectral', 128, 77, 6),
+                  _req(6, 'fvm', 64, 333, 7)]
+  mixed = _by_id(_srv(capacity={64: 3, 128: 2}).run(mix, realtime=False))
+  for i in (0, 1):
+    assert alone[i].status == mixed[i].status == 'done'
+    for a, b in zip(alone[i].state, mixed[i].state):
+      np.testing.assert_array_equal(a, b)
+  assert all(r.status == 'done' for r in mixed.values())
+
+
+def test_server_matches_plain_path():
+  import jax_cfd_amd.base as cfd
+  from jax_cfd_amd import serving
+  dev = torch.device('cuda', 0)
+  res = _by_id(_srv().run([_req(0, 'fvm', 64, 140, 21), _req(1, 'spectral', 64, 140, 22)],
+                          realtime=False))
+  fam = serving.FvmFamily(64, dev)
+  u, v = fam.initial_state(21)
+  plan = cfd._fvm.FvmPlan(fam.spec, dev)
+  uo, vo = plan.step(u, v, 140)
+  np.testing.assert_array_equal(res[0].state[0], uo.cpu().numpy())
+  np.testing.assert_array_equal(res[0].state[1], vo.cpu().numpy())
+  sfam = serving.SpectralFamily(64, dev)
+  (w,) = sfam.initial_state(22)
+  from jax_cfd_amd.spectral import time_stepping
+  step = time_stepping.crank_nicolson_rk4(sfam.equation, sfam.DT)
+  np.testing.assert_array_equal(res[1].state[0], step.multistep(w, 140).cpu().numpy())
+
+
+def test_capacity_limits_concurrency_and_all_complete():
+  reqs = [_req(i, 'fvm', 64, 50 + 17 * i, i) for i in range(7)]
+  srv = _srv(capacity={64: 2})
+  res = srv.run(reqs, realtime=False)
+  assert sorted(r.request_id for r in res) == list(range(7))
+  assert all(r.status == 'done' for r in res)
+  assert srv.pools[('fvm', 64)].capacity == 2
+  # with two slots, at most two requests overlap in time
+  spans = sorted((r.admitted, r.finished) for r in res)
+  for t in [s for s, _ in spans]:
+    assert sum(1 for a, b in spans if a <= t < b) <= 2
+
+
+def test_budget_rejects_and_serializes_pools():
+  from jax_cfd_amd import serving
+  one_64 = serving.FvmFamily(64, torch.device('cuda', 0))
+  slot_bytes = one_64.state_bytes(1) + serving.RolloutServer._scratch_per_slot(one_64)
+  # room for one 64^2 FVM slot (+ plan constants), never for a 512^2 one
+  srv = _srv(hbm_budget_bytes=4 * slot_bytes, capacity={64: 1, 128: 1})
+  res = _by_id(srv.run([_req(0, 'fvm', 64, 40, 1), _req(1, 'fvm', 512, 10, 2),
+                        _req(2, 'spectral', 64, 30, 3)], realtime=False))
+  assert res[1].status == 'rejected'
+  assert res[0].status == 'done' and res[2].status == 'done'
+
+
+def test_poisson_stream_realtime_small():
+  from jax_cfd_amd import serving
+  reqs = serving.poisson_requests(12, rate=200.0, seed=5, sizes=(64, 128),
+                                  steps=(20, 60))
+  import time
+  t = time.perf_counter()
+  res = _srv().run(reqs, realtime=True)
+  summ = serving.summarize(res, time.perf_counter() - t)
+  assert summ['done'] == 12 and summ['p99_latency_s'] >= summ['p50_latency_s'] > 0
+  assert all(r.finished >= r.arrival for r in res)

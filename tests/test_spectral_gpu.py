@@ -127,6 +127,21 @@ def test_batch_equals_single_instances():
     assert torch.equal(batched[i], single)
 
 
+def test_graph_replay_matches_individual_launches():
+  n, b = 64, 2
+  ws = torch.as_tensor(np.stack([_w0(n, s) for s in range(b)]), device=_dev())
+  _, _, eq = _eqs(n)
+  plan = eq.plan(1e-3, b, _dev())
+  plan.set_graphs(False)
+  ref = plan.step(ws, 70)
+  plan.set_graphs(True)
+  for _ in range(3):
+    out = plan.step(ws, 70)
+    torch.cuda.synchronize()
+    assert torch.equal(out, ref)
+    plan.explicit_terms(out)  # unrelated eager work between replays
+
+
 def test_taylor_green_decay():
   """spectral/equations_test.py:263-303 pattern: exp(-2 nu t) decay."""
   n, nu = 64, 1e-2
