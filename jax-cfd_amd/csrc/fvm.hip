@@ -62,6 +62,25 @@ __device__ __forceinline__ float vl_flux(float U, float c_pos, float c_neg, floa
   return up + vl_correction(U, cm1 - cm2, cp1 - c0, c0 - cm1, dt_over_h);
 }
 
+// Flux through the face between c0 = c[k] and cp1 = c[k+1] for
+// advect_van_leer_using_limiters (advection.py:387-395): advect_general with
+// linear velocity interpolation (advection.py:34-116) and
+// apply_tvd_limiter(lax_wendroff, van_leer_limiter) (interpolation.py:96-303).
+__device__ __forceinline__ float vll_flux(float cm1, float c0, float cp1, float cp2, float U,
+                                          float dt_over_h) {
+  const bool pos = U > 0.f;
+  const float low = pos ? c0 : cp1;                       // upwind
+  const float cour = dt_over_h * U;                       // Lax-Wendroff
+  const float d = cp1 - c0;
+  const float high = pos ? c0 + (0.5f * (1.f - cour)) * d : cp1 - (0.5f * (1.f + cour)) * d;
+  const float den = (d != 0.f) ? d : 1.0f;                // safe_div
+  const float r = pos ? (c0 - cm1) / den : (cp2 - cp1) / den;
+  const float onep = 1.f + r;
+  const float phi = (r > 0.f) ? (2.f * r) / ((onep != 0.f) ? onep : 1.0f) : 0.f;  // van_leer_limiter
+  const float x = low - (low - high) * phi;
+  return x * U;
+}
+
 // Rows of the LDS staging window: local row lr holds global row i0 - 3 + lr.
 template <int N>
 struct Window {
@@ -77,10 +96,48 @@ struct Window {
 // loop variable at advection.py:291 makes it), equations.py:102-110 for the
 // sum, finite_differences.py:127-133 for the Laplacian.
 template <int N>
+__device__ __forceinline__ void add_diffusion_forcing(const Window<N>& w, int lr, int j,
+                                                      const FvmParams& p, float fx_j, float& ku,
+                                                      float& kv);
+
+template <int N>
+__device__ __forceinline__ void vll_advection_at(const Window<N>& w, int lr, int j,
+                                                 const FvmParams& p, float& adv_u, float& adv_v) {
+  const float hx = p.dt_over_dx, hy = p.dt_over_dy;
+  // u (offset (1, .5)): x faces between rows a, a+1 with U = .5 u[a] + .5 u[a+1];
+  // y faces between columns b, b+1 with V = .5 v[i, b] + .5 v[i+1, b].
+  auto fx_u = [&](int a) {
+    const float Uf = 0.5f * w.U(a, j) + 0.5f * w.U(a + 1, j);
+    return vll_flux(w.U(a - 1, j), w.U(a, j), w.U(a + 1, j), w.U(a + 2, j), Uf, hx);
+  };
+  auto fy_u = [&](int b) {
+    const float Vf = 0.5f * w.V(lr, b) + 0.5f * w.V(lr + 1, b);
+    return vll_flux(w.U(lr, b - 1), w.U(lr, b), w.U(lr, b + 1), w.U(lr, b + 2), Vf, hy);
+  };
+  adv_u = -((fx_u(lr) - fx_u(lr - 1)) / p.dx + (fy_u(j) - fy_u(j - 1)) / p.dy);
+  // v (offset (.5, 1)): x faces with U = .5 u[a, j] + .5 u[a, j+1]; y faces
+  // with V = .5 v[i, b] + .5 v[i, b+1].
+  auto fx_v = [&](int a) {
+    const float Uf = 0.5f * w.U(a, j) + 0.5f * w.U(a, j + 1);
+    return vll_flux(w.V(a - 1, j), w.V(a, j), w.V(a + 1, j), w.V(a + 2, j), Uf, hx);
+  };
+  auto fy_v = [&](int b) {
+    const float Vf = 0.5f * w.V(lr, b) + 0.5f * w.V(lr, b + 1);
+    return vll_flux(w.V(lr, b - 1), w.V(lr, b), w.V(lr, b + 1), w.V(lr, b + 2), Vf, hy);
+  };
+  adv_v = -((fx_v(lr) - fx_v(lr - 1)) / p.dx + (fy_v(j) - fy_v(j - 1)) / p.dy);
+}
+
+template <int N>
 __device__ __forceinline__ void explicit_terms_at(const Window<N>& w, int lr, int j,
                                                   const FvmParams& p, float fx_j, float& ku,
                                                   float& kv) {
   const float hx = p.dt_over_dx, hy = p.dt_over_dy;
+  if (p.convect == JCFD_CONVECT_VAN_LEER_LIMITERS) {
+    vll_advection_at<N>(w, lr, j, p, ku, kv);
+    add_diffusion_forcing<N>(w, lr, j, p, fx_j, ku, kv);
+    return;
+  }
   // ---- u component (offset (1, .5)) ----
   // x faces at a = i and a = i + 1 (local rows lr, lr + 1)
   auto fx_u = [&](int a) {
@@ -117,6 +174,14 @@ __device__ __forceinline__ void explicit_terms_at(const Window<N>& w, int lr, in
 
   ku = adv_u;
   kv = adv_v;
+  add_diffusion_forcing<N>(w, lr, j, p, fx_j, ku, kv);
+}
+
+// k += nu/rho lap + forcing/rho, in the reference's order (equations.py:102-110).
+template <int N>
+__device__ __forceinline__ void add_diffusion_forcing(const Window<N>& w, int lr, int j,
+                                                      const FvmParams& p, float fx_j, float& ku,
+                                                      float& kv) {
   if (p.has_diffusion) {
     const float uc = w.U(lr, j), vc = w.V(lr, j);
     const float lap_u = ((-2.f * uc) * p.lap_sum + (w.U(lr - 1, j) + w.U(lr + 1, j)) * p.inv_dx_sq) +

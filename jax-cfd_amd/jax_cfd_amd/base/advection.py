@@ -6,7 +6,7 @@ runs the explicit-terms kernel with diffusion and forcing switched off.
 """
 from __future__ import annotations
 
-from jax_cfd_amd import _symbolic
+from jax_cfd_amd import _lib, _symbolic
 from jax_cfd_amd.base import _fvm, boundaries, grids
 
 
@@ -28,13 +28,7 @@ def _probe_term(kind, c, v, dt):
   return grids.GridArray(term, c.offset, c.grid)
 
 
-def advect_van_leer(c: grids.GridVariable, v: grids.GridVariableVector, dt: float,
-                    mode: str = boundaries.Padding.MIRROR) -> grids.GridArray:
-  """Van Leer flux-limited advection of velocity component `c` by `v`
-  (advection.py:210-342, including its limiter-stencil indexing)."""
-  del mode  # periodic only
-  if _symbolic.is_probe(c):
-    return _probe_term('advect_van_leer', c, v, dt)
+def _eager(c, v, dt, convect):
   idx = _component_index(c, v)
   if idx is None or len(v) != 2:
     raise NotImplementedError(
@@ -43,20 +37,29 @@ def advect_van_leer(c: grids.GridVariable, v: grids.GridVariableVector, dt: floa
   grid = grids.consistent_grid(*[u.array for u in v])
   if not boundaries.has_all_periodic_boundary_conditions(*v):
     raise NotImplementedError('non-periodic boundary conditions')
-  spec = _fvm.spec_from_grid(grid, dt=float(dt))
+  spec = _fvm.spec_from_grid(grid, dt=float(dt), convect=convect)
   plan = _fvm.plan_for(spec, v[0].data.device)
   du, dv = plan.explicit_terms(v[0].data, v[1].data)
   return grids.GridArray((du, dv)[idx], c.offset, grid)
 
 
-def advect_van_leer_using_limiters(c, v, dt):
+def advect_van_leer(c: grids.GridVariable, v: grids.GridVariableVector, dt: float,
+                    mode: str = boundaries.Padding.MIRROR) -> grids.GridArray:
+  """Van Leer flux-limited advection of velocity component `c` by `v`
+  (advection.py:210-342, including its limiter-stencil indexing)."""
+  del mode  # periodic only
+  if _symbolic.is_probe(c):
+    return _probe_term('advect_van_leer', c, v, dt)
+  return _eager(c, v, dt, _lib.CONVECT_VAN_LEER)
+
+
+def advect_van_leer_using_limiters(c: grids.GridVariable, v: grids.GridVariableVector,
+                                   dt: float) -> grids.GridArray:
   """Van Leer via TVD-limited Lax-Wendroff (advection.py:387-395): the
-  default `convect` of semi_implicit_navier_stokes.  Recognised by the step
-  factory, which reports that this kernel variant is not built."""
+  default `convect` of semi_implicit_navier_stokes."""
   if _symbolic.is_probe(c):
     return _probe_term('advect_van_leer_using_limiters', c, v, dt)
-  raise NotImplementedError(
-      'advect_van_leer_using_limiters has no GPU kernel yet; use advect_van_leer')
+  return _eager(c, v, dt, _lib.CONVECT_VAN_LEER_LIMITERS)
 
 
 def stable_time_step(max_velocity: float, max_courant_number: float,

@@ -63,16 +63,13 @@ def _read_convection(convect, grid, dt):
     dts.append(t.info['dt'])
   if len(out) != 2 or len(set(kinds)) != 1 or len(set(dts)) != 1:
     raise NotImplementedError('convect must treat both components alike')
-  if kinds[0] == 'advect_van_leer_using_limiters':
-    raise NotImplementedError(
-        'advect_van_leer_using_limiters (the factory default) has no fused '
-        'kernel yet; pass convect=lambda v: tuple(advect_van_leer(u, v, dt) '
-        'for u in v)')
-  if kinds[0] != 'advect_van_leer':
+  kinds_built = {'advect_van_leer': _lib.CONVECT_VAN_LEER,
+                 'advect_van_leer_using_limiters': _lib.CONVECT_VAN_LEER_LIMITERS}
+  if kinds[0] not in kinds_built:
     raise NotImplementedError(f'convection {kinds[0]} is not built')
   if dts[0] != dt:
     raise NotImplementedError('advection dt differing from the step dt')
-  return _lib.CONVECT_VAN_LEER
+  return kinds_built[kinds[0]]
 
 
 def _read_forcing(forcing, grid):

@@ -54,7 +54,7 @@ def _cfg(**kw):
     (dict(n=2048), -2, "grid size"),
     (dict(dx=0.0), -1, "grid steps"),
     (dict(density=0.0), -1, "density"),
-    (dict(convect=1), -2, "advect_van_leer"),
+    (dict(convect=7), -2, "advect_van_leer"),
     (dict(pressure=1), -2, "solve_fast_diag"),
 ])
 def test_plan_create_rejects_before_device(kw, code, msg):

@@ -58,8 +58,18 @@ def test_kolmogorov_only_and_linear_only():
   assert not any(s2.forcing_x)
 
 
+def test_default_convection_is_van_leer_using_limiters():
+  """equations.py:89-92: convect=None means advect_van_leer_using_limiters."""
+  grid = _grid(64)
+  s = cfd.equations.semi_implicit_navier_stokes(1.0, 1e-3, 0.02, grid).spec
+  assert s.convect == _lib.CONVECT_VAN_LEER_LIMITERS
+  s2 = cfd.equations.semi_implicit_navier_stokes(
+      1.0, 1e-3, 0.02, grid, convect=lambda v: tuple(
+          cfd.advection.advect_van_leer_using_limiters(u, v, 0.02) for u in v)).spec
+  assert s2 == s
+
+
 @pytest.mark.parametrize("kw", [
-    dict(convect=None),  # reference default: advect_van_leer_using_limiters
     dict(convect=lambda v: tuple(cfd.advection.advect_van_leer(v[0], v, 0.02) for _ in v)),
     dict(convect=lambda v: tuple(cfd.advection.advect_van_leer(u, v, 0.01) for u in v)),
     dict(convect=_vl(0.02), forcing=cfd.forcings.kolmogorov_forcing(_grid(), swap_xy=True)),

@@ -108,6 +108,36 @@ def test_explicit_terms_parity(cfg):
     assert err <= 1e-5
 
 
+@pytest.mark.parametrize("n,forced", [(64, False), (256, True), (512, True)])
+def test_van_leer_using_limiters_parity(n, forced):
+  """The factory default convection (advection.py:387-395): explicit terms,
+  one step and 100 steps against the oracle's advect_general restatement."""
+  cfd = _cfd()
+  ogrid = _ogrid(n)
+  v0 = oic.filtered_velocity_field(n, ogrid, 2.0, 4)
+  dt = ofvm.stable_time_step(2.0, 0.5, 1e-3, ogrid)
+  forcing = go.Forcing(1.0, 4, -0.1) if forced else None
+  ostep = ofvm.FVMStep(ogrid, dt=dt, viscosity=1e-3, forcing=forcing,
+                       convect='van_leer_limiters')
+  f = (cfd.forcings.simple_turbulence_forcing(_grid(n), 1.0, 4, -0.1, 'kolmogorov')
+       if forced else None)
+  et = cfd.equations.navier_stokes_explicit_terms(1.0, 1e-3, dt, _grid(n), forcing=f)
+  for o, r in zip(et(_state(v0, n)), ostep.explicit_terms(v0)):
+    err = _maxrel(o.data.cpu().numpy(), r.data)
+    print(f"VLL explicit terms n={n}: max rel err {err:.2e}")
+    assert err <= 1e-5
+  step = cfd.equations.semi_implicit_navier_stokes(1.0, 1e-3, dt, _grid(n), forcing=f)
+  for o, r in zip(step(_state(v0, n)), ostep(v0)):
+    assert _maxrel(o.data.cpu().numpy(), r.data) <= 1e-5
+  if n <= 256:
+    ref = ostep.rollout(v0, 100)
+    out = _cfd().funcutils.repeated(step, 100)(_state(v0, n))
+    for o, r in zip(out, ref):
+      err = _rel_l2(o.data.cpu().numpy(), r.data)
+      print(f"VLL 100 steps n={n}: rel L2 {err:.2e}")
+      assert err <= 1e-3
+
+
 @pytest.mark.parametrize("n", [32, 64, 256, 512, 1024])
 def test_projection_parity_and_divergence(n):
   ogrid = _ogrid(n)
