@@ -44,6 +44,8 @@ def _args():
                   help="steps of the 64 x 512^2 spectral batch (0 = skip)")
   ap.add_argument("--served-requests", type=int, default=400,
                   help="requests of the served Poisson stream (0 = skip)")
+  ap.add_argument("--cg-steps", type=int, default=100,
+                  help="steps of the solve_cg variant of the 256^2 run (0 = skip)")
   return ap.parse_args()
 
 
@@ -112,6 +114,35 @@ def _batched_spectral(dev, batch, n, steps, reps):
                    "peak": HBM_PEAK_GBS, "unit": "GB/s",
                    "frac": round(ach / HBM_PEAK_GBS, 4)},
   }
+
+
+def _cg_variant(dev, steps):
+  """The forced 256^2 FVM run with solve_cg (rtol = atol = 1e-6) in place of
+  the fast-diagonal solve."""
+  import functools
+  import numpy as np
+  import torch
+  import jax_cfd_amd.base as cfd
+  grid = cfd.grids.Grid((N, N), domain=((0, 2 * np.pi), (0, 2 * np.pi)))
+  dt = cfd.equations.stable_time_step(VMAX, CFL, NU, grid)
+  step = cfd.equations.semi_implicit_navier_stokes(
+      density=1.0, viscosity=NU, dt=dt, grid=grid,
+      convect=lambda v: tuple(cfd.advection.advect_van_leer(u, v, dt) for u in v),
+      forcing=cfd.forcings.simple_turbulence_forcing(grid, 1.0, 4, -0.1, "kolmogorov"),
+      pressure_solve=functools.partial(cfd.pressure.solve_cg, rtol=1e-6, atol=1e-6))
+  v0 = cfd.initial_conditions.filtered_velocity_field(SEED, grid, VMAX, PEAK)
+  plan = step.plan(dev)
+  u, v = v0[0].data.clone(), v0[1].data.clone()
+  plan.step(u, v, 2, out=(u, v))
+  torch.cuda.synchronize()
+  t0 = time.perf_counter()
+  plan.step(u, v, steps, out=(u, v))
+  torch.cuda.synchronize()
+  wall = time.perf_counter() - t0
+  return {"workload": f"fvm {N}x{N} forced, solve_cg rtol=atol=1e-6", "steps": steps,
+          "value": N * N * steps / wall, "unit": "cell-steps/s",
+          "ms_per_step": wall / steps * 1e3,
+          "cg_iterations_last_step": plan.cg_iterations}
 
 
 def _plain_step_seconds(dev, solver, n, steps=40):
@@ -238,6 +269,9 @@ def main():
                  "GBps": round(kbytes[k] / (kms[k] * 1e-3) / 1e9, 2)} for k in kms}
   print(f"[bench] kernels {json.dumps(kernels)}", file=sys.stderr, flush=True)
 
+  cg = _cg_variant(dev, args.cg_steps) if args.cg_steps > 0 else None
+  if cg:
+    print(f"[bench] cg variant {json.dumps(cg)}", file=sys.stderr, flush=True)
   batched = None
   if args.batched_steps > 0:
     batched = _batched_spectral(dev, 64, 512, args.batched_steps, 20)
@@ -258,7 +292,7 @@ def main():
                  "n": N, "dt": dt, "viscosity": NU, "seed": SEED, "v_max": VMAX,
                  "path": "plain (one instance, 3 kernels per step)"},
       "roofline": roofline, "cpu_baseline": cpu, "kernels": kernels,
-      "batched": batched, "served": served,
+      "batched": batched, "served": served, "cg_variant": cg,
   }
   print(json.dumps(out), flush=True)
 

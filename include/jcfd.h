@@ -134,7 +134,14 @@ int jcfd_fvm_step_slots(jcfd_fvm_plan* plan, jcfd_tensor* u, jcfd_tensor* v,
 int jcfd_fvm_explicit_terms(jcfd_fvm_plan* plan, const jcfd_tensor* u, const jcfd_tensor* v,
                             jcfd_tensor* du, jcfd_tensor* dv, void* stream);
 
-/* pressure.projection(v, solve_fast_diag) (pressure.py:181-198). */
+/* Iterations of the plan's last CG solve (pressure == JCFD_PRESSURE_CG).
+ * Diagnostic with no counterpart in the reference (jax's cg does not report
+ * them).  The CG plan synchronises its stream between chunks of 64
+ * iterations to read its convergence flag. */
+int32_t jcfd_fvm_cg_iterations(const jcfd_fvm_plan* plan);
+
+/* pressure.projection(v, solve) (pressure.py:181-198) with the plan's solve
+ * (solve_fast_diag or solve_cg). */
 int jcfd_fvm_projection(jcfd_fvm_plan* plan, const jcfd_tensor* u, const jcfd_tensor* v,
                         jcfd_tensor* u_out, jcfd_tensor* v_out, void* stream);
 

@@ -223,6 +223,12 @@ struct jcfd_fvm_plan {
   FvmParams params;
   int device = 0;
   int batch = 1;
+  // CG pressure solve (pressure == JCFD_PRESSURE_CG)
+  CgBuffers cg{};
+  float* rhs = nullptr;
+  int* status_host = nullptr;  // pinned
+  int cg_maxiter = 0;
+  int cg_last_iterations = 0;
   float* us = nullptr;
   float* vs = nullptr;
   float2* spec = nullptr;
@@ -288,10 +294,70 @@ FvmBuffers fvm_buffers(const jcfd_fvm_plan* p) {
   return b;
 }
 
+int cg_alloc(jcfd_fvm_plan* p) {
+  const size_t nn = static_cast<size_t>(p->params.n) * p->params.n;
+  const size_t nb = static_cast<size_t>(cg_blocks(p->params.n));
+  CgBuffers& c = p->cg;
+  int rc;
+  if ((rc = device_alloc(&c.x, nn, &p->bytes)) || (rc = device_alloc(&c.r, nn, &p->bytes)) ||
+      (rc = device_alloc(&c.p[0], nn, &p->bytes)) || (rc = device_alloc(&c.p[1], nn, &p->bytes)) ||
+      (rc = device_alloc(&c.ap, nn, &p->bytes)) || (rc = device_alloc(&p->rhs, nn, &p->bytes)) ||
+      (rc = device_alloc(&c.part_rr[0], nb, &p->bytes)) ||
+      (rc = device_alloc(&c.part_rr[1], nb, &p->bytes)) ||
+      (rc = device_alloc(&c.part_pap, nb, &p->bytes)) || (rc = device_alloc(&c.part_bb, nb, &p->bytes)) ||
+      (rc = device_alloc(&c.status, 2, &p->bytes)))
+    return rc;
+  if ((rc = check_hip(hipHostMalloc(reinterpret_cast<void**>(&p->status_host), 2 * sizeof(int),
+                                    hipHostMallocDefault),
+                      "pinned status")))
+    return rc;
+  p->cg_maxiter = p->cfg.cg_maxiter > 0 ? p->cfg.cg_maxiter : static_cast<int>(10 * nn);
+  return JCFD_OK;
+}
+
+constexpr int kCgChunk = 64;  // iterations enqueued between convergence checks
+
+// CG on p->rhs into p->cg.x (solve_cg); reads the convergence flag between
+// chunks of iterations, so this synchronises the stream.
+int cg_solve(jcfd_fvm_plan* p, hipStream_t s) {
+  const float rtol = static_cast<float>(p->cfg.cg_rtol), atol = static_cast<float>(p->cfg.cg_atol);
+  int rc = cg_launch_init(p->params, p->rhs, p->cg, rtol, atol, p->cg_maxiter, s);
+  for (int k = 0; rc == JCFD_OK; k += kCgChunk) {
+    for (int i = 0; i < kCgChunk && rc == JCFD_OK; ++i)
+      rc = cg_launch_iteration(p->params, p->cg, k + i, rtol, atol, p->cg_maxiter, s);
+    if (rc) break;
+    if ((rc = check_hip(hipMemcpyAsync(p->status_host, p->cg.status, 2 * sizeof(int),
+                                       hipMemcpyDeviceToHost, s), "cg status")) ||
+        (rc = check_hip(hipStreamSynchronize(s), "cg status sync")))
+      break;
+    if (p->status_host[0]) {
+      p->cg_last_iterations = p->status_host[1];
+      break;
+    }
+    if (k + kCgChunk > p->cg_maxiter) {
+      p->cg_last_iterations = p->cg_maxiter;
+      break;
+    }
+  }
+  return rc;
+}
+
 // nsteps steps: the first reads b.u/b.v, the rest run in place on
 // b.u_out/b.v_out (replayed from a graph when enabled).
 int fvm_run(jcfd_fvm_plan* p, FvmBuffers b, int64_t nsteps, hipStream_t s) {
   int rc;
+  if (p->cfg.pressure == JCFD_PRESSURE_CG) {
+    b.rhs = p->rhs;
+    b.q = p->cg.x;
+    for (int64_t step = 0; step < nsteps; ++step) {
+      if ((rc = fvm_launch_forward_rhs(p->params, b, s)) || (rc = cg_solve(p, s)) ||
+          (rc = fvm_launch_project_q(p->params, b, s)))
+        return rc;
+      b.u = b.u_out;
+      b.v = b.v_out;
+    }
+    return JCFD_OK;
+  }
   if ((rc = fvm_launch_step(p->params, b, s))) return rc;
   b.u = b.u_out;
   b.v = b.v_out;
@@ -335,8 +401,12 @@ int jcfd_fvm_plan_create(const jcfd_fvm_config* config, jcfd_fvm_plan** plan) {
   if (c.convect != JCFD_CONVECT_VAN_LEER && c.convect != JCFD_CONVECT_VAN_LEER_LIMITERS)
     return fail(JCFD_ERR_UNSUPPORTED,
                 "convection must be advect_van_leer or advect_van_leer_using_limiters");
-  if (c.pressure != JCFD_PRESSURE_FAST_DIAG)
-    return fail(JCFD_ERR_UNSUPPORTED, "only the solve_fast_diag pressure solve is built");
+  if (c.pressure != JCFD_PRESSURE_FAST_DIAG && c.pressure != JCFD_PRESSURE_CG)
+    return fail(JCFD_ERR_UNSUPPORTED, "pressure solve must be solve_fast_diag or solve_cg");
+  if (c.pressure == JCFD_PRESSURE_CG && c.batch > 1)
+    return fail(JCFD_ERR_UNSUPPORTED, "the CG pressure solve is built for one instance per plan");
+  if (c.pressure == JCFD_PRESSURE_CG && (!std::isfinite(c.cg_rtol) || !std::isfinite(c.cg_atol)))
+    return fail(JCFD_ERR_INVALID_ARGUMENT, "CG tolerances must be finite");
 
   auto* p = new (std::nothrow) jcfd_fvm_plan();
   if (p == nullptr) return fail(JCFD_ERR_OUT_OF_MEMORY, "host allocation failed");
@@ -375,7 +445,8 @@ int jcfd_fvm_plan_create(const jcfd_fvm_config* config, jcfd_fvm_plan** plan) {
       (rc = device_alloc(&p->spec, nn / 2, &p->bytes)) ||
       (rc = device_alloc(&p->lam, static_cast<size_t>(n) * (n / 2 + 1), &p->bytes)) ||
       (rc = device_alloc(&p->tw, static_cast<size_t>(n), &p->bytes)) ||
-      (rc = device_alloc(&p->fx, static_cast<size_t>(n), &p->bytes))) {
+      (rc = device_alloc(&p->fx, static_cast<size_t>(n), &p->bytes)) ||
+      (c.pressure == JCFD_PRESSURE_CG && (rc = cg_alloc(p)))) {
     jcfd_fvm_plan_destroy(p);
     return rc;
   }
@@ -401,10 +472,20 @@ int jcfd_fvm_plan_destroy(jcfd_fvm_plan* p) {
   for (void* ptr : {static_cast<void*>(p->us), static_cast<void*>(p->vs), static_cast<void*>(p->spec),
                     static_cast<void*>(p->lam), static_cast<void*>(p->tw), static_cast<void*>(p->fx)})
     if (ptr != nullptr) (void)hipFree(ptr);
+  for (void* ptr : {static_cast<void*>(p->cg.x), static_cast<void*>(p->cg.r),
+                    static_cast<void*>(p->cg.p[0]), static_cast<void*>(p->cg.p[1]),
+                    static_cast<void*>(p->cg.ap), static_cast<void*>(p->cg.part_rr[0]),
+                    static_cast<void*>(p->cg.part_rr[1]), static_cast<void*>(p->cg.part_pap),
+                    static_cast<void*>(p->cg.part_bb), static_cast<void*>(p->cg.status),
+                    static_cast<void*>(p->rhs)})
+    if (ptr != nullptr) (void)hipFree(ptr);
+  if (p->status_host) (void)hipHostFree(p->status_host);
   p->graphs.release();
   delete p;
   return JCFD_OK;
 }
+
+int32_t jcfd_fvm_cg_iterations(const jcfd_fvm_plan* p) { return p ? p->cg_last_iterations : 0; }
 
 int jcfd_fvm_plan_set_graphs(jcfd_fvm_plan* p, int32_t enabled) {
   if (p == nullptr) return fail(JCFD_ERR_INVALID_ARGUMENT, "null plan");
@@ -496,9 +577,23 @@ int jcfd_fvm_projection(jcfd_fvm_plan* p, const jcfd_tensor* u, const jcfd_tenso
       (rc = check_tensor(u_out, "u_out", kDLFloat, 32, n, n, p->device, &pu2)) ||
       (rc = check_tensor(v_out, "v_out", kDLFloat, 32, n, n, p->device, &pv2)))
     return rc;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (p->cfg.pressure == JCFD_PRESSURE_CG) {
+    FvmBuffers b = fvm_buffers(p);
+    b.us = static_cast<float*>(pu);
+    b.vs = static_cast<float*>(pv);
+    b.u_out = static_cast<float*>(pu2);
+    b.v_out = static_cast<float*>(pv2);
+    b.q = p->cg.x;
+    if ((rc = fvm_launch_divergence(p->params, static_cast<const float*>(pu),
+                                    static_cast<const float*>(pv), p->rhs, s)) ||
+        (rc = cg_solve(p, s)))
+      return rc;
+    return fvm_launch_project_q(p->params, b, s);
+  }
   return fvm_launch_projection(p->params, static_cast<const float*>(pu), static_cast<const float*>(pv),
                                static_cast<float*>(pu2), static_cast<float*>(pv2), p->spec, p->lam,
-                               p->tw, static_cast<hipStream_t>(stream));
+                               p->tw, s);
 }
 
 int jcfd_fvm_divergence(jcfd_fvm_plan* p, const jcfd_tensor* u, const jcfd_tensor* v,

@@ -248,7 +248,7 @@ __global__ __launch_bounds__(kThreads) void fvm_rows_forward(FvmParams p, FvmBuf
   }
   __syncthreads();
 
-  if constexpr (MODE == 0 || MODE == 2) {
+  if constexpr (MODE == 0 || MODE == 2 || MODE == 4) {
     Window<N> w{su, sv};
     // rows i0-1 .. i0+RB-1 -> local rows 2 .. RB+2 of the window
     const int first = (MODE == 2) ? 1 : 0;
@@ -285,7 +285,9 @@ __global__ __launch_bounds__(kThreads) void fvm_rows_forward(FvmParams p, FvmBuf
       const float ddx = (sus[(r + 1) * N + j] - sus[r * N + j]) / p.dx;
       const float ddy = (svs[r * N + j] - svs[r * N + ((j - 1) & (N - 1))]) / p.dy;
       zf[e] = ddx + ddy;
+      if constexpr (MODE == 4) b.rhs[(i0 + r) * N + j] = ddx + ddy;
     }
+    if constexpr (MODE == 4) return;
     __syncthreads();
   }
   lds_fft<H, RB, kThreads, 1, H, false, false>(zb, twh, tid);
@@ -349,7 +351,8 @@ __global__ __launch_bounds__(kThreads) void fvm_poisson_cols(FvmBuffers b, float
 }
 
 // Inverse row transforms of q and the velocity correction.
-// MODE 0: velocity correction; MODE 1: write q itself to u_out.
+// MODE 0: velocity correction; MODE 1: write q itself to u_out;
+// MODE 2: velocity correction from the real q of the CG solve (no FFT).
 template <int N, int RB, int MODE>
 __global__ __launch_bounds__(kThreads) void fvm_rows_project(FvmParams p, FvmBuffers b) {
   constexpr int H = N / 2;
@@ -360,18 +363,24 @@ __global__ __launch_bounds__(kThreads) void fvm_rows_project(FvmParams p, FvmBuf
   const int tid = threadIdx.x;
   b = fvm_instance(b, N);
   const int i0 = blockIdx.x * RB;
-  load_twiddles(twn, b.tw, N, 1, tid, kThreads);
-  load_twiddles(twh, b.tw, H, 2, tid, kThreads);
-  __syncthreads();
-  for (int e = tid; e < T * H; e += kThreads) {
-    const int r = e / H, k = e % H;
-    const int gi = (i0 + r) & (N - 1);
-    const float2 pk = b.spec[gi * H + k];
-    const float2 pmk = b.spec[gi * H + ((H - k) & (H - 1))];
-    qb[e] = c2r_combine(pk, pmk, k, twn[k]);
+  if constexpr (MODE == 2) {
+    float* qf = reinterpret_cast<float*>(qb);
+    for (int e = tid; e < T * N; e += kThreads) qf[e] = b.q[((i0 + e / N) & (N - 1)) * N + e % N];
+    __syncthreads();
+  } else {
+    load_twiddles(twn, b.tw, N, 1, tid, kThreads);
+    load_twiddles(twh, b.tw, H, 2, tid, kThreads);
+    __syncthreads();
+    for (int e = tid; e < T * H; e += kThreads) {
+      const int r = e / H, k = e % H;
+      const int gi = (i0 + r) & (N - 1);
+      const float2 pk = b.spec[gi * H + k];
+      const float2 pmk = b.spec[gi * H + ((H - k) & (H - 1))];
+      qb[e] = c2r_combine(pk, pmk, k, twn[k]);
+    }
+    __syncthreads();
+    lds_fft<H, T, kThreads, 1, H, false, true>(qb, twh, tid);
   }
-  __syncthreads();
-  lds_fft<H, T, kThreads, 1, H, false, true>(qb, twh, tid);
   const float* q = reinterpret_cast<const float*>(qb);
   if constexpr (MODE == 1) {
     for (int e = tid; e < RB * N; e += kThreads) b.u_out[i0 * N + e] = q[e];
@@ -492,6 +501,20 @@ template <int N> struct StepOp {
 template <int N> struct ExplicitOp {
   static int run(const FvmParams& p, const FvmBuffers& b, hipStream_t s) { return launch_explicit<N>(p, b, s); }
 };
+template <int N> struct ForwardRhsOp {
+  static int run(const FvmParams& p, const FvmBuffers& b, hipStream_t s) {
+    constexpr int RB = Tiles<N>::RB;
+    hipLaunchKernelGGL((fvm_rows_forward<N, RB, 4>), dim3(N / RB), dim3(kThreads), 0, s, p, b);
+    return check_hip(hipGetLastError(), "fvm forward (CG rhs)");
+  }
+};
+template <int N> struct ProjectQOp {
+  static int run(const FvmParams& p, const FvmBuffers& b, hipStream_t s) {
+    constexpr int RB = Tiles<N>::RB;
+    hipLaunchKernelGGL((fvm_rows_project<N, RB, 2>), dim3(N / RB), dim3(kThreads), 0, s, p, b);
+    return check_hip(hipGetLastError(), "fvm project (CG q)");
+  }
+};
 template <int N> struct OneOp {
   static int run(const FvmParams& p, const FvmBuffers& b, int which, hipStream_t s) {
     return launch_one<N>(p, b, which, s);
@@ -516,6 +539,14 @@ int fvm_launch_step(const FvmParams& p, const FvmBuffers& b, hipStream_t s) {
 
 int fvm_launch_kernel(const FvmParams& p, const FvmBuffers& b, int which, hipStream_t s) {
   return dispatch_n<OneOp>(p.n, p, b, which, s);
+}
+
+int fvm_launch_forward_rhs(const FvmParams& p, const FvmBuffers& b, hipStream_t s) {
+  return dispatch_n<ForwardRhsOp>(p.n, p, b, s);
+}
+
+int fvm_launch_project_q(const FvmParams& p, const FvmBuffers& b, hipStream_t s) {
+  return dispatch_n<ProjectQOp>(p.n, p, b, s);
 }
 
 int fvm_launch_explicit_terms(const FvmParams& p, const float* u, const float* v, float* du,

@@ -47,7 +47,29 @@ struct FvmBuffers {
   // s * n * n / 2).
   const int* slots;
   int count;  // instances in this launch (gridDim.y); <= 0 means 1
+  float* rhs;  // CG variant: div u* (n, n), single instance
+  const float* q;  // CG variant: pressure correction (n, n), single instance
 };
+
+// Conjugate-gradient pressure solve (cg.hip), one instance.
+struct CgBuffers {
+  float* x;            // solution q
+  float* r;            // residual
+  float* p[2];         // search directions (ping-pong by iteration parity)
+  float* ap;           // A p
+  float* part_rr[2];   // per-block partial sums of r.r (by parity)
+  float* part_pap;     // per-block partial sums of p.Ap
+  float* part_bb;      // per-block partial sums of b.b
+  int* status;         // [0] done flag, [1] iterations run
+};
+int cg_blocks(int n);
+int cg_launch_init(const FvmParams& p, const float* rhs, const CgBuffers& c, float rtol, float atol,
+                   int maxiter, hipStream_t s);
+// One CG iteration k (two kernels); a no-op once converged.
+int cg_launch_iteration(const FvmParams& p, const CgBuffers& c, int k, float rtol, float atol,
+                        int maxiter, hipStream_t s);
+int fvm_launch_forward_rhs(const FvmParams& p, const FvmBuffers& b, hipStream_t s);
+int fvm_launch_project_q(const FvmParams& p, const FvmBuffers& b, hipStream_t s);
 
 inline int nb(const FvmBuffers& b) { return b.count > 0 ? b.count : 1; }
 

@@ -78,6 +78,11 @@ class FvmPlan:
   def device_bytes(self) -> int:
     return int(self._lib.jcfd_fvm_plan_device_bytes(self._handle))
 
+  @property
+  def cg_iterations(self) -> int:
+    """Iterations of the last CG pressure solve (CG plans only)."""
+    return int(self._lib.jcfd_fvm_cg_iterations(self._handle))
+
   def set_graphs(self, enabled: bool):
     _lib.check(self._lib.jcfd_fvm_plan_set_graphs(self._handle, int(bool(enabled))),
                'jcfd_fvm_plan_set_graphs')

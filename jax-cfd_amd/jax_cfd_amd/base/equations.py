@@ -168,10 +168,9 @@ def semi_implicit_navier_stokes(
   if diffuse is not diffusion.diffuse:
     raise NotImplementedError('only diffusion.diffuse (explicit) is built')
   if pressure_solve is pressure.solve_fast_diag:
-    psolve = _lib.PRESSURE_FAST_DIAG
-  elif pressure_solve is pressure.solve_cg or (
-      isinstance(pressure_solve, functools.partial) and pressure_solve.func is pressure.solve_cg):
-    raise NotImplementedError('the CG pressure solve is not built yet')
+    solve_kw = {}
+  elif pressure.is_solve_cg(pressure_solve):
+    solve_kw = pressure.cg_options(pressure_solve)
   else:
     raise NotImplementedError(f'pressure_solve {pressure_solve!r} is not built')
   conv = _read_convection(convect, grid, dt)
@@ -179,7 +178,7 @@ def semi_implicit_navier_stokes(
   spec = _fvm.spec_from_grid(
       grid, dt=float(dt), density=float(density),
       viscosity=None if viscosity is None else float(viscosity),
-      convect=conv, pressure=psolve, linear_coefficient=lin, forcing_x=fx)
+      convect=conv, linear_coefficient=lin, forcing_x=fx, **solve_kw)
   return FvmStepFn(grid, spec)
 
 

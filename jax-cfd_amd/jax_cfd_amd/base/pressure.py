@@ -1,13 +1,14 @@
 """Pressure projection (jax_cfd/base/pressure.py:67-198).
 
-``projection`` runs the fast-diagonalisation (rfft) solve on the GPU.  The
-solve callables are markers read by the step factory; called directly they
-return the pressure correction q.
+``projection`` runs the fast-diagonalisation (rfft) or the CG solve on the
+GPU.  ``solve_fast_diag`` called directly returns q; ``solve_cg`` is a marker
+read by ``projection`` and the step factory.
 """
 from __future__ import annotations
 
 from typing import Callable, Optional
 
+from jax_cfd_amd import _lib
 from jax_cfd_amd.base import _fvm, boundaries, grids
 
 
@@ -52,17 +53,44 @@ def _poisson_multiplier(grid, device):
 
 def solve_cg(v, q0=None, pressure_bc=None, rtol: float = 1e-6, atol: float = 1e-6,
              maxiter: Optional[int] = None):
-  """pressure.py:67-112.  Recognised by the step factory (CG variant)."""
-  raise NotImplementedError('solve_cg has no GPU kernel yet; use solve_fast_diag')
+  """pressure.py:67-112: unpreconditioned CG (x0 = 0) on the periodic
+  Laplacian.  A marker: projection(v, solve_cg) and the step factory run it
+  on the GPU (the solution q itself is not returned on its own)."""
+  del v, q0, pressure_bc, rtol, atol, maxiter
+  raise NotImplementedError('use projection(v, solve_cg) or pass solve_cg to '
+                            'semi_implicit_navier_stokes')
+
+
+def cg_options(solve, rtol=1e-6, atol=1e-6, maxiter=None) -> dict:
+  """Plan fields for a solve_cg callable (possibly functools.partial)."""
+  import functools
+  if isinstance(solve, functools.partial):
+    kw = dict(solve.keywords)
+    rtol = kw.get('rtol', rtol)
+    atol = kw.get('atol', atol)
+    maxiter = kw.get('maxiter', maxiter)
+  return dict(pressure=_lib.PRESSURE_CG, cg_rtol=float(rtol), cg_atol=float(atol),
+              cg_maxiter=int(maxiter or 0))
+
+
+def is_solve_cg(solve) -> bool:
+  import functools
+  return solve is solve_cg or (isinstance(solve, functools.partial)
+                               and solve.func is solve_cg)
 
 
 def projection(v: grids.GridVariableVector,
                solve: Callable = solve_fast_diag) -> grids.GridVariableVector:
-  """pressure.py:181-198: v - grad q with q from `solve` (fast-diag on GPU)."""
-  if solve is not solve_fast_diag:
-    raise NotImplementedError('projection on the GPU uses solve_fast_diag')
+  """pressure.py:181-198: v - grad q with q from `solve` (fast-diagonal or
+  CG, both on the GPU)."""
   grid = _check_velocity(v)
-  plan = _fvm.plan_for(_fvm.spec_from_grid(grid), v[0].data.device)
+  if solve is solve_fast_diag:
+    spec = _fvm.spec_from_grid(grid)
+  elif is_solve_cg(solve):
+    spec = _fvm.spec_from_grid(grid, **cg_options(solve))
+  else:
+    raise NotImplementedError('projection on the GPU uses solve_fast_diag or solve_cg')
+  plan = _fvm.plan_for(spec, v[0].data.device)
   uo, vo = plan.projection(v[0].data, v[1].data)
   return tuple(grids.GridVariable(grids.GridArray(d, u.offset, grid), u.bc)
                for d, u in zip((uo, vo), v))

@@ -55,7 +55,8 @@ def _cfg(**kw):
     (dict(dx=0.0), -1, "grid steps"),
     (dict(density=0.0), -1, "density"),
     (dict(convect=7), -2, "advect_van_leer"),
-    (dict(pressure=1), -2, "solve_fast_diag"),
+    (dict(pressure=5), -2, "solve_fast_diag"),
+    (dict(pressure=1, batch=4), -2, "one instance"),
 ])
 def test_plan_create_rejects_before_device(kw, code, msg):
   from jax_cfd_amd import _lib

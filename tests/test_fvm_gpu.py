@@ -138,6 +138,53 @@ def test_van_leer_using_limiters_parity(n, forced):
       assert err <= 1e-3
 
 
+@pytest.mark.parametrize("n", [32, 64, 256])
+def test_cg_projection_matches_fast_diag_and_oracle_cg(n):
+  """solve_cg (pressure.py:67-112, rtol = atol = 1e-6): the projected
+  velocity agrees with the fast-diagonal one and with the oracle's CG to the
+  solver tolerance; iterate-level parity is unpinned (reduction order)."""
+  import functools
+  cfd = _cfd()
+  ogrid = _ogrid(n)
+  rng = np.random.default_rng(n + 1)
+  v = tuple(A((1. + .3 * rng.standard_normal((n, n))).astype(np.float32), o)
+            for o in ogrid.cell_faces)
+  solve = functools.partial(cfd.pressure.solve_cg, rtol=1e-6, atol=1e-6)
+  out = cfd.pressure.projection(_state(v, n), solve)
+  fd_ref = ofvm.projection(v, ogrid)
+  cg_ref = ofvm.projection(v, ogrid, lambda vv, g: ofvm.solve_cg(vv, g, 1e-6, 1e-6))
+  for o, f, c in zip(out, fd_ref, cg_ref):
+    o = o.data.cpu().numpy()
+    print(f"CG projection n={n}: vs fast-diag {_maxrel(o, f.data):.2e}, "
+          f"vs oracle CG {_maxrel(o, c.data):.2e}")
+    assert _maxrel(o, f.data) <= 1e-4 and _maxrel(o, c.data) <= 1e-4
+
+
+def test_cg_step_variant():
+  """The C1 variant with solve_cg in place of solve_fast_diag: one step and
+  20 steps against the fast-diagonal step and the oracle's CG step."""
+  import functools
+  n = 64
+  cfd = _cfd()
+  ogrid, v0, dt, ostep, step_fd = _config(n, 1, 7.0, True)
+  solve = functools.partial(cfd.pressure.solve_cg, rtol=1e-6, atol=1e-6)
+  step = cfd.equations.semi_implicit_navier_stokes(
+      1.0, 1e-3, dt, _grid(n),
+      convect=lambda v: tuple(cfd.advection.advect_van_leer(u, v, dt) for u in v),
+      forcing=cfd.forcings.simple_turbulence_forcing(_grid(n), 1.0, 4, -0.1, 'kolmogorov'),
+      pressure_solve=solve)
+  ostep_cg = ofvm.FVMStep(ogrid, dt=dt, viscosity=1e-3, forcing=ostep.forcing,
+                          pressure='cg')
+  out = cfd.funcutils.repeated(step, 20)(_state(v0, n))
+  ref = ostep_cg.rollout(v0, 20)
+  for o, r in zip(out, ref):
+    err = _rel_l2(o.data.cpu().numpy(), r.data)
+    print(f"CG step 20 steps: rel L2 vs oracle CG {err:.2e}; iterations "
+          f"{step.plan().cg_iterations}")
+    assert err <= 1e-3
+  assert 0 < step.plan().cg_iterations < 10 * n * n
+
+
 @pytest.mark.parametrize("n", [32, 64, 256, 512, 1024])
 def test_projection_parity_and_divergence(n):
   ogrid = _ogrid(n)
