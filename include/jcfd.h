@@ -149,6 +149,22 @@ int jcfd_fvm_projection(jcfd_fvm_plan* plan, const jcfd_tensor* u, const jcfd_te
 int jcfd_fvm_divergence(jcfd_fvm_plan* plan, const jcfd_tensor* u, const jcfd_tensor* v,
                         jcfd_tensor* out, void* stream);
 
+/* Reverse (vector-Jacobian) step of the forward-Euler FVM step with
+ * advect_van_leer: given the state (u, v) a step started from and the
+ * cotangent (lam_u, lam_v) of the state it produced, writes the cotangent
+ * of (u, v): lam* = P lam, out = lam* + dt J(u, v)^T lam*.  The quantity
+ * jax.grad forms through one step of semi_implicit_navier_stokes
+ * (equations.py:120-151, time_stepping.py:88-118); J^T follows jax's
+ * differentiation rules for where / sign / abs / division.  Single-instance
+ * fast-diagonal plans.  Outputs must not alias inputs. */
+int jcfd_fvm_step_vjp(jcfd_fvm_plan* plan, const jcfd_tensor* u, const jcfd_tensor* v,
+                      const jcfd_tensor* lam_u, const jcfd_tensor* lam_v, jcfd_tensor* out_u,
+                      jcfd_tensor* out_v, void* stream);
+
+/* Cotangent of fd.curl_2d: (u_bar, v_bar) from w_bar. */
+int jcfd_fvm_curl_vjp(jcfd_fvm_plan* plan, const jcfd_tensor* w_bar, jcfd_tensor* u_bar,
+                      jcfd_tensor* v_bar, void* stream);
+
 /* fd.curl_2d of a staggered velocity (finite_differences.py:186-193),
  * offset (1, 1). */
 int jcfd_fvm_curl(jcfd_fvm_plan* plan, const jcfd_tensor* u, const jcfd_tensor* v,

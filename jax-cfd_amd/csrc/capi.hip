@@ -664,6 +664,50 @@ int jcfd_fvm_kernel_bytes(const jcfd_fvm_plan* p, int64_t* bytes) {
   return JCFD_OK;
 }
 
+int jcfd_fvm_step_vjp(jcfd_fvm_plan* p, const jcfd_tensor* u, const jcfd_tensor* v,
+                      const jcfd_tensor* lam_u, const jcfd_tensor* lam_v, jcfd_tensor* out_u,
+                      jcfd_tensor* out_v, void* stream) {
+  if (p == nullptr) return fail(JCFD_ERR_INVALID_ARGUMENT, "null plan");
+  if (p->cfg.convect != JCFD_CONVECT_VAN_LEER || p->cfg.pressure != JCFD_PRESSURE_FAST_DIAG)
+    return fail(JCFD_ERR_UNSUPPORTED, "the adjoint is built for advect_van_leer + solve_fast_diag");
+  const int n = p->params.n;
+  void *pu, *pv, *plu, *plv, *pou, *pov;
+  int rc;
+  if ((rc = check_tensor(u, "u", kDLFloat, 32, n, n, p->device, &pu)) ||
+      (rc = check_tensor(v, "v", kDLFloat, 32, n, n, p->device, &pv)) ||
+      (rc = check_tensor(lam_u, "lam_u", kDLFloat, 32, n, n, p->device, &plu)) ||
+      (rc = check_tensor(lam_v, "lam_v", kDLFloat, 32, n, n, p->device, &plv)) ||
+      (rc = check_tensor(out_u, "out_u", kDLFloat, 32, n, n, p->device, &pou)) ||
+      (rc = check_tensor(out_v, "out_v", kDLFloat, 32, n, n, p->device, &pov)))
+    return rc;
+  for (void* o : {pou, pov})
+    for (void* i : {pu, pv, plu, plv})
+      if (o == i) return fail(JCFD_ERR_INVALID_ARGUMENT, "step vjp: outputs must not alias inputs");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  // lam* = P lam into the plan's u*/v* scratch, then the explicit-term transpose.
+  if ((rc = fvm_launch_projection(p->params, static_cast<const float*>(plu),
+                                  static_cast<const float*>(plv), p->us, p->vs, p->spec, p->lam,
+                                  p->tw, s)))
+    return rc;
+  return fvm_launch_adjoint_explicit(p->params, static_cast<const float*>(pu),
+                                     static_cast<const float*>(pv), p->us, p->vs,
+                                     static_cast<float*>(pou), static_cast<float*>(pov), s);
+}
+
+int jcfd_fvm_curl_vjp(jcfd_fvm_plan* p, const jcfd_tensor* w_bar, jcfd_tensor* u_bar,
+                      jcfd_tensor* v_bar, void* stream) {
+  if (p == nullptr) return fail(JCFD_ERR_INVALID_ARGUMENT, "null plan");
+  const int n = p->params.n;
+  void *pw, *pu, *pv;
+  int rc;
+  if ((rc = check_tensor(w_bar, "w_bar", kDLFloat, 32, n, n, p->device, &pw)) ||
+      (rc = check_tensor(u_bar, "u_bar", kDLFloat, 32, n, n, p->device, &pu)) ||
+      (rc = check_tensor(v_bar, "v_bar", kDLFloat, 32, n, n, p->device, &pv)))
+    return rc;
+  return fvm_launch_curl_vjp(p->params, static_cast<const float*>(pw), static_cast<float*>(pu),
+                             static_cast<float*>(pv), static_cast<hipStream_t>(stream));
+}
+
 int jcfd_fvm_curl(jcfd_fvm_plan* p, const jcfd_tensor* u, const jcfd_tensor* v, jcfd_tensor* out,
                   void* stream) {
   if (p == nullptr) return fail(JCFD_ERR_INVALID_ARGUMENT, "null plan");

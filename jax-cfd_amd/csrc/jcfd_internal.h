@@ -68,6 +68,9 @@ int cg_launch_init(const FvmParams& p, const float* rhs, const CgBuffers& c, flo
 // One CG iteration k (two kernels); a no-op once converged.
 int cg_launch_iteration(const FvmParams& p, const CgBuffers& c, int k, float rtol, float atol,
                         int maxiter, hipStream_t s);
+int fvm_launch_adjoint_explicit(const FvmParams& p, const float* u, const float* v, const float* lu,
+                                const float* lv, float* ou, float* ov, hipStream_t s);
+int fvm_launch_curl_vjp(const FvmParams& p, const float* wb, float* ub, float* vb, hipStream_t s);
 int fvm_launch_forward_rhs(const FvmParams& p, const FvmBuffers& b, hipStream_t s);
 int fvm_launch_project_q(const FvmParams& p, const FvmBuffers& b, hipStream_t s);
 

@@ -98,6 +98,8 @@ _SIGNATURES = {
     "jcfd_fvm_projection": (ctypes.c_int, [_P, _P, _P, _P, _P, _P]),
     "jcfd_fvm_divergence": (ctypes.c_int, [_P, _P, _P, _P, _P]),
     "jcfd_fvm_curl": (ctypes.c_int, [_P, _P, _P, _P, _P]),
+    "jcfd_fvm_step_vjp": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _P, _P]),
+    "jcfd_fvm_curl_vjp": (ctypes.c_int, [_P, _P, _P, _P, _P]),
     "jcfd_fvm_time_kernels": (ctypes.c_int, [_P, _P, _P, ctypes.c_int32,
                                              ctypes.POINTER(ctypes.c_double), _P]),
     "jcfd_fvm_kernel_bytes": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int64)]),

@@ -173,6 +173,26 @@ class FvmPlan:
     _lib.check(self._lib.jcfd_fvm_kernel_bytes(self._handle, b), 'jcfd_fvm_kernel_bytes')
     return dict(zip(self.KERNELS, [int(x) for x in b]))
 
+  def step_vjp(self, u, v, lam_u, lam_v):
+    """Cotangent of (u, v) through one step, given the cotangent of its output."""
+    u, v = self._field(u), self._field(v)
+    lu, lv = self._field(lam_u), self._field(lam_v)
+    ou, ov = self._empty(), self._empty()
+    with _lib.dlpack_args() as dl:
+      rc = self._lib.jcfd_fvm_step_vjp(self._handle, dl(u), dl(v), dl(lu), dl(lv), dl(ou),
+                                       dl(ov), _lib.stream_handle(self.device))
+    _lib.check(rc, 'jcfd_fvm_step_vjp')
+    return ou, ov
+
+  def curl_vjp(self, w_bar):
+    wb = self._field(w_bar)
+    ub, vb = self._empty(), self._empty()
+    with _lib.dlpack_args() as dl:
+      rc = self._lib.jcfd_fvm_curl_vjp(self._handle, dl(wb), dl(ub), dl(vb),
+                                       _lib.stream_handle(self.device))
+    _lib.check(rc, 'jcfd_fvm_curl_vjp')
+    return ub, vb
+
   def curl(self, u, v):
     u, v = self._field(u), self._field(v)
     out = self._empty()
