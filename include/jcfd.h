@@ -176,6 +176,36 @@ int jcfd_fvm_curl(jcfd_fvm_plan* plan, const jcfd_tensor* u, const jcfd_tensor* 
 int jcfd_fvm_spectral_filter(jcfd_fvm_plan* plan, const jcfd_tensor* x,
                              const jcfd_tensor* multiplier, jcfd_tensor* out, void* stream);
 
+/* Mixed-size groups: one step of many FVM instances of different grid sizes
+ * (and different physics), each of them a slot of some plan, as ONE
+ * persistent launch per kernel class.  The reference steps one field per
+ * call (funcutils.repeated over step_fn, funcutils.py:82-88; a batch of
+ * same-size fields would be jax.vmap of that); there is no counterpart for
+ * uneven sizes.  Every instance gets exactly the arithmetic of jcfd_fvm_step
+ * on that instance alone. */
+typedef struct jcfd_fvm_group jcfd_fvm_group;
+
+/* One plan's share of a group step: advance, in place, slots[0..n_slots) of
+ * the plan's (batch, n, n) fields u and v.  `slots` is HOST memory (int32,
+ * distinct, in [0, batch)).  Fast-diagonal plans with n <= 512. */
+typedef struct {
+  jcfd_fvm_plan* plan;
+  jcfd_tensor* u;
+  jcfd_tensor* v;
+  const int32_t* slots;
+  int32_t n_slots;
+} jcfd_fvm_group_member;
+
+/* The group holds the device work list (re-uploaded only when the member
+ * set changes) and a captured graph of 32 steps (on by default). */
+int jcfd_fvm_group_create(jcfd_fvm_group** group);
+int jcfd_fvm_group_destroy(jcfd_fvm_group* group);
+int jcfd_fvm_group_set_graphs(jcfd_fvm_group* group, int32_t enabled);
+/* nsteps steps of every member instance.  No (plan, slot) and no (u, slot)
+ * may appear twice.  Calls on one group must be ordered on one stream. */
+int jcfd_fvm_group_step(jcfd_fvm_group* group, const jcfd_fvm_group_member* members,
+                        int32_t n_members, int64_t nsteps, void* stream);
+
 /* ------------------------------------------------------- spectral path */
 typedef struct {
   int32_t n;               /* cells per side; square periodic grid */

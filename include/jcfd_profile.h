@@ -41,6 +41,18 @@ int jcfd_ns2d_time_kernels(jcfd_ns2d_plan* plan, const jcfd_tensor* w_hat, int32
                            double* ms_per_launch, void* stream);
 int jcfd_ns2d_kernel_bytes(const jcfd_ns2d_plan* plan, int64_t* bytes_per_launch);
 
+/* Mixed-size groups: `reps` further group steps of the member set of the
+ * last jcfd_fvm_group_step (their states advance), with events around each
+ * of the three phase launches; ms_per_launch[k] = mean duration of phase k
+ * (the JCFD_FVM_KERNEL_* classes). */
+int jcfd_fvm_group_time_kernels(jcfd_fvm_group* group, int32_t reps, double* ms_per_launch,
+                                void* stream);
+/* Shape of the last group step: workgroups per phase launch, work items per
+ * phase, and the algorithmic bytes per phase launch (the per-instance
+ * jcfd_fvm_kernel_bytes summed over the member instances). */
+int jcfd_fvm_group_launch_shape(const jcfd_fvm_group* group, int32_t* grid, int32_t* items,
+                                int64_t* bytes_per_launch);
+
 #ifdef __cplusplus
 }
 #endif

@@ -2,6 +2,7 @@
 // Kernels live in fvm.hip / spectral.hip; this file only orchestrates.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -734,6 +735,320 @@ int jcfd_fvm_spectral_filter(jcfd_fvm_plan* p, const jcfd_tensor* x, const jcfd_
     return rc;
   return fvm_launch_filter(p->params, static_cast<const float*>(px), static_cast<const float*>(pm),
                            static_cast<float*>(po), p->spec, p->tw, static_cast<hipStream_t>(stream));
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------ mixed-size groups
+struct jcfd_fvm_group {
+  int device = 0;
+  FvmGroupDesc* descs = nullptr;   // device descriptor table
+  FvmGroupItem* items = nullptr;   // device work lists, phase k at items + k * cap_items
+  int* d_counts = nullptr;         // device int[3]: items per phase
+  int cap_descs = 0, cap_items = 0;
+  char* staging = nullptr;         // pinned upload buffer
+  size_t staging_bytes = 0;
+  hipEvent_t staged = nullptr;     // recorded after the last upload
+  bool staged_pending = false;
+  std::vector<char> last;          // the table last uploaded
+  int max_n = 0;
+  int grid[3] = {0, 0, 0};
+  int n_items[3] = {0, 0, 0};   // of the last step
+  int64_t bytes[3] = {0, 0, 0};  // algorithmic bytes per phase launch, last step
+  GraphCache graphs;
+};
+
+namespace {
+
+void group_free_tables(jcfd_fvm_group* g) {
+  for (void* ptr : {static_cast<void*>(g->descs), static_cast<void*>(g->items)})
+    if (ptr != nullptr) (void)hipFree(ptr);
+  g->descs = nullptr;
+  g->items = nullptr;
+  g->cap_descs = g->cap_items = 0;
+}
+
+// Grow the device tables to hold n_desc descriptors and n_item items per
+// phase.  Tables in use by queued work are freed only after a device-wide
+// synchronisation; the graph is keyed on their addresses and so recaptured.
+int group_reserve(jcfd_fvm_group* g, int n_desc, int n_item) {
+  if (n_desc <= g->cap_descs && n_item <= g->cap_items) return JCFD_OK;
+  int rc;
+  if (g->descs != nullptr && (rc = check_hip(hipDeviceSynchronize(), "group regrow sync"))) return rc;
+  const int cd = std::max({n_desc, 2 * g->cap_descs, 16});
+  const int ci = std::max({n_item, 2 * g->cap_items, 256});
+  group_free_tables(g);
+  int64_t unused = 0;
+  if ((rc = device_alloc(&g->descs, static_cast<size_t>(cd), &unused)) ||
+      (rc = device_alloc(&g->items, static_cast<size_t>(3) * ci, &unused))) {
+    group_free_tables(g);
+    return rc;
+  }
+  g->cap_descs = cd;
+  g->cap_items = ci;
+  return JCFD_OK;
+}
+
+// Copy `table` (n_items[4] | descs | items phase 0 | 1 | 2) to the device
+// through the pinned staging buffer, stream-ordered on s.
+int group_upload(jcfd_fvm_group* g, const std::vector<char>& table, int n_desc, const int* counts,
+                 hipStream_t s) {
+  int rc;
+  if (g->staged_pending && (rc = check_hip(hipEventSynchronize(g->staged), "staging reuse")))
+    return rc;
+  g->staged_pending = false;
+  if (table.size() > g->staging_bytes) {
+    if (g->staging != nullptr) (void)hipHostFree(g->staging);
+    g->staging = nullptr;
+    const size_t bytes = std::max(table.size(), 2 * g->staging_bytes);
+    if ((rc = check_hip(hipHostMalloc(reinterpret_cast<void**>(&g->staging), bytes, hipHostMallocDefault),
+                        "pinned group staging")))
+      return rc;
+    g->staging_bytes = bytes;
+  }
+  std::memcpy(g->staging, table.data(), table.size());
+  size_t off = 4 * sizeof(int);
+  if ((rc = check_hip(hipMemcpyAsync(g->d_counts, g->staging, 3 * sizeof(int), hipMemcpyHostToDevice, s),
+                      "group upload")) ||
+      (rc = check_hip(hipMemcpyAsync(g->descs, g->staging + off, n_desc * sizeof(FvmGroupDesc),
+                                     hipMemcpyHostToDevice, s),
+                      "group upload")))
+    return rc;
+  off += n_desc * sizeof(FvmGroupDesc);
+  for (int k = 0; k < 3; ++k) {
+    const size_t bytes = counts[k] * sizeof(FvmGroupItem);
+    if (bytes > 0 &&
+        (rc = check_hip(hipMemcpyAsync(g->items + static_cast<size_t>(k) * g->cap_items, g->staging + off,
+                                       bytes, hipMemcpyHostToDevice, s),
+                        "group upload")))
+      return rc;
+    off += bytes;
+  }
+  if ((rc = check_hip(hipEventRecord(g->staged, s), "staging event"))) return rc;
+  g->staged_pending = true;
+  return JCFD_OK;
+}
+
+FvmGroupQueue group_queue(const jcfd_fvm_group* g) {
+  FvmGroupQueue q{};
+  q.descs = g->descs;
+  for (int k = 0; k < 3; ++k) q.items[k] = g->items + static_cast<size_t>(k) * g->cap_items;
+  q.n_items = g->d_counts;
+  return q;
+}
+
+}  // namespace
+
+extern "C" {
+
+int jcfd_fvm_group_create(jcfd_fvm_group** group) {
+  if (group == nullptr) return fail(JCFD_ERR_INVALID_ARGUMENT, "null argument");
+  *group = nullptr;
+  auto* g = new (std::nothrow) jcfd_fvm_group();
+  if (g == nullptr) return fail(JCFD_ERR_OUT_OF_MEMORY, "host allocation failed");
+  int64_t unused = 0;
+  int rc;
+  if ((rc = check_hip(hipGetDevice(&g->device), "hipGetDevice")) ||
+      (rc = device_alloc(&g->d_counts, 4, &unused)) ||
+      (rc = check_hip(hipMemset(g->d_counts, 0, 4 * sizeof(int)), "zero counts")) ||
+      (rc = check_hip(hipEventCreateWithFlags(&g->staged, hipEventDisableTiming), "staging event"))) {
+    jcfd_fvm_group_destroy(g);
+    return rc;
+  }
+  *group = g;
+  return JCFD_OK;
+}
+
+int jcfd_fvm_group_destroy(jcfd_fvm_group* g) {
+  if (g == nullptr) return JCFD_OK;
+  if (g->staged_pending) (void)hipEventSynchronize(g->staged);
+  g->graphs.release();
+  group_free_tables(g);
+  if (g->d_counts != nullptr) (void)hipFree(g->d_counts);
+  if (g->staging != nullptr) (void)hipHostFree(g->staging);
+  if (g->staged != nullptr) (void)hipEventDestroy(g->staged);
+  delete g;
+  return JCFD_OK;
+}
+
+int jcfd_fvm_group_set_graphs(jcfd_fvm_group* g, int32_t enabled) {
+  if (g == nullptr) return fail(JCFD_ERR_INVALID_ARGUMENT, "null group");
+  g->graphs.enabled = enabled != 0;
+  return JCFD_OK;
+}
+
+int jcfd_fvm_group_step(jcfd_fvm_group* g, const jcfd_fvm_group_member* members, int32_t n_members,
+                        int64_t nsteps, void* stream) {
+  if (g == nullptr) return fail(JCFD_ERR_INVALID_ARGUMENT, "null group");
+  if (n_members < 0 || (n_members > 0 && members == nullptr))
+    return fail(JCFD_ERR_INVALID_ARGUMENT, "members: null array or negative count");
+  if (nsteps < 0) return fail(JCFD_ERR_INVALID_ARGUMENT, "nsteps must be >= 0");
+  // ---- validate and build the descriptor table
+  std::vector<FvmGroupDesc> descs;
+  std::vector<const void*> owners;   // (plan, slot) scratch and (field, slot) state must be unique
+  int rc;
+  for (int m = 0; m < n_members; ++m) {
+    const jcfd_fvm_group_member& mb = members[m];
+    jcfd_fvm_plan* p = mb.plan;
+    const std::string who = "member " + std::to_string(m);
+    if (p == nullptr) return fail(JCFD_ERR_INVALID_ARGUMENT, who + ": null plan");
+    if (p->device != g->device)
+      return fail(JCFD_ERR_INVALID_ARGUMENT, who + ": plan on another device than the group");
+    if (p->cfg.pressure != JCFD_PRESSURE_FAST_DIAG)
+      return fail(JCFD_ERR_UNSUPPORTED, who + ": groups step fast-diagonal plans only");
+    const int n = p->params.n;
+    if (!fvm_group_size_supported(n))
+      return fail(JCFD_ERR_UNSUPPORTED, who + ": grid size " + std::to_string(n) +
+                                            " not built for groups (32..512)");
+    void *pu, *pv;
+    if ((rc = check_batched(mb.u, "u", kDLFloat, 32, p->batch, n, n, p->device, &pu)) ||
+        (rc = check_batched(mb.v, "v", kDLFloat, 32, p->batch, n, n, p->device, &pv)))
+      return fail(rc, who + ": " + jcfd_last_error());
+    if (mb.n_slots < 0 || mb.n_slots > p->batch || (mb.n_slots > 0 && mb.slots == nullptr))
+      return fail(JCFD_ERR_INVALID_ARGUMENT, who + ": slots must hold 0..batch host int32 values");
+    const size_t nn = static_cast<size_t>(n) * n;
+    for (int k = 0; k < mb.n_slots; ++k) {
+      const int s = mb.slots[k];
+      if (s < 0 || s >= p->batch)
+        return fail(JCFD_ERR_INVALID_ARGUMENT, who + ": slot " + std::to_string(s) + " out of range");
+      FvmGroupDesc d{};
+      d.p = p->params;
+      d.b = fvm_buffers(p);
+      d.b.us += s * nn;
+      d.b.vs += s * nn;
+      d.b.spec += s * nn / 2;
+      float* u = static_cast<float*>(pu) + s * nn;
+      float* v = static_cast<float*>(pv) + s * nn;
+      d.b.u = u;
+      d.b.v = v;
+      d.b.u_out = u;
+      d.b.v_out = v;
+      d.b.count = 1;
+      descs.push_back(d);
+      owners.push_back(d.b.us);
+      owners.push_back(u);
+      owners.push_back(v);
+    }
+  }
+  std::vector<const void*> sorted_owners = owners;
+  std::sort(sorted_owners.begin(), sorted_owners.end());
+  if (std::adjacent_find(sorted_owners.begin(), sorted_owners.end()) != sorted_owners.end())
+    return fail(JCFD_ERR_INVALID_ARGUMENT,
+                "group members repeat an instance (same plan slot or same field slot twice)");
+  if (nsteps == 0 || descs.empty()) return JCFD_OK;
+  // largest grids first: their items are the longest, so the tail stays short
+  std::stable_sort(descs.begin(), descs.end(),
+                   [](const FvmGroupDesc& a, const FvmGroupDesc& b) { return a.p.n > b.p.n; });
+  const int n_desc = static_cast<int>(descs.size());
+  const int max_n = descs.front().p.n;
+  std::vector<FvmGroupItem> items[3];
+  for (int k = 0; k < 3; ++k)
+    for (int d = 0; d < n_desc; ++d)
+      for (int t = 0, nt = fvm_group_tiles(descs[d].p.n, k); t < nt; ++t) items[k].push_back({d, t});
+  const int counts[4] = {static_cast<int>(items[0].size()), static_cast<int>(items[1].size()),
+                         static_cast<int>(items[2].size()), 0};
+  std::vector<char> table(4 * sizeof(int) + n_desc * sizeof(FvmGroupDesc) +
+                          (counts[0] + counts[1] + counts[2]) * sizeof(FvmGroupItem));
+  {
+    char* w = table.data();
+    std::memcpy(w, counts, 4 * sizeof(int));
+    w += 4 * sizeof(int);
+    std::memcpy(w, descs.data(), n_desc * sizeof(FvmGroupDesc));
+    w += n_desc * sizeof(FvmGroupDesc);
+    for (int k = 0; k < 3; ++k) {
+      std::memcpy(w, items[k].data(), counts[k] * sizeof(FvmGroupItem));
+      w += counts[k] * sizeof(FvmGroupItem);
+    }
+  }
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const int need_items = std::max({counts[0], counts[1], counts[2]});
+  if (n_desc > g->cap_descs || need_items > g->cap_items) {
+    if ((rc = group_reserve(g, n_desc, need_items))) return rc;
+    g->last.clear();
+  }
+  if (table != g->last) {
+    if ((rc = group_upload(g, table, n_desc, counts, s))) return rc;
+    g->last = table;
+  }
+  if (max_n != g->max_n) {
+    for (int k = 0; k < 3; ++k)
+      if ((rc = fvm_group_grid(k, max_n, &g->grid[k]))) return rc;
+    g->max_n = max_n;
+  }
+  for (int k = 0; k < 3; ++k) g->n_items[k] = counts[k];
+  g->bytes[0] = g->bytes[1] = g->bytes[2] = 0;
+  for (const FvmGroupDesc& d : descs) {
+    const int64_t cells = static_cast<int64_t>(d.p.n) * d.p.n;
+    g->bytes[0] += 20 * cells;
+    g->bytes[1] += 8 * cells;
+    g->bytes[2] += 20 * cells;
+  }
+  const FvmGroupQueue q = group_queue(g);
+  auto one_step = [&](hipStream_t st) {
+    for (int k = 0; k < 3; ++k) {
+      const int r = fvm_group_launch(k, max_n, q, g->grid[k], st);
+      if (r) return r;
+    }
+    return static_cast<int>(JCFD_OK);
+  };
+  int64_t remaining = nsteps;
+  if (g->graphs.enabled && remaining >= kGraphChunk) {
+    auto launch = [&](hipStream_t gs) {
+      for (int i = 0; i < kGraphChunk; ++i) {
+        const int r = one_step(gs);
+        if (r) return r;
+      }
+      return static_cast<int>(JCFD_OK);
+    };
+    if ((rc = g->graphs.run(g->descs, g->items, nullptr, max_n, kGraphChunk, remaining / kGraphChunk, s,
+                            launch)))
+      return rc;
+    remaining %= kGraphChunk;
+  }
+  for (int64_t step = 0; step < remaining; ++step)
+    if ((rc = one_step(s))) return rc;
+  return JCFD_OK;
+}
+
+int jcfd_fvm_group_time_kernels(jcfd_fvm_group* g, int32_t reps, double* ms_per_launch, void* stream) {
+  if (g == nullptr || ms_per_launch == nullptr) return fail(JCFD_ERR_INVALID_ARGUMENT, "null argument");
+  if (reps <= 0) return fail(JCFD_ERR_INVALID_ARGUMENT, "reps must be positive");
+  if (g->max_n == 0 || g->n_items[0] == 0)
+    return fail(JCFD_ERR_INVALID_ARGUMENT, "no group step to time (call jcfd_fvm_group_step first)");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const FvmGroupQueue q = group_queue(g);
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  int rc = JCFD_OK;
+  for (int k = 0; k < 4 && rc == JCFD_OK; ++k) rc = check_hip(hipEventCreate(&ev[k]), "event");
+  double total[3] = {0, 0, 0};
+  for (int r = 0; r < reps && rc == JCFD_OK; ++r) {
+    rc = check_hip(hipEventRecord(ev[0], s), "event record");
+    for (int k = 0; k < 3 && rc == JCFD_OK; ++k) {
+      rc = fvm_group_launch(k, g->max_n, q, g->grid[k], s);
+      if (rc == JCFD_OK) rc = check_hip(hipEventRecord(ev[k + 1], s), "event record");
+    }
+    if (rc == JCFD_OK) rc = check_hip(hipEventSynchronize(ev[3]), "event sync");
+    for (int k = 0; k < 3 && rc == JCFD_OK; ++k) {
+      float ms = 0.f;
+      rc = check_hip(hipEventElapsedTime(&ms, ev[k], ev[k + 1]), "elapsed");
+      total[k] += ms;
+    }
+  }
+  for (int k = 0; k < 3; ++k) ms_per_launch[k] = total[k] / reps;
+  for (hipEvent_t e : ev)
+    if (e != nullptr) (void)hipEventDestroy(e);
+  return rc;
+}
+
+int jcfd_fvm_group_launch_shape(const jcfd_fvm_group* g, int32_t* grid, int32_t* items,
+                                int64_t* bytes_per_launch) {
+  if (g == nullptr) return fail(JCFD_ERR_INVALID_ARGUMENT, "null group");
+  for (int k = 0; k < 3; ++k) {
+    if (grid) grid[k] = g->grid[k];
+    if (items) items[k] = g->n_items[k];
+    if (bytes_per_launch) bytes_per_launch[k] = g->bytes[k];
+  }
+  return JCFD_OK;
 }
 
 }  // extern "C"

@@ -26,6 +26,8 @@
 // contraction in this file) so fp32 results track jax-cfd's to round-off.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "jcfd_fft.h"
 #include "jcfd_internal.h"
 
@@ -210,24 +212,68 @@ __device__ __forceinline__ void stage_rows(float* su, float* sv, const float* __
   }
 }
 
+// LDS layouts of the three step kernels, carved from one byte buffer so the
+// same bodies serve the per-size kernels (static LDS) and the mixed-size
+// group kernels (dynamic LDS sized for the largest grid present).  float2
+// arrays come first so every array stays 8-byte aligned.
+// Arrays a MODE does not use get size 0.
+template <int N, int RB, int MODE>
+struct RowsForwardLds {
+  static constexpr int H = N / 2, WR = RB + 5;
+  static constexpr bool kFft = MODE == 0 || MODE == 1 || MODE == 3;   // row FFT of div / field
+  static constexpr bool kWin = MODE == 0 || MODE == 2 || MODE == 4;   // stencil window
+  static constexpr bool kStar = MODE == 0 || MODE == 1 || MODE == 4;  // u*, v* rows
+  static constexpr int zb = 0;                                  // float2[RB * H]: div rows, then FFTs
+  static constexpr int twn = zb + (kFft ? RB * H * 8 : 0);      // float2[N]
+  static constexpr int twh = twn + (kFft ? N * 8 : 0);          // float2[H]
+  static constexpr int su = twh + (kFft ? H * 8 : 0);           // float[WR * N]: u rows i0-3 .. i0+RB+1
+  static constexpr int sv = su + (kWin ? WR * N * 4 : 0);       // float[WR * N]
+  static constexpr int sus = sv + (kWin ? WR * N * 4 : 0);      // float[(RB + 1) * N]: u* rows i0-1 ..
+  static constexpr int svs = sus + (kStar ? (RB + 1) * N * 4 : 0);  // float[RB * N]: v* rows i0 ..
+  static constexpr int bytes = svs + (kStar ? RB * N * 4 : 0);
+};
+
+template <int N, int W>
+struct PoissonLds {
+  static constexpr int buf = 0;               // float2[N * W]
+  static constexpr int twn = buf + N * W * 8;  // float2[N]
+  static constexpr int bytes = twn + N * 8;
+};
+
+template <int N, int RB, int MODE>
+struct RowsProjectLds {
+  static constexpr int H = N / 2;
+  static constexpr bool kFft = MODE != 2;
+  static constexpr int qb = 0;                          // float2[(RB + 1) * H]
+  static constexpr int twn = qb + (RB + 1) * H * 8;     // float2[N]
+  static constexpr int twh = twn + (kFft ? N * 8 : 0);  // float2[H]
+  static constexpr int bytes = twh + (kFft ? H * 8 : 0);
+};
+
+template <typename T>
+__device__ __forceinline__ T* lds_at(char* smem, int offset) {
+  return reinterpret_cast<T*>(smem + offset);
+}
+
+// Rows [i0, i0 + RB) of one instance (b already points at the instance).
 // MODE 0: full step (write u*, v*, spectrum of div u*).
 // MODE 1: projection only (spectrum of div u for the given u, v).
 // MODE 2: explicit terms only (write k_u, k_v to us, vs).
 // MODE 3: spectrum of the field b.u itself (spectral filtering).
+// MODE 4: CG right-hand side (write div u* to b.rhs, no FFT).
 template <int N, int RB, int MODE>
-__global__ __launch_bounds__(kThreads) void fvm_rows_forward(FvmParams p, FvmBuffers b) {
+__device__ __forceinline__ void rows_forward_body(const FvmParams& p, const FvmBuffers& b, int i0,
+                                                  int tid, char* smem) {
   constexpr int H = N / 2;
-  constexpr int WR = RB + 5;  // staged rows i0-3 .. i0+RB+1
-  __shared__ float su[WR * N];
-  __shared__ float sv[WR * N];
-  __shared__ float sus[(RB + 1) * N];  // u* rows i0-1 .. i0+RB-1
-  __shared__ float svs[RB * N];        // v* rows i0 .. i0+RB-1
-  __shared__ float2 zb[RB * H];        // div rows, then their FFTs
-  __shared__ float2 twn[N];
-  __shared__ float2 twh[H];
-  const int tid = threadIdx.x;
-  b = fvm_instance(b, N);
-  const int i0 = blockIdx.x * RB;
+  constexpr int WR = RB + 5;
+  using L = RowsForwardLds<N, RB, MODE>;
+  float* su = lds_at<float>(smem, L::su);
+  float* sv = lds_at<float>(smem, L::sv);
+  float* sus = lds_at<float>(smem, L::sus);
+  float* svs = lds_at<float>(smem, L::svs);
+  float2* zb = lds_at<float2>(smem, L::zb);
+  float2* twn = lds_at<float2>(smem, L::twn);
+  float2* twh = lds_at<float2>(smem, L::twh);
 
   if constexpr (MODE == 3) {
     float* zf3 = reinterpret_cast<float*>(zb);
@@ -242,13 +288,13 @@ __global__ __launch_bounds__(kThreads) void fvm_rows_forward(FvmParams p, FvmBuf
   } else {
     stage_rows<N, WR>(su, sv, b.u, b.v, i0 - 3, tid);
   }
-  if constexpr (MODE != 2) {
+  if constexpr (L::kFft) {
     load_twiddles(twn, b.tw, N, 1, tid, kThreads);
     load_twiddles(twh, b.tw, H, 2, tid, kThreads);
   }
   __syncthreads();
 
-  if constexpr (MODE == 0 || MODE == 2 || MODE == 4) {
+  if constexpr (L::kWin) {
     Window<N> w{su, sv};
     // rows i0-1 .. i0+RB-1 -> local rows 2 .. RB+2 of the window
     const int first = (MODE == 2) ? 1 : 0;
@@ -297,16 +343,22 @@ __global__ __launch_bounds__(kThreads) void fvm_rows_forward(FvmParams p, FvmBuf
   }
 }
 
-// Column pass of the Poisson solve on a tile of W packed-spectrum columns.
+template <int N, int RB, int MODE>
+__global__ __launch_bounds__(kThreads) void fvm_rows_forward(FvmParams p, FvmBuffers b) {
+  __shared__ __align__(16) char smem[RowsForwardLds<N, RB, MODE>::bytes];
+  rows_forward_body<N, RB, MODE>(p, fvm_instance(b, N), blockIdx.x * RB, threadIdx.x, smem);
+}
+
+// Column pass of the Poisson solve on the tile of W packed-spectrum columns
+// starting at c0 (b already points at the instance).
 template <int N, int W>
-__global__ __launch_bounds__(kThreads) void fvm_poisson_cols(FvmBuffers b, float scale) {
+__device__ __forceinline__ void poisson_cols_body(const FvmBuffers& b, float scale, int c0, int tid,
+                                                  char* smem) {
   constexpr int H = N / 2;
   constexpr int PER = (N * W + kThreads - 1) / kThreads;
-  __shared__ float2 buf[N * W];
-  __shared__ float2 twn[N];
-  const int tid = threadIdx.x;
-  b = fvm_instance(b, N);
-  const int c0 = blockIdx.x * W;
+  using L = PoissonLds<N, W>;
+  float2* buf = lds_at<float2>(smem, L::buf);
+  float2* twn = lds_at<float2>(smem, L::twn);
   for (int e = tid; e < N * W; e += kThreads) {
     const int pp = e / W, c = e % W;
     buf[e] = b.spec[pp * H + c0 + c];
@@ -350,19 +402,25 @@ __global__ __launch_bounds__(kThreads) void fvm_poisson_cols(FvmBuffers b, float
   }
 }
 
-// Inverse row transforms of q and the velocity correction.
+template <int N, int W>
+__global__ __launch_bounds__(kThreads) void fvm_poisson_cols(FvmBuffers b, float scale) {
+  __shared__ __align__(16) char smem[PoissonLds<N, W>::bytes];
+  poisson_cols_body<N, W>(fvm_instance(b, N), scale, blockIdx.x * W, threadIdx.x, smem);
+}
+
+// Inverse row transforms of q and the velocity correction for rows
+// [i0, i0 + RB) (b already points at the instance).
 // MODE 0: velocity correction; MODE 1: write q itself to u_out;
 // MODE 2: velocity correction from the real q of the CG solve (no FFT).
 template <int N, int RB, int MODE>
-__global__ __launch_bounds__(kThreads) void fvm_rows_project(FvmParams p, FvmBuffers b) {
+__device__ __forceinline__ void rows_project_body(const FvmParams& p, const FvmBuffers& b, int i0,
+                                                  int tid, char* smem) {
   constexpr int H = N / 2;
   constexpr int T = RB + 1;
-  __shared__ float2 qb[T * H];
-  __shared__ float2 twn[N];
-  __shared__ float2 twh[H];
-  const int tid = threadIdx.x;
-  b = fvm_instance(b, N);
-  const int i0 = blockIdx.x * RB;
+  using L = RowsProjectLds<N, RB, MODE>;
+  float2* qb = lds_at<float2>(smem, L::qb);
+  float2* twn = lds_at<float2>(smem, L::twn);
+  float2* twh = lds_at<float2>(smem, L::twh);
   if constexpr (MODE == 2) {
     float* qf = reinterpret_cast<float*>(qb);
     for (int e = tid; e < T * N; e += kThreads) qf[e] = b.q[((i0 + e / N) & (N - 1)) * N + e % N];
@@ -395,6 +453,12 @@ __global__ __launch_bounds__(kThreads) void fvm_rows_project(FvmParams p, FvmBuf
     b.u_out[gi * N + j] = b.us[gi * N + j] - gx;
     b.v_out[gi * N + j] = b.vs[gi * N + j] - gy;
   }
+}
+
+template <int N, int RB, int MODE>
+__global__ __launch_bounds__(kThreads) void fvm_rows_project(FvmParams p, FvmBuffers b) {
+  __shared__ __align__(16) char smem[RowsProjectLds<N, RB, MODE>::bytes];
+  rows_project_body<N, RB, MODE>(p, fvm_instance(b, N), blockIdx.x * RB, threadIdx.x, smem);
 }
 
 // curl_2d (finite_differences.py:186-193): forward differences, offset (1, 1).
@@ -527,10 +591,131 @@ template <int N> struct ProjOp {
   static int run(const FvmParams& p, const FvmBuffers& b, hipStream_t s) { return launch_projection<N>(p, b, s); }
 };
 
+// ---- mixed-size group step -------------------------------------------------
+// Each phase is one persistent launch: a grid of as many workgroups as the
+// device holds at once walks the phase's work list (instance, tile) with a
+// fixed stride, so one launch covers every instance of every grid size and
+// the launch arguments never depend on the list's contents (a captured graph
+// stays valid as instances come and go).  The host sorts the list largest
+// grid first, so the stride deals every workgroup a similar mix.  There is
+// no shared work counter: device-scope atomics on one address serialise
+// across the XCDs and cost more than they balance.  Each item runs the same
+// tile body as the per-size kernels, so an instance's arithmetic -- and its
+// result, bit for bit -- does not depend on what it is grouped with.
+template <int N, int PHASE>
+__device__ __forceinline__ void group_tile(const FvmGroupDesc& d, int tile, int tid, char* smem) {
+  constexpr int RB = Tiles<N>::RB, W = Tiles<N>::W;
+  if constexpr (PHASE == 0)
+    rows_forward_body<N, RB, 0>(d.p, d.b, tile * RB, tid, smem);
+  else if constexpr (PHASE == 1)
+    poisson_cols_body<N, W>(d.b, 1.0f, tile * W, tid, smem);
+  else
+    rows_project_body<N, RB, 0>(d.p, d.b, tile * RB, tid, smem);
+}
+
+template <int PHASE>
+__global__ __launch_bounds__(kThreads) void fvm_group_phase(FvmGroupQueue q) {
+  extern __shared__ float4 fvm_group_smem[];
+  char* smem = reinterpret_cast<char*>(fvm_group_smem);
+  const int tid = threadIdx.x;
+  const int total = q.n_items[PHASE];
+  for (int it = blockIdx.x; it < total; it += gridDim.x) {
+    const FvmGroupItem w = q.items[PHASE][it];
+    const FvmGroupDesc d = q.descs[w.desc];
+    // A fresh copy of the thread index per item: keeps the compiler from
+    // hoisting every per-thread address of every tile body out of this loop
+    // (which would hold them all in registers across items).
+    int t = tid;
+    asm volatile("" : "+v"(t));
+    switch (d.p.n) {
+      case 32: group_tile<32, PHASE>(d, w.tile, t, smem); break;
+      case 64: group_tile<64, PHASE>(d, w.tile, t, smem); break;
+      case 128: group_tile<128, PHASE>(d, w.tile, t, smem); break;
+      case 256: group_tile<256, PHASE>(d, w.tile, t, smem); break;
+      case 512: group_tile<512, PHASE>(d, w.tile, t, smem); break;
+      default: break;  // rejected on the host
+    }
+    __syncthreads();  // LDS is reused by the next item
+  }
+}
+
+template <int N>
+constexpr int group_lds_bytes(int phase) {
+  return phase == 0   ? RowsForwardLds<N, Tiles<N>::RB, 0>::bytes
+         : phase == 1 ? PoissonLds<N, Tiles<N>::W>::bytes
+                      : RowsProjectLds<N, Tiles<N>::RB, 0>::bytes;
+}
+
+int group_lds(int phase, int max_n) {
+  int bytes = 0;
+  if (max_n >= 32) bytes = std::max(bytes, group_lds_bytes<32>(phase));
+  if (max_n >= 64) bytes = std::max(bytes, group_lds_bytes<64>(phase));
+  if (max_n >= 128) bytes = std::max(bytes, group_lds_bytes<128>(phase));
+  if (max_n >= 256) bytes = std::max(bytes, group_lds_bytes<256>(phase));
+  if (max_n >= 512) bytes = std::max(bytes, group_lds_bytes<512>(phase));
+  return bytes;
+}
+
+template <int N>
+int group_tiles_n(int phase) {
+  return phase == 1 ? N / 2 / Tiles<N>::W : N / Tiles<N>::RB;
+}
+
+
 }  // namespace
 
 bool fvm_size_supported(int n) {
   return n == 32 || n == 64 || n == 128 || n == 256 || n == 512 || n == 1024;
+}
+
+bool fvm_group_size_supported(int n) { return fvm_size_supported(n) && n <= kFvmGroupMaxN; }
+
+int fvm_group_tiles(int n, int phase) {
+  switch (n) {
+    case 32: return group_tiles_n<32>(phase);
+    case 64: return group_tiles_n<64>(phase);
+    case 128: return group_tiles_n<128>(phase);
+    case 256: return group_tiles_n<256>(phase);
+    case 512: return group_tiles_n<512>(phase);
+    default: return 0;
+  }
+}
+
+// The grid is the number of workgroups of the phase that are resident at
+// once (occupancy for the LDS of the largest grid present, times the CU
+// count).  It does not depend on the work list.
+const void* group_kernel(int phase) {
+  switch (phase) {
+    case 0: return reinterpret_cast<const void*>(&fvm_group_phase<0>);
+    case 1: return reinterpret_cast<const void*>(&fvm_group_phase<1>);
+    default: return reinterpret_cast<const void*>(&fvm_group_phase<2>);
+  }
+}
+
+int fvm_group_grid(int phase, int max_n, int* grid) {
+  int dev = 0, cus = 0, per_cu = 0;
+  int rc;
+  if ((rc = check_hip(hipGetDevice(&dev), "hipGetDevice")) ||
+      (rc = check_hip(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev),
+                      "CU count")) ||
+      (rc = check_hip(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                          &per_cu, group_kernel(phase), kThreads, group_lds(phase, max_n)),
+                      "group occupancy")))
+    return rc;
+  *grid = std::max(1, per_cu) * std::max(1, cus);
+  return JCFD_OK;
+}
+
+int fvm_group_launch(int phase, int max_n, const FvmGroupQueue& q, int grid, hipStream_t s) {
+  if (!fvm_group_size_supported(max_n))
+    return fail(JCFD_ERR_UNSUPPORTED, "group grid size not supported: " + std::to_string(max_n));
+  const size_t lds = static_cast<size_t>(group_lds(phase, max_n));
+  switch (phase) {
+    case 0: hipLaunchKernelGGL(fvm_group_phase<0>, dim3(grid), dim3(kThreads), lds, s, q); break;
+    case 1: hipLaunchKernelGGL(fvm_group_phase<1>, dim3(grid), dim3(kThreads), lds, s, q); break;
+    default: hipLaunchKernelGGL(fvm_group_phase<2>, dim3(grid), dim3(kThreads), lds, s, q);
+  }
+  return check_hip(hipGetLastError(), "fvm group launch");
 }
 
 int fvm_launch_step(const FvmParams& p, const FvmBuffers& b, hipStream_t s) {

@@ -106,6 +106,31 @@ int fvm_launch_filter(const FvmParams& p, const float* x, const float* multiplie
                       float2* spec, const float2* tw, hipStream_t s);
 bool fvm_size_supported(int n);
 
+// Mixed-size group step (fvm.hip): one persistent launch per kernel class
+// (phase 0 rows-forward, 1 Poisson columns, 2 rows-project) advances every
+// instance of a device-resident work list, whatever its grid size.
+struct FvmGroupDesc {
+  FvmParams p;
+  FvmBuffers b;  // pointers already offset to the instance; u_out == u, v_out == v
+};
+struct FvmGroupItem {
+  int desc;  // index into the descriptor table
+  int tile;  // row block (phases 0, 2) or column tile (phase 1)
+};
+struct FvmGroupQueue {
+  const FvmGroupDesc* descs;
+  const FvmGroupItem* items[3];  // work list of each phase
+  const int* n_items;            // device int[3]
+};
+constexpr int kFvmGroupMaxN = 512;
+bool fvm_group_size_supported(int n);
+// Work items per instance of grid size n in `phase`.
+int fvm_group_tiles(int n, int phase);
+// Resident workgroups of `phase` on this device when the largest grid is max_n.
+int fvm_group_grid(int phase, int max_n, int* grid);
+// Launch one phase.
+int fvm_group_launch(int phase, int max_n, const FvmGroupQueue& q, int grid, hipStream_t s);
+
 // Device-side parameters of a pseudospectral stage (spectral.hip).  The
 // wavenumber tables and multipliers follow the reference's fp32 values;
 // beta, g = gamma dt and mu are the stage's CN-RK4 scalars.

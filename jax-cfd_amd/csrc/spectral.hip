@@ -46,13 +46,12 @@ struct NsMult {
   float psi;  // -1 / laplace'  (laplace' = (2 pi i)^2 (kx^2 + ky^2), [0,0] := 1)
 };
 
-__device__ __forceinline__ NsMult ns_mult(const NsParams& q, int p, int c) {
-  const float kx = q.kx[p], ky = q.ky[c];
+__device__ __forceinline__ NsMult ns_mult(const NsParams& q, float kx, float ky, bool origin) {
   NsMult m;
   m.ax = q.two_pi * kx;
   m.ay = q.two_pi * ky;
   const float s = kx * kx + ky * ky;
-  const float lap = (p == 0 && c == 0) ? 1.0f : q.neg_four_pi_sq * s;
+  const float lap = origin ? 1.0f : q.neg_four_pi_sq * s;
   m.psi = -1.0f / lap;
   return m;
 }
@@ -73,29 +72,54 @@ __device__ __forceinline__ float2 ns_spectrum(int m, const NsMult& k, float2 w) 
 
 template <int N, int W>
 struct ColShared {
-  float2 wt[N * W];  // vorticity tile [p][c]
-  float2 wn[N];      // vorticity column N/2 (tile 0 only)
-  float2 buf[N * W]; // work tile
+  float2 buf[N * W];  // work tile [p][c]
+  float2 w0[N];       // tile 0: vorticity column 0
+  float2 wn[N];       // tile 0: vorticity column N/2
   float2 tw[N];
+  float kx[N];        // fftfreq
+  float ky[N / 2 + 1];  // rfftfreq
 };
 
-// Build spectrum m of the tile into sm.buf (scaled by 1/n^2), symmetrising and
-// packing the DC/Nyquist pair into local column 0 of tile 0.
+// Elements of a column tile are owned by threads as e = tid + k * kThreads;
+// each thread keeps its vorticity values in registers across the four spectra
+// (only tile 0's DC/Nyquist columns, which pair rows p and -p, also go through
+// LDS).
 template <int N, int W>
-__device__ __forceinline__ void build_spectrum(ColShared<N, W>& sm, const NsParams& q, int m,
-                                               int c0, int tid) {
+struct WTile {
+  static constexpr int PER = (N * W + kThreads - 1) / kThreads;
+  float2 w[PER];
+};
+
+// Hardware hands workgroups to the 8 XCDs round-robin; give each XCD a
+// contiguous run of tiles so neighbouring tiles (which share cache lines)
+// share an L2.
+__device__ __forceinline__ int xcd_tile(int b, int nb) {
+  return (nb % 8 != 0) ? b : (b % 8) * (nb / 8) + b / 8;
+}
+
+// Build spectrum m of the tile into sm.buf (scaled by 1/n^2), symmetrising and
+// packing the DC/Nyquist pair into local column 0 of tile 0.  The multipliers
+// are recomputed per spectrum from the LDS wavenumber tables (reads after a
+// barrier, which the compiler cannot hoist out of the spectrum loop; keeping
+// them live across it would cost 3 VGPRs per element).
+template <int N, int W>
+__device__ __forceinline__ void build_spectrum(ColShared<N, W>& sm, const WTile<N, W>& wt,
+                                               const NsParams& q, int m, int c0, int tid) {
   constexpr int H = N / 2;
-  for (int e = tid; e < N * W; e += kThreads) {
+#pragma unroll
+  for (int k = 0; k < WTile<N, W>::PER; ++k) {
+    const int e = tid + k * kThreads;
+    if (e >= N * W) break;
     const int p = e / W, c = e % W, cg = c0 + c;
     float2 out;
     if (cg != 0) {
-      out = ns_spectrum(m, ns_mult(q, p, cg), sm.wt[e]);
+      out = ns_spectrum(m, ns_mult(q, sm.kx[p], sm.ky[cg], false), wt.w[k]);
     } else {
       const int pm = (N - p) & (N - 1);
-      const float2 a0 = ns_spectrum(m, ns_mult(q, p, 0), sm.wt[p * W]);
-      const float2 b0 = ns_spectrum(m, ns_mult(q, pm, 0), sm.wt[pm * W]);
-      const float2 ah = ns_spectrum(m, ns_mult(q, p, H), sm.wn[p]);
-      const float2 bh = ns_spectrum(m, ns_mult(q, pm, H), sm.wn[pm]);
+      const float2 a0 = ns_spectrum(m, ns_mult(q, sm.kx[p], sm.ky[0], p == 0), sm.w0[p]);
+      const float2 b0 = ns_spectrum(m, ns_mult(q, sm.kx[pm], sm.ky[0], pm == 0), sm.w0[pm]);
+      const float2 ah = ns_spectrum(m, ns_mult(q, sm.kx[p], sm.ky[H], false), sm.wn[p]);
+      const float2 bh = ns_spectrum(m, ns_mult(q, sm.kx[pm], sm.ky[H], false), sm.wn[pm]);
       const float2 s0 = make_float2(0.5f * (a0.x + b0.x), 0.5f * (a0.y - b0.y));
       const float2 sh = make_float2(0.5f * (ah.x + bh.x), 0.5f * (ah.y - bh.y));
       out = make_float2(s0.x - sh.y, s0.y + sh.x);  // s0 + i sh
@@ -104,15 +128,18 @@ __device__ __forceinline__ void build_spectrum(ColShared<N, W>& sm, const NsPara
   }
 }
 
-// Inverse column transforms of the tile in sm.wt/sm.wn -> G[0..NSPEC): the
-// four derivative spectra (NSPEC = 4) or the field itself (NSPEC = 1).
+// Inverse column transforms of the tile -> G[0..NSPEC): the four derivative
+// spectra (NSPEC = 4) or the field itself (NSPEC = 1).  sm.w0 / sm.wn must be
+// visible (barrier) before the call.
 template <int N, int W, int NSPEC = 4>
-__device__ __forceinline__ void cols_inverse_body(ColShared<N, W>& sm, const NsParams& q,
-                                                  float2* __restrict__ g, int c0, int tid) {
+__device__ __forceinline__ void cols_inverse_body(ColShared<N, W>& sm, const WTile<N, W>& wt,
+                                                  const NsParams& q, float2* __restrict__ g,
+                                                  int c0, int tid) {
   constexpr int H = N / 2;
   const size_t gstride = static_cast<size_t>(N) * H;
+#pragma unroll 1
   for (int m = 0; m < NSPEC; ++m) {
-    build_spectrum<N, W>(sm, q, NSPEC == 1 ? 4 : m, c0, tid);
+    build_spectrum<N, W>(sm, wt, q, NSPEC == 1 ? 4 : m, c0, tid);
     __syncthreads();
     lds_fft<N, W, kThreads, W, 1, true, true>(sm.buf, sm.tw, tid);
     float2* gm = g + m * gstride;
@@ -125,30 +152,44 @@ __device__ __forceinline__ void cols_inverse_body(ColShared<N, W>& sm, const NsP
 }
 
 template <int N, int W>
-__device__ __forceinline__ void load_w_tile(ColShared<N, W>& sm, const float2* __restrict__ w,
-                                            int c0, int tid) {
+__device__ __forceinline__ void load_w_tile(ColShared<N, W>& sm, WTile<N, W>& wt, const NsParams& q,
+                                            const float2* __restrict__ w, int c0, int tid) {
   constexpr int H1 = N / 2 + 1;
-  for (int e = tid; e < N * W; e += kThreads) {
+#pragma unroll
+  for (int k = 0; k < WTile<N, W>::PER; ++k) {
+    const int e = tid + k * kThreads;
+    if (e >= N * W) break;
     const int p = e / W, c = e % W;
-    sm.wt[e] = w[p * H1 + c0 + c];
+    wt.w[k] = w[p * H1 + c0 + c];
   }
   if (c0 == 0)
-    for (int p = tid; p < N; p += kThreads) sm.wn[p] = w[p * H1 + N / 2];
+    for (int p = tid; p < N; p += kThreads) {
+      sm.w0[p] = w[p * H1];
+      sm.wn[p] = w[p * H1 + N / 2];
+    }
+}
+
+template <int N, int W>
+__device__ __forceinline__ void load_wavenumbers(ColShared<N, W>& sm, const NsParams& q, int tid) {
+  for (int p = tid; p < N; p += kThreads) sm.kx[p] = q.kx[p];
+  for (int c = tid; c <= N / 2; c += kThreads) sm.ky[c] = q.ky[c];
 }
 
 template <int N, int W, int NSPEC>
 __global__ __launch_bounds__(kThreads) void ns_cols_inverse(NsParams q, const float2* __restrict__ w,
                                                             float2* __restrict__ g) {
   __shared__ ColShared<N, W> sm;
+  WTile<N, W> wt;
   const int tid = threadIdx.x;
-  const int c0 = blockIdx.x * W;
+  const int c0 = xcd_tile(blockIdx.x, gridDim.x) * W;
   const size_t inst = q.slots ? static_cast<size_t>(q.slots[blockIdx.y]) : blockIdx.y;
   w += inst * static_cast<size_t>(N) * (N / 2 + 1);
   g += inst * NSPEC * static_cast<size_t>(N) * (N / 2);
-  load_w_tile<N, W>(sm, w, c0, tid);
+  load_w_tile<N, W>(sm, wt, q, w, c0, tid);
   load_twiddles(sm.tw, q.tw, N, 1, tid, kThreads);
+  load_wavenumbers(sm, q, tid);
   __syncthreads();
-  cols_inverse_body<N, W, NSPEC>(sm, q, g, c0, tid);
+  cols_inverse_body<N, W, NSPEC>(sm, wt, q, g, c0, tid);
 }
 
 // Packed real row transforms of a batch of real (N, N) fields.
@@ -260,7 +301,7 @@ __global__ __launch_bounds__(kThreads) void ns_cols_forward(NsParams q, const fl
   constexpr int PER = (N * W + kThreads - 1) / kThreads;
   __shared__ ColShared<N, W> sm;
   const int tid = threadIdx.x;
-  const int c0 = blockIdx.x * W;
+  const int c0 = xcd_tile(blockIdx.x, gridDim.x) * W;
   const size_t inst = q.slots ? static_cast<size_t>(q.slots[blockIdx.y]) : blockIdx.y;
   const size_t ext = static_cast<size_t>(N) * H1;
   pin += inst * static_cast<size_t>(N) * H;
@@ -274,6 +315,7 @@ __global__ __launch_bounds__(kThreads) void ns_cols_forward(NsParams q, const fl
     sm.buf[e] = pin[p * H + c0 + c];
   }
   load_twiddles(sm.tw, q.tw, N, 1, tid, kThreads);
+  if constexpr (MODE == 2) load_wavenumbers(sm, q, tid);
   __syncthreads();
   lds_fft<N, W, kThreads, W, 1, true, false>(sm.buf, sm.tw, tid);
 
@@ -347,6 +389,7 @@ __global__ __launch_bounds__(kThreads) void ns_cols_forward(NsParams q, const fl
       u_out[idx] = un;
       return un;
     };
+    [[maybe_unused]] WTile<N, W> wt;
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
       const int e = tid + k * kThreads;
@@ -354,8 +397,11 @@ __global__ __launch_bounds__(kThreads) void ns_cols_forward(NsParams q, const fl
         const int p = e / W, c = e % W, cg = c0 + c;
         const float2 un = update(p, cg, f[k]);
         if constexpr (MODE == 2) {
-          sm.wt[e] = un;
-          if (cg == 0) sm.wn[p] = update(p, H, fh[k]);
+          wt.w[k] = un;
+          if (cg == 0) {
+            sm.w0[p] = un;
+            sm.wn[p] = update(p, H, fh[k]);
+          }
         } else {
           if (cg == 0) update(p, H, fh[k]);
         }
@@ -363,7 +409,7 @@ __global__ __launch_bounds__(kThreads) void ns_cols_forward(NsParams q, const fl
     }
     if constexpr (MODE == 2) {
       __syncthreads();
-      cols_inverse_body<N, W>(sm, q, g, c0, tid);
+      cols_inverse_body<N, W>(sm, wt, q, g, c0, tid);
     }
   }
 }

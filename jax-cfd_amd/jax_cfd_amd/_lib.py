@@ -104,7 +104,22 @@ _SIGNATURES = {
                                              ctypes.POINTER(ctypes.c_double), _P]),
     "jcfd_fvm_kernel_bytes": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int64)]),
     "jcfd_fvm_spectral_filter": (ctypes.c_int, [_P, _P, _P, _P, _P]),
+    "jcfd_fvm_group_create": (ctypes.c_int, [ctypes.POINTER(_P)]),
+    "jcfd_fvm_group_destroy": (ctypes.c_int, [_P]),
+    "jcfd_fvm_group_set_graphs": (ctypes.c_int, [_P, ctypes.c_int32]),
+    "jcfd_fvm_group_step": (ctypes.c_int, [_P, _P, ctypes.c_int32, ctypes.c_int64, _P]),
+    "jcfd_fvm_group_time_kernels": (ctypes.c_int, [_P, ctypes.c_int32,
+                                                   ctypes.POINTER(ctypes.c_double), _P]),
+    "jcfd_fvm_group_launch_shape": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int32),
+                                                   ctypes.POINTER(ctypes.c_int32),
+                                                   ctypes.POINTER(ctypes.c_int64)]),
 }
+
+
+class FvmGroupMember(ctypes.Structure):
+  """jcfd_fvm_group_member."""
+  _fields_ = [("plan", _P), ("u", _P), ("v", _P),
+              ("slots", ctypes.POINTER(ctypes.c_int32)), ("n_slots", ctypes.c_int32)]
 
 _lib = None
 _lock = threading.Lock()

@@ -213,6 +213,76 @@ class FvmPlan:
     return out
 
 
+class FvmGroup:
+  """Mixed-size group stepping (jcfd_fvm_group_*): one persistent launch per
+  kernel class advances every listed slot of every member plan, whatever its
+  grid size, each with exactly the arithmetic of a lone step."""
+
+  def __init__(self, device: Optional[torch.device] = None):
+    lib = _lib.load()
+    self.device = device if device is not None else torch.device(
+        'cuda', torch.cuda.current_device())
+    handle = ctypes.c_void_p()
+    with torch.cuda.device(self.device):
+      _lib.check(lib.jcfd_fvm_group_create(ctypes.byref(handle)), 'jcfd_fvm_group_create')
+    self._handle = handle
+    self._lib = lib
+
+  def set_graphs(self, enabled: bool):
+    _lib.check(self._lib.jcfd_fvm_group_set_graphs(self._handle, int(bool(enabled))),
+               'jcfd_fvm_group_set_graphs')
+
+  def close(self):
+    if getattr(self, '_handle', None) is not None and self._handle.value:
+      self._lib.jcfd_fvm_group_destroy(self._handle)
+      self._handle = None
+
+  def __del__(self):
+    try:
+      self.close()
+    except Exception:  # interpreter shutdown
+      pass
+
+  def step(self, members, nsteps: int):
+    """members: iterable of (plan, u, v, slots) with u, v the plan's
+    (batch, n, n) float32 device fields and slots a sequence of ints; every
+    listed slot advances nsteps steps in place."""
+    members = list(members)
+    arr = (_lib.FvmGroupMember * max(1, len(members)))()
+    keep = []
+    with _lib.dlpack_args() as dl:
+      for k, (plan, u, v, slots) in enumerate(members):
+        idx = np.ascontiguousarray(np.asarray(slots, dtype=np.int32).reshape(-1))
+        keep.append(idx)
+        arr[k].plan = plan._handle
+        arr[k].u = dl(u)
+        arr[k].v = dl(v)
+        arr[k].slots = idx.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+        arr[k].n_slots = int(idx.size)
+      rc = self._lib.jcfd_fvm_group_step(self._handle, ctypes.cast(arr, ctypes.c_void_p),
+                                         len(members), int(nsteps),
+                                         _lib.stream_handle(self.device))
+    _lib.check(rc, 'jcfd_fvm_group_step')
+
+  def time_kernels(self, reps: int = 20):
+    """Mean ms per launch of each phase over `reps` further steps of the last
+    member set (their states advance)."""
+    ms = (ctypes.c_double * 3)()
+    rc = self._lib.jcfd_fvm_group_time_kernels(self._handle, int(reps), ms,
+                                               _lib.stream_handle(self.device))
+    _lib.check(rc, 'jcfd_fvm_group_time_kernels')
+    return dict(zip(FvmPlan.KERNELS, list(ms)))
+
+  def launch_shape(self):
+    """(grid, items, algorithmic bytes) per phase of the last step."""
+    grid, items = (ctypes.c_int32 * 3)(), (ctypes.c_int32 * 3)()
+    nbytes = (ctypes.c_int64 * 3)()
+    _lib.check(self._lib.jcfd_fvm_group_launch_shape(self._handle, grid, items, nbytes),
+               'jcfd_fvm_group_launch_shape')
+    return {k: {'grid': int(g), 'items': int(i), 'bytes': int(b)}
+            for k, g, i, b in zip(FvmPlan.KERNELS, grid, items, nbytes)}
+
+
 @functools.lru_cache(maxsize=64)
 def _cached_plan(spec: FvmSpec, device_index: int) -> FvmPlan:
   return FvmPlan(spec, torch.device('cuda', device_index))
