@@ -115,6 +115,21 @@ int64_t jcfd_fvm_plan_device_bytes(const jcfd_fvm_plan* plan);
  * spectral plans); 0 launches every kernel individually.  Same results. */
 int jcfd_fvm_plan_set_graphs(jcfd_fvm_plan* plan, int32_t enabled);
 
+/* Tile shapes and FFT radix schedule of the three step kernels.  No
+ * counterpart in the reference (XLA picks its own kernels); set by the
+ * project's autotuner (jax_cfd_amd.autotune), which times the menu on the
+ * device.  Tile shapes never change results; radix 4 instead of 8 changes
+ * the FFT rounding (results stay within fp32 round-off of each other). */
+typedef struct {
+  int32_t rows_forward; /* rows per workgroup of the stencil kernel: 1, 2, 4, 8 */
+  int32_t poisson_cols; /* spectrum columns per workgroup: 1, 2, 4, 8, 16 */
+  int32_t rows_project; /* rows per workgroup of the projection kernel: 1, 2, 4, 8 */
+  int32_t radix;        /* largest FFT radix: 8 or 4 */
+} jcfd_fvm_tiles;
+int jcfd_fvm_plan_get_tiles(const jcfd_fvm_plan* plan, jcfd_fvm_tiles* tiles);
+/* Rejects (JCFD_ERR_UNSUPPORTED) a shape that is not built for the plan's n. */
+int jcfd_fvm_plan_set_tiles(jcfd_fvm_plan* plan, const jcfd_fvm_tiles* tiles);
+
 /* nsteps applications of step_fn (time_stepping.py:88-104 forward Euler +
  * pressure.py:181-198 projection), i.e. funcutils.repeated(step_fn, nsteps)
  * (funcutils.py:82-88).  u_out/v_out may alias u/v. */

@@ -22,7 +22,9 @@ extern "C" {
 #define JCFD_FVM_KERNEL_ROWS_PROJECT 2 /* inverse row FFT + correction */
 #define JCFD_FVM_KERNEL_COUNT 3
 
-/* ms_per_launch[k] = mean duration of kernel class k over `reps` launches. */
+/* ms_per_launch[k] = mean duration of kernel class k over `reps` launches
+ * with the plan's tile shapes; u, v are the plan's (batch, n, n) fields (or
+ * (n, n) for one instance) and every instance is processed. */
 int jcfd_fvm_time_kernels(jcfd_fvm_plan* plan, const jcfd_tensor* u, const jcfd_tensor* v,
                           int32_t reps, double* ms_per_launch, void* stream);
 

@@ -24,17 +24,34 @@ FLAGS = [
 
 
 def build(verbose: bool = False) -> str:
-  srcs = [os.path.join(CSRC, s) for s in SOURCES]
+  """Compile every source to an object in parallel, then link the library."""
+  objdir = os.path.join(HERE, "_build")
+  os.makedirs(objdir, exist_ok=True)
+  jobs = []
+  for src in SOURCES:
+    obj = os.path.join(objdir, src.replace(".hip", ".o"))
+    cmd = [HIPCC, *FLAGS, "-c", "-I", INCLUDE, "-I", CSRC, "-o", obj, os.path.join(CSRC, src)]
+    cmd.remove("-shared")
+    if verbose:
+      print(" ".join(cmd), flush=True)
+    jobs.append((obj, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE,
+                                       text=True)))
+  failed = False
+  for obj, proc in jobs:
+    out, err = proc.communicate()
+    if proc.returncode != 0:
+      sys.stderr.write(out + err)
+      failed = True
+    elif verbose and err.strip():
+      sys.stderr.write(err)
+  if failed:
+    raise RuntimeError("hipcc failed")
   tmp = OUT + ".tmp"
-  cmd = [HIPCC, *FLAGS, "-I", INCLUDE, "-I", CSRC, "-o", tmp, *srcs]
-  if verbose:
-    print(" ".join(cmd), flush=True)
-  res = subprocess.run(cmd, capture_output=True, text=True)
+  link = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *[o for o, _ in jobs]]
+  res = subprocess.run(link, capture_output=True, text=True)
   if res.returncode != 0:
     sys.stderr.write(res.stdout + res.stderr)
-    raise RuntimeError(f"hipcc failed with exit code {res.returncode}")
-  if verbose and res.stderr.strip():
-    sys.stderr.write(res.stderr)
+    raise RuntimeError(f"link failed with exit code {res.returncode}")
   os.replace(tmp, OUT)
   return OUT
 

@@ -73,7 +73,7 @@ struct Slot {
 };
 
 // Input slots of the four flux families relative to the flux position (a, b),
-// exactly as fvm.hip's explicit_terms_at reads them.
+// exactly as fvm.hip's advect_van_leer face fluxes (face_fx_u .. face_fy_v) read them.
 __constant__ Slot kSlots[4][8] = {
     // Fx_u(a, b): U = .5 u[a-1] + .5 u[a]; c_pos u[a-1], c_neg u[a]; limiter u[a-2..a+1]
     {{-1, 0, 0}, {0, 0, 0}, {-1, 0, 0}, {0, 0, 0}, {-2, 0, 0}, {-1, 0, 0}, {0, 0, 0}, {1, 0, 0}},

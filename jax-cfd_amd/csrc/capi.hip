@@ -437,6 +437,7 @@ int jcfd_fvm_plan_create(const jcfd_fvm_config* config, jcfd_fvm_plan** plan) {
   q.has_forcing = c.has_forcing ? 1 : 0;
   q.lin = static_cast<float>(c.linear_coefficient);
   q.convect = c.convect;
+  (void)fvm_default_tiles(&q);
 
   p->batch = c.batch > 1 ? c.batch : 1;
   p->graphs.enabled = p->batch > 1;
@@ -491,6 +492,34 @@ int32_t jcfd_fvm_cg_iterations(const jcfd_fvm_plan* p) { return p ? p->cg_last_i
 int jcfd_fvm_plan_set_graphs(jcfd_fvm_plan* p, int32_t enabled) {
   if (p == nullptr) return fail(JCFD_ERR_INVALID_ARGUMENT, "null plan");
   p->graphs.enabled = enabled != 0;
+  return JCFD_OK;
+}
+
+int jcfd_fvm_plan_get_tiles(const jcfd_fvm_plan* p, jcfd_fvm_tiles* tiles) {
+  if (p == nullptr || tiles == nullptr) return fail(JCFD_ERR_INVALID_ARGUMENT, "null argument");
+  tiles->rows_forward = p->params.rb_fwd;
+  tiles->poisson_cols = p->params.w_cols;
+  tiles->rows_project = p->params.rb_proj;
+  tiles->radix = p->params.radix;
+  return JCFD_OK;
+}
+
+int jcfd_fvm_plan_set_tiles(jcfd_fvm_plan* p, const jcfd_fvm_tiles* tiles) {
+  if (p == nullptr || tiles == nullptr) return fail(JCFD_ERR_INVALID_ARGUMENT, "null argument");
+  const int n = p->params.n;
+  if (tiles->radix != 8 && tiles->radix != 4)
+    return fail(JCFD_ERR_UNSUPPORTED, "radix schedule must be 8 or 4");
+  const int32_t t[3] = {tiles->rows_forward, tiles->poisson_cols, tiles->rows_project};
+  const char* names[3] = {"rows_forward", "poisson_cols", "rows_project"};
+  for (int k = 0; k < 3; ++k)
+    if (!fvm_tile_supported(n, k, t[k]))
+      return fail(JCFD_ERR_UNSUPPORTED, std::string(names[k]) + " tile " + std::to_string(t[k]) +
+                                            " not built for n = " + std::to_string(n));
+  p->params.rb_fwd = t[0];
+  p->params.w_cols = t[1];
+  p->params.rb_proj = t[2];
+  p->params.radix = tiles->radix;
+  p->graphs.release();  // captured launches carry the old shapes
   return JCFD_OK;
 }
 
@@ -619,11 +648,12 @@ int jcfd_fvm_time_kernels(jcfd_fvm_plan* p, const jcfd_tensor* u, const jcfd_ten
   const int n = p->params.n;
   void *pu, *pv;
   int rc;
-  if ((rc = check_tensor(u, "u", kDLFloat, 32, n, n, p->device, &pu)) ||
-      (rc = check_tensor(v, "v", kDLFloat, 32, n, n, p->device, &pv)))
+  if ((rc = check_batched(u, "u", kDLFloat, 32, p->batch, n, n, p->device, &pu)) ||
+      (rc = check_batched(v, "v", kDLFloat, 32, p->batch, n, n, p->device, &pv)))
     return rc;
   hipStream_t s = static_cast<hipStream_t>(stream);
   FvmBuffers b{};
+  b.count = p->batch;
   b.u = static_cast<const float*>(pu);
   b.v = static_cast<const float*>(pv);
   b.us = p->us;
@@ -896,6 +926,8 @@ int jcfd_fvm_group_step(jcfd_fvm_group* g, const jcfd_fvm_group_member* members,
       return fail(JCFD_ERR_INVALID_ARGUMENT, who + ": plan on another device than the group");
     if (p->cfg.pressure != JCFD_PRESSURE_FAST_DIAG)
       return fail(JCFD_ERR_UNSUPPORTED, who + ": groups step fast-diagonal plans only");
+    if (p->params.radix != 8)
+      return fail(JCFD_ERR_UNSUPPORTED, who + ": groups run the radix-8 FFT schedule only");
     const int n = p->params.n;
     if (!fvm_group_size_supported(n))
       return fail(JCFD_ERR_UNSUPPORTED, who + ": grid size " + std::to_string(n) +

@@ -92,113 +92,91 @@ struct Window {
   __device__ __forceinline__ float V(int lr, int j) const { return sv[lr * N + (j & (N - 1))]; }
 };
 
-// Explicit terms k_u, k_v at (local row lr, column j); see advection.py:210-342
-// for the face fluxes (the limiter stencil of the cross-direction faces is
-// read one cell over along the component's staggered axis, as the reference's
-// loop variable at advection.py:291 makes it), equations.py:102-110 for the
-// sum, finite_differences.py:127-133 for the Laplacian.
+// Face fluxes of the convection term, each evaluated once per face.  With
+// the indexing below a cell at local window row lr, column j reads
+//   adv_u = -((FX(lr + 1, j) - FX(lr, j)) / dx + (FY(lr, j) - FY(lr, j - 1)) / dy)
+//   adv_v = -((GX(lr + 1, j) - GX(lr, j)) / dx + (GY(lr, j) - GY(lr, j - 1)) / dy)
+// for u (offset (1, .5)) and v (offset (.5, 1)).  advect_van_leer
+// (advection.py:210-342): upwind flux plus the limited correction, whose
+// stencil for the cross-direction faces is read one cell over along the
+// component's staggered axis, as the reference's loop variable at
+// advection.py:291 makes it.  advect_van_leer_using_limiters
+// (advection.py:387-395): advect_general with linear face velocities and
+// the TVD Lax-Wendroff / van_leer_limiter face value.  The operation order
+// of each face and of the differences is the reference's.
 template <int N>
-__device__ __forceinline__ void add_diffusion_forcing(const Window<N>& w, int lr, int j,
-                                                      const FvmParams& p, float fx_j, float& ku,
-                                                      float& kv);
-
-template <int N>
-__device__ __forceinline__ void vll_advection_at(const Window<N>& w, int lr, int j,
-                                                 const FvmParams& p, float& adv_u, float& adv_v) {
-  const float hx = p.dt_over_dx, hy = p.dt_over_dy;
-  // u (offset (1, .5)): x faces between rows a, a+1 with U = .5 u[a] + .5 u[a+1];
-  // y faces between columns b, b+1 with V = .5 v[i, b] + .5 v[i+1, b].
-  auto fx_u = [&](int a) {
-    const float Uf = 0.5f * w.U(a, j) + 0.5f * w.U(a + 1, j);
-    return vll_flux(w.U(a - 1, j), w.U(a, j), w.U(a + 1, j), w.U(a + 2, j), Uf, hx);
-  };
-  auto fy_u = [&](int b) {
-    const float Vf = 0.5f * w.V(lr, b) + 0.5f * w.V(lr + 1, b);
-    return vll_flux(w.U(lr, b - 1), w.U(lr, b), w.U(lr, b + 1), w.U(lr, b + 2), Vf, hy);
-  };
-  adv_u = -((fx_u(lr) - fx_u(lr - 1)) / p.dx + (fy_u(j) - fy_u(j - 1)) / p.dy);
-  // v (offset (.5, 1)): x faces with U = .5 u[a, j] + .5 u[a, j+1]; y faces
-  // with V = .5 v[i, b] + .5 v[i, b+1].
-  auto fx_v = [&](int a) {
-    const float Uf = 0.5f * w.U(a, j) + 0.5f * w.U(a, j + 1);
-    return vll_flux(w.V(a - 1, j), w.V(a, j), w.V(a + 1, j), w.V(a + 2, j), Uf, hx);
-  };
-  auto fy_v = [&](int b) {
-    const float Vf = 0.5f * w.V(lr, b) + 0.5f * w.V(lr, b + 1);
-    return vll_flux(w.V(lr, b - 1), w.V(lr, b), w.V(lr, b + 1), w.V(lr, b + 2), Vf, hy);
-  };
-  adv_v = -((fx_v(lr) - fx_v(lr - 1)) / p.dx + (fy_v(j) - fy_v(j - 1)) / p.dy);
-}
-
-template <int N>
-__device__ __forceinline__ void explicit_terms_at(const Window<N>& w, int lr, int j,
-                                                  const FvmParams& p, float fx_j, float& ku,
-                                                  float& kv) {
-  const float hx = p.dt_over_dx, hy = p.dt_over_dy;
+__device__ __forceinline__ float face_fx_u(const Window<N>& w, int a, int j, const FvmParams& p) {
+  const float h = p.dt_over_dx;
   if (p.convect == JCFD_CONVECT_VAN_LEER_LIMITERS) {
-    vll_advection_at<N>(w, lr, j, p, ku, kv);
-    add_diffusion_forcing<N>(w, lr, j, p, fx_j, ku, kv);
-    return;
+    const int c = a - 1;  // face between rows c and c + 1
+    const float Uf = 0.5f * w.U(c, j) + 0.5f * w.U(c + 1, j);
+    return vll_flux(w.U(c - 1, j), w.U(c, j), w.U(c + 1, j), w.U(c + 2, j), Uf, h);
   }
-  // ---- u component (offset (1, .5)) ----
-  // x faces at a = i and a = i + 1 (local rows lr, lr + 1)
-  auto fx_u = [&](int a) {
-    const float Uf = 0.5f * w.U(a - 1, j) + 0.5f * w.U(a, j);
-    return vl_flux(Uf, w.U(a - 1, j), w.U(a, j), w.U(a - 2, j), w.U(a - 1, j), w.U(a, j),
-                   w.U(a + 1, j), hx);
-  };
-  // y faces at (a, b) = (i + 1, j) and (i + 1, j - 1)
-  auto fy_u = [&](int a, int b) {
-    const float Vf = 0.5f * w.V(a - 1, b) + 0.5f * w.V(a, b);
-    return vl_flux(Vf, w.U(a - 1, b), w.U(a - 1, b + 1), w.U(a, b - 1), w.U(a, b), w.U(a, b + 1),
-                   w.U(a, b + 2), hy);
-  };
-  const float dfx_u = (fx_u(lr + 1) - fx_u(lr)) / p.dx;
-  const float dfy_u = (fy_u(lr + 1, j) - fy_u(lr + 1, j - 1)) / p.dy;
-  const float adv_u = -(dfx_u + dfy_u);
-
-  // ---- v component (offset (.5, 1)) ----
-  // x faces at (a, b) = (i, j + 1) and (i - 1, j + 1)
-  auto fx_v = [&](int a, int b) {
-    const float Uf = 0.5f * w.U(a, b - 1) + 0.5f * w.U(a, b);
-    return vl_flux(Uf, w.V(a, b - 1), w.V(a + 1, b - 1), w.V(a - 1, b), w.V(a, b), w.V(a + 1, b),
-                   w.V(a + 2, b), hx);
-  };
-  // y faces at b = j + 1 and b = j (row i)
-  auto fy_v = [&](int b) {
-    const float Vf = 0.5f * w.V(lr, b - 1) + 0.5f * w.V(lr, b);
-    return vl_flux(Vf, w.V(lr, b - 1), w.V(lr, b), w.V(lr, b - 2), w.V(lr, b - 1), w.V(lr, b),
-                   w.V(lr, b + 1), hy);
-  };
-  const float dfx_v = (fx_v(lr, j + 1) - fx_v(lr - 1, j + 1)) / p.dx;
-  const float dfy_v = (fy_v(j + 1) - fy_v(j)) / p.dy;
-  const float adv_v = -(dfx_v + dfy_v);
-
-  ku = adv_u;
-  kv = adv_v;
-  add_diffusion_forcing<N>(w, lr, j, p, fx_j, ku, kv);
+  const float Uf = 0.5f * w.U(a - 1, j) + 0.5f * w.U(a, j);
+  return vl_flux(Uf, w.U(a - 1, j), w.U(a, j), w.U(a - 2, j), w.U(a - 1, j), w.U(a, j), w.U(a + 1, j), h);
 }
 
-// k += nu/rho lap + forcing/rho, in the reference's order (equations.py:102-110).
 template <int N>
-__device__ __forceinline__ void add_diffusion_forcing(const Window<N>& w, int lr, int j,
-                                                      const FvmParams& p, float fx_j, float& ku,
-                                                      float& kv) {
+__device__ __forceinline__ float face_fy_u(const Window<N>& w, int lr, int b, const FvmParams& p) {
+  const float h = p.dt_over_dy;
+  if (p.convect == JCFD_CONVECT_VAN_LEER_LIMITERS) {
+    const float Vf = 0.5f * w.V(lr, b) + 0.5f * w.V(lr + 1, b);
+    return vll_flux(w.U(lr, b - 1), w.U(lr, b), w.U(lr, b + 1), w.U(lr, b + 2), Vf, h);
+  }
+  const int a = lr + 1;
+  const float Vf = 0.5f * w.V(a - 1, b) + 0.5f * w.V(a, b);
+  return vl_flux(Vf, w.U(a - 1, b), w.U(a - 1, b + 1), w.U(a, b - 1), w.U(a, b), w.U(a, b + 1),
+                 w.U(a, b + 2), h);
+}
+
+template <int N>
+__device__ __forceinline__ float face_fx_v(const Window<N>& w, int a, int j, const FvmParams& p) {
+  const float h = p.dt_over_dx;
+  const int c = a - 1;
+  if (p.convect == JCFD_CONVECT_VAN_LEER_LIMITERS) {
+    const float Uf = 0.5f * w.U(c, j) + 0.5f * w.U(c, j + 1);
+    return vll_flux(w.V(c - 1, j), w.V(c, j), w.V(c + 1, j), w.V(c + 2, j), Uf, h);
+  }
+  const int b = j + 1;
+  const float Uf = 0.5f * w.U(c, b - 1) + 0.5f * w.U(c, b);
+  return vl_flux(Uf, w.V(c, b - 1), w.V(c + 1, b - 1), w.V(c - 1, b), w.V(c, b), w.V(c + 1, b),
+                 w.V(c + 2, b), h);
+}
+
+template <int N>
+__device__ __forceinline__ float face_fy_v(const Window<N>& w, int lr, int j, const FvmParams& p) {
+  const float h = p.dt_over_dy;
+  if (p.convect == JCFD_CONVECT_VAN_LEER_LIMITERS) {
+    const float Vf = 0.5f * w.V(lr, j) + 0.5f * w.V(lr, j + 1);
+    return vll_flux(w.V(lr, j - 1), w.V(lr, j), w.V(lr, j + 1), w.V(lr, j + 2), Vf, h);
+  }
+  const int b = j + 1;
+  const float Vf = 0.5f * w.V(lr, b - 1) + 0.5f * w.V(lr, b);
+  return vl_flux(Vf, w.V(lr, b - 1), w.V(lr, b), w.V(lr, b - 2), w.V(lr, b - 1), w.V(lr, b),
+                 w.V(lr, b + 1), h);
+}
+
+// f / rho; x / 1 is x exactly, so the common rho = 1 skips the division.
+__device__ __forceinline__ float over_rho(float f, float rho) { return rho == 1.0f ? f : f / rho; }
+
+// k += nu/rho lap + forcing/rho for one component, in the reference's order
+// (equations.py:102-110, finite_differences.py:127-133 for the Laplacian).
+// V_COMP selects v (no forcing column) instead of u.
+template <int N, bool V_COMP>
+__device__ __forceinline__ float add_diffusion_forcing(const Window<N>& w, int lr, int j,
+                                                       const FvmParams& p, float fx_j, float k) {
+  auto c = [&](int r, int jj) { return V_COMP ? w.V(r, jj) : w.U(r, jj); };
   if (p.has_diffusion) {
-    const float uc = w.U(lr, j), vc = w.V(lr, j);
-    const float lap_u = ((-2.f * uc) * p.lap_sum + (w.U(lr - 1, j) + w.U(lr + 1, j)) * p.inv_dx_sq) +
-                        (w.U(lr, j - 1) + w.U(lr, j + 1)) * p.inv_dy_sq;
-    const float lap_v = ((-2.f * vc) * p.lap_sum + (w.V(lr - 1, j) + w.V(lr + 1, j)) * p.inv_dx_sq) +
-                        (w.V(lr, j - 1) + w.V(lr, j + 1)) * p.inv_dy_sq;
-    ku = ku + p.nu * lap_u;
-    kv = kv + p.nu * lap_v;
+    const float cc = c(lr, j);
+    const float lap = ((-2.f * cc) * p.lap_sum + (c(lr - 1, j) + c(lr + 1, j)) * p.inv_dx_sq) +
+                      (c(lr, j - 1) + c(lr, j + 1)) * p.inv_dy_sq;
+    k = k + p.nu * lap;
   }
   if (p.has_forcing) {
-    const float fu = p.lin * w.U(lr, j) + fx_j;
-    const float fv = p.lin * w.V(lr, j) + 0.f;
-    ku = ku + fu / p.rho;
-    kv = kv + fv / p.rho;
+    const float f = p.lin * c(lr, j) + (V_COMP ? 0.f : fx_j);
+    k = k + over_rho(f, p.rho);
   }
+  return k;
 }
 
 template <int N, int ROWS>
@@ -223,14 +201,24 @@ struct RowsForwardLds {
   static constexpr bool kFft = MODE == 0 || MODE == 1 || MODE == 3;   // row FFT of div / field
   static constexpr bool kWin = MODE == 0 || MODE == 2 || MODE == 4;   // stencil window
   static constexpr bool kStar = MODE == 0 || MODE == 1 || MODE == 4;  // u*, v* rows
-  static constexpr int zb = 0;                                  // float2[RB * H]: div rows, then FFTs
-  static constexpr int twn = zb + (kFft ? RB * H * 8 : 0);      // float2[N]
-  static constexpr int twh = twn + (kFft ? N * 8 : 0);          // float2[H]
-  static constexpr int su = twh + (kFft ? H * 8 : 0);           // float[WR * N]: u rows i0-3 .. i0+RB+1
-  static constexpr int sv = su + (kWin ? WR * N * 4 : 0);       // float[WR * N]
-  static constexpr int sus = sv + (kWin ? WR * N * 4 : 0);      // float[(RB + 1) * N]: u* rows i0-1 ..
-  static constexpr int svs = sus + (kStar ? (RB + 1) * N * 4 : 0);  // float[RB * N]: v* rows i0 ..
-  static constexpr int bytes = svs + (kStar ? RB * N * 4 : 0);
+  // Window rows of the cells whose explicit terms are formed: u at LO_U..HI
+  // (global rows i0-1.. for the backward difference of div u*, or i0.. for
+  // MODE 2), v at 3..HI (global rows i0..i0+RB-1).
+  static constexpr int LO_U = MODE == 2 ? 3 : 2, HI = RB + 2;
+  static constexpr int NU = HI - LO_U + 1, NV = RB;
+  // Rows of the face-flux area: x faces + y faces of one component, or the
+  // u*, v* rows afterwards.
+  static constexpr int FR = kWin ? (2 * NU + 1) : (kStar ? 2 * RB + 1 : 0);
+  static constexpr int twn = 0;                              // float2[N]
+  static constexpr int twh = twn + (kFft ? N * 8 : 0);       // float2[H]
+  // float2[RB * H]: div rows, then their FFTs; shares the window's space when
+  // there is a window (the window is dead by then)
+  static constexpr int zb = twh + (kFft ? H * 8 : 0);
+  static constexpr int su = zb + ((kFft && !kWin) ? RB * H * 8 : 0);  // float[WR * N]: rows i0-3 ..
+  static constexpr int sv = su + (kWin ? WR * N * 4 : 0);              // float[WR * N]
+  static constexpr int flux = sv + (kWin ? WR * N * 4 : 0);            // float[FR * N]
+  static constexpr int bytes = flux + FR * N * 4;
+  static_assert(!(kFft && kWin) || RB * H * 8 <= 2 * WR * N * 4, "zb must fit in the window");
 };
 
 template <int N, int W>
@@ -261,17 +249,19 @@ __device__ __forceinline__ T* lds_at(char* smem, int offset) {
 // MODE 2: explicit terms only (write k_u, k_v to us, vs).
 // MODE 3: spectrum of the field b.u itself (spectral filtering).
 // MODE 4: CG right-hand side (write div u* to b.rhs, no FFT).
-template <int N, int RB, int MODE>
+template <int N, int RB, int MODE, int R = 8>
 __device__ __forceinline__ void rows_forward_body(const FvmParams& p, const FvmBuffers& b, int i0,
                                                   int tid, char* smem) {
   constexpr int H = N / 2;
   constexpr int WR = RB + 5;
   using L = RowsForwardLds<N, RB, MODE>;
+  constexpr int LO_U = L::LO_U, NU = L::NU, NV = L::NV;
   float* su = lds_at<float>(smem, L::su);
   float* sv = lds_at<float>(smem, L::sv);
-  float* sus = lds_at<float>(smem, L::sus);
-  float* svs = lds_at<float>(smem, L::svs);
-  float2* zb = lds_at<float2>(smem, L::zb);
+  float* flux = lds_at<float>(smem, L::flux);
+  float* sus = flux;                                   // u* rows i0-1 .. i0+RB-1
+  float* svs = flux + (RB + 1) * N;                    // v* rows i0 .. i0+RB-1
+  float2* zb = lds_at<float2>(smem, L::kWin ? L::su : L::zb);
   float2* twn = lds_at<float2>(smem, L::twn);
   float2* twh = lds_at<float2>(smem, L::twh);
 
@@ -295,31 +285,76 @@ __device__ __forceinline__ void rows_forward_body(const FvmParams& p, const FvmB
   __syncthreads();
 
   if constexpr (L::kWin) {
-    Window<N> w{su, sv};
-    // rows i0-1 .. i0+RB-1 -> local rows 2 .. RB+2 of the window
-    const int first = (MODE == 2) ? 1 : 0;
-    for (int e = tid + first * N; e < (RB + 1) * N; e += kThreads) {
-      const int r = e / N, j = e % N;  // r = 0 -> row i0 - 1
-      const int lr = r + 2;
-      const float fxj = p.has_forcing ? b.fx[j] : 0.f;
-      float ku, kv;
-      explicit_terms_at<N>(w, lr, j, p, fxj, ku, kv);
-      const int gi = (i0 - 1 + r) & (N - 1);
-      if constexpr (MODE == 2) {
-        b.us[gi * N + j] = ku;
-        b.vs[gi * N + j] = kv;
-      } else {
-        const float ust = w.U(lr, j) + p.dt * ku;
-        sus[e] = ust;
-        if (r >= 1) {
-          const float vst = w.V(lr, j) + p.dt * kv;
-          svs[e - N] = vst;
-          b.us[gi * N + j] = ust;
-          b.vs[gi * N + j] = vst;
-        }
+    const Window<N> w{su, sv};
+    constexpr int PU = (NU * N + kThreads - 1) / kThreads;
+    constexpr int PV = (NV * N + kThreads - 1) / kThreads;
+    float ru[PU], rv[PV];  // explicit terms (MODE 2) or u*, v* of this thread's cells
+    // ---- u: x faces of rows LO_U .. HI + 1, y faces of rows LO_U .. HI
+    for (int e = tid; e < (NU + 1) * N; e += kThreads) flux[e] = face_fx_u(w, LO_U + e / N, e % N, p);
+    for (int e = tid; e < NU * N; e += kThreads)
+      flux[(NU + 1) * N + e] = face_fy_u(w, LO_U + e / N, e % N, p);
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < PU; ++k) {
+      const int e = tid + k * kThreads;
+      if (e < NU * N) {
+        const int r = e / N, j = e % N, lr = LO_U + r;
+        const float* fy = flux + (NU + 1) * N + r * N;
+        const float adv = -((flux[(r + 1) * N + j] - flux[r * N + j]) / p.dx +
+                            (fy[j] - fy[(j - 1) & (N - 1)]) / p.dy);
+        const float fxj = p.has_forcing ? b.fx[j] : 0.f;
+        const float ku = add_diffusion_forcing<N, false>(w, lr, j, p, fxj, adv);
+        ru[k] = (MODE == 2) ? ku : w.U(lr, j) + p.dt * ku;
       }
     }
-    if constexpr (MODE == 2) return;
+    __syncthreads();
+    // ---- v: x faces of rows 3 .. HI + 1, y faces of rows 3 .. HI
+    for (int e = tid; e < (NV + 1) * N; e += kThreads) flux[e] = face_fx_v(w, 3 + e / N, e % N, p);
+    for (int e = tid; e < NV * N; e += kThreads)
+      flux[(NV + 1) * N + e] = face_fy_v(w, 3 + e / N, e % N, p);
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < PV; ++k) {
+      const int e = tid + k * kThreads;
+      if (e < NV * N) {
+        const int r = e / N, j = e % N, lr = 3 + r;
+        const float* fy = flux + (NV + 1) * N + r * N;
+        const float adv = -((flux[(r + 1) * N + j] - flux[r * N + j]) / p.dx +
+                            (fy[j] - fy[(j - 1) & (N - 1)]) / p.dy);
+        const float kv = add_diffusion_forcing<N, true>(w, lr, j, p, 0.f, adv);
+        rv[k] = (MODE == 2) ? kv : w.V(lr, j) + p.dt * kv;
+      }
+    }
+    if constexpr (MODE == 2) {
+#pragma unroll
+      for (int k = 0; k < PU; ++k) {
+        const int e = tid + k * kThreads;
+        if (e < NU * N) b.us[(i0 + e / N) * N + e % N] = ru[k];
+      }
+#pragma unroll
+      for (int k = 0; k < PV; ++k) {
+        const int e = tid + k * kThreads;
+        if (e < NV * N) b.vs[(i0 + e / N) * N + e % N] = rv[k];
+      }
+      return;
+    }
+    __syncthreads();  // the flux area becomes the u*, v* rows
+#pragma unroll
+    for (int k = 0; k < PU; ++k) {
+      const int e = tid + k * kThreads;
+      if (e < NU * N) {
+        sus[e] = ru[k];
+        if (e >= N) b.us[(i0 - 1) * N + e] = ru[k];  // rows i0 .. (row i0-1 is another block's)
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < PV; ++k) {
+      const int e = tid + k * kThreads;
+      if (e < NV * N) {
+        svs[e] = rv[k];
+        b.vs[i0 * N + e] = rv[k];
+      }
+    }
     __syncthreads();
   }
 
@@ -336,22 +371,22 @@ __device__ __forceinline__ void rows_forward_body(const FvmParams& p, const FvmB
     if constexpr (MODE == 4) return;
     __syncthreads();
   }
-  lds_fft<H, RB, kThreads, 1, H, false, false>(zb, twh, tid);
+  lds_fft<H, RB, kThreads, 1, H, false, false, R>(zb, twh, tid);
   for (int e = tid; e < RB * H; e += kThreads) {
     const int r = e / H, k = e % H;
     b.spec[(i0 + r) * H + k] = r2c_combine(zb + r * H, k, H, twn[k]);
   }
 }
 
-template <int N, int RB, int MODE>
+template <int N, int RB, int MODE, int R = 8>
 __global__ __launch_bounds__(kThreads) void fvm_rows_forward(FvmParams p, FvmBuffers b) {
   __shared__ __align__(16) char smem[RowsForwardLds<N, RB, MODE>::bytes];
-  rows_forward_body<N, RB, MODE>(p, fvm_instance(b, N), blockIdx.x * RB, threadIdx.x, smem);
+  rows_forward_body<N, RB, MODE, R>(p, fvm_instance(b, N), blockIdx.x * RB, threadIdx.x, smem);
 }
 
 // Column pass of the Poisson solve on the tile of W packed-spectrum columns
 // starting at c0 (b already points at the instance).
-template <int N, int W>
+template <int N, int W, int R = 8>
 __device__ __forceinline__ void poisson_cols_body(const FvmBuffers& b, float scale, int c0, int tid,
                                                   char* smem) {
   constexpr int H = N / 2;
@@ -365,7 +400,7 @@ __device__ __forceinline__ void poisson_cols_body(const FvmBuffers& b, float sca
   }
   load_twiddles(twn, b.tw, N, 1, tid, kThreads);
   __syncthreads();
-  lds_fft<N, W, kThreads, W, 1, true, false>(buf, twn, tid);
+  lds_fft<N, W, kThreads, W, 1, true, false, R>(buf, twn, tid);
   float2 nv[PER];
 #pragma unroll
   for (int q = 0; q < PER; ++q) {
@@ -395,24 +430,24 @@ __device__ __forceinline__ void poisson_cols_body(const FvmBuffers& b, float sca
     if (e < N * W) buf[e] = nv[q];
   }
   __syncthreads();
-  lds_fft<N, W, kThreads, W, 1, true, true>(buf, twn, tid);
+  lds_fft<N, W, kThreads, W, 1, true, true, R>(buf, twn, tid);
   for (int e = tid; e < N * W; e += kThreads) {
     const int pp = e / W, c = e % W;
     b.spec[pp * H + c0 + c] = buf[e];
   }
 }
 
-template <int N, int W>
+template <int N, int W, int R = 8>
 __global__ __launch_bounds__(kThreads) void fvm_poisson_cols(FvmBuffers b, float scale) {
   __shared__ __align__(16) char smem[PoissonLds<N, W>::bytes];
-  poisson_cols_body<N, W>(fvm_instance(b, N), scale, blockIdx.x * W, threadIdx.x, smem);
+  poisson_cols_body<N, W, R>(fvm_instance(b, N), scale, blockIdx.x * W, threadIdx.x, smem);
 }
 
 // Inverse row transforms of q and the velocity correction for rows
 // [i0, i0 + RB) (b already points at the instance).
 // MODE 0: velocity correction; MODE 1: write q itself to u_out;
 // MODE 2: velocity correction from the real q of the CG solve (no FFT).
-template <int N, int RB, int MODE>
+template <int N, int RB, int MODE, int R = 8>
 __device__ __forceinline__ void rows_project_body(const FvmParams& p, const FvmBuffers& b, int i0,
                                                   int tid, char* smem) {
   constexpr int H = N / 2;
@@ -437,7 +472,7 @@ __device__ __forceinline__ void rows_project_body(const FvmParams& p, const FvmB
       qb[e] = c2r_combine(pk, pmk, k, twn[k]);
     }
     __syncthreads();
-    lds_fft<H, T, kThreads, 1, H, false, true>(qb, twh, tid);
+    lds_fft<H, T, kThreads, 1, H, false, true, R>(qb, twh, tid);
   }
   const float* q = reinterpret_cast<const float*>(qb);
   if constexpr (MODE == 1) {
@@ -455,10 +490,10 @@ __device__ __forceinline__ void rows_project_body(const FvmParams& p, const FvmB
   }
 }
 
-template <int N, int RB, int MODE>
+template <int N, int RB, int MODE, int R = 8>
 __global__ __launch_bounds__(kThreads) void fvm_rows_project(FvmParams p, FvmBuffers b) {
   __shared__ __align__(16) char smem[RowsProjectLds<N, RB, MODE>::bytes];
-  rows_project_body<N, RB, MODE>(p, fvm_instance(b, N), blockIdx.x * RB, threadIdx.x, smem);
+  rows_project_body<N, RB, MODE, R>(p, fvm_instance(b, N), blockIdx.x * RB, threadIdx.x, smem);
 }
 
 // curl_2d (finite_differences.py:186-193): forward differences, offset (1, 1).
@@ -495,29 +530,111 @@ template <> struct Tiles<256> { static constexpr int RB = 4, W = 8; };
 template <> struct Tiles<512> { static constexpr int RB = 2, W = 8; };
 template <> struct Tiles<1024> { static constexpr int RB = 1, W = 4; };
 
-template <int N>
-int launch_step(const FvmParams& p, const FvmBuffers& b, hipStream_t s) {
-  constexpr int RB = Tiles<N>::RB, W = Tiles<N>::W;
-  hipLaunchKernelGGL((fvm_rows_forward<N, RB, 0>), dim3(N / RB, nb(b)), dim3(kThreads), 0, s, p, b);
-  hipLaunchKernelGGL((fvm_poisson_cols<N, W>), dim3(N / 2 / W, nb(b)), dim3(kThreads), 0, s, b, 1.0f);
-  hipLaunchKernelGGL((fvm_rows_project<N, RB, 0>), dim3(N / RB, nb(b)), dim3(kThreads), 0, s, p, b);
-  return check_hip(hipGetLastError(), "fvm step launch");
+// ---- tunable step kernels ---------------------------------------------------
+// The three step kernels are built for a menu of tile shapes and both radix
+// schedules; a plan's FvmParams carries its choice (rb_fwd, w_cols, rb_proj,
+// radix).  Tile shapes only regroup the same per-cell / per-column
+// arithmetic, so they never change a result; the radix schedule changes the
+// FFTs' rounding.
+constexpr int kTileLdsMax = 96 * 1024;
+
+template <int N, int RB>
+constexpr bool fwd_ok() {
+  return RB <= N / 4 && RowsForwardLds<N, RB, 0>::bytes <= kTileLdsMax;
+}
+template <int N, int W>
+constexpr bool cols_ok() {
+  return W <= N / 2 && PoissonLds<N, W>::bytes <= kTileLdsMax;
+}
+template <int N, int RB>
+constexpr bool proj_ok() {
+  return RB <= N / 4 && RowsProjectLds<N, RB, 0>::bytes <= kTileLdsMax;
+}
+
+// which: 0 rows-forward (tile = rows per block), 1 Poisson columns (tile =
+// columns per tile), 2 rows-project (tile = rows per block).  Returns
+// JCFD_ERR_UNSUPPORTED for a shape not on the menu of size N.
+template <int N, int T, int R>
+int launch_variant(int which, const FvmParams& p, const FvmBuffers& b, hipStream_t s) {
+  if (which == 0) {
+    if constexpr (fwd_ok<N, T>()) {
+      hipLaunchKernelGGL((fvm_rows_forward<N, T, 0, R>), dim3(N / T, nb(b)), dim3(kThreads), 0, s, p, b);
+      return JCFD_OK;
+    }
+  } else if (which == 1) {
+    if constexpr (cols_ok<N, T>()) {
+      hipLaunchKernelGGL((fvm_poisson_cols<N, T, R>), dim3(N / 2 / T, nb(b)), dim3(kThreads), 0, s, b, 1.0f);
+      return JCFD_OK;
+    }
+  } else {
+    if constexpr (proj_ok<N, T>()) {
+      hipLaunchKernelGGL((fvm_rows_project<N, T, 0, R>), dim3(N / T, nb(b)), dim3(kThreads), 0, s, p, b);
+      return JCFD_OK;
+    }
+  }
+  return JCFD_ERR_UNSUPPORTED;
+}
+
+template <int N, int R>
+int launch_tile(int which, int tile, const FvmParams& p, const FvmBuffers& b, hipStream_t s) {
+  switch (tile) {
+    case 1: return launch_variant<N, 1, R>(which, p, b, s);
+    case 2: return launch_variant<N, 2, R>(which, p, b, s);
+    case 4: return launch_variant<N, 4, R>(which, p, b, s);
+    case 8: return launch_variant<N, 8, R>(which, p, b, s);
+    case 16: return which == 1 ? launch_variant<N, 16, R>(which, p, b, s) : JCFD_ERR_UNSUPPORTED;
+    default: return JCFD_ERR_UNSUPPORTED;
+  }
 }
 
 template <int N>
 int launch_one(const FvmParams& p, const FvmBuffers& b, int which, hipStream_t s) {
-  constexpr int RB = Tiles<N>::RB, W = Tiles<N>::W;
-  switch (which) {
-    case 0:
-      hipLaunchKernelGGL((fvm_rows_forward<N, RB, 0>), dim3(N / RB, nb(b)), dim3(kThreads), 0, s, p, b);
-      break;
-    case 1:
-      hipLaunchKernelGGL((fvm_poisson_cols<N, W>), dim3(N / 2 / W, nb(b)), dim3(kThreads), 0, s, b, 1.0f);
-      break;
-    default:
-      hipLaunchKernelGGL((fvm_rows_project<N, RB, 0>), dim3(N / RB, nb(b)), dim3(kThreads), 0, s, p, b);
-  }
+  const int tile = which == 0 ? p.rb_fwd : (which == 1 ? p.w_cols : p.rb_proj);
+  const int rc = p.radix == 4 ? launch_tile<N, 4>(which, tile, p, b, s)
+                              : launch_tile<N, 8>(which, tile, p, b, s);
+  if (rc) return fail(rc, "fvm tile shape not built for n = " + std::to_string(N));
   return check_hip(hipGetLastError(), "fvm kernel launch");
+}
+
+template <int N>
+int launch_step(const FvmParams& p, const FvmBuffers& b, hipStream_t s) {
+  int rc;
+  for (int k = 0; k < 3; ++k)
+    if ((rc = launch_one<N>(p, b, k, s))) return rc;
+  return JCFD_OK;
+}
+
+template <int N>
+bool tile_on_menu(int which, int tile) {
+  auto row = [](int t) {
+    switch (t) {
+      case 1: return fwd_ok<N, 1>();
+      case 2: return fwd_ok<N, 2>();
+      case 4: return fwd_ok<N, 4>();
+      case 8: return fwd_ok<N, 8>();
+      default: return false;
+    }
+  };
+  auto proj = [](int t) {
+    switch (t) {
+      case 1: return proj_ok<N, 1>();
+      case 2: return proj_ok<N, 2>();
+      case 4: return proj_ok<N, 4>();
+      case 8: return proj_ok<N, 8>();
+      default: return false;
+    }
+  };
+  auto col = [](int t) {
+    switch (t) {
+      case 1: return cols_ok<N, 1>();
+      case 2: return cols_ok<N, 2>();
+      case 4: return cols_ok<N, 4>();
+      case 8: return cols_ok<N, 8>();
+      case 16: return cols_ok<N, 16>();
+      default: return false;
+    }
+  };
+  return which == 0 ? row(tile) : (which == 1 ? col(tile) : proj(tile));
 }
 
 template <int N>
@@ -669,6 +786,29 @@ bool fvm_size_supported(int n) {
 }
 
 bool fvm_group_size_supported(int n) { return fvm_size_supported(n) && n <= kFvmGroupMaxN; }
+
+template <int N> struct DefaultTilesOp {
+  static int run(FvmParams* p) {
+    p->rb_fwd = Tiles<N>::RB;
+    p->w_cols = Tiles<N>::W;
+    p->rb_proj = Tiles<N>::RB;
+    p->radix = 8;
+    return JCFD_OK;
+  }
+};
+template <int N> struct TileMenuOp {
+  static int run(int which, int tile, bool* ok) {
+    *ok = tile_on_menu<N>(which, tile);
+    return JCFD_OK;
+  }
+};
+
+int fvm_default_tiles(FvmParams* p) { return dispatch_n<DefaultTilesOp>(p->n, p); }
+
+bool fvm_tile_supported(int n, int which, int tile) {
+  bool ok = false;
+  return dispatch_n<TileMenuOp>(n, which, tile, &ok) == JCFD_OK && ok;
+}
 
 int fvm_group_tiles(int n, int phase) {
   switch (n) {

@@ -1,7 +1,8 @@
 // Workgroup-level FFT building blocks for gfx950 (device code only).
 //
 // Transforms live in LDS and are done in place with Stockham auto-sort passes
-// (radix 8, then 4, then 2): each pass loads R values per butterfly into
+// (radix 8, then 4, then 2; or at most 4, a schedule the tile tuner may pick):
+// each pass loads R values per butterfly into
 // registers, waits on a barrier, applies twiddles and an R-point DFT in
 // registers, and writes the results back in auto-sorted order.  A workgroup
 // runs T independent transforms of length N at once; element e of transform t
@@ -136,22 +137,25 @@ __device__ __forceinline__ void stockham_pass(float2* buf, const float2* tw, int
   __syncthreads();
 }
 
-template <int N, int NS, int T, int NT, int ES, int TS, bool TFAST, bool INV>
+// Radix schedule: the largest radix <= MAXR (8 or 4) that divides what is
+// left, largest first (N = 512, MAXR 8: 8 8 8; MAXR 4: 4 4 4 4 2).
+template <int N, int NS, int T, int NT, int ES, int TS, bool TFAST, bool INV, int MAXR>
 __device__ __forceinline__ void fft_passes(float2* buf, const float2* tw, int tid) {
   if constexpr (NS < N) {
     constexpr int REM = N / NS;
-    constexpr int R = (REM % 8 == 0) ? 8 : ((REM % 4 == 0) ? 4 : 2);
+    constexpr int R = (MAXR >= 8 && REM % 8 == 0) ? 8 : ((REM % 4 == 0) ? 4 : 2);
     stockham_pass<N, R, NS, T, NT, ES, TS, TFAST, INV>(buf, tw, tid);
-    fft_passes<N, NS * R, T, NT, ES, TS, TFAST, INV>(buf, tw, tid);
+    fft_passes<N, NS * R, T, NT, ES, TS, TFAST, INV, MAXR>(buf, tw, tid);
   }
 }
 
 // T in-place complex transforms of length N in LDS.  Callers must have a
 // barrier between writing `buf` and calling this; it ends with a barrier.
-template <int N, int T, int NT, int ES, int TS, bool TFAST, bool INV>
+template <int N, int T, int NT, int ES, int TS, bool TFAST, bool INV, int MAXR = 8>
 __device__ __forceinline__ void lds_fft(float2* buf, const float2* tw, int tid) {
   static_assert((N & (N - 1)) == 0 && N >= 2, "power-of-two length");
-  fft_passes<N, 1, T, NT, ES, TS, TFAST, INV>(buf, tw, tid);
+  static_assert(MAXR == 8 || MAXR == 4, "radix schedule");
+  fft_passes<N, 1, T, NT, ES, TS, TFAST, INV, MAXR>(buf, tw, tid);
 }
 
 // Copy a length-N twiddle table to LDS (stride 1 in the table of length

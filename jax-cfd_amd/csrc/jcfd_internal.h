@@ -28,6 +28,9 @@ struct FvmParams {
   int has_diffusion;
   int has_forcing;
   int convect;       // JCFD_CONVECT_*
+  // Step-kernel tile shapes and FFT radix schedule (host side; see
+  // fvm_default_tiles / fvm_tile_supported).
+  int rb_fwd, w_cols, rb_proj, radix;
 };
 
 // Pointers of one FVM instance for one step.
@@ -105,6 +108,11 @@ int fvm_launch_curl(const FvmParams& p, const float* u, const float* v, float* o
 int fvm_launch_filter(const FvmParams& p, const float* x, const float* multiplier, float* out,
                       float2* spec, const float2* tw, hipStream_t s);
 bool fvm_size_supported(int n);
+// Default tile shapes and radix of size p->n into p.
+int fvm_default_tiles(FvmParams* p);
+// Whether kernel class `which` (0 rows-forward, 1 Poisson columns,
+// 2 rows-project) is built with tile `tile` for size n.
+bool fvm_tile_supported(int n, int which, int tile);
 
 // Mixed-size group step (fvm.hip): one persistent launch per kernel class
 // (phase 0 rows-forward, 1 Poisson columns, 2 rows-project) advances every

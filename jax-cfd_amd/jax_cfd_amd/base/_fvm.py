@@ -87,6 +87,20 @@ class FvmPlan:
     _lib.check(self._lib.jcfd_fvm_plan_set_graphs(self._handle, int(bool(enabled))),
                'jcfd_fvm_plan_set_graphs')
 
+  @property
+  def tiles(self) -> dict:
+    """Tile shapes and radix schedule of the step kernels."""
+    t = _lib.FvmTiles()
+    _lib.check(self._lib.jcfd_fvm_plan_get_tiles(self._handle, ctypes.byref(t)),
+               'jcfd_fvm_plan_get_tiles')
+    return {k: int(getattr(t, k)) for k, _ in _lib.FvmTiles._fields_}
+
+  def set_tiles(self, rows_forward: int, poisson_cols: int, rows_project: int,
+                radix: int = 8):
+    t = _lib.FvmTiles(rows_forward, poisson_cols, rows_project, radix)
+    _lib.check(self._lib.jcfd_fvm_plan_set_tiles(self._handle, ctypes.byref(t)),
+               'jcfd_fvm_plan_set_tiles')
+
   def close(self):
     if getattr(self, '_handle', None) is not None and self._handle.value:
       self._lib.jcfd_fvm_plan_destroy(self._handle)

@@ -1,0 +1,98 @@
+"""Tile-shape menu and radix schedules of the FVM step kernels on the GPU.
+Every tile shape must give results bit-identical to the default shape; the
+radix-4 schedule must stay within fp32 round-off of radix 8 and of the
+oracle (one-step max relative error <= 1e-5); the autotuner's choice must
+step identically to an untuned plan."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import fvm as ofvm
+from oracle.grid_ops import Grid as OGrid
+
+pytestmark = pytest.mark.gpu
+TWO_PI = 2 * np.pi
+
+
+def _dev():
+  return torch.device('cuda', 0)
+
+
+def _family(n):
+  from jax_cfd_amd import serving
+  return serving.FvmFamily(n, _dev())
+
+
+def _plan(fam, batch=1):
+  import jax_cfd_amd.base as cfd
+  return cfd._fvm.FvmPlan(fam.spec, _dev(), batch=batch)
+
+
+@pytest.mark.parametrize("n", [32, 256, 512])
+def test_every_tile_shape_is_bit_identical(n):
+  fam = _family(n)
+  plan = _plan(fam)
+  u0, v0 = fam.initial_state(3)
+  ref = plan.step(u0, v0, 3)
+  default = plan.tiles
+  tried = 0
+  for cls, menu in (('rows_forward', (1, 2, 4, 8)), ('poisson_cols', (1, 2, 4, 8, 16)),
+                    ('rows_project', (1, 2, 4, 8))):
+    for t in menu:
+      cfg = dict(default)
+      cfg[cls] = t
+      try:
+        plan.set_tiles(cfg['rows_forward'], cfg['poisson_cols'], cfg['rows_project'], 8)
+      except NotImplementedError:
+        continue
+      print(f"n={n} {cls}={t}", flush=True)
+      out = plan.step(u0, v0, 3)
+      assert torch.equal(out[0], ref[0]) and torch.equal(out[1], ref[1]), (cls, t)
+      tried += 1
+  assert tried >= 8
+
+
+@pytest.mark.parametrize("n", [64, 256])
+def test_radix4_schedule_within_roundoff(n):
+  fam = _family(n)
+  plan = _plan(fam)
+  u0, v0 = fam.initial_state(5)
+  r8 = plan.step(u0, v0, 1)
+  d = plan.tiles
+  plan.set_tiles(d['rows_forward'], d['poisson_cols'], d['rows_project'], 4)
+  r4 = plan.step(u0, v0, 1)
+  scale = float(torch.max(torch.abs(r8[0])))
+  diff = max(float(torch.max(torch.abs(a - b))) for a, b in zip(r4, r8))
+  print(f"n={n} radix 4 vs 8: max |diff| / max|u| = {diff / scale:.2e}", flush=True)
+  assert diff <= 1e-5 * scale
+  # against the oracle
+  from oracle import grid_ops as go
+  og = OGrid.from_domain((n, n), ((0, TWO_PI), (0, TWO_PI)))
+  ostep = ofvm.FVMStep(og, dt=fam.dt, viscosity=1e-3, forcing=go.Forcing(1.0, 4, -0.1))
+  ref = ostep(ofvm.velocity(u0.cpu().numpy(), v0.cpu().numpy(), og))
+  for got, want in zip(r4, ref):
+    want = np.asarray(want.data, np.float64)
+    err = np.max(np.abs(got.cpu().numpy() - want)) / np.max(np.abs(want))
+    print(f"n={n} radix 4 vs oracle: max rel err {err:.2e}", flush=True)
+    assert err <= 1e-5
+
+
+def test_autotuned_plan_steps_identically_and_group_rejects_radix4(tmp_path, monkeypatch):
+  from jax_cfd_amd import autotune
+  from jax_cfd_amd.base import _fvm
+  monkeypatch.setenv('JCFD_TUNE_CACHE', str(tmp_path / 'tiles.json'))
+  fam = _family(128)
+  plan = _plan(fam, batch=4)
+  u = torch.stack([fam.initial_state(s)[0] for s in range(4)])
+  v = torch.stack([fam.initial_state(s)[1] for s in range(4)])
+  ref = plan.step(u, v, 4)
+  res = autotune.tune_fvm(plan, radices=(8,), reps=10)
+  print("tuned:", {k: res[k] for k in ('rows_forward', 'poisson_cols', 'rows_project', 'radix')},
+        flush=True)
+  out = plan.step(u, v, 4)
+  assert torch.equal(out[0], ref[0]) and torch.equal(out[1], ref[1])
+  d = plan.tiles
+  plan.set_tiles(d['rows_forward'], d['poisson_cols'], d['rows_project'], 4)
+  group = _fvm.FvmGroup(_dev())
+  with pytest.raises(NotImplementedError):
+    group.step([(plan, u, v, [0])], 1)
