@@ -44,6 +44,9 @@ def _args():
                   help="steps of the 64 x 512^2 spectral batch (0 = skip)")
   ap.add_argument("--served-requests", type=int, default=400,
                   help="requests of the served Poisson stream (0 = skip)")
+  ap.add_argument("--group-steps", type=int, default=256,
+                  help="steps of the mixed-size group line (0 skips it)")
+  ap.add_argument("--adjoint", type=int, default=1, help="time the C4 gradient (0 skips)")
   ap.add_argument("--cg-steps", type=int, default=100,
                   help="steps of the solve_cg variant of the 256^2 run (0 = skip)")
   return ap.parse_args()
@@ -91,6 +94,8 @@ def _batched_spectral(dev, batch, n, steps, reps):
   fam = serving.SpectralFamily(n, dev)
   w = torch.stack([fam.initial_state(s)[0] for s in range(batch)])
   plan = fam.make_plan(batch)
+  from jax_cfd_amd import autotune
+  tiles = autotune.tune_ns2d(plan, radices=(8, 4), use_cache=False)
   plan.step(w, 8, out=w)
   torch.cuda.synchronize()
   t0 = time.perf_counter()
@@ -105,6 +110,7 @@ def _batched_spectral(dev, batch, n, steps, reps):
   ach = kb[dom] / (kms[dom] * 1e-3) / 1e9
   return {
       "workload": f"spectral {batch}x{n}^2 forced, CN-RK4 dt 1e-3",
+      "tiles": {k: tiles[k] for k in ("cols", "rows", "radix")},
       "value": batch * n * n * steps / wall, "unit": "cell-steps/s",
       "steps": steps, "ms_per_step": wall / steps * 1e3,
       "kernels": {k: {"ms_per_launch": round(kms[k], 5), "launches_per_step":
@@ -147,12 +153,14 @@ def _cg_variant(dev, steps):
 
 def _plain_step_seconds(dev, solver, n, steps=40):
   """Seconds per step of one instance on the plain path (one instance per
-  launch sequence, no graphs)."""
+  launch sequence, no graphs, tile shapes tuned for one instance)."""
   import torch
   from jax_cfd_amd import serving
+  from jax_cfd_amd import autotune
   fam = serving.FAMILIES[solver](n, dev)
   plan = fam.make_plan(1)
   plan.set_graphs(False)
+  (autotune.tune_fvm if solver == "fvm" else autotune.tune_ns2d)(plan, radices=(8,))
   init = fam.initial_state(1)
   if solver == "fvm":
     u, v = init
@@ -174,7 +182,7 @@ def _served(dev, count, rate, seed):
   import numpy as np
   from jax_cfd_amd import serving
   reqs = serving.poisson_requests(count, rate=rate, seed=seed)
-  srv = serving.RolloutServer(hbm_budget_bytes=200e9)
+  srv = serving.RolloutServer(hbm_budget_bytes=200e9, tune=True)
   warm = [serving.RolloutRequest(10 ** 6 + i, s, n, 40, i)
           for i, (s, n) in enumerate((s, n) for s in ("fvm", "spectral")
                                      for n in (64, 128, 256, 512))]
@@ -217,6 +225,107 @@ def _served(dev, count, rate, seed):
   return summ
 
 
+def _group_mixed(dev, mix, steps):
+  """Uneven sizes in one persistent launch per kernel class (jcfd_fvm_group)
+  against the same instances as one batched launch sequence per size on
+  concurrent streams (tuned tiles).  FVM physics of the 256^2 run at every
+  size, seeds 0.. per size."""
+  import torch
+  from jax_cfd_amd import autotune, serving
+  from jax_cfd_amd.base import _fvm
+  members, pools = [], []
+  for n, b in mix.items():
+    fam = serving.FvmFamily(n, dev)
+    u = torch.stack([fam.initial_state(s)[0] for s in range(b)])
+    v = torch.stack([fam.initial_state(s)[1] for s in range(b)])
+    gplan = fam.make_plan(b)
+    members.append((gplan, u, v, list(range(b))))
+    pplan = fam.make_plan(b)
+    autotune.tune_fvm(pplan, radices=(8,))
+    pools.append((pplan, u.clone(), v.clone(),
+                  torch.arange(b, dtype=torch.int32, device=dev), torch.cuda.Stream(device=dev)))
+  cells = sum(n * n * b for n, b in mix.items())
+  group = _fvm.FvmGroup(dev)
+  group.step(members, 32)
+  torch.cuda.synchronize()
+  t0 = time.perf_counter()
+  group.step(members, steps)
+  torch.cuda.synchronize()
+  g_wall = time.perf_counter() - t0
+  shape = group.launch_shape()
+
+  def run_pools(k):
+    for plan, u, v, idx, st in pools:
+      st.wait_stream(torch.cuda.current_stream())
+      with torch.cuda.stream(st):
+        plan.step_slots(u, v, idx, k)
+    for *_, st in pools:
+      torch.cuda.current_stream().wait_stream(st)
+  run_pools(32)
+  torch.cuda.synchronize()
+  t0 = time.perf_counter()
+  run_pools(steps)
+  torch.cuda.synchronize()
+  p_wall = time.perf_counter() - t0
+  same = all(torch.equal(m[1], p[1]) and torch.equal(m[2], p[2]) for m, p in zip(members, pools))
+  tk = group.time_kernels(10)  # advances the group's members further
+  return {"workload": "fvm mixed sizes " + ", ".join(f"{b}x{n}^2" for n, b in mix.items()),
+          "steps": steps, "group_value": cells * steps / g_wall,
+          "pools_value": cells * steps / p_wall, "unit": "cell-steps/s",
+          "group_ms_per_step": g_wall / steps * 1e3,
+          "bitwise_equal_to_pools": bool(same),
+          "phases": {k: {"grid": shape[k]["grid"], "items": shape[k]["items"],
+                         "ms_per_launch": round(tk[k], 5),
+                         "GBps": round(shape[k]["bytes"] / (tk[k] * 1e-3) / 1e9, 1)}
+                     for k in tk}}
+
+
+def _adjoint(dev, n=256, steps=64, every=8, reps=3):
+  """Gradient of the vorticity MSE after 64 steps of the forced 256^2 run
+  with respect to the initial velocity (hand-written adjoint, checkpoints
+  every 8 steps), against the forward rollout alone."""
+  import numpy as np
+  import torch
+  import jax_cfd_amd.base as cfd
+  from jax_cfd_amd import adjoint
+  grid = cfd.grids.Grid((n, n), domain=((0, 2 * np.pi), (0, 2 * np.pi)))
+  dt = cfd.equations.stable_time_step(VMAX, CFL, NU, grid)
+  step = cfd.equations.semi_implicit_navier_stokes(
+      density=1.0, viscosity=NU, dt=dt, grid=grid,
+      convect=lambda v: tuple(cfd.advection.advect_van_leer(u, v, dt) for u in v),
+      forcing=cfd.forcings.simple_turbulence_forcing(grid, 1.0, 4, -0.1, "kolmogorov"))
+  v0 = cfd.initial_conditions.filtered_velocity_field(SEED, grid, VMAX, PEAK)
+  vt = cfd.initial_conditions.filtered_velocity_field(2, grid, VMAX, PEAK)
+  plan = step.plan(dev)
+  ut, vt_ = plan.step(vt[0].data, vt[1].data, steps)
+  target = adjoint.curl_2d(ut, vt_, grid).detach()
+  roll = adjoint.rollout(step, steps, every)
+
+  def grad():
+    u0 = v0[0].data.clone().requires_grad_(True)
+    w0 = v0[1].data.clone().requires_grad_(True)
+    u, v = roll(u0, w0)
+    loss = ((adjoint.curl_2d(u, v, grid) - target) ** 2).mean()
+    return torch.autograd.grad(loss, (u0, w0))
+  grad()
+  torch.cuda.synchronize()
+  t0 = time.perf_counter()
+  for _ in range(reps):
+    g = grad()
+  torch.cuda.synchronize()
+  g_s = (time.perf_counter() - t0) / reps
+  assert all(torch.isfinite(x).all() for x in g)
+  t0 = time.perf_counter()
+  for _ in range(reps):
+    plan.step(v0[0].data, v0[1].data, steps)
+  torch.cuda.synchronize()
+  f_s = (time.perf_counter() - t0) / reps
+  return {"workload": f"fvm {n}^2, {steps}-step rollout, checkpoints every {every}, "
+                      "d(vorticity MSE)/d(initial velocity)",
+          "ms_per_gradient": g_s * 1e3, "ms_forward_only": f_s * 1e3,
+          "gradient_over_forward": g_s / f_s}
+
+
 def main():
   args = _args()
   if args.gpus != 1:
@@ -242,8 +351,11 @@ def main():
   v0 = cfd.initial_conditions.filtered_velocity_field(SEED, grid, VMAX, PEAK)
   plan = step.plan(dev)
   u, v = v0[0].data.clone(), v0[1].data.clone()
-  print(f"[bench] workload fvm {N}x{N} forced, dt={dt:.6g}; warmup {args.warmup}",
-        file=sys.stderr, flush=True)
+  from jax_cfd_amd import autotune
+  tuned = autotune.tune_fvm(plan, u.clone(), v.clone(), radices=(8, 4), use_cache=False)
+  tiles = {k: tuned[k] for k in ("rows_forward", "poisson_cols", "rows_project", "radix")}
+  print(f"[bench] workload fvm {N}x{N} forced, dt={dt:.6g}; warmup {args.warmup}; "
+        f"tuned tiles {tiles}", file=sys.stderr, flush=True)
   plan.step(u, v, args.warmup, out=(u, v))
   torch.cuda.synchronize()
   t0 = time.perf_counter()
@@ -276,6 +388,13 @@ def main():
   if args.batched_steps > 0:
     batched = _batched_spectral(dev, 64, 512, args.batched_steps, 20)
     print(f"[bench] batched {json.dumps(batched)}", file=sys.stderr, flush=True)
+  group = None
+  if args.group_steps > 0:
+    group = _group_mixed(dev, {64: 64, 128: 32, 256: 16, 512: 8}, args.group_steps)
+    print(f"[bench] group {json.dumps(group)}", file=sys.stderr, flush=True)
+  adj = _adjoint(dev) if args.adjoint else None
+  if adj:
+    print(f"[bench] adjoint {json.dumps(adj)}", file=sys.stderr, flush=True)
   served = None
   if args.served_requests > 0:
     served = _served(dev, args.served_requests, 200.0, 0)
@@ -290,9 +409,10 @@ def main():
       "data": "synthetic (seeded filtered_velocity_field)",
       "config": {"workload": "fvm-256x256-kolmogorov-forced-vanleer-fastdiag",
                  "n": N, "dt": dt, "viscosity": NU, "seed": SEED, "v_max": VMAX,
-                 "path": "plain (one instance, 3 kernels per step)"},
+                 "path": "plain (one instance, 3 kernels per step)", "tiles": tiles},
       "roofline": roofline, "cpu_baseline": cpu, "kernels": kernels,
-      "batched": batched, "served": served, "cg_variant": cg,
+      "batched": batched, "served": served, "cg_variant": cg, "group": group,
+      "adjoint": adj,
   }
   print(json.dumps(out), flush=True)
 

@@ -246,6 +246,17 @@ int64_t jcfd_ns2d_plan_device_bytes(const jcfd_ns2d_plan* plan);
 /* As jcfd_fvm_plan_set_graphs, for the spectral plan. */
 int jcfd_ns2d_plan_set_graphs(jcfd_ns2d_plan* plan, int32_t enabled);
 
+/* Tile shapes and FFT radix schedule of the step kernels (see
+ * jcfd_fvm_tiles; no counterpart in the reference).  Tile shapes never
+ * change results; radix 4 changes the FFT rounding. */
+typedef struct {
+  int32_t cols;  /* packed spectrum columns per workgroup of the column kernels: 1..16 */
+  int32_t rows;  /* rows per workgroup of the row (nonlinear product) kernel: 1, 2, 4 */
+  int32_t radix; /* largest FFT radix: 8 or 4 */
+} jcfd_ns2d_tiles;
+int jcfd_ns2d_plan_get_tiles(const jcfd_ns2d_plan* plan, jcfd_ns2d_tiles* tiles);
+int jcfd_ns2d_plan_set_tiles(jcfd_ns2d_plan* plan, const jcfd_ns2d_tiles* tiles);
+
 /* nsteps CN-RK4 steps of every instance; w_hat/w_out complex64 of shape
  * (batch, n, n/2 + 1), or (n, n/2 + 1) when batch == 1.  w_out may alias
  * w_hat.  funcutils.repeated(step_fn, nsteps) (funcutils.py:82-88). */

@@ -1232,8 +1232,36 @@ int jcfd_ns2d_plan_create(const jcfd_ns2d_config* config, jcfd_ns2d_plan** plan)
     q.mu = static_cast<float>(0.5 * c.dt * (kCnAlpha[k + 1] - kCnAlpha[k]));
     q.read_h = k > 0 ? 1 : 0;
     q.slots = nullptr;
+    (void)ns_default_tiles(&q, n);
   }
   *plan = p;
+  return JCFD_OK;
+}
+
+int jcfd_ns2d_plan_get_tiles(const jcfd_ns2d_plan* p, jcfd_ns2d_tiles* tiles) {
+  if (p == nullptr || tiles == nullptr) return fail(JCFD_ERR_INVALID_ARGUMENT, "null argument");
+  tiles->cols = p->stages[0].w_cols;
+  tiles->rows = p->stages[0].rb_rows;
+  tiles->radix = p->stages[0].radix;
+  return JCFD_OK;
+}
+
+int jcfd_ns2d_plan_set_tiles(jcfd_ns2d_plan* p, const jcfd_ns2d_tiles* tiles) {
+  if (p == nullptr || tiles == nullptr) return fail(JCFD_ERR_INVALID_ARGUMENT, "null argument");
+  if (tiles->radix != 8 && tiles->radix != 4)
+    return fail(JCFD_ERR_UNSUPPORTED, "radix schedule must be 8 or 4");
+  if (!ns_tile_supported(p->n, 0, tiles->cols))
+    return fail(JCFD_ERR_UNSUPPORTED, "column tile " + std::to_string(tiles->cols) +
+                                          " not built for n = " + std::to_string(p->n));
+  if (!ns_tile_supported(p->n, 1, tiles->rows))
+    return fail(JCFD_ERR_UNSUPPORTED, "row tile " + std::to_string(tiles->rows) +
+                                          " not built for n = " + std::to_string(p->n));
+  for (NsParams& q : p->stages) {
+    q.w_cols = tiles->cols;
+    q.rb_rows = tiles->rows;
+    q.radix = tiles->radix;
+  }
+  p->graphs.release();  // captured launches carry the old shapes
   return JCFD_OK;
 }
 

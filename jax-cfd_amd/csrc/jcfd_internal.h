@@ -158,6 +158,9 @@ struct NsParams {
   float beta, g, mu;
   int read_h;
   const int* slots;       // instance of launch row blockIdx.y, or null = blockIdx.y
+  // Step-kernel tile shapes and FFT radix schedule (host side; see
+  // ns_default_tiles / ns_tile_supported).
+  int w_cols, rb_rows, radix;
 };
 
 struct NsBuffers {
@@ -169,6 +172,11 @@ struct NsBuffers {
 };
 
 bool ns_size_supported(int n);
+// Default tile shapes and radix for size n into q.
+int ns_default_tiles(NsParams* q, int n);
+// Whether step kernel class `which` (0 cols-inverse, 1 rows, 2/3 cols-forward)
+// is built with tile `tile` (columns per tile; rows per workgroup for 1).
+bool ns_tile_supported(int n, int which, int tile);
 int ns_launch_step(int n, const NsParams* stages, const NsBuffers& b, int batch, hipStream_t s);
 int ns_launch_explicit_terms(int n, const NsParams& q, const float2* w, float2* out, float2* g,
                              float2* pbuf, int batch, hipStream_t s);

@@ -131,7 +131,7 @@ __device__ __forceinline__ void build_spectrum(ColShared<N, W>& sm, const WTile<
 // Inverse column transforms of the tile -> G[0..NSPEC): the four derivative
 // spectra (NSPEC = 4) or the field itself (NSPEC = 1).  sm.w0 / sm.wn must be
 // visible (barrier) before the call.
-template <int N, int W, int NSPEC = 4>
+template <int N, int W, int NSPEC = 4, int R = 8>
 __device__ __forceinline__ void cols_inverse_body(ColShared<N, W>& sm, const WTile<N, W>& wt,
                                                   const NsParams& q, float2* __restrict__ g,
                                                   int c0, int tid) {
@@ -141,7 +141,7 @@ __device__ __forceinline__ void cols_inverse_body(ColShared<N, W>& sm, const WTi
   for (int m = 0; m < NSPEC; ++m) {
     build_spectrum<N, W>(sm, wt, q, NSPEC == 1 ? 4 : m, c0, tid);
     __syncthreads();
-    lds_fft<N, W, kThreads, W, 1, true, true>(sm.buf, sm.tw, tid);
+    lds_fft<N, W, kThreads, W, 1, true, true, R>(sm.buf, sm.tw, tid);
     float2* gm = g + m * gstride;
     for (int e = tid; e < N * W; e += kThreads) {
       const int p = e / W, c = e % W;
@@ -175,7 +175,7 @@ __device__ __forceinline__ void load_wavenumbers(ColShared<N, W>& sm, const NsPa
   for (int c = tid; c <= N / 2; c += kThreads) sm.ky[c] = q.ky[c];
 }
 
-template <int N, int W, int NSPEC>
+template <int N, int W, int NSPEC, int R = 8>
 __global__ __launch_bounds__(kThreads) void ns_cols_inverse(NsParams q, const float2* __restrict__ w,
                                                             float2* __restrict__ g) {
   __shared__ ColShared<N, W> sm;
@@ -189,7 +189,7 @@ __global__ __launch_bounds__(kThreads) void ns_cols_inverse(NsParams q, const fl
   load_twiddles(sm.tw, q.tw, N, 1, tid, kThreads);
   load_wavenumbers(sm, q, tid);
   __syncthreads();
-  cols_inverse_body<N, W, NSPEC>(sm, wt, q, g, c0, tid);
+  cols_inverse_body<N, W, NSPEC, R>(sm, wt, q, g, c0, tid);
 }
 
 // Packed real row transforms of a batch of real (N, N) fields.
@@ -243,7 +243,7 @@ __global__ __launch_bounds__(kThreads) void rows_c2r(NsParams q, const float2* _
   for (int e = tid; e < RB * N; e += kThreads) x[static_cast<size_t>(i0) * N + e] = zf[e];
 }
 
-template <int N, int RB>
+template <int N, int RB, int R = 8>
 __global__ __launch_bounds__(kThreads) void ns_rows(NsParams q, const float2* __restrict__ g,
                                                     float2* __restrict__ pout) {
   constexpr int H = N / 2;
@@ -267,7 +267,7 @@ __global__ __launch_bounds__(kThreads) void ns_rows(NsParams q, const float2* __
     rows[e] = c2r_combine(row[k], row[(H - k) & (H - 1)], k, twn[k]);
   }
   __syncthreads();
-  lds_fft<H, 4 * RB, kThreads, 1, H, false, true>(rows, twh, tid);
+  lds_fft<H, 4 * RB, kThreads, 1, H, false, true, R>(rows, twh, tid);
   // advection = -(grad_x * vx + grad_y * vy)  (spectral/equations.py:170)
   const float* gx = reinterpret_cast<const float*>(rows);
   const float* gy = gx + RB * N;
@@ -276,7 +276,7 @@ __global__ __launch_bounds__(kThreads) void ns_rows(NsParams q, const float2* __
   float* pr = reinterpret_cast<float*>(prod);
   for (int e = tid; e < RB * N; e += kThreads) pr[e] = -(gx[e] * vx[e] + gy[e] * vy[e]);
   __syncthreads();
-  lds_fft<H, RB, kThreads, 1, H, false, false>(prod, twh, tid);
+  lds_fft<H, RB, kThreads, 1, H, false, false, R>(prod, twh, tid);
   for (int e = tid; e < RB * H; e += kThreads) {
     const int r = e / H, k = e % H;
     pout[static_cast<size_t>(i0 + r) * H + k] = r2c_combine(prod + r * H, k, H, twn[k]);
@@ -292,7 +292,7 @@ __device__ __forceinline__ float ns_filter(const NsParams& q, int p, int c) {
 
 // MODE 0: write F (explicit terms) to u_out.  MODE 1: RK/CN update of (u, h).
 // MODE 2: update, then the next stage's inverse column work on the new tile.
-template <int N, int W, int MODE>
+template <int N, int W, int MODE, int R = 8>
 __global__ __launch_bounds__(kThreads) void ns_cols_forward(NsParams q, const float2* __restrict__ pin,
                                                             const float2* u_in, float2* u_out,
                                                             float2* h, float2* __restrict__ g) {
@@ -317,7 +317,7 @@ __global__ __launch_bounds__(kThreads) void ns_cols_forward(NsParams q, const fl
   load_twiddles(sm.tw, q.tw, N, 1, tid, kThreads);
   if constexpr (MODE == 2) load_wavenumbers(sm, q, tid);
   __syncthreads();
-  lds_fft<N, W, kThreads, W, 1, true, false>(sm.buf, sm.tw, tid);
+  lds_fft<N, W, kThreads, W, 1, true, false, R>(sm.buf, sm.tw, tid);
 
   // F at (p, cg) for this thread's elements; tile 0 also produces column N/2.
   float2 f[PER], fh[PER];
@@ -409,7 +409,7 @@ __global__ __launch_bounds__(kThreads) void ns_cols_forward(NsParams q, const fl
     }
     if constexpr (MODE == 2) {
       __syncthreads();
-      cols_inverse_body<N, W>(sm, wt, q, g, c0, tid);
+      cols_inverse_body<N, W, 4, R>(sm, wt, q, g, c0, tid);
     }
   }
 }
@@ -448,30 +448,102 @@ struct NsOps {
     hipLaunchKernelGGL((ns_rows<N, RB>), dim3(N / RB, batch), dim3(kThreads), 0, s, q, g, pbuf);
     return check_hip(hipGetLastError(), "ns_rows");
   }
-  static int cols_forward(int mode, const NsParams& q, const float2* pbuf, const float2* u_in,
-                          float2* u_out, float2* h, float2* g, int batch, hipStream_t s) {
-    const dim3 grid(N / 2 / W, batch);
-    if (mode == 0)
-      hipLaunchKernelGGL((ns_cols_forward<N, W, 0>), grid, dim3(kThreads), 0, s, q, pbuf, u_in, u_out, h, g);
-    else if (mode == 1)
-      hipLaunchKernelGGL((ns_cols_forward<N, W, 1>), grid, dim3(kThreads), 0, s, q, pbuf, u_in, u_out, h, g);
-    else
-      hipLaunchKernelGGL((ns_cols_forward<N, W, 2>), grid, dim3(kThreads), 0, s, q, pbuf, u_in, u_out, h, g);
+  // Forward column transforms of P -> explicit terms F (no update).
+  static int explicit_cols(const NsParams& q, const float2* pbuf, float2* out, int batch, hipStream_t s) {
+    hipLaunchKernelGGL((ns_cols_forward<N, W, 0>), dim3(N / 2 / W, batch), dim3(kThreads), 0, s, q, pbuf,
+                       nullptr, out, nullptr, nullptr);
     return check_hip(hipGetLastError(), "ns_cols_forward");
   }
 };
 
+// ---- tunable step kernels ---------------------------------------------------
+// The step's column kernels are built for a menu of tile widths W and its row
+// kernel for a menu of rows per workgroup RB, each with both radix schedules;
+// NsParams carries a plan's choice (w_cols, rb_rows, radix).  Tile shapes
+// regroup the same per-column / per-row arithmetic and never change results.
+constexpr int kNsLdsMax = 96 * 1024;
+
+template <int N, int W>
+constexpr bool ns_cols_ok() {
+  return W <= N / 2 && N * W <= 16 * kThreads && sizeof(ColShared<N, W>) <= kNsLdsMax;
+}
+template <int N, int RB>
+constexpr bool ns_rows_ok() {
+  return RB <= N / 4 && (5 * RB * (N / 2) + N + N / 2) * 8 <= kNsLdsMax;
+}
+
+// which: 0 cols-inverse, 1 rows, 2 cols-forward + update + next inverse,
+// 3 cols-forward + final update.  JCFD_ERR_UNSUPPORTED off the menu.
+template <int N, int T, int R>
+int ns_variant(int which, const NsParams& q, const float2* a, const float2* u_in, float2* u_out,
+               float2* h, float2* g, int batch, hipStream_t s) {
+  if (which == 1) {
+    if constexpr (ns_rows_ok<N, T>()) {
+      hipLaunchKernelGGL((ns_rows<N, T, R>), dim3(N / T, batch), dim3(kThreads), 0, s, q, a, g);
+      return JCFD_OK;
+    }
+    return JCFD_ERR_UNSUPPORTED;
+  }
+  if constexpr (ns_cols_ok<N, T>()) {
+    const dim3 grid(N / 2 / T, batch);
+    if (which == 0)
+      hipLaunchKernelGGL((ns_cols_inverse<N, T, 4, R>), grid, dim3(kThreads), 0, s, q, a, g);
+    else if (which == 2)
+      hipLaunchKernelGGL((ns_cols_forward<N, T, 2, R>), grid, dim3(kThreads), 0, s, q, a, u_in, u_out, h, g);
+    else
+      hipLaunchKernelGGL((ns_cols_forward<N, T, 1, R>), grid, dim3(kThreads), 0, s, q, a, u_in, u_out, h, g);
+    return JCFD_OK;
+  }
+  return JCFD_ERR_UNSUPPORTED;
+}
+
+template <int N, int R>
+int ns_tile(int which, int tile, const NsParams& q, const float2* a, const float2* u_in, float2* u_out,
+            float2* h, float2* g, int batch, hipStream_t s) {
+  switch (tile) {
+    case 1: return ns_variant<N, 1, R>(which, q, a, u_in, u_out, h, g, batch, s);
+    case 2: return ns_variant<N, 2, R>(which, q, a, u_in, u_out, h, g, batch, s);
+    case 4: return ns_variant<N, 4, R>(which, q, a, u_in, u_out, h, g, batch, s);
+    case 8: return which == 1 ? JCFD_ERR_UNSUPPORTED : ns_variant<N, 8, R>(which, q, a, u_in, u_out, h, g, batch, s);
+    case 16: return which == 1 ? JCFD_ERR_UNSUPPORTED : ns_variant<N, 16, R>(which, q, a, u_in, u_out, h, g, batch, s);
+    default: return JCFD_ERR_UNSUPPORTED;
+  }
+}
+
+// One tuned launch.  For rows (which 1) `a` is G and the product spectrum is
+// written to g; for the column kernels `a` is the state (0) or P (2, 3).
+template <int N>
+int ns_launch(int which, const NsParams& q, const float2* a, const float2* u_in, float2* u_out,
+              float2* h, float2* g, int batch, hipStream_t s) {
+  const int tile = which == 1 ? q.rb_rows : q.w_cols;
+  const int rc = q.radix == 4 ? ns_tile<N, 4>(which, tile, q, a, u_in, u_out, h, g, batch, s)
+                              : ns_tile<N, 8>(which, tile, q, a, u_in, u_out, h, g, batch, s);
+  if (rc) return fail(rc, "spectral tile shape not built for n = " + std::to_string(N));
+  return check_hip(hipGetLastError(), "spectral step launch");
+}
+
 template <int N>
 int ns_step_impl(const NsParams* stages, const NsBuffers& b, int batch, hipStream_t s) {
-  int rc = NsOps<N>::cols_inverse(stages[0], b.u_in, b.g, batch, s);
+  int rc = ns_launch<N>(0, stages[0], b.u_in, nullptr, nullptr, nullptr, b.g, batch, s);
   for (int k = 0; k < 5 && rc == JCFD_OK; ++k) {
-    rc = NsOps<N>::rows(stages[k], b.g, b.pbuf, batch, s);
+    rc = ns_launch<N>(1, stages[k], b.g, nullptr, nullptr, nullptr, b.pbuf, batch, s);
     if (rc) break;
     const float2* u_src = (k == 0) ? b.u_in : b.u_out;
-    rc = NsOps<N>::cols_forward(k < 4 ? 2 : 1, stages[k], b.pbuf, u_src, b.u_out, b.h, b.g,
-                                batch, s);
+    rc = ns_launch<N>(k < 4 ? 2 : 3, stages[k], b.pbuf, u_src, b.u_out, b.h, b.g, batch, s);
   }
   return rc;
+}
+
+template <int N>
+bool ns_tile_on_menu(int which, int tile) {
+  switch (tile) {
+    case 1: return which == 1 ? ns_rows_ok<N, 1>() : ns_cols_ok<N, 1>();
+    case 2: return which == 1 ? ns_rows_ok<N, 2>() : ns_cols_ok<N, 2>();
+    case 4: return which == 1 ? ns_rows_ok<N, 4>() : ns_cols_ok<N, 4>();
+    case 8: return which != 1 && ns_cols_ok<N, 8>();
+    case 16: return which != 1 && ns_cols_ok<N, 16>();
+    default: return false;
+  }
 }
 
 template <int N>
@@ -479,7 +551,7 @@ int ns_explicit_impl(const NsParams& q, const float2* w, float2* out, float2* g,
                      int batch, hipStream_t s) {
   int rc = NsOps<N>::cols_inverse(q, w, g, batch, s);
   if (!rc) rc = NsOps<N>::rows(q, g, pbuf, batch, s);
-  if (!rc) rc = NsOps<N>::cols_forward(0, q, pbuf, nullptr, out, nullptr, nullptr, batch, s);
+  if (!rc) rc = NsOps<N>::explicit_cols(q, pbuf, out, batch, s);
   return rc;
 }
 
@@ -491,13 +563,43 @@ namespace {
 template <int N>
 int ns_one_impl(int which, const NsParams* stages, const NsBuffers& b, int batch, hipStream_t s) {
   switch (which) {
-    case 0: return NsOps<N>::cols_inverse(stages[0], b.u_in, b.g, batch, s);
-    case 1: return NsOps<N>::rows(stages[0], b.g, b.pbuf, batch, s);
-    case 2: return NsOps<N>::cols_forward(2, stages[1], b.pbuf, b.u_out, b.u_out, b.h, b.g, batch, s);
-    default: return NsOps<N>::cols_forward(1, stages[4], b.pbuf, b.u_out, b.u_out, b.h, b.g, batch, s);
+    case 0: return ns_launch<N>(0, stages[0], b.u_in, nullptr, nullptr, nullptr, b.g, batch, s);
+    case 1: return ns_launch<N>(1, stages[0], b.g, nullptr, nullptr, nullptr, b.pbuf, batch, s);
+    case 2: return ns_launch<N>(2, stages[1], b.pbuf, b.u_out, b.u_out, b.h, b.g, batch, s);
+    default: return ns_launch<N>(3, stages[4], b.pbuf, b.u_out, b.u_out, b.h, b.g, batch, s);
   }
 }
+
+template <int N> struct NsDefaultOp {
+  static void run(NsParams* q) {
+    q->w_cols = NsTiles<N>::W;
+    q->rb_rows = NsTiles<N>::RB;
+    q->radix = 8;
+  }
+};
 }  // namespace
+
+int ns_default_tiles(NsParams* q, int n) {
+  switch (n) {
+    case 32: NsDefaultOp<32>::run(q); return JCFD_OK;
+    case 64: NsDefaultOp<64>::run(q); return JCFD_OK;
+    case 128: NsDefaultOp<128>::run(q); return JCFD_OK;
+    case 256: NsDefaultOp<256>::run(q); return JCFD_OK;
+    case 512: NsDefaultOp<512>::run(q); return JCFD_OK;
+    default: return fail(JCFD_ERR_UNSUPPORTED, "spectral grid size not built");
+  }
+}
+
+bool ns_tile_supported(int n, int which, int tile) {
+  switch (n) {
+    case 32: return ns_tile_on_menu<32>(which, tile);
+    case 64: return ns_tile_on_menu<64>(which, tile);
+    case 128: return ns_tile_on_menu<128>(which, tile);
+    case 256: return ns_tile_on_menu<256>(which, tile);
+    case 512: return ns_tile_on_menu<512>(which, tile);
+    default: return false;
+  }
+}
 
 int ns_launch_kernel(int n, int which, const NsParams* stages, const NsBuffers& b, int batch,
                      hipStream_t s) {

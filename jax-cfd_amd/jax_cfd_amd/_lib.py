@@ -104,6 +104,8 @@ _SIGNATURES = {
                                              ctypes.POINTER(ctypes.c_double), _P]),
     "jcfd_fvm_kernel_bytes": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int64)]),
     "jcfd_fvm_spectral_filter": (ctypes.c_int, [_P, _P, _P, _P, _P]),
+    "jcfd_ns2d_plan_get_tiles": (ctypes.c_int, [_P, _P]),
+    "jcfd_ns2d_plan_set_tiles": (ctypes.c_int, [_P, _P]),
     "jcfd_fvm_plan_get_tiles": (ctypes.c_int, [_P, _P]),
     "jcfd_fvm_plan_set_tiles": (ctypes.c_int, [_P, _P]),
     "jcfd_fvm_group_create": (ctypes.c_int, [ctypes.POINTER(_P)]),
@@ -122,6 +124,11 @@ class FvmTiles(ctypes.Structure):
   """jcfd_fvm_tiles."""
   _fields_ = [("rows_forward", ctypes.c_int32), ("poisson_cols", ctypes.c_int32),
               ("rows_project", ctypes.c_int32), ("radix", ctypes.c_int32)]
+
+
+class Ns2dTiles(ctypes.Structure):
+  """jcfd_ns2d_tiles."""
+  _fields_ = [("cols", ctypes.c_int32), ("rows", ctypes.c_int32), ("radix", ctypes.c_int32)]
 
 
 class FvmGroupMember(ctypes.Structure):

@@ -166,12 +166,20 @@ FAMILIES = {'fvm': FvmFamily, 'spectral': SpectralFamily}
 
 # ------------------------------------------------------------------ pools
 class Pool:
-  """Device-resident slots of one (solver, n) with one batched plan."""
+  """Device-resident slots of one (solver, n) with one batched plan.  With
+  `tune`, the plan's tile shapes are autotuned for its capacity (radix-8
+  schedule kept, so results stay bit-identical to untuned plans)."""
 
-  def __init__(self, family, capacity: int):
+  def __init__(self, family, capacity: int, tune: bool = False):
     self.family = family
     self.capacity = capacity
     self.plan = family.make_plan(capacity)
+    if tune:
+      from jax_cfd_amd import autotune
+      if family.name == 'fvm':
+        autotune.tune_fvm(self.plan, radices=(8,))
+      else:
+        autotune.tune_ns2d(self.plan, radices=(8,))
     self.state = family.alloc_state(capacity)
     self.stream = torch.cuda.Stream(device=family.device)  # default priority
     # active slot list; a fixed buffer so captured graphs stay valid
@@ -240,7 +248,8 @@ class RolloutServer:
 
   def __init__(self, hbm_budget_bytes: float = 200e9,
                capacity: Optional[Dict[int, int]] = None, max_chunk: int = 128,
-               device: Optional[torch.device] = None):
+               device: Optional[torch.device] = None, tune: bool = False):
+    self.tune = bool(tune)
     self.device = device if device is not None else torch.device(
         'cuda', torch.cuda.current_device())
     self.budget = float(hbm_budget_bytes)
@@ -275,7 +284,7 @@ class RolloutServer:
     cap = int(max(1, min(cap, (self.budget - self.reserved) // per_slot)))
     if self.reserved + cap * per_slot > self.budget:
       return None
-    pool = Pool(fam, cap)
+    pool = Pool(fam, cap, tune=self.tune)
     self.pools[key] = pool
     self.reserved += pool.bytes
     return pool

@@ -68,6 +68,19 @@ class Ns2dPlan:
     _lib.check(self._lib.jcfd_ns2d_plan_set_graphs(self._handle, int(bool(enabled))),
                'jcfd_ns2d_plan_set_graphs')
 
+  @property
+  def tiles(self) -> dict:
+    """Tile shapes and radix schedule of the step kernels."""
+    t = _lib.Ns2dTiles()
+    _lib.check(self._lib.jcfd_ns2d_plan_get_tiles(self._handle, ctypes.byref(t)),
+               'jcfd_ns2d_plan_get_tiles')
+    return {k: int(getattr(t, k)) for k, _ in _lib.Ns2dTiles._fields_}
+
+  def set_tiles(self, cols: int, rows: int, radix: int = 8):
+    t = _lib.Ns2dTiles(cols, rows, radix)
+    _lib.check(self._lib.jcfd_ns2d_plan_set_tiles(self._handle, ctypes.byref(t)),
+               'jcfd_ns2d_plan_set_tiles')
+
   def close(self):
     if getattr(self, '_handle', None) is not None and self._handle.value:
       self._lib.jcfd_ns2d_plan_destroy(self._handle)

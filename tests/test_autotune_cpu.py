@@ -83,5 +83,44 @@ def test_measure_restores_tiles_on_error():
     raise RuntimeError('device error')
   plan.time_kernels = boom
   with pytest.raises(RuntimeError):
-    autotune.measure_fvm(plan, None, None, radices=(8,), reps=1)
+    autotune.measure(plan, autotune.FVM, lambda: plan.time_kernels(None, None, 1), radices=(8,))
   assert plan.tiles == dict(rows_forward=4, poisson_cols=8, rows_project=4, radix=8)
+
+
+class FakeNsPlan:
+  """Spectral stand-in: the column tile sets three kernels launched 1, 4 and
+  1 times per step; ns_cols_inverse alone prefers 4, the 4x-launched
+  ns_cols_update_inverse prefers 8, so the weighted choice is 8."""
+
+  class spec:
+    n = 128
+
+  batch = 2
+  device = 'cpu'
+
+  def __init__(self):
+    self._tiles = dict(cols=8, rows=2, radix=8)
+
+  @property
+  def tiles(self):
+    return dict(self._tiles)
+
+  def set_tiles(self, cols, rows, radix=8):
+    if rows == 8 or rows == 16:
+      raise NotImplementedError('row tile off the menu')
+    self._tiles = dict(cols=cols, rows=rows, radix=radix)
+
+  def time_kernels(self, w, reps):
+    t = self._tiles
+    return {'ns_cols_inverse': 1 + abs(t['cols'] - 4),
+            'ns_rows': 1 + abs(t['rows'] - 1),
+            'ns_cols_update_inverse': 1 + abs(t['cols'] - 8),
+            'ns_cols_update': 1.0}
+
+
+def test_spectral_choice_weights_kernels_by_launches(tmp_path, monkeypatch):
+  monkeypatch.setenv('JCFD_TUNE_CACHE', str(tmp_path / 'tiles.json'))
+  plan = FakeNsPlan()
+  res = autotune.tune_ns2d(plan, w=object(), reps=1)
+  assert plan.tiles == dict(cols=8, rows=1, radix=8)
+  assert res['timings']['8']['rows'].keys() == {'1', '2', '4'}

@@ -96,3 +96,47 @@ def test_autotuned_plan_steps_identically_and_group_rejects_radix4(tmp_path, mon
   group = _fvm.FvmGroup(_dev())
   with pytest.raises(NotImplementedError):
     group.step([(plan, u, v, [0])], 1)
+
+
+def _spectral(n, batch=1):
+  from jax_cfd_amd import serving
+  from jax_cfd_amd.spectral import _ns2d
+  fam = serving.SpectralFamily(n, _dev())
+  return fam, _ns2d.Ns2dPlan(fam.spec, batch, _dev())
+
+
+@pytest.mark.parametrize("n", [64, 512])
+def test_spectral_tile_shapes_bit_identical_and_radix4_close(n):
+  fam, plan = _spectral(n)
+  (w0,) = fam.initial_state(7)
+  ref = plan.step(w0, 3)
+  default = plan.tiles
+  tried = 0
+  for cols in (1, 2, 4, 8, 16):
+    for rows in (1, 2, 4):
+      try:
+        plan.set_tiles(cols, rows, 8)
+      except NotImplementedError:
+        continue
+      print(f"spectral n={n} cols={cols} rows={rows}", flush=True)
+      out = plan.step(w0, 3)
+      assert torch.equal(out, ref), (cols, rows)
+      tried += 1
+  assert tried >= 8
+  plan.set_tiles(default['cols'], default['rows'], 4)
+  r4 = plan.step(w0, 3)
+  rel = float(torch.linalg.vector_norm(r4 - ref) / torch.linalg.vector_norm(ref))
+  print(f"spectral n={n} radix 4 vs 8 after 3 steps: rel L2 {rel:.2e}", flush=True)
+  assert rel <= 1e-5
+
+
+def test_spectral_autotune_batched(tmp_path, monkeypatch):
+  from jax_cfd_amd import autotune
+  monkeypatch.setenv('JCFD_TUNE_CACHE', str(tmp_path / 'tiles.json'))
+  fam, plan = _spectral(128, batch=4)
+  w = torch.stack([fam.initial_state(s)[0] for s in range(4)])
+  ref = plan.step(w, 3)
+  res = autotune.tune_ns2d(plan, radices=(8,), reps=5)
+  print("spectral tuned:", {k: res[k] for k in ('cols', 'rows', 'radix')}, flush=True)
+  out = plan.step(w, 3)
+  assert torch.equal(out, ref)
