@@ -49,7 +49,50 @@ def _args():
   ap.add_argument("--adjoint", type=int, default=1, help="time the C4 gradient (0 skips)")
   ap.add_argument("--cg-steps", type=int, default=100,
                   help="steps of the solve_cg variant of the 256^2 run (0 = skip)")
+  ap.add_argument("--copy-mib", type=int, default=0,
+                  help="first copy this many MiB device-to-device once: a kernel of "
+                       "known traffic that calibrates rocprofv3's FETCH_SIZE/WRITE_SIZE")
+  ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
+                  help="summarise two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE; "
+                       "both made with --copy-mib) into HBM bytes per dispatch, then exit")
   return ap.parse_args()
+
+
+def _pmc_rows(root, counter):
+  """{kernel name: [counter value per dispatch]} from rocprofv3's
+  *counter_collection.csv files under `root`."""
+  import csv
+  out = {}
+  for dirpath, _, files in os.walk(root):
+    for fn in files:
+      if not fn.endswith("counter_collection.csv"):
+        continue
+      with open(os.path.join(dirpath, fn), newline="") as f:
+        for row in csv.DictReader(f):
+          if row.get("Counter_Name") != counter:
+            continue
+          out.setdefault(row["Kernel_Name"], []).append(float(row["Counter_Value"]))
+  return out
+
+
+def _pmc_summary(fetch_dir, write_dir, copy_mib):
+  """HBM bytes per dispatch of every kernel: FETCH_SIZE and WRITE_SIZE (KiB
+  per dispatch), each scaled so that the calibration copy of `copy_mib` MiB
+  reads and writes exactly its size."""
+  fetch, write = _pmc_rows(fetch_dir, "FETCH_SIZE"), _pmc_rows(write_dir, "WRITE_SIZE")
+  mean = lambda xs: sum(xs) / len(xs)  # noqa: E731
+  copy_kib = copy_mib * 1024.0
+  # the calibration copy is the kernel whose mean fetch is closest to its size
+  cand = [k for k in fetch if k in write]
+  ref = min(cand, key=lambda k: abs(mean(fetch[k]) - copy_kib) + abs(mean(write[k]) - copy_kib))
+  f_scale, w_scale = copy_kib / mean(fetch[ref]), copy_kib / mean(write[ref])
+  kernels = {}
+  for k in cand:
+    kernels[k] = {"dispatches": len(fetch[k]),
+                  "fetch_bytes": mean(fetch[k]) * 1024.0 * f_scale,
+                  "write_bytes": mean(write[k]) * 1024.0 * w_scale}
+  return {"calibration_kernel": ref, "copy_mib": copy_mib,
+          "fetch_scale": f_scale, "write_scale": w_scale, "kernels": kernels}
 
 
 def _traffic(kernel, workload):
@@ -328,6 +371,9 @@ def _adjoint(dev, n=256, steps=64, every=8, reps=3):
 
 def main():
   args = _args()
+  if args.pmc_summary:
+    print(json.dumps(_pmc_summary(*args.pmc_summary, args.copy_mib), indent=1))
+    return
   if args.gpus != 1:
     print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported here "
           "(independent instances shard across GPUs with no communication; "
@@ -339,6 +385,12 @@ def main():
 
   dev = torch.device("cuda", 0)
   torch.cuda.set_device(dev)
+  if args.copy_mib > 0:
+    src = torch.ones(args.copy_mib * 2 ** 18, dtype=torch.float32, device=dev)
+    dst = torch.empty_like(src)
+    torch.mul(src, 2.0, out=dst)  # one elementwise kernel: reads and writes copy_mib MiB
+    torch.cuda.synchronize()
+    del src, dst
   grid = cfd.grids.Grid((N, N), domain=((0, 2 * np.pi), (0, 2 * np.pi)))
   dt = cfd.equations.stable_time_step(VMAX, CFL, NU, grid)
   forcing = cfd.forcings.simple_turbulence_forcing(

@@ -734,25 +734,49 @@ template <int PHASE>
 __global__ __launch_bounds__(kThreads) void fvm_group_phase(FvmGroupQueue q) {
   extern __shared__ float4 fvm_group_smem[];
   char* smem = reinterpret_cast<char*>(fvm_group_smem);
+  constexpr int kDescWords = static_cast<int>(sizeof(FvmGroupDesc) / 4);
+  static_assert(sizeof(FvmGroupDesc) % 4 == 0 && kDescWords <= 64, "descriptor fits one wave");
   const int tid = threadIdx.x;
+  const int lane = tid & 63;
   const int total = q.n_items[PHASE];
-  for (int it = blockIdx.x; it < total; it += gridDim.x) {
-    const FvmGroupItem w = q.items[PHASE][it];
-    const FvmGroupDesc d = q.descs[w.desc];
+  // The next item's tile and descriptor are fetched while the current item
+  // runs: lane k of every wave loads dword k of the descriptor into one
+  // register, and readlane broadcasts it when the item starts.
+  auto fetch = [&](int idx, int& tile, unsigned& word) {
+    const FvmGroupItem w = q.items[PHASE][idx];
+    tile = w.tile;
+    word = lane < kDescWords ? reinterpret_cast<const unsigned*>(q.descs + w.desc)[lane] : 0u;
+  };
+  int it = blockIdx.x;
+  if (it >= total) return;
+  int tile = 0;
+  unsigned word = 0;
+  fetch(it, tile, word);
+  for (;;) {
+    FvmGroupDesc d;
+    unsigned* dw = reinterpret_cast<unsigned*>(&d);
+#pragma unroll
+    for (int k = 0; k < kDescWords; ++k)
+      dw[k] = static_cast<unsigned>(__builtin_amdgcn_readlane(static_cast<int>(word), k));
+    const int cur = tile;
+    const int nxt = it + static_cast<int>(gridDim.x);
+    if (nxt < total) fetch(nxt, tile, word);
     // A fresh copy of the thread index per item: keeps the compiler from
     // hoisting every per-thread address of every tile body out of this loop
     // (which would hold them all in registers across items).
     int t = tid;
     asm volatile("" : "+v"(t));
     switch (d.p.n) {
-      case 32: group_tile<32, PHASE>(d, w.tile, t, smem); break;
-      case 64: group_tile<64, PHASE>(d, w.tile, t, smem); break;
-      case 128: group_tile<128, PHASE>(d, w.tile, t, smem); break;
-      case 256: group_tile<256, PHASE>(d, w.tile, t, smem); break;
-      case 512: group_tile<512, PHASE>(d, w.tile, t, smem); break;
+      case 32: group_tile<32, PHASE>(d, cur, t, smem); break;
+      case 64: group_tile<64, PHASE>(d, cur, t, smem); break;
+      case 128: group_tile<128, PHASE>(d, cur, t, smem); break;
+      case 256: group_tile<256, PHASE>(d, cur, t, smem); break;
+      case 512: group_tile<512, PHASE>(d, cur, t, smem); break;
       default: break;  // rejected on the host
     }
     __syncthreads();  // LDS is reused by the next item
+    if (nxt >= total) break;
+    it = nxt;
   }
 }
 
