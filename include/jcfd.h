@@ -121,7 +121,8 @@ int jcfd_fvm_plan_set_graphs(jcfd_fvm_plan* plan, int32_t enabled);
  * device.  Tile shapes never change results; radix 4 instead of 8 changes
  * the FFT rounding (results stay within fp32 round-off of each other). */
 typedef struct {
-  int32_t rows_forward; /* rows per workgroup of the stencil kernel: 1, 2, 4, 8 */
+  int32_t rows_forward; /* rows per workgroup of the stencil kernel: 1, 2, 4, 8; 16, 32
+                           march over the rows with a ring of staged rows (n 256, 512) */
   int32_t poisson_cols; /* spectrum columns per workgroup: 1, 2, 4, 8, 16 */
   int32_t rows_project; /* rows per workgroup of the projection kernel: 1, 2, 4, 8 */
   int32_t radix;        /* largest FFT radix: 8 or 4 */

@@ -92,6 +92,21 @@ struct Window {
   __device__ __forceinline__ float V(int lr, int j) const { return sv[lr * N + (j & (N - 1))]; }
 };
 
+// Byte offset into a workgroup's LDS buffer as a typed pointer.
+template <typename T>
+__device__ __forceinline__ T* lds_at(char* smem, int offset) {
+  return reinterpret_cast<T*>(smem + offset);
+}
+
+// The same rows held in a ring of K slots: local row a lives in slot a % K.
+template <int N, int K>
+struct RingWindow {
+  const float* su;
+  const float* sv;
+  __device__ __forceinline__ float U(int a, int j) const { return su[(a & (K - 1)) * N + (j & (N - 1))]; }
+  __device__ __forceinline__ float V(int a, int j) const { return sv[(a & (K - 1)) * N + (j & (N - 1))]; }
+};
+
 // Face fluxes of the convection term, each evaluated once per face.  With
 // the indexing below a cell at local window row lr, column j reads
 //   adv_u = -((FX(lr + 1, j) - FX(lr, j)) / dx + (FY(lr, j) - FY(lr, j - 1)) / dy)
@@ -104,8 +119,8 @@ struct Window {
 // (advection.py:387-395): advect_general with linear face velocities and
 // the TVD Lax-Wendroff / van_leer_limiter face value.  The operation order
 // of each face and of the differences is the reference's.
-template <int N>
-__device__ __forceinline__ float face_fx_u(const Window<N>& w, int a, int j, const FvmParams& p) {
+template <typename WIN>
+__device__ __forceinline__ float face_fx_u(const WIN& w, int a, int j, const FvmParams& p) {
   const float h = p.dt_over_dx;
   if (p.convect == JCFD_CONVECT_VAN_LEER_LIMITERS) {
     const int c = a - 1;  // face between rows c and c + 1
@@ -116,8 +131,8 @@ __device__ __forceinline__ float face_fx_u(const Window<N>& w, int a, int j, con
   return vl_flux(Uf, w.U(a - 1, j), w.U(a, j), w.U(a - 2, j), w.U(a - 1, j), w.U(a, j), w.U(a + 1, j), h);
 }
 
-template <int N>
-__device__ __forceinline__ float face_fy_u(const Window<N>& w, int lr, int b, const FvmParams& p) {
+template <typename WIN>
+__device__ __forceinline__ float face_fy_u(const WIN& w, int lr, int b, const FvmParams& p) {
   const float h = p.dt_over_dy;
   if (p.convect == JCFD_CONVECT_VAN_LEER_LIMITERS) {
     const float Vf = 0.5f * w.V(lr, b) + 0.5f * w.V(lr + 1, b);
@@ -129,8 +144,8 @@ __device__ __forceinline__ float face_fy_u(const Window<N>& w, int lr, int b, co
                  w.U(a, b + 2), h);
 }
 
-template <int N>
-__device__ __forceinline__ float face_fx_v(const Window<N>& w, int a, int j, const FvmParams& p) {
+template <typename WIN>
+__device__ __forceinline__ float face_fx_v(const WIN& w, int a, int j, const FvmParams& p) {
   const float h = p.dt_over_dx;
   const int c = a - 1;
   if (p.convect == JCFD_CONVECT_VAN_LEER_LIMITERS) {
@@ -143,8 +158,8 @@ __device__ __forceinline__ float face_fx_v(const Window<N>& w, int a, int j, con
                  w.V(c + 2, b), h);
 }
 
-template <int N>
-__device__ __forceinline__ float face_fy_v(const Window<N>& w, int lr, int j, const FvmParams& p) {
+template <typename WIN>
+__device__ __forceinline__ float face_fy_v(const WIN& w, int lr, int j, const FvmParams& p) {
   const float h = p.dt_over_dy;
   if (p.convect == JCFD_CONVECT_VAN_LEER_LIMITERS) {
     const float Vf = 0.5f * w.V(lr, j) + 0.5f * w.V(lr, j + 1);
@@ -162,8 +177,8 @@ __device__ __forceinline__ float over_rho(float f, float rho) { return rho == 1.
 // k += nu/rho lap + forcing/rho for one component, in the reference's order
 // (equations.py:102-110, finite_differences.py:127-133 for the Laplacian).
 // V_COMP selects v (no forcing column) instead of u.
-template <int N, bool V_COMP>
-__device__ __forceinline__ float add_diffusion_forcing(const Window<N>& w, int lr, int j,
+template <bool V_COMP, typename WIN>
+__device__ __forceinline__ float add_diffusion_forcing(const WIN& w, int lr, int j,
                                                        const FvmParams& p, float fx_j, float k) {
   auto c = [&](int r, int jj) { return V_COMP ? w.V(r, jj) : w.U(r, jj); };
   if (p.has_diffusion) {
@@ -221,6 +236,148 @@ struct RowsForwardLds {
   static_assert(!(kFft && kWin) || RB * H * 8 <= 2 * WR * N * 4, "zb must fit in the window");
 };
 
+// Marching variant of kernel A (full step only), for n >= 256: a workgroup
+// walks RB rows i0-1 .. i0+RB-1 one at a time over a ring of 8 staged rows of
+// u and v (each row read once per workgroup, plus a 4-row halo), fetching
+// the next row into registers while the current row is computed.  x-face
+// fluxes carry over from row to row in registers; y-face fluxes and v* of
+// the current row go through LDS for the column neighbours; div rows collect
+// FB at a time and are transformed together.  Same per-cell arithmetic as
+// rows_forward_body, so the same results bit for bit.
+template <int N, int RB>
+struct MarchLds {
+  static constexpr int H = N / 2, K = 8, FB = 4;
+  static constexpr int twn = 0;                  // float2[N]
+  static constexpr int twh = twn + N * 8;        // float2[H]
+  static constexpr int zb = twh + H * 8;         // float2[FB * H]: div rows, then FFTs
+  static constexpr int su = zb + FB * H * 8;     // float[K * N]: ring of u rows
+  static constexpr int sv = su + K * N * 4;      // float[K * N]: ring of v rows
+  static constexpr int fy = sv + K * N * 4;      // float[N]: y-face fluxes of u, current row
+  static constexpr int gy = fy + N * 4;          // float[N]: y-face fluxes of v, current row
+  static constexpr int vs = gy + N * 4;          // float[N]: v* of the current row
+  static constexpr int bytes = vs + N * 4;
+};
+
+template <int N, int RB, int R = 8>
+__device__ __forceinline__ void rows_forward_march_body(const FvmParams& p, const FvmBuffers& b,
+                                                        int i0, int tid, char* smem) {
+  using L = MarchLds<N, RB>;
+  constexpr int H = N / 2, K = L::K, FB = L::FB, CPT = N / kThreads;
+  static_assert(N % kThreads == 0 && RB % FB == 0, "march shape");
+  float* su = lds_at<float>(smem, L::su);
+  float* sv = lds_at<float>(smem, L::sv);
+  float* fy = lds_at<float>(smem, L::fy);
+  float* gy = lds_at<float>(smem, L::gy);
+  float* vs = lds_at<float>(smem, L::vs);
+  float2* zb = lds_at<float2>(smem, L::zb);
+  float* zf = reinterpret_cast<float*>(zb);
+  float2* twn = lds_at<float2>(smem, L::twn);
+  float2* twh = lds_at<float2>(smem, L::twh);
+  const RingWindow<N, K> w{su, sv};
+  // ring row a holds global row i0 - 3 + a
+  auto grow = [&](int a) { return (i0 - 3 + a) & (N - 1); };
+  for (int e = tid; e < 5 * N; e += kThreads) {
+    const int a = e / N, j = e % N;
+    su[(a & (K - 1)) * N + j] = b.u[grow(a) * N + j];
+    sv[(a & (K - 1)) * N + j] = b.v[grow(a) * N + j];
+  }
+  load_twiddles(twn, b.tw, N, 1, tid, kThreads);
+  load_twiddles(twh, b.tw, H, 2, tid, kThreads);
+  __syncthreads();
+  float fx_prev[CPT], gx_prev[CPT], us_prev[CPT], fxj[CPT];
+#pragma unroll
+  for (int c = 0; c < CPT; ++c) {
+    const int j = tid + c * kThreads;
+    fx_prev[c] = face_fx_u(w, 2, j, p);  // x face below the first u row (i0 - 1)
+    gx_prev[c] = face_fx_v(w, 3, j, p);  // x face below the first v row (i0)
+    fxj[c] = p.has_forcing ? b.fx[j] : 0.f;
+    us_prev[c] = 0.f;
+  }
+  // step s: cells of ring row a = s + 2 (global row i0 - 1 + s); u for every
+  // s, v and div for s >= 1.  Reads ring rows a - 1 .. a + 2.
+  for (int s = 0; s <= RB; ++s) {
+    const int a = s + 2;
+    const bool vrow = s >= 1;
+    const bool more = s < RB;
+    float pu[CPT], pv[CPT];
+    if (more) {
+#pragma unroll
+      for (int c = 0; c < CPT; ++c) {
+        const int j = tid + c * kThreads;
+        pu[c] = b.u[grow(a + 3) * N + j];
+        pv[c] = b.v[grow(a + 3) * N + j];
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < CPT; ++c) {
+      const int j = tid + c * kThreads;
+      fy[j] = face_fy_u(w, a, j, p);
+      if (vrow) gy[j] = face_fy_v(w, a, j, p);
+    }
+    __syncthreads();
+    float us_cur[CPT];
+#pragma unroll
+    for (int c = 0; c < CPT; ++c) {
+      const int j = tid + c * kThreads;
+      const int jm = (j - 1) & (N - 1);
+      const float fx_next = face_fx_u(w, a + 1, j, p);
+      const float adv_u = -((fx_next - fx_prev[c]) / p.dx + (fy[j] - fy[jm]) / p.dy);
+      const float ku = add_diffusion_forcing<false>(w, a, j, p, fxj[c], adv_u);
+      us_cur[c] = w.U(a, j) + p.dt * ku;
+      fx_prev[c] = fx_next;
+      if (vrow) {
+        const float gx_next = face_fx_v(w, a + 1, j, p);
+        const float adv_v = -((gx_next - gx_prev[c]) / p.dx + (gy[j] - gy[jm]) / p.dy);
+        const float kv = add_diffusion_forcing<true>(w, a, j, p, 0.f, adv_v);
+        const float vst = w.V(a, j) + p.dt * kv;
+        gx_prev[c] = gx_next;
+        vs[j] = vst;
+        const int gi = i0 - 1 + s;
+        b.us[gi * N + j] = us_cur[c];
+        b.vs[gi * N + j] = vst;
+      }
+    }
+    __syncthreads();  // v* row complete; y-face rows read
+    if (vrow) {
+      const int d = s - 1;  // div row i0 + d
+#pragma unroll
+      for (int c = 0; c < CPT; ++c) {
+        const int j = tid + c * kThreads;
+        // div u* (finite_differences.py:136-143)
+        const float ddx = (us_cur[c] - us_prev[c]) / p.dx;
+        const float ddy = (vs[j] - vs[(j - 1) & (N - 1)]) / p.dy;
+        zf[(d % FB) * N + j] = ddx + ddy;
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < CPT; ++c) us_prev[c] = us_cur[c];
+    if (more) {
+#pragma unroll
+      for (int c = 0; c < CPT; ++c) {
+        const int j = tid + c * kThreads;
+        su[((a + 3) & (K - 1)) * N + j] = pu[c];
+        sv[((a + 3) & (K - 1)) * N + j] = pv[c];
+      }
+    }
+    __syncthreads();
+    if (vrow && (s - 1) % FB == FB - 1) {
+      const int d0 = s - FB;  // first div row of this batch
+      lds_fft<H, FB, kThreads, 1, H, false, false, R>(zb, twh, tid);
+      for (int e = tid; e < FB * H; e += kThreads) {
+        const int r = e / H, k = e % H;
+        b.spec[(i0 + d0 + r) * H + k] = r2c_combine(zb + r * H, k, H, twn[k]);
+      }
+      // zb is written again only after the next step's first barrier
+    }
+  }
+}
+
+template <int N, int RB, int R = 8>
+__global__ __launch_bounds__(kThreads) void fvm_rows_forward_march(FvmParams p, FvmBuffers b) {
+  __shared__ __align__(16) char smem[MarchLds<N, RB>::bytes];
+  rows_forward_march_body<N, RB, R>(p, fvm_instance(b, N), blockIdx.x * RB, threadIdx.x, smem);
+}
+
 template <int N, int W>
 struct PoissonLds {
   static constexpr int buf = 0;               // float2[N * W]
@@ -237,11 +394,6 @@ struct RowsProjectLds {
   static constexpr int twh = twn + (kFft ? N * 8 : 0);  // float2[H]
   static constexpr int bytes = twh + (kFft ? H * 8 : 0);
 };
-
-template <typename T>
-__device__ __forceinline__ T* lds_at(char* smem, int offset) {
-  return reinterpret_cast<T*>(smem + offset);
-}
 
 // Rows [i0, i0 + RB) of one instance (b already points at the instance).
 // MODE 0: full step (write u*, v*, spectrum of div u*).
@@ -303,7 +455,7 @@ __device__ __forceinline__ void rows_forward_body(const FvmParams& p, const FvmB
         const float adv = -((flux[(r + 1) * N + j] - flux[r * N + j]) / p.dx +
                             (fy[j] - fy[(j - 1) & (N - 1)]) / p.dy);
         const float fxj = p.has_forcing ? b.fx[j] : 0.f;
-        const float ku = add_diffusion_forcing<N, false>(w, lr, j, p, fxj, adv);
+        const float ku = add_diffusion_forcing<false>(w, lr, j, p, fxj, adv);
         ru[k] = (MODE == 2) ? ku : w.U(lr, j) + p.dt * ku;
       }
     }
@@ -321,7 +473,7 @@ __device__ __forceinline__ void rows_forward_body(const FvmParams& p, const FvmB
         const float* fy = flux + (NV + 1) * N + r * N;
         const float adv = -((flux[(r + 1) * N + j] - flux[r * N + j]) / p.dx +
                             (fy[j] - fy[(j - 1) & (N - 1)]) / p.dy);
-        const float kv = add_diffusion_forcing<N, true>(w, lr, j, p, 0.f, adv);
+        const float kv = add_diffusion_forcing<true>(w, lr, j, p, 0.f, adv);
         rv[k] = (MODE == 2) ? kv : w.V(lr, j) + p.dt * kv;
       }
     }
@@ -538,9 +690,13 @@ template <> struct Tiles<1024> { static constexpr int RB = 1, W = 4; };
 // FFTs' rounding.
 constexpr int kTileLdsMax = 96 * 1024;
 
+// rows_forward tiles 1..8 stage RB + 5 rows at once; 16 and 32 march (n >= 256).
 template <int N, int RB>
 constexpr bool fwd_ok() {
-  return RB <= N / 4 && RowsForwardLds<N, RB, 0>::bytes <= kTileLdsMax;
+  if constexpr (RB >= 16)
+    return N >= kThreads && N <= 512 && RB <= N / 8 && MarchLds<N, RB>::bytes <= kTileLdsMax;
+  else
+    return RB <= N / 4 && RowsForwardLds<N, RB, 0>::bytes <= kTileLdsMax;
 }
 template <int N, int W>
 constexpr bool cols_ok() {
@@ -558,16 +714,19 @@ template <int N, int T, int R>
 int launch_variant(int which, const FvmParams& p, const FvmBuffers& b, hipStream_t s) {
   if (which == 0) {
     if constexpr (fwd_ok<N, T>()) {
-      hipLaunchKernelGGL((fvm_rows_forward<N, T, 0, R>), dim3(N / T, nb(b)), dim3(kThreads), 0, s, p, b);
+      if constexpr (T >= 16)
+        hipLaunchKernelGGL((fvm_rows_forward_march<N, T, R>), dim3(N / T, nb(b)), dim3(kThreads), 0, s, p, b);
+      else
+        hipLaunchKernelGGL((fvm_rows_forward<N, T, 0, R>), dim3(N / T, nb(b)), dim3(kThreads), 0, s, p, b);
       return JCFD_OK;
     }
   } else if (which == 1) {
-    if constexpr (cols_ok<N, T>()) {
+    if constexpr (T <= 16 && cols_ok<N, T>()) {
       hipLaunchKernelGGL((fvm_poisson_cols<N, T, R>), dim3(N / 2 / T, nb(b)), dim3(kThreads), 0, s, b, 1.0f);
       return JCFD_OK;
     }
   } else {
-    if constexpr (proj_ok<N, T>()) {
+    if constexpr (T <= 8 && proj_ok<N, T>()) {
       hipLaunchKernelGGL((fvm_rows_project<N, T, 0, R>), dim3(N / T, nb(b)), dim3(kThreads), 0, s, p, b);
       return JCFD_OK;
     }
@@ -582,7 +741,8 @@ int launch_tile(int which, int tile, const FvmParams& p, const FvmBuffers& b, hi
     case 2: return launch_variant<N, 2, R>(which, p, b, s);
     case 4: return launch_variant<N, 4, R>(which, p, b, s);
     case 8: return launch_variant<N, 8, R>(which, p, b, s);
-    case 16: return which == 1 ? launch_variant<N, 16, R>(which, p, b, s) : JCFD_ERR_UNSUPPORTED;
+    case 16: return which == 2 ? JCFD_ERR_UNSUPPORTED : launch_variant<N, 16, R>(which, p, b, s);
+    case 32: return which == 0 ? launch_variant<N, 32, R>(which, p, b, s) : JCFD_ERR_UNSUPPORTED;
     default: return JCFD_ERR_UNSUPPORTED;
   }
 }
@@ -612,6 +772,8 @@ bool tile_on_menu(int which, int tile) {
       case 2: return fwd_ok<N, 2>();
       case 4: return fwd_ok<N, 4>();
       case 8: return fwd_ok<N, 8>();
+      case 16: return fwd_ok<N, 16>();
+      case 32: return fwd_ok<N, 32>();
       default: return false;
     }
   };

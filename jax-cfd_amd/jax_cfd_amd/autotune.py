@@ -2,7 +2,8 @@
 
 The step kernels are built for a menu of tile shapes and two FFT radix
 schedules (largest radix 8 or 4).  FVM plans tune the rows per workgroup of
-the stencil and projection kernels and the spectrum columns per workgroup of
+the stencil kernel (16 and 32 select its marching form, n >= 256) and of the
+projection kernel and the spectrum columns per workgroup of
 the Poisson column kernel; spectral plans tune the packed-spectrum columns
 per workgroup of their three column kernels and the rows per workgroup of
 the nonlinear-product row kernel.  Which combination is fastest depends on
@@ -45,7 +46,7 @@ class Tunables:
 
 FVM = Tunables(
     'fvm',
-    menu={'rows_forward': (1, 2, 4, 8), 'poisson_cols': (1, 2, 4, 8, 16),
+    menu={'rows_forward': (1, 2, 4, 8, 16, 32), 'poisson_cols': (1, 2, 4, 8, 16),
           'rows_project': (1, 2, 4, 8)},
     weights={'rows_forward': {'fvm_rows_forward': 1},
              'poisson_cols': {'fvm_poisson_cols': 1},

@@ -36,7 +36,7 @@ def test_every_tile_shape_is_bit_identical(n):
   ref = plan.step(u0, v0, 3)
   default = plan.tiles
   tried = 0
-  for cls, menu in (('rows_forward', (1, 2, 4, 8)), ('poisson_cols', (1, 2, 4, 8, 16)),
+  for cls, menu in (('rows_forward', (1, 2, 4, 8, 16, 32)), ('poisson_cols', (1, 2, 4, 8, 16)),
                     ('rows_project', (1, 2, 4, 8))):
     for t in menu:
       cfg = dict(default)
