@@ -78,6 +78,9 @@ const char* jcfd_version(void);
 #define JCFD_CONVECT_VAN_LEER_LIMITERS 1 /* advection.py:387-395 */
 #define JCFD_PRESSURE_FAST_DIAG 0        /* pressure.py:115-157 */
 #define JCFD_PRESSURE_CG 1               /* pressure.py:67-112 */
+#define JCFD_DIFFUSION_EXPLICIT 0        /* semi_implicit_navier_stokes: diffuse (diffusion.py:35-37) */
+#define JCFD_DIFFUSION_IMPLICIT_FAST_DIAG 1 /* implicit_diffusion_navier_stokes (equations.py:154-195)
+                                                with diffusion.solve_fast_diag (diffusion.py:170-212) */
 
 typedef struct {
   int32_t n;               /* cells per side; square periodic grid */
@@ -97,6 +100,11 @@ typedef struct {
                               (= zero) */
   int32_t batch;           /* instances the plan holds scratch for (<= 1: one);
                               batched fields are (batch, n, n) */
+  int32_t diffusion;       /* JCFD_DIFFUSION_*: explicit nu/rho lap in the explicit
+                              terms, or the step of implicit_diffusion_navier_stokes
+                              (convection + forcing, Euler, projection, then
+                              (1 - viscosity dt L)^-1 per component); needs
+                              has_viscosity */
 } jcfd_fvm_config;
 
 typedef struct jcfd_fvm_plan jcfd_fvm_plan;
@@ -160,6 +168,13 @@ int32_t jcfd_fvm_cg_iterations(const jcfd_fvm_plan* plan);
  * (solve_fast_diag or solve_cg). */
 int jcfd_fvm_projection(jcfd_fvm_plan* plan, const jcfd_tensor* u, const jcfd_tensor* v,
                         jcfd_tensor* u_out, jcfd_tensor* v_out, void* stream);
+
+/* diffusion.solve_fast_diag(v, viscosity, dt) (diffusion.py:170-212),
+ * periodic: each component becomes (1 - viscosity dt L)^-1 u, computed as
+ * u + irfft2(D rfft2(u)).  Implicit-diffusion plans only (the plan's
+ * viscosity and dt).  u_out/v_out may alias u/v. */
+int jcfd_fvm_diffuse(jcfd_fvm_plan* plan, const jcfd_tensor* u, const jcfd_tensor* v,
+                     jcfd_tensor* u_out, jcfd_tensor* v_out, void* stream);
 
 /* fd.divergence of a staggered velocity (finite_differences.py:136-143). */
 int jcfd_fvm_divergence(jcfd_fvm_plan* plan, const jcfd_tensor* u, const jcfd_tensor* v,

@@ -136,6 +136,24 @@ std::vector<float> poisson_multiplier(int n, double dx, double dy) {
   return lam;
 }
 
+// D / n^2 for the implicit diffusion solve (diffusion.py:180-185 through
+// fast_diagonalization.py:196-225): D = (dt nu) s / (1 - (dt nu) s) with s the
+// summed periodic Laplacian eigenvalues, in fp64, rfft layout on the last axis.
+std::vector<float> diffusion_multiplier(int n, double dx, double dy, double nu, double dt) {
+  const int h1 = n / 2 + 1;
+  std::vector<float> m(static_cast<size_t>(n) * h1);
+  const double dt_nu = dt * nu;
+  for (int p = 0; p < n; ++p) {
+    const double lx = (-2.0 / (dx * dx)) + 2.0 * std::cos(2.0 * M_PI * p / n) / (dx * dx);
+    for (int c = 0; c < h1; ++c) {
+      const double ly = (-2.0 / (dy * dy)) + 2.0 * std::cos(2.0 * M_PI * c / n) / (dy * dy);
+      const double x = dt_nu * (lx + ly);
+      m[static_cast<size_t>(p) * h1 + c] = static_cast<float>(x / (1.0 - x) / (static_cast<double>(n) * n));
+    }
+  }
+  return m;
+}
+
 // Replays `chunk` steps captured once into a hipGraph.  Capture happens on a
 // plan-owned non-blocking stream (default priority); events order it after
 // the caller's prior work and the caller's later work after it.  The graph
@@ -236,6 +254,7 @@ struct jcfd_fvm_plan {
   float* lam = nullptr;
   float2* tw = nullptr;
   float* fx = nullptr;
+  float* dlam = nullptr;  // implicit diffusion multiplier / n^2 (diffusion == implicit)
   int64_t bytes = 0;
   GraphCache graphs;
 };
@@ -345,6 +364,14 @@ int cg_solve(jcfd_fvm_plan* p, hipStream_t s) {
 
 // nsteps steps: the first reads b.u/b.v, the rest run in place on
 // b.u_out/b.v_out (replayed from a graph when enabled).
+// One step: the fast-diagonal step kernels, then (implicit diffusion plans)
+// the diffusion solve in place on the output.
+int fvm_full_step(const jcfd_fvm_plan* p, const FvmBuffers& b, hipStream_t s) {
+  int rc = fvm_launch_step(p->params, b, s);
+  if (rc == JCFD_OK && p->dlam != nullptr) rc = fvm_launch_diffuse(p->params, b, p->dlam, s);
+  return rc;
+}
+
 int fvm_run(jcfd_fvm_plan* p, FvmBuffers b, int64_t nsteps, hipStream_t s) {
   int rc;
   if (p->cfg.pressure == JCFD_PRESSURE_CG) {
@@ -354,19 +381,20 @@ int fvm_run(jcfd_fvm_plan* p, FvmBuffers b, int64_t nsteps, hipStream_t s) {
       if ((rc = fvm_launch_forward_rhs(p->params, b, s)) || (rc = cg_solve(p, s)) ||
           (rc = fvm_launch_project_q(p->params, b, s)))
         return rc;
+      if (p->dlam != nullptr && (rc = fvm_launch_diffuse(p->params, b, p->dlam, s))) return rc;
       b.u = b.u_out;
       b.v = b.v_out;
     }
     return JCFD_OK;
   }
-  if ((rc = fvm_launch_step(p->params, b, s))) return rc;
+  if ((rc = fvm_full_step(p, b, s))) return rc;
   b.u = b.u_out;
   b.v = b.v_out;
   int64_t remaining = nsteps - 1;
   if (p->graphs.enabled && remaining >= kGraphChunk) {
     auto launch = [&](hipStream_t gs) {
       for (int i = 0; i < kGraphChunk; ++i) {
-        const int r = fvm_launch_step(p->params, b, gs);
+        const int r = fvm_full_step(p, b, gs);
         if (r) return r;
       }
       return static_cast<int>(JCFD_OK);
@@ -377,7 +405,7 @@ int fvm_run(jcfd_fvm_plan* p, FvmBuffers b, int64_t nsteps, hipStream_t s) {
     remaining %= kGraphChunk;
   }
   for (int64_t step = 0; step < remaining; ++step)
-    if ((rc = fvm_launch_step(p->params, b, s))) return rc;
+    if ((rc = fvm_full_step(p, b, s))) return rc;
   return JCFD_OK;
 }
 }  // namespace
@@ -408,6 +436,10 @@ int jcfd_fvm_plan_create(const jcfd_fvm_config* config, jcfd_fvm_plan** plan) {
     return fail(JCFD_ERR_UNSUPPORTED, "the CG pressure solve is built for one instance per plan");
   if (c.pressure == JCFD_PRESSURE_CG && (!std::isfinite(c.cg_rtol) || !std::isfinite(c.cg_atol)))
     return fail(JCFD_ERR_INVALID_ARGUMENT, "CG tolerances must be finite");
+  if (c.diffusion != JCFD_DIFFUSION_EXPLICIT && c.diffusion != JCFD_DIFFUSION_IMPLICIT_FAST_DIAG)
+    return fail(JCFD_ERR_UNSUPPORTED, "diffusion must be explicit or implicit fast-diagonal");
+  if (c.diffusion == JCFD_DIFFUSION_IMPLICIT_FAST_DIAG && (!c.has_viscosity || !std::isfinite(c.viscosity)))
+    return fail(JCFD_ERR_INVALID_ARGUMENT, "implicit diffusion needs a finite viscosity");
 
   auto* p = new (std::nothrow) jcfd_fvm_plan();
   if (p == nullptr) return fail(JCFD_ERR_OUT_OF_MEMORY, "host allocation failed");
@@ -431,7 +463,8 @@ int jcfd_fvm_plan_create(const jcfd_fvm_config* config, jcfd_fvm_plan** plan) {
   q.dt = static_cast<float>(c.dt);
   q.dt_over_dx = static_cast<float>(c.dt / c.dx);
   q.dt_over_dy = static_cast<float>(c.dt / c.dy);
-  q.has_diffusion = c.has_viscosity ? 1 : 0;
+  // implicit-diffusion plans leave the Laplacian out of the explicit terms
+  q.has_diffusion = (c.has_viscosity && c.diffusion == JCFD_DIFFUSION_EXPLICIT) ? 1 : 0;
   q.nu = static_cast<float>(c.has_viscosity ? c.viscosity / c.density : 0.0);
   q.rho = static_cast<float>(c.density);
   q.has_forcing = c.has_forcing ? 1 : 0;
@@ -448,7 +481,9 @@ int jcfd_fvm_plan_create(const jcfd_fvm_config* config, jcfd_fvm_plan** plan) {
       (rc = device_alloc(&p->lam, static_cast<size_t>(n) * (n / 2 + 1), &p->bytes)) ||
       (rc = device_alloc(&p->tw, static_cast<size_t>(n), &p->bytes)) ||
       (rc = device_alloc(&p->fx, static_cast<size_t>(n), &p->bytes)) ||
-      (c.pressure == JCFD_PRESSURE_CG && (rc = cg_alloc(p)))) {
+      (c.pressure == JCFD_PRESSURE_CG && (rc = cg_alloc(p))) ||
+      (c.diffusion == JCFD_DIFFUSION_IMPLICIT_FAST_DIAG &&
+       (rc = device_alloc(&p->dlam, static_cast<size_t>(n) * (n / 2 + 1), &p->bytes)))) {
     jcfd_fvm_plan_destroy(p);
     return rc;
   }
@@ -465,6 +500,14 @@ int jcfd_fvm_plan_create(const jcfd_fvm_config* config, jcfd_fvm_plan** plan) {
     jcfd_fvm_plan_destroy(p);
     return rc;
   }
+  if (p->dlam != nullptr) {
+    const std::vector<float> dm = diffusion_multiplier(n, c.dx, c.dy, c.viscosity, c.dt);
+    if ((rc = check_hip(hipMemcpy(p->dlam, dm.data(), dm.size() * sizeof(float), hipMemcpyHostToDevice),
+                        "copy diffusion multiplier"))) {
+      jcfd_fvm_plan_destroy(p);
+      return rc;
+    }
+  }
   *plan = p;
   return JCFD_OK;
 }
@@ -472,7 +515,8 @@ int jcfd_fvm_plan_create(const jcfd_fvm_config* config, jcfd_fvm_plan** plan) {
 int jcfd_fvm_plan_destroy(jcfd_fvm_plan* p) {
   if (p == nullptr) return JCFD_OK;
   for (void* ptr : {static_cast<void*>(p->us), static_cast<void*>(p->vs), static_cast<void*>(p->spec),
-                    static_cast<void*>(p->lam), static_cast<void*>(p->tw), static_cast<void*>(p->fx)})
+                    static_cast<void*>(p->lam), static_cast<void*>(p->tw), static_cast<void*>(p->fx),
+                    static_cast<void*>(p->dlam)})
     if (ptr != nullptr) (void)hipFree(ptr);
   for (void* ptr : {static_cast<void*>(p->cg.x), static_cast<void*>(p->cg.r),
                     static_cast<void*>(p->cg.p[0]), static_cast<void*>(p->cg.p[1]),
@@ -626,6 +670,31 @@ int jcfd_fvm_projection(jcfd_fvm_plan* p, const jcfd_tensor* u, const jcfd_tenso
                                p->tw, s);
 }
 
+int jcfd_fvm_diffuse(jcfd_fvm_plan* p, const jcfd_tensor* u, const jcfd_tensor* v, jcfd_tensor* u_out,
+                     jcfd_tensor* v_out, void* stream) {
+  if (p == nullptr) return fail(JCFD_ERR_INVALID_ARGUMENT, "null plan");
+  if (p->dlam == nullptr)
+    return fail(JCFD_ERR_UNSUPPORTED, "jcfd_fvm_diffuse needs an implicit-diffusion plan");
+  const int n = p->params.n;
+  void *pu, *pv, *pu2, *pv2;
+  int rc;
+  if ((rc = check_tensor(u, "u", kDLFloat, 32, n, n, p->device, &pu)) ||
+      (rc = check_tensor(v, "v", kDLFloat, 32, n, n, p->device, &pv)) ||
+      (rc = check_tensor(u_out, "u_out", kDLFloat, 32, n, n, p->device, &pu2)) ||
+      (rc = check_tensor(v_out, "v_out", kDLFloat, 32, n, n, p->device, &pv2)))
+    return rc;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const size_t bytes = sizeof(float) * static_cast<size_t>(n) * n;
+  if (pu2 != pu && (rc = check_hip(hipMemcpyAsync(pu2, pu, bytes, hipMemcpyDeviceToDevice, s), "copy")))
+    return rc;
+  if (pv2 != pv && (rc = check_hip(hipMemcpyAsync(pv2, pv, bytes, hipMemcpyDeviceToDevice, s), "copy")))
+    return rc;
+  FvmBuffers b = fvm_buffers(p);
+  b.u_out = static_cast<float*>(pu2);
+  b.v_out = static_cast<float*>(pv2);
+  return fvm_launch_diffuse(p->params, b, p->dlam, s);
+}
+
 int jcfd_fvm_divergence(jcfd_fvm_plan* p, const jcfd_tensor* u, const jcfd_tensor* v,
                         jcfd_tensor* out, void* stream) {
   if (p == nullptr) return fail(JCFD_ERR_INVALID_ARGUMENT, "null plan");
@@ -699,8 +768,10 @@ int jcfd_fvm_step_vjp(jcfd_fvm_plan* p, const jcfd_tensor* u, const jcfd_tensor*
                       const jcfd_tensor* lam_u, const jcfd_tensor* lam_v, jcfd_tensor* out_u,
                       jcfd_tensor* out_v, void* stream) {
   if (p == nullptr) return fail(JCFD_ERR_INVALID_ARGUMENT, "null plan");
-  if (p->cfg.convect != JCFD_CONVECT_VAN_LEER || p->cfg.pressure != JCFD_PRESSURE_FAST_DIAG)
-    return fail(JCFD_ERR_UNSUPPORTED, "the adjoint is built for advect_van_leer + solve_fast_diag");
+  if (p->cfg.convect != JCFD_CONVECT_VAN_LEER || p->cfg.pressure != JCFD_PRESSURE_FAST_DIAG ||
+      p->cfg.diffusion != JCFD_DIFFUSION_EXPLICIT)
+    return fail(JCFD_ERR_UNSUPPORTED,
+                "the adjoint is built for advect_van_leer + explicit diffusion + solve_fast_diag");
   const int n = p->params.n;
   void *pu, *pv, *plu, *plv, *pou, *pov;
   int rc;
@@ -928,6 +999,8 @@ int jcfd_fvm_group_step(jcfd_fvm_group* g, const jcfd_fvm_group_member* members,
       return fail(JCFD_ERR_UNSUPPORTED, who + ": groups step fast-diagonal plans only");
     if (p->params.radix != 8)
       return fail(JCFD_ERR_UNSUPPORTED, who + ": groups run the radix-8 FFT schedule only");
+    if (p->dlam != nullptr)
+      return fail(JCFD_ERR_UNSUPPORTED, who + ": groups step explicit-diffusion plans only");
     const int n = p->params.n;
     if (!fvm_group_size_supported(n))
       return fail(JCFD_ERR_UNSUPPORTED, who + ": grid size " + std::to_string(n) +

@@ -598,7 +598,8 @@ __global__ __launch_bounds__(kThreads) void fvm_poisson_cols(FvmBuffers b, float
 // Inverse row transforms of q and the velocity correction for rows
 // [i0, i0 + RB) (b already points at the instance).
 // MODE 0: velocity correction; MODE 1: write q itself to u_out;
-// MODE 2: velocity correction from the real q of the CG solve (no FFT).
+// MODE 2: velocity correction from the real q of the CG solve (no FFT);
+// MODE 3: u_out = u + q (the implicit diffusion solve).
 template <int N, int RB, int MODE, int R = 8>
 __device__ __forceinline__ void rows_project_body(const FvmParams& p, const FvmBuffers& b, int i0,
                                                   int tid, char* smem) {
@@ -629,6 +630,11 @@ __device__ __forceinline__ void rows_project_body(const FvmParams& p, const FvmB
   const float* q = reinterpret_cast<const float*>(qb);
   if constexpr (MODE == 1) {
     for (int e = tid; e < RB * N; e += kThreads) b.u_out[i0 * N + e] = q[e];
+    return;
+  }
+  if constexpr (MODE == 3) {
+    // implicit diffusion: u + (1 - nu dt L)^-1 (nu dt L) u (diffusion.py:198-208)
+    for (int e = tid; e < RB * N; e += kThreads) b.u_out[i0 * N + e] = b.u[i0 * N + e] + q[e];
     return;
   }
   for (int e = tid; e < RB * N; e += kThreads) {
@@ -825,6 +831,24 @@ int launch_filter(const FvmParams& p, const FvmBuffers& b, hipStream_t s) {
   return check_hip(hipGetLastError(), "spectral filter launch");
 }
 
+// In place on b.u_out and b.v_out (every instance of b): u <- u + irfft2(m .
+// rfft2(u)) with m = D / n^2 the implicit-diffusion multiplier.
+template <int N>
+int launch_diffuse(const FvmParams& p, const FvmBuffers& b, const float* mult, hipStream_t s) {
+  constexpr int RB = Tiles<N>::RB, W = Tiles<N>::W;
+  for (int c = 0; c < 2; ++c) {
+    FvmBuffers f = b;
+    float* field = c == 0 ? b.u_out : b.v_out;
+    f.u = field;
+    f.u_out = field;
+    f.lam = mult;
+    hipLaunchKernelGGL((fvm_rows_forward<N, RB, 3>), dim3(N / RB, nb(f)), dim3(kThreads), 0, s, p, f);
+    hipLaunchKernelGGL((fvm_poisson_cols<N, W>), dim3(N / 2 / W, nb(f)), dim3(kThreads), 0, s, f, 1.0f);
+    hipLaunchKernelGGL((fvm_rows_project<N, RB, 3>), dim3(N / RB, nb(f)), dim3(kThreads), 0, s, p, f);
+  }
+  return check_hip(hipGetLastError(), "implicit diffusion launch");
+}
+
 template <template <int> class F, typename... Args>
 int dispatch_n(int n, Args&&... args) {
   switch (n) {
@@ -865,6 +889,11 @@ template <int N> struct OneOp {
 };
 template <int N> struct FilterOp {
   static int run(const FvmParams& p, const FvmBuffers& b, hipStream_t s) { return launch_filter<N>(p, b, s); }
+};
+template <int N> struct DiffuseOp {
+  static int run(const FvmParams& p, const FvmBuffers& b, const float* mult, hipStream_t s) {
+    return launch_diffuse<N>(p, b, mult, s);
+  }
 };
 template <int N> struct ProjOp {
   static int run(const FvmParams& p, const FvmBuffers& b, hipStream_t s) { return launch_projection<N>(p, b, s); }
@@ -1046,6 +1075,10 @@ int fvm_group_launch(int phase, int max_n, const FvmGroupQueue& q, int grid, hip
 
 int fvm_launch_step(const FvmParams& p, const FvmBuffers& b, hipStream_t s) {
   return dispatch_n<StepOp>(p.n, p, b, s);
+}
+
+int fvm_launch_diffuse(const FvmParams& p, const FvmBuffers& b, const float* mult, hipStream_t s) {
+  return dispatch_n<DiffuseOp>(p.n, p, b, mult, s);
 }
 
 int fvm_launch_kernel(const FvmParams& p, const FvmBuffers& b, int which, hipStream_t s) {

@@ -95,6 +95,9 @@ __device__ __forceinline__ FvmBuffers fvm_instance(FvmBuffers b, int n) {
 
 // Kernel launchers (fvm.hip).  All are stream-ordered and asynchronous.
 int fvm_launch_step(const FvmParams& p, const FvmBuffers& b, hipStream_t s);
+// Implicit diffusion solve in place on b.u_out / b.v_out (every instance of b)
+// with the multiplier table `mult` ((n, n/2 + 1) floats, 1/n^2 folded in).
+int fvm_launch_diffuse(const FvmParams& p, const FvmBuffers& b, const float* mult, hipStream_t s);
 // One kernel class of the step (0 rows-forward, 1 Poisson columns, 2 rows-project).
 int fvm_launch_kernel(const FvmParams& p, const FvmBuffers& b, int which, hipStream_t s);
 int fvm_launch_explicit_terms(const FvmParams& p, const float* u, const float* v,

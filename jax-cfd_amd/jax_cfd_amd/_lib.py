@@ -28,6 +28,8 @@ CONVECT_VAN_LEER = 0
 CONVECT_VAN_LEER_LIMITERS = 1
 PRESSURE_FAST_DIAG = 0
 PRESSURE_CG = 1
+DIFFUSION_EXPLICIT = 0
+DIFFUSION_IMPLICIT_FAST_DIAG = 1
 
 
 class BudgetExceededError(RuntimeError):
@@ -54,6 +56,7 @@ class FvmConfig(ctypes.Structure):
       ("linear_coefficient", ctypes.c_double),
       ("forcing_x", ctypes.POINTER(ctypes.c_float)),
       ("batch", ctypes.c_int32),
+      ("diffusion", ctypes.c_int32),
   ]
 
 
@@ -97,6 +100,7 @@ _SIGNATURES = {
     "jcfd_fvm_explicit_terms": (ctypes.c_int, [_P, _P, _P, _P, _P, _P]),
     "jcfd_fvm_projection": (ctypes.c_int, [_P, _P, _P, _P, _P, _P]),
     "jcfd_fvm_divergence": (ctypes.c_int, [_P, _P, _P, _P, _P]),
+    "jcfd_fvm_diffuse": (ctypes.c_int, [_P, _P, _P, _P, _P, _P]),
     "jcfd_fvm_curl": (ctypes.c_int, [_P, _P, _P, _P, _P]),
     "jcfd_fvm_step_vjp": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _P, _P]),
     "jcfd_fvm_curl_vjp": (ctypes.c_int, [_P, _P, _P, _P, _P]),

@@ -28,6 +28,7 @@ class FvmSpec:
   cg_maxiter: int = 0
   linear_coefficient: Optional[float] = None  # None: no forcing at all
   forcing_x: Optional[Tuple[float, ...]] = None  # f_x(y_j), n values
+  diffusion: int = _lib.DIFFUSION_EXPLICIT
 
 
 def spec_from_grid(grid, **kw) -> FvmSpec:
@@ -67,6 +68,7 @@ class FvmPlan:
         raise ValueError('forcing_x must hold n values')
       cfg.forcing_x = self._fx.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
     cfg.batch = self.batch
+    cfg.diffusion = spec.diffusion
     handle = ctypes.c_void_p()
     with torch.cuda.device(self.device):
       _lib.check(lib.jcfd_fvm_plan_create(ctypes.byref(cfg), ctypes.byref(handle)),
@@ -159,6 +161,16 @@ class FvmPlan:
       rc = self._lib.jcfd_fvm_projection(self._handle, dl(u), dl(v), dl(uo), dl(vo),
                                          _lib.stream_handle(self.device))
     _lib.check(rc, 'jcfd_fvm_projection')
+    return uo, vo
+
+  def diffuse(self, u, v):
+    """Implicit diffusion solve (implicit-diffusion plans): (1 - nu dt L)^-1."""
+    u, v = self._field(u), self._field(v)
+    uo, vo = self._empty(), self._empty()
+    with _lib.dlpack_args() as dl:
+      rc = self._lib.jcfd_fvm_diffuse(self._handle, dl(u), dl(v), dl(uo), dl(vo),
+                                      _lib.stream_handle(self.device))
+    _lib.check(rc, 'jcfd_fvm_diffuse')
     return uo, vo
 
   def divergence(self, u, v):

@@ -1,7 +1,7 @@
 """Navier-Stokes step factories (jax_cfd/base/equations.py).
 
-``semi_implicit_navier_stokes`` keeps the reference signature
-(equations.py:120-151) and returns a ``step_fn(v) -> v`` for the state
+``semi_implicit_navier_stokes`` and ``implicit_diffusion_navier_stokes``
+keep the reference signatures (equations.py:120-151, 154-195) and returns a ``step_fn(v) -> v`` for the state
 ``(GridVariable(u), GridVariable(v))``.  The operator callables are read
 once on probe velocities (see jax_cfd_amd._symbolic) and mapped onto one
 fused GPU plan; combinations the kernels do not implement raise
@@ -179,6 +179,37 @@ def semi_implicit_navier_stokes(
       grid, dt=float(dt), density=float(density),
       viscosity=None if viscosity is None else float(viscosity),
       convect=conv, linear_coefficient=lin, forcing_x=fx, **solve_kw)
+  return FvmStepFn(grid, spec)
+
+
+def implicit_diffusion_navier_stokes(
+    density: float,
+    viscosity: float,
+    dt: float,
+    grid: grids.Grid,
+    convect: Optional[Callable] = None,
+    diffusion_solve: Callable = diffusion.solve_fast_diag,
+    pressure_solve: Callable = pressure.solve_fast_diag,
+    forcing: Optional[Callable] = None,
+) -> FvmStepFn:
+  """Returns step_fn of Navier-Stokes with implicit diffusion
+  (equations.py:154-195): convection (+ forcing / density), forward Euler,
+  projection, then diffusion_solve(v, viscosity, dt)."""
+  if not diffusion.is_solve_fast_diag(diffusion_solve):
+    raise NotImplementedError(f'diffusion_solve {diffusion_solve!r} is not built '
+                              '(diffusion.solve_fast_diag is)')
+  if pressure_solve is pressure.solve_fast_diag:
+    solve_kw = {}
+  elif pressure.is_solve_cg(pressure_solve):
+    solve_kw = pressure.cg_options(pressure_solve)
+  else:
+    raise NotImplementedError(f'pressure_solve {pressure_solve!r} is not built')
+  conv = _read_convection(convect, grid, dt)
+  lin, fx = _read_forcing(forcing, grid)
+  spec = _fvm.spec_from_grid(
+      grid, dt=float(dt), density=float(density), viscosity=float(viscosity),
+      convect=conv, linear_coefficient=lin, forcing_x=fx,
+      diffusion=_lib.DIFFUSION_IMPLICIT_FAST_DIAG, **solve_kw)
   return FvmStepFn(grid, spec)
 
 

@@ -8,14 +8,20 @@ equations.py:120-151) with ``time_stepper=forward_euler``
 ``advect_van_leer_using_limiters`` convection, explicit diffusion, optional
 Kolmogorov + linear forcing, and ``projection`` (pressure.py:181-198) with
 either ``solve_fast_diag`` (pressure.py:115-157 via fast_diagonalization.py:
-199-266) or ``solve_cg`` (pressure.py:67-112).
+199-266) or ``solve_cg`` (pressure.py:67-112).  ``ImplicitDiffusionStep``
+mirrors ``implicit_diffusion_navier_stokes`` (equations.py:154-195) with
+``diffusion.solve_fast_diag`` (diffusion.py:170-212).
 
 Pinning (tests/test_oracle_fvm.py): the 4x4 periodic Poisson KAT
 ``x = b - mean(b)`` (pressure_test.py:197-213); circulant pseudo-inverse vs a
 dense ``numpy.linalg.lstsq`` brute force (fast_diagonalization_test.py:
 58-130); projection divergence (pressure_test.py:91-110); divergence and
 momentum invariants over many steps (equations_test.py:84-163); the
-Taylor-Green closed form (validation_test.py:34-169).  ``solve_cg`` follows the
+Taylor-Green closed form (validation_test.py:34-169); for the implicit
+diffusion solve, constant fields unchanged (diffusion_test.py:58-76), the
+residual of (1 - nu dt L) x = b (diffusion_test.py:31-42), a dense
+``numpy.linalg.solve`` brute force and the single-Fourier-mode closed form.
+``solve_cg`` follows the
 published conjugate-gradient algorithm of jax.scipy.sparse.linalg.cg (no
 preconditioner, stop when |r|^2 <= max(tol^2 |b|^2, atol^2)); it is checked
 against the fast-diagonalisation solve, but its individual iterates are
@@ -195,6 +201,76 @@ class FVMStep:
 
   def __call__(self, v: Sequence[A]) -> Tuple[A, ...]:
     return projection(self.intermediate(v), self.grid, self.solve)
+
+  def rollout(self, v: Sequence[A], steps: int) -> Tuple[A, ...]:
+    for _ in range(steps):
+      v = self(v)
+    return v
+
+
+# ------------------------------------------------------- implicit diffusion
+def implicit_diffusion_diagonal(grid: Grid, nu: float, dt: float, dtype=np.float32):
+  """The diagonal of (1 - nu dt L)^-1 - 1 in the Laplacian's Fourier basis.
+
+  diffusion.py:180-185: func(x) = (dt nu) x / (1 - (dt nu) x), applied by
+  fast_diagonalization.transform (fast_diagonalization.py:196-225, 'rfft')
+  to the summed eigenvalues of the periodic 1-D Laplacians (fft per axis,
+  rfft on the last), formed in fp64/complex128 and cast to the complex type
+  of `dtype` (jnp.asarray with x64 disabled).
+  """
+  eig = laplacian_eigenvalues(grid)
+  summed = functools.reduce(np.add.outer, eig)
+  dt_nu_x = (dt * nu) * summed
+  diag = dt_nu_x / (1 - dt_nu_x)
+  ctype = np.complex64 if np.dtype(dtype) == np.float32 else np.complex128
+  return diag.astype(ctype)
+
+
+def solve_fast_diag_diffusion(v: Sequence[A], nu: float, dt: float,
+                              grid: Grid) -> Tuple[A, ...]:
+  """diffusion.solve_fast_diag (diffusion.py:170-212), periodic: each
+  component becomes u + irfftn(D rfftn(u)) = (1 - nu dt L)^-1 u."""
+  out = []
+  for u in v:
+    dtype = u.data.dtype.type
+    diag = implicit_diffusion_diagonal(grid, nu, dt, dtype)
+    t = np.fft.irfftn(diag * np.fft.rfftn(u.data), s=u.data.shape,
+                      axes=tuple(range(u.data.ndim)))
+    out.append(A(u.data + t.real.astype(dtype), u.offset))
+  return tuple(out)
+
+
+@dataclasses.dataclass
+class ImplicitDiffusionStep:
+  """implicit_diffusion_navier_stokes (equations.py:154-195), restated:
+  dv/dt = convection (+ forcing / density) (sum_fields, equations.py:41-42),
+  v <- v + dv/dt * dt, projection, then the implicit diffusion solve with
+  the (kinematic) `viscosity` as given (equations.py:191-193)."""
+  grid: Grid
+  dt: float
+  density: float = 1.0
+  viscosity: float = 1e-3
+  forcing: Optional[go.Forcing] = None
+  convect: str = 'van_leer_limiters'  # the factory default (equations.py:165-168)
+  pressure: str = 'fast_diag'
+  cg_rtol: float = 1e-6
+  cg_atol: float = 1e-6
+
+  def accelerations(self, v: Sequence[A]):
+    conv = FVMStep(self.grid, self.dt, convect=self.convect).convection(v)
+    dv = [0 + c for c in conv]
+    if self.forcing is not None:
+      f = self.forcing(v, self.grid)
+      dv = [d + fi / self.density for d, fi in zip(dv, f)]
+    return dv
+
+  def __call__(self, v: Sequence[A]) -> Tuple[A, ...]:
+    dvdt = self.accelerations(v)
+    v = tuple(A(u.data + d * self.dt, u.offset) for u, d in zip(v, dvdt))
+    solve = FVMStep(self.grid, self.dt, pressure=self.pressure, cg_rtol=self.cg_rtol,
+                    cg_atol=self.cg_atol).solve
+    v = projection(v, self.grid, solve)
+    return solve_fast_diag_diffusion(v, self.viscosity, self.dt, self.grid)
 
   def rollout(self, v: Sequence[A], steps: int) -> Tuple[A, ...]:
     for _ in range(steps):

@@ -101,3 +101,29 @@ def test_grid_matches_reference_geometry():
   assert kx.shape == (8, 5)
   np.testing.assert_allclose(2 * np.pi * ky[0], np.arange(5), atol=1e-5)
   np.testing.assert_allclose(2 * np.pi * kx[:, 0], [0, 1, 2, 3, -4, -3, -2, -1], atol=1e-5)
+
+
+def test_implicit_diffusion_factory_spec():
+  """implicit_diffusion_navier_stokes (equations.py:154-195): default
+  convection is advect_van_leer_using_limiters; the diffusion solve maps to
+  the implicit fast-diagonal plan with the kinematic viscosity as given."""
+  grid = _grid(128)
+  step = cfd.equations.implicit_diffusion_navier_stokes(2.0, 1e-3, 0.01, grid)
+  s = step.spec
+  assert s.diffusion == _lib.DIFFUSION_IMPLICIT_FAST_DIAG
+  assert s.convect == _lib.CONVECT_VAN_LEER_LIMITERS
+  assert s.viscosity == 1e-3 and s.density == 2.0 and s.linear_coefficient is None
+  step = cfd.equations.implicit_diffusion_navier_stokes(
+      1.0, 1e-3, 0.01, grid, convect=_vl(0.01),
+      diffusion_solve=functools.partial(cfd.diffusion.solve_fast_diag, implementation='rfft'),
+      pressure_solve=functools.partial(cfd.pressure.solve_cg, rtol=1e-5))
+  assert step.spec.convect == _lib.CONVECT_VAN_LEER
+  assert step.spec.pressure == _lib.PRESSURE_CG and step.spec.cg_rtol == 1e-5
+  with pytest.raises(NotImplementedError):
+    cfd.equations.implicit_diffusion_navier_stokes(
+        1.0, 1e-3, 0.01, grid, diffusion_solve=lambda v, nu, dt: v)
+  with pytest.raises(NotImplementedError):
+    cfd.equations.implicit_diffusion_navier_stokes(
+        1.0, 1e-3, 0.01, grid,
+        diffusion_solve=functools.partial(cfd.diffusion.solve_fast_diag,
+                                          implementation='matmul'))

@@ -389,3 +389,68 @@ def test_taylor_green_convergence():
     errs.append(max(np.abs(a.data - b.data).max() for a, b in zip(v, ref)))
   assert errs[1] < 0.1
   assert errs[0] / errs[1] > 2.5
+
+
+# ------------------------------------------------------ implicit diffusion
+def _dense_laplacian(grid):
+  """Kronecker sum of the periodic 1-D second-difference matrices."""
+  mats = []
+  for n, h in zip(grid.shape, grid.step):
+    m = -2 * np.eye(n) + np.roll(np.eye(n), 1, 0) + np.roll(np.eye(n), -1, 0)
+    mats.append(m / h ** 2)
+  nx, ny = grid.shape
+  return np.kron(mats[0], np.eye(ny)) + np.kron(np.eye(nx), mats[1])
+
+
+def test_implicit_diffusion_keeps_constants():
+  """diffusion_test.py:58-76: constant velocity is unchanged (atol 1e-6)."""
+  grid = Grid.from_step((100, 100), (1., 1.))
+  v = tuple(A(np.ones((100, 100), np.float32), o) for o in grid.cell_faces)
+  out = fvm.solve_fast_diag_diffusion(v, 1.0, 0.1, grid)
+  for a, b in zip(out, v):
+    np.testing.assert_allclose(a.data, b.data, atol=1e-6)
+
+
+def test_implicit_diffusion_residual():
+  """diffusion_test.py:31-42: x - nu dt lap(x) == b for random b (4x4,
+  step 1, nu = 1, dt = 0.5), with the oracle's periodic Laplacian."""
+  grid = Grid.from_step((4, 4), (1., 1.))
+  b = np.random.RandomState(0).randn(4, 4)
+  (x,) = fvm.solve_fast_diag_diffusion((A(b, (1, .5)),), 1.0, 0.5, grid)
+  resid = x.data - 1.0 * 0.5 * go.laplacian(x, grid).data
+  np.testing.assert_allclose(resid, b, atol=1e-12)
+
+
+def test_implicit_diffusion_dense_brute_force():
+  """(1 - nu dt L)^-1 b by a dense solve on a rectangular periodic grid."""
+  grid = Grid.from_domain((8, 6), ((0, 2.), (0, 3.)))
+  b = np.random.RandomState(1).randn(8, 6)
+  nu, dt = 0.3, 0.7
+  dense = np.linalg.solve(np.eye(48) - nu * dt * _dense_laplacian(grid), b.ravel())
+  (x,) = fvm.solve_fast_diag_diffusion((A(b, (.5, 1)),), nu, dt, grid)
+  np.testing.assert_allclose(x.data.ravel(), dense, atol=1e-12)
+
+
+def test_implicit_diffusion_fourier_mode():
+  """A single mode cos(2 pi k i / n) is scaled by 1 / (1 - nu dt lambda_k),
+  lambda_k = (2 cos(2 pi k / n) - 2) / h^2."""
+  n, k, nu, dt = 32, 3, 0.05, 0.2
+  grid = Grid.from_domain((n, n), ((0, 2 * np.pi), (0, 2 * np.pi)))
+  h = grid.step[0]
+  i = np.arange(n)
+  mode = np.cos(2 * np.pi * k * i / n)[:, None] * np.ones((1, n))
+  (x,) = fvm.solve_fast_diag_diffusion((A(mode, (1, .5)),), nu, dt, grid)
+  lam = (2 * np.cos(2 * np.pi * k / n) - 2) / h ** 2
+  np.testing.assert_allclose(x.data, mode / (1 - nu * dt * lam), atol=1e-12)
+
+
+def test_implicit_diffusion_step_invariants():
+  """equations_test.py:198-245 (the implicit_diffusion_navier_stokes cases):
+  divergence stays ~0 and momentum is conserved without forcing."""
+  grid = Grid.from_step((32, 32), (1., 1.))
+  step = fvm.ImplicitDiffusionStep(grid, dt=1e-2, viscosity=1e-4)
+  v0 = fvm.projection(_sinusoidal_velocity(grid), grid)
+  v = step.rollout(v0, 50)
+  assert np.abs(go.divergence(v, grid).data).max() < 1e-10
+  for a, b in zip(v0, v):
+    assert abs(a.data.sum() - b.data.sum()) < 1e-9
