@@ -105,6 +105,11 @@ typedef struct {
                               (convection + forcing, Euler, projection, then
                               (1 - viscosity dt L)^-1 per component); needs
                               has_viscosity */
+  int32_t rk_stages;       /* Butcher tableau of navier_stokes_rk (time_stepping.py:59-106):
+                              0 or 1 with rk_b[0] = 1 is forward Euler (fused
+                              kernels); up to 4 stages otherwise */
+  double rk_a[6];          /* a[i-1][j], i = 1 .. stages-1, j < i, packed row by row */
+  double rk_b[4];
 } jcfd_fvm_config;
 
 typedef struct jcfd_fvm_plan jcfd_fvm_plan;

@@ -255,6 +255,12 @@ struct jcfd_fvm_plan {
   float2* tw = nullptr;
   float* fx = nullptr;
   float* dlam = nullptr;  // implicit diffusion multiplier / n^2 (diffusion == implicit)
+  // Runge-Kutta tableau steps (rk_stages > 1 or b != [1]): stage slopes k_i,
+  // the projected stage state, and the combinations (stages 1.., final).
+  int rk_stages = 0;
+  float* kbuf = nullptr;   // [stage][u, v][batch][n][n]
+  float* sbuf = nullptr;   // [u, v][batch][n][n]
+  RkTerms* rk_terms = nullptr;  // device, rk_stages entries
   int64_t bytes = 0;
   GraphCache graphs;
 };
@@ -364,10 +370,40 @@ int cg_solve(jcfd_fvm_plan* p, hipStream_t s) {
 
 // nsteps steps: the first reads b.u/b.v, the rest run in place on
 // b.u_out/b.v_out (replayed from a graph when enabled).
-// One step: the fast-diagonal step kernels, then (implicit diffusion plans)
-// the diffusion solve in place on the output.
+// One navier_stokes_rk step (time_stepping.py:83-104): k0 = F(u0); for each
+// later stage, u_i = P(u0 + dt sum a k) and k_i = F(u_i); then
+// P(u0 + dt sum b k) into b.u_out.
+int fvm_rk_step(const jcfd_fvm_plan* p, const FvmBuffers& b, hipStream_t s) {
+  const size_t field = static_cast<size_t>(p->params.n) * p->params.n * p->batch;
+  auto ku = [&](int i) { return p->kbuf + (2 * i) * field; };
+  auto kv = [&](int i) { return p->kbuf + (2 * i + 1) * field; };
+  FvmBuffers e = b;
+  e.us = ku(0);
+  e.vs = kv(0);
+  int rc = fvm_launch_rk_explicit(p->params, e, s);
+  for (int i = 1; i < p->rk_stages && rc == JCFD_OK; ++i) {
+    FvmBuffers c = b;
+    c.rk = p->rk_terms + (i - 1);
+    c.u_out = p->sbuf;
+    c.v_out = p->sbuf + field;
+    if ((rc = fvm_launch_rk_project(p->params, c, s))) break;
+    FvmBuffers e2 = b;
+    e2.u = p->sbuf;
+    e2.v = p->sbuf + field;
+    e2.us = ku(i);
+    e2.vs = kv(i);
+    rc = fvm_launch_rk_explicit(p->params, e2, s);
+  }
+  if (rc) return rc;
+  FvmBuffers c = b;
+  c.rk = p->rk_terms + (p->rk_stages - 1);
+  return fvm_launch_rk_project(p->params, c, s);
+}
+
+// One step: the fast-diagonal step kernels (or the Runge-Kutta stages), then
+// (implicit diffusion plans) the diffusion solve in place on the output.
 int fvm_full_step(const jcfd_fvm_plan* p, const FvmBuffers& b, hipStream_t s) {
-  int rc = fvm_launch_step(p->params, b, s);
+  int rc = p->rk_terms != nullptr ? fvm_rk_step(p, b, s) : fvm_launch_step(p->params, b, s);
   if (rc == JCFD_OK && p->dlam != nullptr) rc = fvm_launch_diffuse(p->params, b, p->dlam, s);
   return rc;
 }
@@ -440,6 +476,16 @@ int jcfd_fvm_plan_create(const jcfd_fvm_config* config, jcfd_fvm_plan** plan) {
     return fail(JCFD_ERR_UNSUPPORTED, "diffusion must be explicit or implicit fast-diagonal");
   if (c.diffusion == JCFD_DIFFUSION_IMPLICIT_FAST_DIAG && (!c.has_viscosity || !std::isfinite(c.viscosity)))
     return fail(JCFD_ERR_INVALID_ARGUMENT, "implicit diffusion needs a finite viscosity");
+  if (c.rk_stages < 0 || c.rk_stages > 4)
+    return fail(JCFD_ERR_UNSUPPORTED, "Runge-Kutta tableaux of 1 to 4 stages are built");
+  for (int k = 0; k < 6; ++k)
+    if (!std::isfinite(c.rk_a[k])) return fail(JCFD_ERR_INVALID_ARGUMENT, "rk_a must be finite");
+  for (int k = 0; k < 4; ++k)
+    if (!std::isfinite(c.rk_b[k])) return fail(JCFD_ERR_INVALID_ARGUMENT, "rk_b must be finite");
+  const bool generic_rk = c.rk_stages > 1 || (c.rk_stages == 1 && c.rk_b[0] != 1.0);
+  if (generic_rk && (c.pressure != JCFD_PRESSURE_FAST_DIAG || c.diffusion != JCFD_DIFFUSION_EXPLICIT))
+    return fail(JCFD_ERR_UNSUPPORTED,
+                "Runge-Kutta tableaux are built with solve_fast_diag and explicit diffusion");
 
   auto* p = new (std::nothrow) jcfd_fvm_plan();
   if (p == nullptr) return fail(JCFD_ERR_OUT_OF_MEMORY, "host allocation failed");
@@ -483,7 +529,10 @@ int jcfd_fvm_plan_create(const jcfd_fvm_config* config, jcfd_fvm_plan** plan) {
       (rc = device_alloc(&p->fx, static_cast<size_t>(n), &p->bytes)) ||
       (c.pressure == JCFD_PRESSURE_CG && (rc = cg_alloc(p))) ||
       (c.diffusion == JCFD_DIFFUSION_IMPLICIT_FAST_DIAG &&
-       (rc = device_alloc(&p->dlam, static_cast<size_t>(n) * (n / 2 + 1), &p->bytes)))) {
+       (rc = device_alloc(&p->dlam, static_cast<size_t>(n) * (n / 2 + 1), &p->bytes))) ||
+      (generic_rk && ((rc = device_alloc(&p->kbuf, 2 * c.rk_stages * nn, &p->bytes)) ||
+                      (rc = device_alloc(&p->sbuf, 2 * nn, &p->bytes)) ||
+                      (rc = device_alloc(&p->rk_terms, static_cast<size_t>(c.rk_stages), &p->bytes))))) {
     jcfd_fvm_plan_destroy(p);
     return rc;
   }
@@ -499,6 +548,34 @@ int jcfd_fvm_plan_create(const jcfd_fvm_config* config, jcfd_fvm_plan** plan) {
                       "copy forcing"))) {
     jcfd_fvm_plan_destroy(p);
     return rc;
+  }
+  if (generic_rk) {
+    // terms of stage i = 1 .. stages-1 (row i-1 of a), then b; zero
+    // coefficients are left out, as the reference's sums skip them
+    p->rk_stages = c.rk_stages;
+    std::vector<RkTerms> terms(c.rk_stages);
+    auto fill = [&](RkTerms& t, const double* coef, int count) {
+      t = RkTerms{};
+      for (int j = 0; j < count; ++j) {
+        if (coef[j] == 0.0) continue;
+        t.ku[t.n] = p->kbuf + (2 * j) * nn;
+        t.kv[t.n] = p->kbuf + (2 * j + 1) * nn;
+        t.c[t.n] = static_cast<float>(coef[j]);
+        ++t.n;
+      }
+    };
+    int row = 0;
+    for (int i = 1; i < c.rk_stages; ++i) {
+      fill(terms[i - 1], c.rk_a + row, i);
+      row += i;
+    }
+    fill(terms[c.rk_stages - 1], c.rk_b, c.rk_stages);
+    if ((rc = check_hip(hipMemcpy(p->rk_terms, terms.data(), terms.size() * sizeof(RkTerms),
+                                  hipMemcpyHostToDevice),
+                        "copy Runge-Kutta terms"))) {
+      jcfd_fvm_plan_destroy(p);
+      return rc;
+    }
   }
   if (p->dlam != nullptr) {
     const std::vector<float> dm = diffusion_multiplier(n, c.dx, c.dy, c.viscosity, c.dt);
@@ -516,7 +593,8 @@ int jcfd_fvm_plan_destroy(jcfd_fvm_plan* p) {
   if (p == nullptr) return JCFD_OK;
   for (void* ptr : {static_cast<void*>(p->us), static_cast<void*>(p->vs), static_cast<void*>(p->spec),
                     static_cast<void*>(p->lam), static_cast<void*>(p->tw), static_cast<void*>(p->fx),
-                    static_cast<void*>(p->dlam)})
+                    static_cast<void*>(p->dlam), static_cast<void*>(p->kbuf), static_cast<void*>(p->sbuf),
+                    static_cast<void*>(p->rk_terms)})
     if (ptr != nullptr) (void)hipFree(ptr);
   for (void* ptr : {static_cast<void*>(p->cg.x), static_cast<void*>(p->cg.r),
                     static_cast<void*>(p->cg.p[0]), static_cast<void*>(p->cg.p[1]),
@@ -769,7 +847,7 @@ int jcfd_fvm_step_vjp(jcfd_fvm_plan* p, const jcfd_tensor* u, const jcfd_tensor*
                       jcfd_tensor* out_v, void* stream) {
   if (p == nullptr) return fail(JCFD_ERR_INVALID_ARGUMENT, "null plan");
   if (p->cfg.convect != JCFD_CONVECT_VAN_LEER || p->cfg.pressure != JCFD_PRESSURE_FAST_DIAG ||
-      p->cfg.diffusion != JCFD_DIFFUSION_EXPLICIT)
+      p->cfg.diffusion != JCFD_DIFFUSION_EXPLICIT || p->rk_terms != nullptr)
     return fail(JCFD_ERR_UNSUPPORTED,
                 "the adjoint is built for advect_van_leer + explicit diffusion + solve_fast_diag");
   const int n = p->params.n;
@@ -999,8 +1077,9 @@ int jcfd_fvm_group_step(jcfd_fvm_group* g, const jcfd_fvm_group_member* members,
       return fail(JCFD_ERR_UNSUPPORTED, who + ": groups step fast-diagonal plans only");
     if (p->params.radix != 8)
       return fail(JCFD_ERR_UNSUPPORTED, who + ": groups run the radix-8 FFT schedule only");
-    if (p->dlam != nullptr)
-      return fail(JCFD_ERR_UNSUPPORTED, who + ": groups step explicit-diffusion plans only");
+    if (p->dlam != nullptr || p->rk_terms != nullptr)
+      return fail(JCFD_ERR_UNSUPPORTED,
+                  who + ": groups step forward-Euler explicit-diffusion plans only");
     const int n = p->params.n;
     if (!fvm_group_size_supported(n))
       return fail(JCFD_ERR_UNSUPPORTED, who + ": grid size " + std::to_string(n) +

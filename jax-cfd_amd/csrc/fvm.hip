@@ -213,9 +213,9 @@ __device__ __forceinline__ void stage_rows(float* su, float* sv, const float* __
 template <int N, int RB, int MODE>
 struct RowsForwardLds {
   static constexpr int H = N / 2, WR = RB + 5;
-  static constexpr bool kFft = MODE == 0 || MODE == 1 || MODE == 3;   // row FFT of div / field
-  static constexpr bool kWin = MODE == 0 || MODE == 2 || MODE == 4;   // stencil window
-  static constexpr bool kStar = MODE == 0 || MODE == 1 || MODE == 4;  // u*, v* rows
+  static constexpr bool kFft = MODE == 0 || MODE == 1 || MODE == 3 || MODE == 5;  // row FFT
+  static constexpr bool kWin = MODE == 0 || MODE == 2 || MODE == 4;                // stencil window
+  static constexpr bool kStar = MODE == 0 || MODE == 1 || MODE == 4 || MODE == 5;  // u*, v* rows
   // Window rows of the cells whose explicit terms are formed: u at LO_U..HI
   // (global rows i0-1.. for the backward difference of div u*, or i0.. for
   // MODE 2), v at 3..HI (global rows i0..i0+RB-1).
@@ -401,6 +401,9 @@ struct RowsProjectLds {
 // MODE 2: explicit terms only (write k_u, k_v to us, vs).
 // MODE 3: spectrum of the field b.u itself (spectral filtering).
 // MODE 4: CG right-hand side (write div u* to b.rhs, no FFT).
+// MODE 5: Runge-Kutta combination u* = u0 + dt * (0 + sum c_t k_t) with
+//         u0 = (b.u, b.v) and the terms of *b.rk (time_stepping.py:91-101);
+//         write u*, v*, spectrum of div u*.
 template <int N, int RB, int MODE, int R = 8>
 __device__ __forceinline__ void rows_forward_body(const FvmParams& p, const FvmBuffers& b, int i0,
                                                   int tid, char* smem) {
@@ -426,6 +429,27 @@ __device__ __forceinline__ void rows_forward_body(const FvmParams& p, const FvmB
       const int lr = e / N, j = e % N;
       sus[e] = b.u[((i0 - 1 + lr) & (N - 1)) * N + j];
       if (lr < RB) svs[e] = b.v[((i0 + lr) & (N - 1)) * N + j];
+    }
+  } else if constexpr (MODE == 5) {
+    const RkTerms& t = *b.rk;
+    const size_t off = b.inst * static_cast<size_t>(N) * N;
+    const float dt = p.dt;
+    for (int e = tid; e < (RB + 1) * N; e += kThreads) {
+      const int lr = e / N, j = e % N;
+      const int gu = ((i0 - 1 + lr) & (N - 1)) * N + j;
+      float acc = 0.f;
+      for (int k = 0; k < t.n; ++k) acc = acc + t.c[k] * t.ku[k][off + gu];
+      const float ust = b.u[gu] + dt * acc;
+      sus[e] = ust;
+      if (lr >= 1) b.us[gu] = ust;
+      if (lr < RB) {
+        const int gv = ((i0 + lr) & (N - 1)) * N + j;
+        float accv = 0.f;
+        for (int k = 0; k < t.n; ++k) accv = accv + t.c[k] * t.kv[k][off + gv];
+        const float vst = b.v[gv] + dt * accv;
+        svs[e] = vst;
+        b.vs[gv] = vst;
+      }
     }
   } else {
     stage_rows<N, WR>(su, sv, b.u, b.v, i0 - 3, tid);
@@ -849,6 +873,22 @@ int launch_diffuse(const FvmParams& p, const FvmBuffers& b, const float* mult, h
   return check_hip(hipGetLastError(), "implicit diffusion launch");
 }
 
+template <int N>
+int launch_rk_explicit(const FvmParams& p, const FvmBuffers& b, hipStream_t s) {
+  constexpr int RB = Tiles<N>::RB;
+  hipLaunchKernelGGL((fvm_rows_forward<N, RB, 2>), dim3(N / RB, nb(b)), dim3(kThreads), 0, s, p, b);
+  return check_hip(hipGetLastError(), "rk explicit-terms launch");
+}
+
+template <int N>
+int launch_rk_project(const FvmParams& p, const FvmBuffers& b, hipStream_t s) {
+  constexpr int RB = Tiles<N>::RB, W = Tiles<N>::W;
+  hipLaunchKernelGGL((fvm_rows_forward<N, RB, 5>), dim3(N / RB, nb(b)), dim3(kThreads), 0, s, p, b);
+  hipLaunchKernelGGL((fvm_poisson_cols<N, W>), dim3(N / 2 / W, nb(b)), dim3(kThreads), 0, s, b, 1.0f);
+  hipLaunchKernelGGL((fvm_rows_project<N, RB, 0>), dim3(N / RB, nb(b)), dim3(kThreads), 0, s, p, b);
+  return check_hip(hipGetLastError(), "rk projection launch");
+}
+
 template <template <int> class F, typename... Args>
 int dispatch_n(int n, Args&&... args) {
   switch (n) {
@@ -894,6 +934,12 @@ template <int N> struct DiffuseOp {
   static int run(const FvmParams& p, const FvmBuffers& b, const float* mult, hipStream_t s) {
     return launch_diffuse<N>(p, b, mult, s);
   }
+};
+template <int N> struct RkExplicitOp {
+  static int run(const FvmParams& p, const FvmBuffers& b, hipStream_t s) { return launch_rk_explicit<N>(p, b, s); }
+};
+template <int N> struct RkProjectOp {
+  static int run(const FvmParams& p, const FvmBuffers& b, hipStream_t s) { return launch_rk_project<N>(p, b, s); }
 };
 template <int N> struct ProjOp {
   static int run(const FvmParams& p, const FvmBuffers& b, hipStream_t s) { return launch_projection<N>(p, b, s); }
@@ -1079,6 +1125,14 @@ int fvm_launch_step(const FvmParams& p, const FvmBuffers& b, hipStream_t s) {
 
 int fvm_launch_diffuse(const FvmParams& p, const FvmBuffers& b, const float* mult, hipStream_t s) {
   return dispatch_n<DiffuseOp>(p.n, p, b, mult, s);
+}
+
+int fvm_launch_rk_explicit(const FvmParams& p, const FvmBuffers& b, hipStream_t s) {
+  return dispatch_n<RkExplicitOp>(p.n, p, b, s);
+}
+
+int fvm_launch_rk_project(const FvmParams& p, const FvmBuffers& b, hipStream_t s) {
+  return dispatch_n<RkProjectOp>(p.n, p, b, s);
 }
 
 int fvm_launch_kernel(const FvmParams& p, const FvmBuffers& b, int which, hipStream_t s) {

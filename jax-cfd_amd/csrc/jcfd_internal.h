@@ -33,6 +33,16 @@ struct FvmParams {
   int rb_fwd, w_cols, rb_proj, radix;
 };
 
+// One Runge-Kutta combination u* = u0 + dt * (0 + c[0] k[0] + ... ) with the
+// reference's term order (zero coefficients left out by the host); k arrays
+// are batch-sized like the state (instance s at s * n * n).
+struct RkTerms {
+  const float* ku[4];
+  const float* kv[4];
+  float c[4];
+  int n;
+};
+
 // Pointers of one FVM instance for one step.
 struct FvmBuffers {
   const float* u;        // state in (n, n)
@@ -52,6 +62,8 @@ struct FvmBuffers {
   int count;  // instances in this launch (gridDim.y); <= 0 means 1
   float* rhs;  // CG variant: div u* (n, n), single instance
   const float* q;  // CG variant: pressure correction (n, n), single instance
+  const RkTerms* rk;  // Runge-Kutta combination (device memory) of rows-forward mode 5
+  size_t inst;        // instance index of this block (set by fvm_instance)
 };
 
 // Conjugate-gradient pressure solve (cg.hip), one instance.
@@ -90,6 +102,7 @@ __device__ __forceinline__ FvmBuffers fvm_instance(FvmBuffers b, int n) {
   if (b.u_out) b.u_out += st;
   if (b.v_out) b.v_out += st;
   if (b.spec) b.spec += st / 2;
+  b.inst = s;
   return b;
 }
 
@@ -98,6 +111,12 @@ int fvm_launch_step(const FvmParams& p, const FvmBuffers& b, hipStream_t s);
 // Implicit diffusion solve in place on b.u_out / b.v_out (every instance of b)
 // with the multiplier table `mult` ((n, n/2 + 1) floats, 1/n^2 folded in).
 int fvm_launch_diffuse(const FvmParams& p, const FvmBuffers& b, const float* mult, hipStream_t s);
+// Runge-Kutta pieces (every instance of b, default tile shapes):
+// explicit terms of (b.u, b.v) into (b.us, b.vs) rows of all instances, and
+// the combination b.u + dt * sum(b.rk) -> (b.us, b.vs) + its div spectrum,
+// Poisson columns and velocity correction into (b.u_out, b.v_out).
+int fvm_launch_rk_explicit(const FvmParams& p, const FvmBuffers& b, hipStream_t s);
+int fvm_launch_rk_project(const FvmParams& p, const FvmBuffers& b, hipStream_t s);
 // One kernel class of the step (0 rows-forward, 1 Poisson columns, 2 rows-project).
 int fvm_launch_kernel(const FvmParams& p, const FvmBuffers& b, int which, hipStream_t s);
 int fvm_launch_explicit_terms(const FvmParams& p, const float* u, const float* v,

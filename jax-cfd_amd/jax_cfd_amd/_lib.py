@@ -57,6 +57,9 @@ class FvmConfig(ctypes.Structure):
       ("forcing_x", ctypes.POINTER(ctypes.c_float)),
       ("batch", ctypes.c_int32),
       ("diffusion", ctypes.c_int32),
+      ("rk_stages", ctypes.c_int32),
+      ("rk_a", ctypes.c_double * 6),
+      ("rk_b", ctypes.c_double * 4),
   ]
 
 

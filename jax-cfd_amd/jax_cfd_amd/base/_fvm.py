@@ -29,6 +29,9 @@ class FvmSpec:
   linear_coefficient: Optional[float] = None  # None: no forcing at all
   forcing_x: Optional[Tuple[float, ...]] = None  # f_x(y_j), n values
   diffusion: int = _lib.DIFFUSION_EXPLICIT
+  rk_stages: int = 0  # 0: forward Euler (fused); else the Butcher tableau below
+  rk_a: Tuple[float, ...] = ()  # a[i-1][j] packed row by row
+  rk_b: Tuple[float, ...] = ()
 
 
 def spec_from_grid(grid, **kw) -> FvmSpec:
@@ -69,6 +72,11 @@ class FvmPlan:
       cfg.forcing_x = self._fx.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
     cfg.batch = self.batch
     cfg.diffusion = spec.diffusion
+    cfg.rk_stages = spec.rk_stages
+    for k, x in enumerate(spec.rk_a):
+      cfg.rk_a[k] = x
+    for k, x in enumerate(spec.rk_b):
+      cfg.rk_b[k] = x
     handle = ctypes.c_void_p()
     with torch.cuda.device(self.device):
       _lib.check(lib.jcfd_fvm_plan_create(ctypes.byref(cfg), ctypes.byref(handle)),

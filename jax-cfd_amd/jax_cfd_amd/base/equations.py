@@ -163,8 +163,12 @@ def semi_implicit_navier_stokes(
     time_stepper: Callable = time_stepping.forward_euler,
 ) -> FvmStepFn:
   """Returns step_fn for explicit-diffusion Navier-Stokes (equations.py:120-151)."""
-  if time_stepper is not time_stepping.forward_euler:
-    raise NotImplementedError('only time_stepper=forward_euler is built')
+  tableau = time_stepping.tableau_of(time_stepper)
+  if tableau is None:
+    raise NotImplementedError(
+        'time_stepper must be forward_euler, midpoint_rk2, heun_rk2, classic_rk4 or '
+        'functools.partial(navier_stokes_rk, tableau)')
+  rk = time_stepping.plan_fields(tableau)
   if diffuse is not diffusion.diffuse:
     raise NotImplementedError('only diffusion.diffuse (explicit) is built')
   if pressure_solve is pressure.solve_fast_diag:
@@ -175,10 +179,12 @@ def semi_implicit_navier_stokes(
     raise NotImplementedError(f'pressure_solve {pressure_solve!r} is not built')
   conv = _read_convection(convect, grid, dt)
   lin, fx = _read_forcing(forcing, grid)
+  if rk and solve_kw:
+    raise NotImplementedError('Runge-Kutta tableaux are built with solve_fast_diag')
   spec = _fvm.spec_from_grid(
       grid, dt=float(dt), density=float(density),
       viscosity=None if viscosity is None else float(viscosity),
-      convect=conv, linear_coefficient=lin, forcing_x=fx, **solve_kw)
+      convect=conv, linear_coefficient=lin, forcing_x=fx, **solve_kw, **rk)
   return FvmStepFn(grid, spec)
 
 

@@ -3,8 +3,8 @@
 TEST INFRASTRUCTURE ONLY (see oracle/__init__.py).
 
 ``FVMStep`` mirrors ``semi_implicit_navier_stokes`` (jax_cfd/base/
-equations.py:120-151) with ``time_stepper=forward_euler``
-(time_stepping.py:59-118), an ``advect_van_leer`` or
+equations.py:120-151) with ``time_stepper`` forward_euler, midpoint_rk2,
+heun_rk2 or classic_rk4 (navier_stokes_rk, time_stepping.py:59-158), an ``advect_van_leer`` or
 ``advect_van_leer_using_limiters`` convection, explicit diffusion, optional
 Kolmogorov + linear forcing, and ``projection`` (pressure.py:181-198) with
 either ``solve_fast_diag`` (pressure.py:115-157 via fast_diagonalization.py:
@@ -21,7 +21,9 @@ Taylor-Green closed form (validation_test.py:34-169); for the implicit
 diffusion solve, constant fields unchanged (diffusion_test.py:58-76), the
 residual of (1 - nu dt L) x = b (diffusion_test.py:31-42), a dense
 ``numpy.linalg.solve`` brute force and the single-Fourier-mode closed form.
-``solve_cg`` follows the
+The Runge-Kutta driver is pinned by the reference's harmonic-oscillator test
+(time_stepping_test.py:27-90, its per-stepper tolerances).  ``solve_cg``
+follows the
 published conjugate-gradient algorithm of jax.scipy.sparse.linalg.cg (no
 preconditioner, stop when |r|^2 <= max(tol^2 |b|^2, atol^2)); it is checked
 against the fast-diagonalisation solve, but its individual iterates are
@@ -148,6 +150,40 @@ def stable_time_step(max_velocity: float, max_courant_number: float,
   return dt
 
 
+# ------------------------------------------------------- Runge-Kutta steppers
+# time_stepping.py:109-158: (a, b) of forward_euler, midpoint_rk2, heun_rk2,
+# classic_rk4.
+TABLEAUS = {
+    'forward_euler': ((), (1,)),
+    'midpoint_rk2': (((1 / 2,),), (0, 1)),
+    'heun_rk2': (((1,),), (1 / 2, 1 / 2)),
+    'classic_rk4': (((1 / 2,), (0, 1 / 2), (0, 0, 1)), (1 / 6, 1 / 3, 1 / 3, 1 / 6)),
+}
+
+
+def navier_stokes_rk(a, b, explicit_terms: Callable, projection_fn: Callable,
+                     dt: float) -> Callable:
+  """time_stepping.py:59-106 on tuples of arrays: stage states are projected,
+  sums skip zero coefficients and start from the integer 0, as the
+  reference's ``sum(...)`` does."""
+  def combine(u0, coefs, k):
+    terms = [(c, kj) for c, kj in zip(coefs, k) if c]
+    out = []
+    for comp in range(len(u0)):
+      acc = 0
+      for c, kj in terms:
+        acc = acc + c * kj[comp]
+      out.append(u0[comp] + dt * acc)
+    return tuple(out)
+
+  def step(u0):
+    k = [explicit_terms(u0)]
+    for i in range(1, len(b)):
+      k.append(explicit_terms(projection_fn(combine(u0, a[i - 1], k))))
+    return projection_fn(combine(u0, b, k))
+  return step
+
+
 @dataclasses.dataclass
 class FVMStep:
   """semi_implicit_navier_stokes + forward_euler, restated.
@@ -165,6 +201,7 @@ class FVMStep:
   pressure: str = 'fast_diag'
   cg_rtol: float = 1e-6
   cg_atol: float = 1e-6
+  time_stepper: str = 'forward_euler'
 
   def convection(self, v: Sequence[A]):
     if self.convect == 'van_leer':
@@ -200,7 +237,14 @@ class FVMStep:
                  for u, k in zip(v, k0))
 
   def __call__(self, v: Sequence[A]) -> Tuple[A, ...]:
-    return projection(self.intermediate(v), self.grid, self.solve)
+    if self.time_stepper == 'forward_euler':
+      return projection(self.intermediate(v), self.grid, self.solve)
+    a, b = TABLEAUS[self.time_stepper]
+    offsets = [u.offset for u in v]
+    wrap = lambda xs: tuple(A(x, o) for x, o in zip(xs, offsets))  # noqa: E731
+    F = lambda xs: tuple(k.data for k in self.explicit_terms(wrap(xs)))  # noqa: E731
+    P = lambda xs: tuple(u.data for u in projection(wrap(xs), self.grid, self.solve))  # noqa: E731
+    return wrap(navier_stokes_rk(a, b, F, P, self.dt)(tuple(u.data for u in v)))
 
   def rollout(self, v: Sequence[A], steps: int) -> Tuple[A, ...]:
     for _ in range(steps):

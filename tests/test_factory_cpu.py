@@ -73,7 +73,11 @@ def test_default_convection_is_van_leer_using_limiters():
     dict(convect=lambda v: tuple(cfd.advection.advect_van_leer(v[0], v, 0.02) for _ in v)),
     dict(convect=lambda v: tuple(cfd.advection.advect_van_leer(u, v, 0.01) for u in v)),
     dict(convect=_vl(0.02), forcing=cfd.forcings.kolmogorov_forcing(_grid(), swap_xy=True)),
-    dict(convect=_vl(0.02), time_stepper=cfd.time_stepping.classic_rk4),
+    dict(convect=_vl(0.02), time_stepper=lambda equation, dt: None),
+    dict(convect=_vl(0.02), time_stepper=functools.partial(
+        cfd.time_stepping.navier_stokes_rk,
+        cfd.time_stepping.ButcherTableau(a=[[1], [1, 1], [1, 1, 1], [1, 1, 1, 1]],
+                                         b=[1, 1, 1, 1, 1]))),
     dict(convect=_vl(0.02), diffuse=lambda c, nu: c.array),
 ])
 def test_unsupported_compositions_raise(kw):
@@ -127,3 +131,28 @@ def test_implicit_diffusion_factory_spec():
         1.0, 1e-3, 0.01, grid,
         diffusion_solve=functools.partial(cfd.diffusion.solve_fast_diag,
                                           implementation='matmul'))
+
+
+@pytest.mark.parametrize("stepper,stages,a,b", [
+    ('midpoint_rk2', 2, (0.5,), (0.0, 1.0)),
+    ('heun_rk2', 2, (1.0,), (0.5, 0.5)),
+    ('classic_rk4', 4, (0.5, 0.0, 0.5, 0.0, 0.0, 1.0), (1 / 6, 1 / 3, 1 / 3, 1 / 6))])
+def test_runge_kutta_time_steppers_map_to_tableau(stepper, stages, a, b):
+  """time_stepping.py:109-158: named steppers carry their tableaux into the
+  plan; forward Euler stays on the fused kernels."""
+  grid = _grid(64)
+  step = cfd.equations.semi_implicit_navier_stokes(
+      1.0, 1e-3, 0.01, grid, convect=_vl(0.01),
+      time_stepper=getattr(cfd.time_stepping, stepper))
+  assert (step.spec.rk_stages, step.spec.rk_a, step.spec.rk_b) == (stages, a, b)
+  euler = cfd.equations.semi_implicit_navier_stokes(1.0, 1e-3, 0.01, grid, convect=_vl(0.01))
+  assert euler.spec.rk_stages == 0
+  custom = cfd.equations.semi_implicit_navier_stokes(
+      1.0, 1e-3, 0.01, grid, convect=_vl(0.01),
+      time_stepper=functools.partial(cfd.time_stepping.navier_stokes_rk,
+                                     cfd.time_stepping.ButcherTableau(a=[], b=[2.0])))
+  assert (custom.spec.rk_stages, custom.spec.rk_b) == (1, (2.0,))
+  with pytest.raises(NotImplementedError):  # Runge-Kutta with the CG solve is not built
+    cfd.equations.semi_implicit_navier_stokes(
+        1.0, 1e-3, 0.01, grid, convect=_vl(0.01), pressure_solve=cfd.pressure.solve_cg,
+        time_stepper=getattr(cfd.time_stepping, stepper))

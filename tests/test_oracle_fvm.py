@@ -454,3 +454,63 @@ def test_implicit_diffusion_step_invariants():
   assert np.abs(go.divergence(v, grid).data).max() < 1e-10
   for a, b in zip(v0, v):
     assert abs(a.data.sum() - b.data.sum()) < 1e-9
+
+
+# ------------------------------------------------------ Runge-Kutta steppers
+@pytest.mark.parametrize("name,atol", [('forward_euler', 1e-2), ('midpoint_rk2', 3e-5),
+                                       ('heun_rk2', 3e-5), ('classic_rk4', 4e-7)])
+def test_rk_harmonic_oscillator(name, atol):
+  """time_stepping_test.py:27-90: x' = (x1, -x0) from (1, 1) with the
+  identity projection, dt 1e-2, trajectory of 5 x 20 steps against the
+  closed form, with the reference's tolerance for each stepper (fp64)."""
+  a, b = fvm.TABLEAUS[name]
+  F = lambda x: (x[1], -x[0])  # noqa: E731
+  step = fvm.navier_stokes_rk(a, b, F, lambda x: x, 1e-2)
+  x = (np.float64(1.0), np.float64(1.0))
+  theta, r = np.arctan(1.0), np.sqrt(2.0)
+  for outer in range(5):
+    for _ in range(20):
+      x = step(x)
+    t = 1e-2 * 20 * (outer + 1)
+    np.testing.assert_allclose(x, (r * np.sin(t + theta), r * np.cos(t + theta)),
+                               atol=atol, rtol=0)
+
+
+def test_rk_orders_of_accuracy():
+  """Global error on the oscillator falls as dt^p: p = 1, 2, 2, 4."""
+  for name, order in (('forward_euler', 1), ('midpoint_rk2', 2), ('heun_rk2', 2),
+                      ('classic_rk4', 4)):
+    a, b = fvm.TABLEAUS[name]
+    errs = []
+    for dt in (0.02, 0.01):
+      step = fvm.navier_stokes_rk(a, b, lambda x: (x[1], -x[0]), lambda x: x, dt)
+      x = (1.0, 0.0)
+      for _ in range(int(round(1.0 / dt))):
+        x = step(x)
+      errs.append(np.hypot(x[0] - np.cos(1.0), x[1] + np.sin(1.0)))
+    assert 0.8 * order < np.log2(errs[0] / errs[1]) < 1.25 * order + 0.3, (name, errs)
+
+
+def test_rk_forward_euler_equals_euler_step():
+  """The generic driver with b = [1] is the fused Euler step bit for bit."""
+  grid = Grid.from_step((16, 16), (1., 1.))
+  v0 = fvm.projection(_sinusoidal_velocity(grid), grid)
+  e = fvm.FVMStep(grid, dt=1e-2, viscosity=1e-3)
+  a, b = fvm.TABLEAUS['forward_euler']
+  F = lambda xs: tuple(k.data for k in e.explicit_terms(tuple(A(x, u.offset) for x, u in zip(xs, v0))))  # noqa: E731
+  P = lambda xs: tuple(u.data for u in fvm.projection(tuple(A(x, u.offset) for x, u in zip(xs, v0)), grid))  # noqa: E731
+  out = fvm.navier_stokes_rk(a, b, F, P, 1e-2)(tuple(u.data for u in v0))
+  for x, y in zip(out, e(v0)):
+    np.testing.assert_array_equal(x, y.data)
+
+
+def test_rk4_step_invariants():
+  """equations_test.py:84-163 pattern with classic_rk4: divergence ~0 and
+  momentum conserved."""
+  grid = Grid.from_step((32, 32), (1., 1.))
+  step = fvm.FVMStep(grid, dt=1e-2, viscosity=1e-4, time_stepper='classic_rk4')
+  v0 = fvm.projection(_sinusoidal_velocity(grid), grid)
+  v = step.rollout(v0, 20)
+  assert np.abs(go.divergence(v, grid).data).max() < 1e-10
+  for a_, b_ in zip(v0, v):
+    assert abs(a_.data.sum() - b_.data.sum()) < 1e-9
