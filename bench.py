@@ -59,39 +59,45 @@ def _args():
 
 
 def _pmc_rows(root, counter):
-  """{kernel name: [counter value per dispatch]} from rocprofv3's
+  """[(dispatch id, kernel name, counter value)] from rocprofv3's
   *counter_collection.csv files under `root`."""
   import csv
-  out = {}
+  out = []
   for dirpath, _, files in os.walk(root):
     for fn in files:
       if not fn.endswith("counter_collection.csv"):
         continue
       with open(os.path.join(dirpath, fn), newline="") as f:
         for row in csv.DictReader(f):
-          if row.get("Counter_Name") != counter:
-            continue
-          out.setdefault(row["Kernel_Name"], []).append(float(row["Counter_Value"]))
+          if row.get("Counter_Name") == counter:
+            out.append((int(row["Dispatch_Id"]), row["Kernel_Name"], float(row["Counter_Value"])))
   return out
 
 
 def _pmc_summary(fetch_dir, write_dir, copy_mib):
   """HBM bytes per dispatch of every kernel: FETCH_SIZE and WRITE_SIZE (KiB
-  per dispatch), each scaled so that the calibration copy of `copy_mib` MiB
-  reads and writes exactly its size."""
+  per dispatch) of the two runs, each scaled so that the calibration copy
+  (--copy-mib: the process's first large dispatch, which reads and writes
+  copy_mib MiB) reads and writes exactly its size."""
   fetch, write = _pmc_rows(fetch_dir, "FETCH_SIZE"), _pmc_rows(write_dir, "WRITE_SIZE")
-  mean = lambda xs: sum(xs) / len(xs)  # noqa: E731
   copy_kib = copy_mib * 1024.0
-  # the calibration copy is the kernel whose mean fetch is closest to its size
-  cand = [k for k in fetch if k in write]
-  ref = min(cand, key=lambda k: abs(mean(fetch[k]) - copy_kib) + abs(mean(write[k]) - copy_kib))
-  f_scale, w_scale = copy_kib / mean(fetch[ref]), copy_kib / mean(write[ref])
-  kernels = {}
-  for k in cand:
-    kernels[k] = {"dispatches": len(fetch[k]),
-                  "fetch_bytes": mean(fetch[k]) * 1024.0 * f_scale,
-                  "write_bytes": mean(write[k]) * 1024.0 * w_scale}
-  return {"calibration_kernel": ref, "copy_mib": copy_mib,
+
+  def calibration(rows):  # the first dispatch of the run that moves a sizeable part of it
+    return min((r for r in rows if r[2] >= 0.25 * copy_kib), key=lambda r: r[0])
+  f_ref, w_ref = calibration(fetch), calibration(write)
+  f_scale, w_scale = copy_kib / f_ref[2], copy_kib / w_ref[2]
+  per = {}
+  for rows, key, scale in ((fetch, "fetch_bytes", f_scale), (write, "write_bytes", w_scale)):
+    for did, name, val in rows:
+      if (did, name) in ((f_ref[0], f_ref[1]), (w_ref[0], w_ref[1])):
+        continue  # the calibration copy itself
+      per.setdefault(name, {}).setdefault(key, []).append(val * 1024.0 * scale)
+  kernels = {k: {"dispatches": len(v.get("fetch_bytes", [])),
+                 "fetch_bytes": sum(v["fetch_bytes"]) / len(v["fetch_bytes"]),
+                 "write_bytes": sum(v["write_bytes"]) / len(v["write_bytes"])}
+             for k, v in per.items() if v.get("fetch_bytes") and v.get("write_bytes")}
+  return {"copy_mib": copy_mib, "fetch_calibration": {"kernel": f_ref[1], "kib": f_ref[2]},
+          "write_calibration": {"kernel": w_ref[1], "kib": w_ref[2]},
           "fetch_scale": f_scale, "write_scale": w_scale, "kernels": kernels}
 
 

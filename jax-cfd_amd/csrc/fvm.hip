@@ -375,7 +375,8 @@ __device__ __forceinline__ void rows_forward_march_body(const FvmParams& p, cons
 template <int N, int RB, int R = 8>
 __global__ __launch_bounds__(kThreads) void fvm_rows_forward_march(FvmParams p, FvmBuffers b) {
   __shared__ __align__(16) char smem[MarchLds<N, RB>::bytes];
-  rows_forward_march_body<N, RB, R>(p, fvm_instance(b, N), blockIdx.x * RB, threadIdx.x, smem);
+  rows_forward_march_body<N, RB, R>(p, fvm_instance(b, N), xcd_tile(blockIdx.x, gridDim.x) * RB,
+                                     threadIdx.x, smem);
 }
 
 template <int N, int W>
@@ -557,7 +558,8 @@ __device__ __forceinline__ void rows_forward_body(const FvmParams& p, const FvmB
 template <int N, int RB, int MODE, int R = 8>
 __global__ __launch_bounds__(kThreads) void fvm_rows_forward(FvmParams p, FvmBuffers b) {
   __shared__ __align__(16) char smem[RowsForwardLds<N, RB, MODE>::bytes];
-  rows_forward_body<N, RB, MODE, R>(p, fvm_instance(b, N), blockIdx.x * RB, threadIdx.x, smem);
+  rows_forward_body<N, RB, MODE, R>(p, fvm_instance(b, N), xcd_tile(blockIdx.x, gridDim.x) * RB,
+                                    threadIdx.x, smem);
 }
 
 // Column pass of the Poisson solve on the tile of W packed-spectrum columns
@@ -616,7 +618,8 @@ __device__ __forceinline__ void poisson_cols_body(const FvmBuffers& b, float sca
 template <int N, int W, int R = 8>
 __global__ __launch_bounds__(kThreads) void fvm_poisson_cols(FvmBuffers b, float scale) {
   __shared__ __align__(16) char smem[PoissonLds<N, W>::bytes];
-  poisson_cols_body<N, W, R>(fvm_instance(b, N), scale, blockIdx.x * W, threadIdx.x, smem);
+  poisson_cols_body<N, W, R>(fvm_instance(b, N), scale, xcd_tile(blockIdx.x, gridDim.x) * W,
+                             threadIdx.x, smem);
 }
 
 // Inverse row transforms of q and the velocity correction for rows
@@ -675,7 +678,8 @@ __device__ __forceinline__ void rows_project_body(const FvmParams& p, const FvmB
 template <int N, int RB, int MODE, int R = 8>
 __global__ __launch_bounds__(kThreads) void fvm_rows_project(FvmParams p, FvmBuffers b) {
   __shared__ __align__(16) char smem[RowsProjectLds<N, RB, MODE>::bytes];
-  rows_project_body<N, RB, MODE, R>(p, fvm_instance(b, N), blockIdx.x * RB, threadIdx.x, smem);
+  rows_project_body<N, RB, MODE, R>(p, fvm_instance(b, N), xcd_tile(blockIdx.x, gridDim.x) * RB,
+                                    threadIdx.x, smem);
 }
 
 // curl_2d (finite_differences.py:186-193): forward differences, offset (1, 1).

@@ -91,6 +91,16 @@ int fvm_launch_project_q(const FvmParams& p, const FvmBuffers& b, hipStream_t s)
 
 inline int nb(const FvmBuffers& b) { return b.count > 0 ? b.count : 1; }
 
+// Workgroups are dispatched round-robin over the 8 XCDs (each with its own
+// L2): workgroup b of a row of nb runs on XCD b % 8.  This maps it to tile
+// (b % 8) * (nb / 8) + b / 8 so every XCD gets a contiguous run of tiles, and
+// neighbouring tiles -- which share halo rows or cache lines -- share an L2.
+// (blockIdx.y rows of nb workgroups each keep the same XCD pattern when nb is
+// a multiple of 8; otherwise the identity is kept.)
+__device__ __forceinline__ int xcd_tile(int b, int nb) {
+  return (nb % 8 != 0) ? b : (b % 8) * (nb / 8) + b / 8;
+}
+
 // Per-instance view of FvmBuffers for the instance this block works on.
 __device__ __forceinline__ FvmBuffers fvm_instance(FvmBuffers b, int n) {
   const size_t s = b.slots ? static_cast<size_t>(b.slots[blockIdx.y]) : blockIdx.y;

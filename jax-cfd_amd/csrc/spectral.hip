@@ -90,13 +90,6 @@ struct WTile {
   float2 w[PER];
 };
 
-// Hardware hands workgroups to the 8 XCDs round-robin; give each XCD a
-// contiguous run of tiles so neighbouring tiles (which share cache lines)
-// share an L2.
-__device__ __forceinline__ int xcd_tile(int b, int nb) {
-  return (nb % 8 != 0) ? b : (b % 8) * (nb / 8) + b / 8;
-}
-
 // Build spectrum m of the tile into sm.buf (scaled by 1/n^2), symmetrising and
 // packing the DC/Nyquist pair into local column 0 of tile 0.  The multipliers
 // are recomputed per spectrum from the LDS wavenumber tables (reads after a
@@ -201,7 +194,7 @@ __global__ __launch_bounds__(kThreads) void rows_r2c(NsParams q, const float* __
   __shared__ float2 twn[N];
   __shared__ float2 twh[H];
   const int tid = threadIdx.x;
-  const int i0 = blockIdx.x * RB;
+  const int i0 = xcd_tile(blockIdx.x, gridDim.x) * RB;
   const size_t inst = q.slots ? static_cast<size_t>(q.slots[blockIdx.y]) : blockIdx.y;
   x += inst * static_cast<size_t>(N) * N;
   pout += inst * static_cast<size_t>(N) * H;
@@ -225,7 +218,7 @@ __global__ __launch_bounds__(kThreads) void rows_c2r(NsParams q, const float2* _
   __shared__ float2 twn[N];
   __shared__ float2 twh[H];
   const int tid = threadIdx.x;
-  const int i0 = blockIdx.x * RB;
+  const int i0 = xcd_tile(blockIdx.x, gridDim.x) * RB;
   const size_t inst = q.slots ? static_cast<size_t>(q.slots[blockIdx.y]) : blockIdx.y;
   x += inst * static_cast<size_t>(N) * N;
   pin += inst * static_cast<size_t>(N) * H;
@@ -252,7 +245,7 @@ __global__ __launch_bounds__(kThreads) void ns_rows(NsParams q, const float2* __
   __shared__ float2 twn[N];
   __shared__ float2 twh[H];
   const int tid = threadIdx.x;
-  const int i0 = blockIdx.x * RB;
+  const int i0 = xcd_tile(blockIdx.x, gridDim.x) * RB;
   const size_t inst = q.slots ? static_cast<size_t>(q.slots[blockIdx.y]) : blockIdx.y;
   const size_t gstride = static_cast<size_t>(N) * H;
   g += inst * 4 * gstride;
