@@ -70,8 +70,22 @@ def _pmc_rows(root, counter):
       with open(os.path.join(dirpath, fn), newline="") as f:
         for row in csv.DictReader(f):
           if row.get("Counter_Name") == counter:
-            out.append((int(row["Dispatch_Id"]), row["Kernel_Name"], float(row["Counter_Value"])))
+            out.append((int(row["Dispatch_Id"]), _kernel_key(row["Kernel_Name"], row["Grid_Size"]),
+                        float(row["Counter_Value"])))
   return out
+
+
+def _kernel_key(name, grid):
+  """'fvm_rows_forward<256, 1, 0, 4> grid=65536': the kernel's short name and
+  its launch size in work-items (one launch configuration)."""
+  short = name.replace("void ", "", 1).replace("jcfd::(anonymous namespace)::", "")
+  depth, cut = 0, len(short)
+  for i, ch in enumerate(short):  # cut at the parameter list, not inside <...>
+    depth += (ch == "<") - (ch == ">")
+    if ch == "(" and depth == 0:
+      cut = i
+      break
+  return f"{short[:cut]} grid={int(grid)}"
 
 
 def _pmc_summary(fetch_dir, write_dir, copy_mib):
@@ -101,14 +115,24 @@ def _pmc_summary(fetch_dir, write_dir, copy_mib):
           "fetch_scale": f_scale, "write_scale": w_scale, "kernels": kernels}
 
 
-def _traffic(kernel, workload):
-  """HBM bytes per launch from the committed rocprofv3 counter summary."""
+def _traffic(key):
+  """HBM bytes (fetched + written) per launch of one launch configuration
+  (see _kernel_key) from the committed rocprofv3 counter summary, or None."""
   path = os.path.join(ROOT, "profiles", "traffic.json")
   try:
     with open(path) as f:
-      return json.load(f).get(workload, {}).get(kernel)
-  except (OSError, ValueError):
+      k = json.load(f)["kernels"].get(key)
+  except (OSError, ValueError, KeyError):
     return None
+  return None if k is None else round(k["fetch_bytes"] + k["write_bytes"])
+
+
+def _fvm_kernel_key(n, tiles):
+  """_kernel_key of the plain plan's stencil kernel for its tuned tiles."""
+  rb, radix = tiles["rows_forward"], tiles["radix"]
+  name = (f"fvm_rows_forward_march<{n}, {rb}, {radix}>" if rb >= 16
+          else f"fvm_rows_forward<{n}, {rb}, 0, {radix}>")
+  return f"{name} grid={n // rb * 256}"
 
 
 def _cpu_baseline(seconds):
@@ -429,10 +453,11 @@ def main():
   kbytes = plan.kernel_bytes()
   dominant = max(kms, key=kms.get)
   achieved = kbytes[dominant] / (kms[dominant] * 1e-3) / 1e9
-  workload = f"fvm{N}"
   roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
               "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
-              "traffic": _traffic(dominant, workload), "kernel": dominant,
+              "traffic": (_traffic(_fvm_kernel_key(N, tiles))
+                          if dominant == "fvm_rows_forward" else None),
+              "kernel": dominant,
               "bytes_per_launch": kbytes[dominant],
               "ms_per_launch": round(kms[dominant], 6)}
   kernels = {k: {"ms_per_launch": round(kms[k], 6), "algorithmic_bytes": kbytes[k],

@@ -240,7 +240,7 @@ class Pool:
     return out
 
 
-DEFAULT_CAPACITY = {64: 256, 128: 128, 256: 64, 512: 32}
+DEFAULT_CAPACITY = {64: 512, 128: 256, 256: 128, 512: 64}
 
 
 class RolloutServer:
