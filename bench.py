@@ -168,7 +168,7 @@ def _batched_spectral(dev, batch, n, steps, reps):
   w = torch.stack([fam.initial_state(s)[0] for s in range(batch)])
   plan = fam.make_plan(batch)
   from jax_cfd_amd import autotune
-  tiles = autotune.tune_ns2d(plan, radices=(8, 4), use_cache=False)
+  tiles = autotune.tune_ns2d(plan, radices=(8, 4))
   plan.step(w, 8, out=w)
   torch.cuda.synchronize()
   t0 = time.perf_counter()
@@ -434,7 +434,9 @@ def main():
   plan = step.plan(dev)
   u, v = v0[0].data.clone(), v0[1].data.clone()
   from jax_cfd_amd import autotune
-  tuned = autotune.tune_fvm(plan, u.clone(), v.clone(), radices=(8, 4), use_cache=False)
+  # tile choices are cached per (device, size, batch): a second run of the same
+  # command (e.g. under a profiler) steps with the same kernels
+  tuned = autotune.tune_fvm(plan, u.clone(), v.clone(), radices=(8, 4))
   tiles = {k: tuned[k] for k in ("rows_forward", "poisson_cols", "rows_project", "radix")}
   print(f"[bench] workload fvm {N}x{N} forced, dt={dt:.6g}; warmup {args.warmup}; "
         f"tuned tiles {tiles}", file=sys.stderr, flush=True)
