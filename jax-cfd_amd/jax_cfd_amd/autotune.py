@@ -86,8 +86,12 @@ def _store_cache(path: str, key: str, value: dict) -> None:
 
 
 def _device_name(device) -> str:
+  """ISA and compute-unit count (e.g. 'gfx950x256'): read from the HSA agent,
+  so the key is the same whether or not the marketing-name table is present
+  (it is absent under some profilers)."""
   try:
-    return torch.cuda.get_device_name(device)
+    props = torch.cuda.get_device_properties(device)
+    return f"{props.gcnArchName.split(':')[0]}x{props.multi_processor_count}"
   except Exception:  # no device query available
     return 'unknown'
 
