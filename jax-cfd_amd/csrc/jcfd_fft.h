@@ -165,6 +165,40 @@ __device__ __forceinline__ void load_twiddles(float2* dst, const float2* __restr
   for (int k = tid; k < n; k += nt) dst[k] = src[k * table_stride];
 }
 
+// A length-N twiddle table (entries k * STRIDE of the source table) staged
+// through registers: load() issues all of a thread's global loads, store()
+// writes them to LDS.  Splitting the two lets a kernel keep the loads in
+// flight together with its data loads (a loop with a run-time bound waits on
+// each load before the next is issued).
+template <int N, int STRIDE, int NT>
+struct TwiddleStage {
+  static constexpr int PER = (N + NT - 1) / NT;
+  float2 v[PER];
+  __device__ __forceinline__ void load(const float2* __restrict__ src, int tid) {
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int i = tid + k * NT;
+      if (N % NT == 0 || i < N) v[k] = src[i * STRIDE];
+    }
+  }
+  __device__ __forceinline__ void store(float2* dst, int tid) const {
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int i = tid + k * NT;
+      if (N % NT == 0 || i < N) dst[i] = v[k];
+    }
+  }
+};
+
+// Table loads of compile-time length: loads first, then the LDS writes.  No
+// barrier.
+template <int N, int STRIDE, int NT>
+__device__ __forceinline__ void load_twiddles_n(float2* dst, const float2* __restrict__ src, int tid) {
+  TwiddleStage<N, STRIDE, NT> t;
+  t.load(src, tid);
+  t.store(dst, tid);
+}
+
 // After an in-place length-n/2 forward transform of z (row in LDS), the
 // packed half-spectrum word k of the real row: needs Z[k] and Z[n/2 - k].
 // twn = exp(-2 pi i k / n) for this k.

@@ -38,6 +38,37 @@ namespace {
 
 constexpr int kThreads = 256;
 
+// Stage TOT elements through registers, up to CH per thread at a time: every
+// load of a chunk is issued before the chunk's first write, so a chunk costs
+// one memory latency (a loop with a run-time bound waits on each load).
+template <int TOT, int CH, typename Load, typename Store>
+__device__ __forceinline__ void staged(int tid, Load&& load, Store&& store) {
+  constexpr int PER = (TOT + kThreads - 1) / kThreads;
+  constexpr int C = PER < CH ? PER : CH;
+  using V = decltype(load(0));
+#pragma unroll 1
+  for (int k0 = 0; k0 < PER; k0 += C) {
+    V v[C];
+#pragma unroll
+    for (int k = 0; k < C; ++k) {
+      const int e = tid + (k0 + k) * kThreads;
+      if (e < TOT) v[k] = load(e);
+    }
+#pragma unroll
+    for (int k = 0; k < C; ++k) {
+      const int e = tid + (k0 + k) * kThreads;
+      if (e < TOT) store(e, v[k]);
+    }
+  }
+}
+
+struct F2Pair {
+  float2 a, b;
+};
+struct F2Triple {
+  float2 a, b, t;
+};
+
 // Per-element multipliers of the four derivative spectra, following the
 // reference's fp32/complex64 operation order.
 struct NsMult {
@@ -136,9 +167,17 @@ __device__ __forceinline__ void cols_inverse_body(ColShared<N, W>& sm, const WTi
     __syncthreads();
     lds_fft<N, W, kThreads, W, 1, true, true, R>(sm.buf, sm.tw, tid);
     float2* gm = g + m * gstride;
-    for (int e = tid; e < N * W; e += kThreads) {
-      const int p = e / W, c = e % W;
-      gm[p * H + c0 + c] = sm.buf[e];
+    constexpr int PER = WTile<N, W>::PER;
+    float2 o[PER];
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int e = tid + k * kThreads;
+      if ((N * W) % kThreads == 0 || e < N * W) o[k] = sm.buf[e];
+    }
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int e = tid + k * kThreads;
+      if ((N * W) % kThreads == 0 || e < N * W) gm[(e / W) * H + c0 + e % W] = o[k];
     }
     __syncthreads();
   }
@@ -155,17 +194,66 @@ __device__ __forceinline__ void load_w_tile(ColShared<N, W>& sm, WTile<N, W>& wt
     const int p = e / W, c = e % W;
     wt.w[k] = w[p * H1 + c0 + c];
   }
-  if (c0 == 0)
-    for (int p = tid; p < N; p += kThreads) {
-      sm.w0[p] = w[p * H1];
-      sm.wn[p] = w[p * H1 + N / 2];
+  if (c0 == 0) {
+    constexpr int PN = (N + kThreads - 1) / kThreads;
+    float2 a[PN], b[PN];
+#pragma unroll
+    for (int k = 0; k < PN; ++k) {
+      const int p = tid + k * kThreads;
+      if (N % kThreads == 0 || p < N) {
+        a[k] = w[p * H1];
+        b[k] = w[p * H1 + N / 2];
+      }
     }
+#pragma unroll
+    for (int k = 0; k < PN; ++k) {
+      const int p = tid + k * kThreads;
+      if (N % kThreads == 0 || p < N) {
+        sm.w0[p] = a[k];
+        sm.wn[p] = b[k];
+      }
+    }
+  }
 }
+
+// Wavenumber tables staged through registers (load() issues the global
+// loads, store() writes LDS), like TwiddleStage.
+template <int N>
+struct WavenumberStage {
+  static constexpr int PX = (N + kThreads - 1) / kThreads;
+  static constexpr int PY = (N / 2 + 1 + kThreads - 1) / kThreads;
+  float x[PX], y[PY];
+  __device__ __forceinline__ void load(const NsParams& q, int tid) {
+#pragma unroll
+    for (int k = 0; k < PX; ++k) {
+      const int i = tid + k * kThreads;
+      if (N % kThreads == 0 || i < N) x[k] = q.kx[i];
+    }
+#pragma unroll
+    for (int k = 0; k < PY; ++k) {
+      const int i = tid + k * kThreads;
+      if (i <= N / 2) y[k] = q.ky[i];
+    }
+  }
+  __device__ __forceinline__ void store(float* kx, float* ky, int tid) const {
+#pragma unroll
+    for (int k = 0; k < PX; ++k) {
+      const int i = tid + k * kThreads;
+      if (N % kThreads == 0 || i < N) kx[i] = x[k];
+    }
+#pragma unroll
+    for (int k = 0; k < PY; ++k) {
+      const int i = tid + k * kThreads;
+      if (i <= N / 2) ky[i] = y[k];
+    }
+  }
+};
 
 template <int N, int W>
 __device__ __forceinline__ void load_wavenumbers(ColShared<N, W>& sm, const NsParams& q, int tid) {
-  for (int p = tid; p < N; p += kThreads) sm.kx[p] = q.kx[p];
-  for (int c = tid; c <= N / 2; c += kThreads) sm.ky[c] = q.ky[c];
+  WavenumberStage<N> st;
+  st.load(q, tid);
+  st.store(sm.kx, sm.ky, tid);
 }
 
 template <int N, int W, int NSPEC, int R = 8>
@@ -178,9 +266,13 @@ __global__ __launch_bounds__(kThreads) void ns_cols_inverse(NsParams q, const fl
   const size_t inst = q.slots ? static_cast<size_t>(q.slots[blockIdx.y]) : blockIdx.y;
   w += inst * static_cast<size_t>(N) * (N / 2 + 1);
   g += inst * NSPEC * static_cast<size_t>(N) * (N / 2);
+  TwiddleStage<N, 1, kThreads> tw;
+  WavenumberStage<N> wn;
+  tw.load(q.tw, tid);
+  wn.load(q, tid);
   load_w_tile<N, W>(sm, wt, q, w, c0, tid);
-  load_twiddles(sm.tw, q.tw, N, 1, tid, kThreads);
-  load_wavenumbers(sm, q, tid);
+  tw.store(sm.tw, tid);
+  wn.store(sm.kx, sm.ky, tid);
   __syncthreads();
   cols_inverse_body<N, W, NSPEC, R>(sm, wt, q, g, c0, tid);
 }
@@ -198,10 +290,15 @@ __global__ __launch_bounds__(kThreads) void rows_r2c(NsParams q, const float* __
   const size_t inst = q.slots ? static_cast<size_t>(q.slots[blockIdx.y]) : blockIdx.y;
   x += inst * static_cast<size_t>(N) * N;
   pout += inst * static_cast<size_t>(N) * H;
-  load_twiddles(twn, q.tw, N, 1, tid, kThreads);
-  load_twiddles(twh, q.tw, H, 2, tid, kThreads);
+  TwiddleStage<N, 1, kThreads> ta;
+  TwiddleStage<H, 2, kThreads> tb;
+  ta.load(q.tw, tid);
+  tb.load(q.tw, tid);
+  const float* xr = x + static_cast<size_t>(i0) * N;
   float* zf = reinterpret_cast<float*>(zb);
-  for (int e = tid; e < RB * N; e += kThreads) zf[e] = x[static_cast<size_t>(i0) * N + e];
+  staged<RB * N, 16>(tid, [&](int e) { return xr[e]; }, [&](int e, float v) { zf[e] = v; });
+  ta.store(twn, tid);
+  tb.store(twh, tid);
   __syncthreads();
   lds_fft<H, RB, kThreads, 1, H, false, false>(zb, twh, tid);
   for (int e = tid; e < RB * H; e += kThreads) {
@@ -215,21 +312,23 @@ __global__ __launch_bounds__(kThreads) void rows_c2r(NsParams q, const float2* _
                                                      float* __restrict__ x) {
   constexpr int H = N / 2;
   __shared__ float2 zb[RB * H];
-  __shared__ float2 twn[N];
   __shared__ float2 twh[H];
   const int tid = threadIdx.x;
   const int i0 = xcd_tile(blockIdx.x, gridDim.x) * RB;
   const size_t inst = q.slots ? static_cast<size_t>(q.slots[blockIdx.y]) : blockIdx.y;
   x += inst * static_cast<size_t>(N) * N;
   pin += inst * static_cast<size_t>(N) * H;
-  load_twiddles(twn, q.tw, N, 1, tid, kThreads);
-  load_twiddles(twh, q.tw, H, 2, tid, kThreads);
-  __syncthreads();
-  for (int e = tid; e < RB * H; e += kThreads) {
-    const int r = e / H, k = e % H;
-    const float2* row = pin + static_cast<size_t>(i0 + r) * H;
-    zb[e] = c2r_combine(row[k], row[(H - k) & (H - 1)], k, twn[k]);
-  }
+  TwiddleStage<H, 2, kThreads> tb;
+  tb.load(q.tw, tid);
+  staged<RB * H, 8>(
+      tid,
+      [&](int e) {
+        const int r = e / H, k = e % H;
+        const float2* row = pin + static_cast<size_t>(i0 + r) * H;
+        return F2Triple{row[k], row[(H - k) & (H - 1)], q.tw[k]};
+      },
+      [&](int e, F2Triple v) { zb[e] = c2r_combine(v.a, v.b, e % H, v.t); });
+  tb.store(twh, tid);
   __syncthreads();
   lds_fft<H, RB, kThreads, 1, H, false, true>(zb, twh, tid);
   const float* zf = reinterpret_cast<const float*>(zb);
@@ -250,14 +349,43 @@ __global__ __launch_bounds__(kThreads) void ns_rows(NsParams q, const float2* __
   const size_t gstride = static_cast<size_t>(N) * H;
   g += inst * 4 * gstride;
   pout += inst * gstride;
-  load_twiddles(twn, q.tw, N, 1, tid, kThreads);
-  load_twiddles(twh, q.tw, H, 2, tid, kThreads);
-  __syncthreads();
-  for (int e = tid; e < 4 * RB * H; e += kThreads) {
-    const int t = e / H, k = e % H;
-    const int m = t / RB, r = t % RB;
-    const float2* row = g + m * gstride + static_cast<size_t>(i0 + r) * H;
-    rows[e] = c2r_combine(row[k], row[(H - k) & (H - 1)], k, twn[k]);
+  // Load the four spectra's rows in chunks (all loads of a chunk issued
+  // together); the twiddle tables go to LDS while the first chunk is in
+  // flight.
+  constexpr int TOT = 4 * RB * H;
+  constexpr int PER = (TOT + kThreads - 1) / kThreads;
+  constexpr int C = PER < 8 ? PER : 8;
+  TwiddleStage<N, 1, kThreads> ta;
+  TwiddleStage<H, 2, kThreads> tb;
+  ta.load(q.tw, tid);
+  tb.load(q.tw, tid);
+#pragma unroll 1
+  for (int k0 = 0; k0 < PER; k0 += C) {
+    float2 a[C], b[C];
+#pragma unroll
+    for (int k = 0; k < C; ++k) {
+      const int e = tid + (k0 + k) * kThreads;
+      if (e < TOT) {
+        const int t = e / H, kk = e % H;
+        const int m = t / RB, r = t % RB;
+        const float2* row = g + m * gstride + static_cast<size_t>(i0 + r) * H;
+        a[k] = row[kk];
+        b[k] = row[(H - kk) & (H - 1)];
+      }
+    }
+    if (k0 == 0) {
+      ta.store(twn, tid);
+      tb.store(twh, tid);
+      __syncthreads();
+    }
+#pragma unroll
+    for (int k = 0; k < C; ++k) {
+      const int e = tid + (k0 + k) * kThreads;
+      if (e < TOT) {
+        const int kk = e % H;
+        rows[e] = c2r_combine(a[k], b[k], kk, twn[kk]);
+      }
+    }
   }
   __syncthreads();
   lds_fft<H, 4 * RB, kThreads, 1, H, false, true, R>(rows, twh, tid);
@@ -267,13 +395,15 @@ __global__ __launch_bounds__(kThreads) void ns_rows(NsParams q, const float2* __
   const float* vx = gy + RB * N;
   const float* vy = vx + RB * N;
   float* pr = reinterpret_cast<float*>(prod);
-  for (int e = tid; e < RB * N; e += kThreads) pr[e] = -(gx[e] * vx[e] + gy[e] * vy[e]);
+  staged<RB * N, 8>(
+      tid, [&](int e) { return -(gx[e] * vx[e] + gy[e] * vy[e]); },
+      [&](int e, float v) { pr[e] = v; });
   __syncthreads();
   lds_fft<H, RB, kThreads, 1, H, false, false, R>(prod, twh, tid);
-  for (int e = tid; e < RB * H; e += kThreads) {
-    const int r = e / H, k = e % H;
-    pout[static_cast<size_t>(i0 + r) * H + k] = r2c_combine(prod + r * H, k, H, twn[k]);
-  }
+  float2* po = pout + static_cast<size_t>(i0) * H;
+  staged<RB * H, 8>(
+      tid, [&](int e) { return r2c_combine(prod + (e / H) * H, e % H, H, twn[e % H]); },
+      [&](int e, float2 v) { po[e] = v; });
 }
 
 // 2/3 rule mask (spectral/utils.py:128-134).
@@ -303,12 +433,42 @@ __global__ __launch_bounds__(kThreads) void ns_cols_forward(NsParams q, const fl
   if (h) h += inst * ext;
   if (g) g += inst * 4 * static_cast<size_t>(N) * H;
 
-  for (int e = tid; e < N * W; e += kThreads) {
-    const int p = e / W, c = e % W;
-    sm.buf[e] = pin[p * H + c0 + c];
+  // Every global load of the kernel's first half is issued up front: the
+  // tile of P, the state (u, h) at this thread's elements for the update
+  // (each element is read and written by its owner only, so reading u and h
+  // before the transform is safe also when u_in == u_out), the twiddles.
+  // Issue order matters: a wait for a load also waits for every load issued
+  // before it, so the tables (needed first) go first and (u, h) (needed
+  // after the transform) last.
+  TwiddleStage<N, 1, kThreads> tws;
+  [[maybe_unused]] WavenumberStage<N> wns;
+  tws.load(q.tw, tid);
+  if constexpr (MODE != 0) wns.load(q, tid);
+  float2 pv[PER];
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const int e = tid + k * kThreads;
+    if ((N * W) % kThreads == 0 || e < N * W) pv[k] = pin[(e / W) * H + c0 + e % W];
   }
-  load_twiddles(sm.tw, q.tw, N, 1, tid, kThreads);
-  if constexpr (MODE == 2) load_wavenumbers(sm, q, tid);
+  [[maybe_unused]] float2 u_pre[MODE == 0 ? 1 : PER], h_pre[MODE == 0 ? 1 : PER];
+  if constexpr (MODE != 0) {
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int e = tid + k * kThreads;
+      if ((N * W) % kThreads == 0 || e < N * W) {
+        const size_t idx = static_cast<size_t>(e / W) * H1 + c0 + e % W;
+        u_pre[k] = u_in[idx];
+        h_pre[k] = q.read_h ? h[idx] : make_float2(0.f, 0.f);
+      }
+    }
+  }
+  tws.store(sm.tw, tid);
+  if constexpr (MODE != 0) wns.store(sm.kx, sm.ky, tid);
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const int e = tid + k * kThreads;
+    if ((N * W) % kThreads == 0 || e < N * W) sm.buf[e] = pv[k];
+  }
   __syncthreads();
   lds_fft<N, W, kThreads, W, 1, true, false, R>(sm.buf, sm.tw, tid);
 
@@ -362,17 +522,14 @@ __global__ __launch_bounds__(kThreads) void ns_cols_forward(NsParams q, const fl
     }
     return;
   } else {
-    // h = F + beta h;  u = (u + g h + mu L u) / (1 - mu L)
-    auto update = [&](int p, int cg, float2 fv) -> float2 {
+    // h = F + beta h;  u = (u + g h + mu L u) / (1 - mu L), with the old
+    // (u, h) at (p, cg) given.
+    auto update_with = [&](int p, int cg, float2 fv, float2 uo, float2 ho) -> float2 {
       const size_t idx = static_cast<size_t>(p) * H1 + cg;
       float2 hv = fv;
-      if (q.read_h) {
-        const float2 ho = h[idx];
-        hv = make_float2(fv.x + q.beta * ho.x, fv.y + q.beta * ho.y);
-      }
+      if (q.read_h) hv = make_float2(fv.x + q.beta * ho.x, fv.y + q.beta * ho.y);
       h[idx] = hv;
-      const float2 uo = u_in[idx];
-      const float kx = q.kx[p], ky = q.ky[cg];
+      const float kx = sm.kx[p], ky = sm.ky[cg];
       const float lap = q.neg_four_pi_sq * (kx * kx + ky * ky);
       const float L = q.nu * lap - q.drag;
       const float2 lu = make_float2(L * uo.x, L * uo.y);
@@ -382,21 +539,27 @@ __global__ __launch_bounds__(kThreads) void ns_cols_forward(NsParams q, const fl
       u_out[idx] = un;
       return un;
     };
+    // Column N/2 (tile 0 only): (u, h) read here.
+    auto update_nyquist = [&](int p, float2 fv) -> float2 {
+      const size_t idx = static_cast<size_t>(p) * H1 + H;
+      const float2 ho = q.read_h ? h[idx] : make_float2(0.f, 0.f);
+      return update_with(p, H, fv, u_in[idx], ho);
+    };
     [[maybe_unused]] WTile<N, W> wt;
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
       const int e = tid + k * kThreads;
       if (e < N * W) {
         const int p = e / W, c = e % W, cg = c0 + c;
-        const float2 un = update(p, cg, f[k]);
+        const float2 un = update_with(p, cg, f[k], u_pre[k], h_pre[k]);
         if constexpr (MODE == 2) {
           wt.w[k] = un;
           if (cg == 0) {
             sm.w0[p] = un;
-            sm.wn[p] = update(p, H, fh[k]);
+            sm.wn[p] = update_nyquist(p, fh[k]);
           }
         } else {
-          if (cg == 0) update(p, H, fh[k]);
+          if (cg == 0) update_nyquist(p, fh[k]);
         }
       }
     }
