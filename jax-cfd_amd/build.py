@@ -20,6 +20,8 @@ FLAGS = [
     "-Wall", "-Wno-unused-result",
     # no packed-fp32 instructions (v_pk_*_f32) in any kernel
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # no multiply-add contraction anywhere (fixed fp32 operation order)
+    "-ffp-contract=off",
 ]
 
 

@@ -194,15 +194,21 @@ __device__ __forceinline__ float add_diffusion_forcing(const WIN& w, int lr, int
   return k;
 }
 
+// Rows row0 .. row0 + ROWS - 1 (periodic) of u and v into LDS, all loads of
+// a chunk issued before its first LDS write.
 template <int N, int ROWS>
 __device__ __forceinline__ void stage_rows(float* su, float* sv, const float* __restrict__ u,
                                            const float* __restrict__ v, int row0, int tid) {
-  for (int e = tid; e < ROWS * N; e += kThreads) {
-    const int lr = e / N, j = e % N;
-    const int gi = (row0 + lr) & (N - 1);
-    su[e] = u[gi * N + j];
-    sv[e] = v[gi * N + j];
-  }
+  staged<ROWS * N, 8, kThreads>(
+      tid,
+      [&](int e) {
+        const int gi = (row0 + e / N) & (N - 1), j = e % N;
+        return make_float2(u[gi * N + j], v[gi * N + j]);
+      },
+      [&](int e, float2 x) {
+        su[e] = x.x;
+        sv[e] = x.y;
+      });
 }
 
 // LDS layouts of the three step kernels, carved from one byte buffer so the
@@ -276,21 +282,32 @@ __device__ __forceinline__ void rows_forward_march_body(const FvmParams& p, cons
   const RingWindow<N, K> w{su, sv};
   // ring row a holds global row i0 - 3 + a
   auto grow = [&](int a) { return (i0 - 3 + a) & (N - 1); };
-  for (int e = tid; e < 5 * N; e += kThreads) {
-    const int a = e / N, j = e % N;
-    su[(a & (K - 1)) * N + j] = b.u[grow(a) * N + j];
-    sv[(a & (K - 1)) * N + j] = b.v[grow(a) * N + j];
-  }
-  load_twiddles(twn, b.tw, N, 1, tid, kThreads);
-  load_twiddles(twh, b.tw, H, 2, tid, kThreads);
-  __syncthreads();
+  TwiddleStage<N, 1, kThreads> tsn;
+  TwiddleStage<H, 2, kThreads> tsh;
+  tsn.load(b.tw, tid);
+  tsh.load(b.tw, tid);
   float fx_prev[CPT], gx_prev[CPT], us_prev[CPT], fxj[CPT];
+#pragma unroll
+  for (int c = 0; c < CPT; ++c) fxj[c] = p.has_forcing ? b.fx[tid + c * kThreads] : 0.f;
+  staged<5 * N, 8, kThreads>(
+      tid,
+      [&](int e) {
+        const int g = grow(e / N) * N + e % N;
+        return make_float2(b.u[g], b.v[g]);
+      },
+      [&](int e, float2 x) {
+        const int a = e / N, j = e % N;
+        su[(a & (K - 1)) * N + j] = x.x;
+        sv[(a & (K - 1)) * N + j] = x.y;
+      });
+  tsn.store(twn, tid);
+  tsh.store(twh, tid);
+  __syncthreads();
 #pragma unroll
   for (int c = 0; c < CPT; ++c) {
     const int j = tid + c * kThreads;
     fx_prev[c] = face_fx_u(w, 2, j, p);  // x face below the first u row (i0 - 1)
     gx_prev[c] = face_fx_v(w, 3, j, p);  // x face below the first v row (i0)
-    fxj[c] = p.has_forcing ? b.fx[j] : 0.f;
     us_prev[c] = 0.f;
   }
   // step s: cells of ring row a = s + 2 (global row i0 - 1 + s); u for every
@@ -391,8 +408,7 @@ struct RowsProjectLds {
   static constexpr int H = N / 2;
   static constexpr bool kFft = MODE != 2;
   static constexpr int qb = 0;                          // float2[(RB + 1) * H]
-  static constexpr int twn = qb + (RB + 1) * H * 8;     // float2[N]
-  static constexpr int twh = twn + (kFft ? N * 8 : 0);  // float2[H]
+  static constexpr int twh = qb + (RB + 1) * H * 8;     // float2[H]
   static constexpr int bytes = twh + (kFft ? H * 8 : 0);
 };
 
@@ -421,16 +437,37 @@ __device__ __forceinline__ void rows_forward_body(const FvmParams& p, const FvmB
   float2* twn = lds_at<float2>(smem, L::twn);
   float2* twh = lds_at<float2>(smem, L::twh);
 
+  // Twiddle and forcing loads are issued first and written to LDS after the
+  // field rows' loads (a wait for a load also waits for every earlier one).
+  [[maybe_unused]] TwiddleStage<N, 1, kThreads> tsn;
+  [[maybe_unused]] TwiddleStage<H, 2, kThreads> tsh;
+  if constexpr (L::kFft) {
+    tsn.load(b.tw, tid);
+    tsh.load(b.tw, tid);
+  }
+  constexpr int PU = (NU * N + kThreads - 1) / kThreads;
+  [[maybe_unused]] float fxr[L::kWin ? PU : 1];
+  if constexpr (L::kWin) {
+#pragma unroll
+    for (int k = 0; k < PU; ++k) fxr[k] = p.has_forcing ? b.fx[(tid + k * kThreads) % N] : 0.f;
+  }
   if constexpr (MODE == 3) {
     float* zf3 = reinterpret_cast<float*>(zb);
-    for (int e = tid; e < RB * N; e += kThreads) zf3[e] = b.u[(i0 + e / N) * N + e % N];
+    const float* ur = b.u + static_cast<size_t>(i0) * N;
+    staged<RB * N, 16, kThreads>(tid, [&](int e) { return ur[e]; }, [&](int e, float x) { zf3[e] = x; });
   } else if constexpr (MODE == 1) {
     // u* = u: stage rows i0-1 .. i0+RB-1 of u and rows i0 .. of v directly.
-    for (int e = tid; e < (RB + 1) * N; e += kThreads) {
-      const int lr = e / N, j = e % N;
-      sus[e] = b.u[((i0 - 1 + lr) & (N - 1)) * N + j];
-      if (lr < RB) svs[e] = b.v[((i0 + lr) & (N - 1)) * N + j];
-    }
+    staged<(RB + 1) * N, 8, kThreads>(
+        tid,
+        [&](int e) {
+          const int lr = e / N, j = e % N;
+          return make_float2(b.u[((i0 - 1 + lr) & (N - 1)) * N + j],
+                             lr < RB ? b.v[((i0 + lr) & (N - 1)) * N + j] : 0.f);
+        },
+        [&](int e, float2 x) {
+          sus[e] = x.x;
+          if (e < RB * N) svs[e] = x.y;
+        });
   } else if constexpr (MODE == 5) {
     const RkTerms& t = *b.rk;
     const size_t off = b.inst * static_cast<size_t>(N) * N;
@@ -456,14 +493,13 @@ __device__ __forceinline__ void rows_forward_body(const FvmParams& p, const FvmB
     stage_rows<N, WR>(su, sv, b.u, b.v, i0 - 3, tid);
   }
   if constexpr (L::kFft) {
-    load_twiddles(twn, b.tw, N, 1, tid, kThreads);
-    load_twiddles(twh, b.tw, H, 2, tid, kThreads);
+    tsn.store(twn, tid);
+    tsh.store(twh, tid);
   }
   __syncthreads();
 
   if constexpr (L::kWin) {
     const Window<N> w{su, sv};
-    constexpr int PU = (NU * N + kThreads - 1) / kThreads;
     constexpr int PV = (NV * N + kThreads - 1) / kThreads;
     float ru[PU], rv[PV];  // explicit terms (MODE 2) or u*, v* of this thread's cells
     // ---- u: x faces of rows LO_U .. HI + 1, y faces of rows LO_U .. HI
@@ -479,8 +515,7 @@ __device__ __forceinline__ void rows_forward_body(const FvmParams& p, const FvmB
         const float* fy = flux + (NU + 1) * N + r * N;
         const float adv = -((flux[(r + 1) * N + j] - flux[r * N + j]) / p.dx +
                             (fy[j] - fy[(j - 1) & (N - 1)]) / p.dy);
-        const float fxj = p.has_forcing ? b.fx[j] : 0.f;
-        const float ku = add_diffusion_forcing<false>(w, lr, j, p, fxj, adv);
+        const float ku = add_diffusion_forcing<false>(w, lr, j, p, fxr[k], adv);
         ru[k] = (MODE == 2) ? ku : w.U(lr, j) + p.dt * ku;
       }
     }
@@ -572,11 +607,20 @@ __device__ __forceinline__ void poisson_cols_body(const FvmBuffers& b, float sca
   using L = PoissonLds<N, W>;
   float2* buf = lds_at<float2>(smem, L::buf);
   float2* twn = lds_at<float2>(smem, L::twn);
-  for (int e = tid; e < N * W; e += kThreads) {
-    const int pp = e / W, c = e % W;
-    buf[e] = b.spec[pp * H + c0 + c];
+  TwiddleStage<N, 1, kThreads> tsn;
+  tsn.load(b.tw, tid);
+  staged<N * W, 8, kThreads>(
+      tid, [&](int e) { return b.spec[(e / W) * H + c0 + e % W]; },
+      [&](int e, float2 x) { buf[e] = x; });
+  // the multipliers of this thread's elements arrive during the transform
+  float lam[PER];
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    const int e0 = tid + q * kThreads;
+    const int e = e0 < N * W ? e0 : N * W - 1;
+    lam[q] = b.lam[(e / W) * (H + 1) + c0 + e % W];
   }
-  load_twiddles(twn, b.tw, N, 1, tid, kThreads);
+  tsn.store(twn, tid);
   __syncthreads();
   lds_fft<N, W, kThreads, W, 1, true, false, R>(buf, twn, tid);
   float2 nv[PER];
@@ -592,12 +636,12 @@ __device__ __forceinline__ void poisson_cols_body(const FvmBuffers& b, float sca
         const float2 f0 = make_float2(0.5f * (y.x + ym.x), 0.5f * (y.y + ym.y));
         const float2 d = csub(y, ym);
         const float2 fh = make_float2(0.5f * d.y, -0.5f * d.x);
-        const float l0 = b.lam[pp * (H + 1)] * scale;
+        const float l0 = lam[q] * scale;
         const float lh = b.lam[pp * (H + 1) + H] * scale;
         const float2 g0 = cscale(f0, l0), gh = cscale(fh, lh);
         nv[q] = make_float2(g0.x - gh.y, g0.y + gh.x);  // g0 + i gh
       } else {
-        nv[q] = cscale(y, b.lam[pp * (H + 1) + cg] * scale);
+        nv[q] = cscale(y, lam[q] * scale);
       }
     }
   }
@@ -609,10 +653,9 @@ __device__ __forceinline__ void poisson_cols_body(const FvmBuffers& b, float sca
   }
   __syncthreads();
   lds_fft<N, W, kThreads, W, 1, true, true, R>(buf, twn, tid);
-  for (int e = tid; e < N * W; e += kThreads) {
-    const int pp = e / W, c = e % W;
-    b.spec[pp * H + c0 + c] = buf[e];
-  }
+  staged<N * W, 16, kThreads>(
+      tid, [&](int e) { return buf[e]; },
+      [&](int e, float2 x) { b.spec[(e / W) * H + c0 + e % W] = x; });
 }
 
 template <int N, int W, int R = 8>
@@ -633,45 +676,69 @@ __device__ __forceinline__ void rows_project_body(const FvmParams& p, const FvmB
   constexpr int H = N / 2;
   constexpr int T = RB + 1;
   using L = RowsProjectLds<N, RB, MODE>;
+  constexpr int PR = (RB * N + kThreads - 1) / kThreads;
   float2* qb = lds_at<float2>(smem, L::qb);
-  float2* twn = lds_at<float2>(smem, L::twn);
   float2* twh = lds_at<float2>(smem, L::twh);
+  // (u*, v*) (MODES 0, 2) or u (MODE 3) at this thread's cells: loaded
+  // before the transform, used after it
+  [[maybe_unused]] float pa[PR], pb[PR];
+  auto prefetch = [&]() {
+#pragma unroll
+    for (int k = 0; k < PR; ++k) {
+      const int e0 = tid + k * kThreads;
+      const int e = i0 * N + (e0 < RB * N ? e0 : RB * N - 1);
+      if constexpr (MODE == 0 || MODE == 2) {
+        pa[k] = b.us[e];
+        pb[k] = b.vs[e];
+      } else if constexpr (MODE == 3) {
+        pa[k] = b.u[e];
+      }
+    }
+  };
   if constexpr (MODE == 2) {
     float* qf = reinterpret_cast<float*>(qb);
-    for (int e = tid; e < T * N; e += kThreads) qf[e] = b.q[((i0 + e / N) & (N - 1)) * N + e % N];
+    staged<T * N, 16, kThreads>(
+        tid, [&](int e) { return b.q[((i0 + e / N) & (N - 1)) * N + e % N]; },
+        [&](int e, float x) { qf[e] = x; });
+    prefetch();
     __syncthreads();
   } else {
-    load_twiddles(twn, b.tw, N, 1, tid, kThreads);
-    load_twiddles(twh, b.tw, H, 2, tid, kThreads);
-    __syncthreads();
-    for (int e = tid; e < T * H; e += kThreads) {
-      const int r = e / H, k = e % H;
-      const int gi = (i0 + r) & (N - 1);
-      const float2 pk = b.spec[gi * H + k];
-      const float2 pmk = b.spec[gi * H + ((H - k) & (H - 1))];
-      qb[e] = c2r_combine(pk, pmk, k, twn[k]);
-    }
+    TwiddleStage<H, 2, kThreads> tsh;
+    tsh.load(b.tw, tid);
+    // the combine's twiddle exp(-2 pi i k / n) is entry k of the table
+    staged<T * H, 8, kThreads>(
+        tid,
+        [&](int e) {
+          const int k = e % H;
+          const float2* row = b.spec + ((i0 + e / H) & (N - 1)) * H;
+          return F2Triple{row[k], row[(H - k) & (H - 1)], b.tw[k]};
+        },
+        [&](int e, F2Triple x) { qb[e] = c2r_combine(x.a, x.b, e % H, x.t); });
+    prefetch();
+    tsh.store(twh, tid);
     __syncthreads();
     lds_fft<H, T, kThreads, 1, H, false, true, R>(qb, twh, tid);
   }
   const float* q = reinterpret_cast<const float*>(qb);
-  if constexpr (MODE == 1) {
-    for (int e = tid; e < RB * N; e += kThreads) b.u_out[i0 * N + e] = q[e];
-    return;
-  }
-  if constexpr (MODE == 3) {
-    // implicit diffusion: u + (1 - nu dt L)^-1 (nu dt L) u (diffusion.py:198-208)
-    for (int e = tid; e < RB * N; e += kThreads) b.u_out[i0 * N + e] = b.u[i0 * N + e] + q[e];
-    return;
-  }
-  for (int e = tid; e < RB * N; e += kThreads) {
-    const int r = e / N, j = e % N;
-    const int gi = i0 + r;
-    const float qc = q[r * N + j];
-    const float gx = (q[(r + 1) * N + j] + (-qc)) / p.dx;
-    const float gy = (q[r * N + ((j + 1) & (N - 1))] + (-qc)) / p.dy;
-    b.u_out[gi * N + j] = b.us[gi * N + j] - gx;
-    b.v_out[gi * N + j] = b.vs[gi * N + j] - gy;
+#pragma unroll
+  for (int k = 0; k < PR; ++k) {
+    const int e = tid + k * kThreads;
+    if (e < RB * N) {
+      if constexpr (MODE == 1) {
+        b.u_out[i0 * N + e] = q[e];
+      } else if constexpr (MODE == 3) {
+        // implicit diffusion: u + (1 - nu dt L)^-1 (nu dt L) u (diffusion.py:198-208)
+        b.u_out[i0 * N + e] = pa[k] + q[e];
+      } else {
+        const int r = e / N, j = e % N;
+        const int gi = i0 + r;
+        const float qc = q[r * N + j];
+        const float gx = (q[(r + 1) * N + j] + (-qc)) / p.dx;
+        const float gy = (q[r * N + ((j + 1) & (N - 1))] + (-qc)) / p.dy;
+        b.u_out[gi * N + j] = pa[k] - gx;
+        b.v_out[gi * N + j] = pb[k] - gy;
+      }
+    }
   }
 }
 

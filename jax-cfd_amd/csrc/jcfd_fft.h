@@ -24,6 +24,11 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+// No multiply-add contraction: whether the backend fuses a product into an
+// add depends on the surrounding code after inlining, so contracted helpers
+// could round differently from one kernel (or tile shape) to the next.
+#pragma clang fp contract(off)
+
 namespace jcfd {
 
 __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
@@ -174,11 +179,14 @@ template <int N, int STRIDE, int NT>
 struct TwiddleStage {
   static constexpr int PER = (N + NT - 1) / NT;
   float2 v[PER];
+  // Loads are unconditional (surplus threads read entry N - 1): a load under
+  // a branch makes the compiler's wait before the first use of any later
+  // load a wait for all of them.
   __device__ __forceinline__ void load(const float2* __restrict__ src, int tid) {
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
       const int i = tid + k * NT;
-      if (N % NT == 0 || i < N) v[k] = src[i * STRIDE];
+      v[k] = src[(N % NT == 0 || i < N ? i : N - 1) * STRIDE];
     }
   }
   __device__ __forceinline__ void store(float2* dst, int tid) const {
@@ -188,6 +196,36 @@ struct TwiddleStage {
       if (N % NT == 0 || i < N) dst[i] = v[k];
     }
   }
+};
+
+// Stage TOT elements through registers, up to CH per thread at a time
+// (element e = tid + k * NT): every load of a chunk is issued before the
+// chunk's first write, so a chunk costs one memory latency.  load() is
+// called unconditionally (surplus lanes load element TOT - 1), store() only
+// for e < TOT.
+template <int TOT, int CH, int NT, typename Load, typename Store>
+__device__ __forceinline__ void staged(int tid, Load&& load, Store&& store) {
+  constexpr int PER = (TOT + NT - 1) / NT;
+  constexpr int C = PER < CH ? PER : CH;
+  using V = decltype(load(0));
+#pragma unroll 1
+  for (int k0 = 0; k0 < PER; k0 += C) {
+    V v[C];
+#pragma unroll
+    for (int k = 0; k < C; ++k) {
+      const int e = tid + (k0 + k) * NT;
+      v[k] = load(TOT % NT == 0 || e < TOT ? e : TOT - 1);
+    }
+#pragma unroll
+    for (int k = 0; k < C; ++k) {
+      const int e = tid + (k0 + k) * NT;
+      if (e < TOT) store(e, v[k]);
+    }
+  }
+}
+
+struct F2Triple {
+  float2 a, b, t;
 };
 
 // Table loads of compile-time length: loads first, then the LDS writes.  No

@@ -38,37 +38,6 @@ namespace {
 
 constexpr int kThreads = 256;
 
-// Stage TOT elements through registers, up to CH per thread at a time: every
-// load of a chunk is issued before the chunk's first write, so a chunk costs
-// one memory latency (a loop with a run-time bound waits on each load).
-template <int TOT, int CH, typename Load, typename Store>
-__device__ __forceinline__ void staged(int tid, Load&& load, Store&& store) {
-  constexpr int PER = (TOT + kThreads - 1) / kThreads;
-  constexpr int C = PER < CH ? PER : CH;
-  using V = decltype(load(0));
-#pragma unroll 1
-  for (int k0 = 0; k0 < PER; k0 += C) {
-    V v[C];
-#pragma unroll
-    for (int k = 0; k < C; ++k) {
-      const int e = tid + (k0 + k) * kThreads;
-      if (e < TOT) v[k] = load(e);
-    }
-#pragma unroll
-    for (int k = 0; k < C; ++k) {
-      const int e = tid + (k0 + k) * kThreads;
-      if (e < TOT) store(e, v[k]);
-    }
-  }
-}
-
-struct F2Pair {
-  float2 a, b;
-};
-struct F2Triple {
-  float2 a, b, t;
-};
-
 // Per-element multipliers of the four derivative spectra, following the
 // reference's fp32/complex64 operation order.
 struct NsMult {
@@ -227,12 +196,12 @@ struct WavenumberStage {
 #pragma unroll
     for (int k = 0; k < PX; ++k) {
       const int i = tid + k * kThreads;
-      if (N % kThreads == 0 || i < N) x[k] = q.kx[i];
+      x[k] = q.kx[N % kThreads == 0 || i < N ? i : N - 1];
     }
 #pragma unroll
     for (int k = 0; k < PY; ++k) {
       const int i = tid + k * kThreads;
-      if (i <= N / 2) y[k] = q.ky[i];
+      y[k] = q.ky[i <= N / 2 ? i : N / 2];
     }
   }
   __device__ __forceinline__ void store(float* kx, float* ky, int tid) const {
@@ -296,7 +265,7 @@ __global__ __launch_bounds__(kThreads) void rows_r2c(NsParams q, const float* __
   tb.load(q.tw, tid);
   const float* xr = x + static_cast<size_t>(i0) * N;
   float* zf = reinterpret_cast<float*>(zb);
-  staged<RB * N, 16>(tid, [&](int e) { return xr[e]; }, [&](int e, float v) { zf[e] = v; });
+  staged<RB * N, 16, kThreads>(tid, [&](int e) { return xr[e]; }, [&](int e, float v) { zf[e] = v; });
   ta.store(twn, tid);
   tb.store(twh, tid);
   __syncthreads();
@@ -320,7 +289,7 @@ __global__ __launch_bounds__(kThreads) void rows_c2r(NsParams q, const float2* _
   pin += inst * static_cast<size_t>(N) * H;
   TwiddleStage<H, 2, kThreads> tb;
   tb.load(q.tw, tid);
-  staged<RB * H, 8>(
+  staged<RB * H, 8, kThreads>(
       tid,
       [&](int e) {
         const int r = e / H, k = e % H;
@@ -395,13 +364,13 @@ __global__ __launch_bounds__(kThreads) void ns_rows(NsParams q, const float2* __
   const float* vx = gy + RB * N;
   const float* vy = vx + RB * N;
   float* pr = reinterpret_cast<float*>(prod);
-  staged<RB * N, 8>(
+  staged<RB * N, 8, kThreads>(
       tid, [&](int e) { return -(gx[e] * vx[e] + gy[e] * vy[e]); },
       [&](int e, float v) { pr[e] = v; });
   __syncthreads();
   lds_fft<H, RB, kThreads, 1, H, false, false, R>(prod, twh, tid);
   float2* po = pout + static_cast<size_t>(i0) * H;
-  staged<RB * H, 8>(
+  staged<RB * H, 8, kThreads>(
       tid, [&](int e) { return r2c_combine(prod + (e / H) * H, e % H, H, twn[e % H]); },
       [&](int e, float2 v) { po[e] = v; });
 }
