@@ -278,6 +278,7 @@ struct jcfd_ns2d_plan {
   float2* h = nullptr;
   float2* g = nullptr;
   float2* pbuf = nullptr;
+  float2* ws = nullptr;  // the state in the step kernels' internal layout
   int64_t bytes = 0;
   GraphCache graphs;
 };
@@ -1239,21 +1240,26 @@ int jcfd_fvm_group_launch_shape(const jcfd_fvm_group* g, int32_t* grid, int32_t*
 
 // ------------------------------------------------------------ spectral path
 namespace {
-// nsteps CN-RK4 steps of `count` instances (slots, or 0..count-1): the first
-// reads b.u_in, the rest run in place on b.u_out (graph replay when enabled).
-int ns_run(jcfd_ns2d_plan* p, NsBuffers b, const int* slots, int count, int64_t nsteps,
-           hipStream_t s) {
+// nsteps CN-RK4 steps of `count` instances (slots, or 0..count-1) from
+// w_in to w_out (external layout; may alias): the state is moved into the
+// plan's internal-layout buffer, stepped in place there (graph replay when
+// enabled) and moved back.
+int ns_run(jcfd_ns2d_plan* p, const float2* w_in, float2* w_out, const int* slots, int count,
+           int64_t nsteps, hipStream_t s) {
   NsParams stages[5];
   for (int k = 0; k < 5; ++k) {
     stages[k] = p->stages[k];
     stages[k].slots = slots;
   }
+  NsBuffers b{};
+  b.u_in = p->ws;
+  b.u_out = p->ws;
   b.h = p->h;
   b.g = p->g;
   b.pbuf = p->pbuf;
   int rc;
+  if ((rc = ns_launch_state_convert(p->n, true, w_in, p->ws, slots, count, s))) return rc;
   if ((rc = ns_launch_step(p->n, stages, b, count, s))) return rc;
-  b.u_in = b.u_out;
   int64_t remaining = nsteps - 1;
   if (p->graphs.enabled && remaining >= kGraphChunk) {
     auto launch = [&](hipStream_t gs) {
@@ -1270,7 +1276,7 @@ int ns_run(jcfd_ns2d_plan* p, NsBuffers b, const int* slots, int count, int64_t 
   }
   for (int64_t step = 0; step < remaining; ++step)
     if ((rc = ns_launch_step(p->n, stages, b, count, s))) return rc;
-  return JCFD_OK;
+  return ns_launch_state_convert(p->n, false, p->ws, w_out, slots, count, s);
 }
 }  // namespace
 
@@ -1280,7 +1286,8 @@ int jcfd_ns2d_plan_destroy(jcfd_ns2d_plan* p) {
   if (p == nullptr) return JCFD_OK;
   for (void* ptr : {static_cast<void*>(p->kx), static_cast<void*>(p->ky), static_cast<void*>(p->tw),
                     static_cast<void*>(p->force_idx), static_cast<void*>(p->force_val),
-                    static_cast<void*>(p->h), static_cast<void*>(p->g), static_cast<void*>(p->pbuf)})
+                    static_cast<void*>(p->h), static_cast<void*>(p->g), static_cast<void*>(p->pbuf),
+                    static_cast<void*>(p->ws)})
     if (ptr != nullptr) (void)hipFree(ptr);
   p->graphs.release();
   delete p;
@@ -1338,7 +1345,8 @@ int jcfd_ns2d_plan_create(const jcfd_ns2d_config* config, jcfd_ns2d_plan** plan)
   if ((rc = device_alloc(&p->kx, n, &p->bytes)) || (rc = device_alloc(&p->ky, h1, &p->bytes)) ||
       (rc = device_alloc(&p->tw, n, &p->bytes)) || (rc = device_alloc(&p->force_idx, 2 * nf, &p->bytes)) ||
       (rc = device_alloc(&p->force_val, nf, &p->bytes)) || (rc = device_alloc(&p->h, B * ext, &p->bytes)) ||
-      (rc = device_alloc(&p->g, B * 4 * half, &p->bytes)) || (rc = device_alloc(&p->pbuf, B * half, &p->bytes))) {
+      (rc = device_alloc(&p->g, B * 4 * half, &p->bytes)) || (rc = device_alloc(&p->pbuf, B * half, &p->bytes)) ||
+      (rc = device_alloc(&p->ws, B * ext, &p->bytes))) {
     jcfd_ns2d_plan_destroy(p);
     return rc;
   }
@@ -1434,10 +1442,7 @@ int jcfd_ns2d_step(jcfd_ns2d_plan* p, const jcfd_tensor* w_hat, jcfd_tensor* w_o
     if (pin != pout) return check_hip(hipMemcpyAsync(pout, pin, bytes, hipMemcpyDeviceToDevice, s), "copy");
     return JCFD_OK;
   }
-  NsBuffers b{};
-  b.u_in = static_cast<const float2*>(pin);
-  b.u_out = static_cast<float2*>(pout);
-  return ns_run(p, b, nullptr, p->batch, nsteps, s);
+  return ns_run(p, static_cast<const float2*>(pin), static_cast<float2*>(pout), nullptr, p->batch, nsteps, s);
 }
 
 int jcfd_ns2d_step_slots(jcfd_ns2d_plan* p, jcfd_tensor* w_hat, const jcfd_tensor* slots,
@@ -1450,10 +1455,8 @@ int jcfd_ns2d_step_slots(jcfd_ns2d_plan* p, jcfd_tensor* w_hat, const jcfd_tenso
       (rc = check_slots(slots, p->batch, p->device, &ps, &count)))
     return rc;
   if (nsteps == 0 || count == 0) return JCFD_OK;
-  NsBuffers b{};
-  b.u_in = static_cast<const float2*>(pw);
-  b.u_out = static_cast<float2*>(pw);
-  return ns_run(p, b, static_cast<const int*>(ps), count, nsteps, static_cast<hipStream_t>(stream));
+  return ns_run(p, static_cast<const float2*>(pw), static_cast<float2*>(pw), static_cast<const int*>(ps),
+                count, nsteps, static_cast<hipStream_t>(stream));
 }
 
 int jcfd_ns2d_explicit_terms(jcfd_ns2d_plan* p, const jcfd_tensor* w_hat, jcfd_tensor* out,
@@ -1465,9 +1468,11 @@ int jcfd_ns2d_explicit_terms(jcfd_ns2d_plan* p, const jcfd_tensor* w_hat, jcfd_t
       (rc = check_spectral(out, "out", p->batch, p->n, p->device, &pout)))
     return rc;
   if (pin == pout) return fail(JCFD_ERR_INVALID_ARGUMENT, "explicit terms: out must not alias w_hat");
-  return ns_launch_explicit_terms(p->n, p->stages[0], static_cast<const float2*>(pin),
-                                  static_cast<float2*>(pout), p->g, p->pbuf, p->batch,
-                                  static_cast<hipStream_t>(stream));
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if ((rc = ns_launch_state_convert(p->n, true, static_cast<const float2*>(pin), p->ws, nullptr, p->batch, s)))
+    return rc;
+  return ns_launch_explicit_terms(p->n, p->stages[0], p->ws, static_cast<float2*>(pout), p->g, p->pbuf,
+                                  p->batch, s);
 }
 
 int jcfd_ns2d_time_kernels(jcfd_ns2d_plan* p, const jcfd_tensor* w_hat, int32_t reps,
@@ -1483,13 +1488,14 @@ int jcfd_ns2d_time_kernels(jcfd_ns2d_plan* p, const jcfd_tensor* w_hat, int32_t 
   int64_t unused = 0;
   if ((rc = device_alloc(&tmp, count, &unused))) return rc;
   NsBuffers b{};
-  b.u_in = static_cast<const float2*>(pin);
+  b.u_in = p->ws;
   b.u_out = tmp;
   b.h = p->h;
   b.g = p->g;
   b.pbuf = p->pbuf;
   hipEvent_t e0 = nullptr, e1 = nullptr;
-  rc = check_hip(hipEventCreate(&e0), "event");
+  rc = ns_launch_state_convert(p->n, true, static_cast<const float2*>(pin), p->ws, nullptr, p->batch, s);
+  if (!rc) rc = check_hip(hipEventCreate(&e0), "event");
   if (!rc) rc = check_hip(hipEventCreate(&e1), "event");
   if (!rc) rc = ns_launch_step(p->n, p->stages, b, p->batch, s);  // fills every scratch buffer
   for (int k = 0; k < JCFD_NS2D_KERNEL_COUNT && rc == JCFD_OK; ++k) {

@@ -196,11 +196,11 @@ struct NsParams {
 };
 
 struct NsBuffers {
-  const float2* u_in;  // state in, (batch, n, n/2+1)
-  float2* u_out;       // state out
-  float2* h;           // low-storage RK register, external layout
-  float2* g;           // 4 packed spectra per instance, (batch, 4, n, n/2)
-  float2* pbuf;        // packed product spectrum, (batch, n, n/2)
+  const float2* u_in;  // state in, (batch, n, n/2+1) words, internal layout
+  float2* u_out;       // state out, internal layout
+  float2* h;           // low-storage RK register, internal layout
+  float2* g;           // 4 packed spectra per instance, (batch, 4, n x n/2), tiled
+  float2* pbuf;        // packed product spectrum, (batch, n x n/2), tiled
 };
 
 bool ns_size_supported(int n);
@@ -209,7 +209,15 @@ int ns_default_tiles(NsParams* q, int n);
 // Whether step kernel class `which` (0 cols-inverse, 1 rows, 2/3 cols-forward)
 // is built with tile `tile` (columns per tile; rows per workgroup for 1).
 bool ns_tile_supported(int n, int which, int tile);
+// The step kernels keep the state (and h, G, P) in an internal layout of the
+// same size as the external (n, n/2 + 1) one: column tiles of 4 words, each
+// a contiguous block, then the Nyquist column.  ns_launch_state_convert maps
+// `count` instances (slots[i], or i) between the two.
+int ns_launch_state_convert(int n, bool to_internal, const float2* src, float2* dst, const int* slots,
+                            int count, hipStream_t s);
+// One CN-RK4 step; b.u_in / b.u_out / b.h in the internal state layout.
 int ns_launch_step(int n, const NsParams* stages, const NsBuffers& b, int batch, hipStream_t s);
+// Explicit terms of w (internal state layout) into out (external layout).
 int ns_launch_explicit_terms(int n, const NsParams& q, const float2* w, float2* out, float2* g,
                              float2* pbuf, int batch, hipStream_t s);
 int ns_launch_rfft2(int n, const NsParams& q, const float* x, float2* out, float2* pbuf, int batch,

@@ -38,6 +38,25 @@ namespace {
 
 constexpr int kThreads = 256;
 
+// Internal layout of packed half spectra (n rows x n/2 words: the four
+// derivative spectra G and the product spectrum P): column tiles of kTw
+// words, each tile one contiguous block of n rows x kTw words.  A column
+// kernel's tile is then a contiguous block, and a row kernel's rows cover
+// whole cache lines in every tile (4 rows x 4 words = 128 bytes), where the
+// row-major layout left 32-byte pieces of 2 KB rows to both.
+constexpr int kTw = 4;
+template <int N>
+__device__ __forceinline__ int tix(int p, int c) {
+  return ((c / kTw) * N + p) * kTw + (c % kTw);
+}
+// Internal state layout (n rows x (n/2 + 1) words, the same size as the
+// external rfft2 layout): columns 0 .. n/2 - 1 tiled as above, then column
+// n/2 (the Nyquist column) as n consecutive words.
+template <int N>
+__device__ __forceinline__ int six(int p, int c) {
+  return c < N / 2 ? tix<N>(p, c) : (N / 2) * N + p;
+}
+
 // Per-element multipliers of the four derivative spectra, following the
 // reference's fp32/complex64 operation order.
 struct NsMult {
@@ -91,12 +110,13 @@ struct WTile {
 };
 
 // Build spectrum m of the tile into sm.buf (scaled by 1/n^2), symmetrising and
-// packing the DC/Nyquist pair into local column 0 of tile 0.  The multipliers
-// are recomputed per spectrum from the LDS wavenumber tables (reads after a
-// barrier, which the compiler cannot hoist out of the spectrum loop; keeping
-// them live across it would cost 3 VGPRs per element).
+// packing the DC/Nyquist pair into local column 0 of tile 0.  psi[k] is
+// -1/laplacian at this thread's element k (columns other than 0; one
+// division per element for all four spectra); 2 pi kx, 2 pi ky come from
+// the LDS wavenumber tables.
 template <int N, int W>
 __device__ __forceinline__ void build_spectrum(ColShared<N, W>& sm, const WTile<N, W>& wt,
+                                               const float (&psi)[WTile<N, W>::PER],
                                                const NsParams& q, int m, int c0, int tid) {
   constexpr int H = N / 2;
 #pragma unroll
@@ -106,7 +126,11 @@ __device__ __forceinline__ void build_spectrum(ColShared<N, W>& sm, const WTile<
     const int p = e / W, c = e % W, cg = c0 + c;
     float2 out;
     if (cg != 0) {
-      out = ns_spectrum(m, ns_mult(q, sm.kx[p], sm.ky[cg], false), wt.w[k]);
+      NsMult mk;
+      mk.ax = q.two_pi * sm.kx[p];
+      mk.ay = q.two_pi * sm.ky[cg];
+      mk.psi = psi[k];
+      out = ns_spectrum(m, mk, wt.w[k]);
     } else {
       const int pm = (N - p) & (N - 1);
       const float2 a0 = ns_spectrum(m, ns_mult(q, sm.kx[p], sm.ky[0], p == 0), sm.w0[p]);
@@ -129,14 +153,21 @@ __device__ __forceinline__ void cols_inverse_body(ColShared<N, W>& sm, const WTi
                                                   const NsParams& q, float2* __restrict__ g,
                                                   int c0, int tid) {
   constexpr int H = N / 2;
+  constexpr int PER = WTile<N, W>::PER;
   const size_t gstride = static_cast<size_t>(N) * H;
+  float psi[PER];
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const int e = tid + k * kThreads;
+    const int p = (e < N * W ? e : N * W - 1) / W, cg = c0 + e % W;
+    psi[k] = (NSPEC == 4 && cg != 0) ? ns_mult(q, sm.kx[p], sm.ky[cg], false).psi : 0.f;
+  }
 #pragma unroll 1
   for (int m = 0; m < NSPEC; ++m) {
-    build_spectrum<N, W>(sm, wt, q, NSPEC == 1 ? 4 : m, c0, tid);
+    build_spectrum<N, W>(sm, wt, psi, q, NSPEC == 1 ? 4 : m, c0, tid);
     __syncthreads();
     lds_fft<N, W, kThreads, W, 1, true, true, R>(sm.buf, sm.tw, tid);
     float2* gm = g + m * gstride;
-    constexpr int PER = WTile<N, W>::PER;
     float2 o[PER];
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
@@ -146,22 +177,25 @@ __device__ __forceinline__ void cols_inverse_body(ColShared<N, W>& sm, const WTi
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
       const int e = tid + k * kThreads;
-      if ((N * W) % kThreads == 0 || e < N * W) gm[(e / W) * H + c0 + e % W] = o[k];
+      if ((N * W) % kThreads == 0 || e < N * W) gm[tix<N>(e / W, c0 + e % W)] = o[k];
     }
     __syncthreads();
   }
 }
 
-template <int N, int W>
+// The state tile (internal layout when TILED, else the external rfft2
+// layout) into registers; tile 0 also stages columns 0 and N/2 in LDS.
+template <int N, int W, bool TILED>
 __device__ __forceinline__ void load_w_tile(ColShared<N, W>& sm, WTile<N, W>& wt, const NsParams& q,
                                             const float2* __restrict__ w, int c0, int tid) {
   constexpr int H1 = N / 2 + 1;
+  auto at = [&](int p, int c) { return TILED ? six<N>(p, c) : p * H1 + c; };
 #pragma unroll
   for (int k = 0; k < WTile<N, W>::PER; ++k) {
     const int e = tid + k * kThreads;
     if (e >= N * W) break;
     const int p = e / W, c = e % W;
-    wt.w[k] = w[p * H1 + c0 + c];
+    wt.w[k] = w[at(p, c0 + c)];
   }
   if (c0 == 0) {
     constexpr int PN = (N + kThreads - 1) / kThreads;
@@ -170,8 +204,8 @@ __device__ __forceinline__ void load_w_tile(ColShared<N, W>& sm, WTile<N, W>& wt
     for (int k = 0; k < PN; ++k) {
       const int p = tid + k * kThreads;
       if (N % kThreads == 0 || p < N) {
-        a[k] = w[p * H1];
-        b[k] = w[p * H1 + N / 2];
+        a[k] = w[at(p, 0)];
+        b[k] = w[at(p, N / 2)];
       }
     }
 #pragma unroll
@@ -239,7 +273,9 @@ __global__ __launch_bounds__(kThreads) void ns_cols_inverse(NsParams q, const fl
   WavenumberStage<N> wn;
   tw.load(q.tw, tid);
   wn.load(q, tid);
-  load_w_tile<N, W>(sm, wt, q, w, c0, tid);
+  // the step (NSPEC = 4) reads the internal state, irfft2 (NSPEC = 1) the
+  // caller's array
+  load_w_tile<N, W, NSPEC == 4>(sm, wt, q, w, c0, tid);
   tw.store(sm.tw, tid);
   wn.store(sm.kx, sm.ky, tid);
   __syncthreads();
@@ -272,7 +308,7 @@ __global__ __launch_bounds__(kThreads) void rows_r2c(NsParams q, const float* __
   lds_fft<H, RB, kThreads, 1, H, false, false>(zb, twh, tid);
   for (int e = tid; e < RB * H; e += kThreads) {
     const int r = e / H, k = e % H;
-    pout[static_cast<size_t>(i0 + r) * H + k] = r2c_combine(zb + r * H, k, H, twn[k]);
+    pout[tix<N>(i0 + r, k)] = r2c_combine(zb + r * H, k, H, twn[k]);
   }
 }
 
@@ -293,8 +329,7 @@ __global__ __launch_bounds__(kThreads) void rows_c2r(NsParams q, const float2* _
       tid,
       [&](int e) {
         const int r = e / H, k = e % H;
-        const float2* row = pin + static_cast<size_t>(i0 + r) * H;
-        return F2Triple{row[k], row[(H - k) & (H - 1)], q.tw[k]};
+        return F2Triple{pin[tix<N>(i0 + r, k)], pin[tix<N>(i0 + r, (H - k) & (H - 1))], q.tw[k]};
       },
       [&](int e, F2Triple v) { zb[e] = c2r_combine(v.a, v.b, e % H, v.t); });
   tb.store(twh, tid);
@@ -337,9 +372,9 @@ __global__ __launch_bounds__(kThreads) void ns_rows(NsParams q, const float2* __
       if (e < TOT) {
         const int t = e / H, kk = e % H;
         const int m = t / RB, r = t % RB;
-        const float2* row = g + m * gstride + static_cast<size_t>(i0 + r) * H;
-        a[k] = row[kk];
-        b[k] = row[(H - kk) & (H - 1)];
+        const float2* gm = g + m * gstride;
+        a[k] = gm[tix<N>(i0 + r, kk)];
+        b[k] = gm[tix<N>(i0 + r, (H - kk) & (H - 1))];
       }
     }
     if (k0 == 0) {
@@ -369,10 +404,36 @@ __global__ __launch_bounds__(kThreads) void ns_rows(NsParams q, const float2* __
       [&](int e, float v) { pr[e] = v; });
   __syncthreads();
   lds_fft<H, RB, kThreads, 1, H, false, false, R>(prod, twh, tid);
-  float2* po = pout + static_cast<size_t>(i0) * H;
   staged<RB * H, 8, kThreads>(
       tid, [&](int e) { return r2c_combine(prod + (e / H) * H, e % H, H, twn[e % H]); },
-      [&](int e, float2 v) { po[e] = v; });
+      [&](int e, float2 v) { pout[tix<N>(i0 + e / H, e % H)] = v; });
+}
+
+// External rfft2 layout <-> internal state layout of one instance per
+// blockIdx.y (slots[blockIdx.y] when given); one word per thread, walked in
+// the internal order (contiguous internal side).
+template <int N, bool TO_INTERNAL>
+__global__ __launch_bounds__(kThreads) void ns_state_convert(const float2* __restrict__ src,
+                                                             float2* __restrict__ dst,
+                                                             const int* __restrict__ slots) {
+  constexpr int H = N / 2, H1 = H + 1, TOT = N * H1;
+  const int t = blockIdx.x * kThreads + threadIdx.x;
+  if (t >= TOT) return;
+  const size_t inst = slots ? static_cast<size_t>(slots[blockIdx.y]) : blockIdx.y;
+  int p, c;
+  if (t < N * H) {
+    const int tile = t / (N * kTw), r = t % (N * kTw);
+    p = r / kTw;
+    c = tile * kTw + r % kTw;
+  } else {
+    p = t - N * H;
+    c = H;
+  }
+  const size_t base = inst * TOT;
+  if (TO_INTERNAL)
+    dst[base + t] = src[base + static_cast<size_t>(p) * H1 + c];
+  else
+    dst[base + static_cast<size_t>(p) * H1 + c] = src[base + t];
 }
 
 // 2/3 rule mask (spectral/utils.py:128-134).
@@ -382,7 +443,8 @@ __device__ __forceinline__ float ns_filter(const NsParams& q, int p, int c) {
   return (row && c < q.filt_cols) ? 1.0f : 0.0f;
 }
 
-// MODE 0: write F (explicit terms) to u_out.  MODE 1: RK/CN update of (u, h).
+// MODE 0: write F (explicit terms) to u_out in the external layout.
+// MODE 1: RK/CN update of (u, h) (internal state layout).
 // MODE 2: update, then the next stage's inverse column work on the new tile.
 template <int N, int W, int MODE, int R = 8>
 __global__ __launch_bounds__(kThreads) void ns_cols_forward(NsParams q, const float2* __restrict__ pin,
@@ -417,7 +479,7 @@ __global__ __launch_bounds__(kThreads) void ns_cols_forward(NsParams q, const fl
 #pragma unroll
   for (int k = 0; k < PER; ++k) {
     const int e = tid + k * kThreads;
-    if ((N * W) % kThreads == 0 || e < N * W) pv[k] = pin[(e / W) * H + c0 + e % W];
+    if ((N * W) % kThreads == 0 || e < N * W) pv[k] = pin[tix<N>(e / W, c0 + e % W)];
   }
   [[maybe_unused]] float2 u_pre[MODE == 0 ? 1 : PER], h_pre[MODE == 0 ? 1 : PER];
   if constexpr (MODE != 0) {
@@ -425,7 +487,7 @@ __global__ __launch_bounds__(kThreads) void ns_cols_forward(NsParams q, const fl
     for (int k = 0; k < PER; ++k) {
       const int e = tid + k * kThreads;
       if ((N * W) % kThreads == 0 || e < N * W) {
-        const size_t idx = static_cast<size_t>(e / W) * H1 + c0 + e % W;
+        const int idx = six<N>(e / W, c0 + e % W);
         u_pre[k] = u_in[idx];
         h_pre[k] = q.read_h ? h[idx] : make_float2(0.f, 0.f);
       }
@@ -441,101 +503,78 @@ __global__ __launch_bounds__(kThreads) void ns_cols_forward(NsParams q, const fl
   __syncthreads();
   lds_fft<N, W, kThreads, W, 1, true, false, R>(sm.buf, sm.tw, tid);
 
-  // F at (p, cg) for this thread's elements; tile 0 also produces column N/2.
-  float2 f[PER], fh[PER];
+  // h = F + beta h;  u = (u + g h + mu L u) / (1 - mu L), with the old
+  // (u, h) at (p, cg) given.
+  auto update_with = [&](int p, int cg, float2 fv, float2 uo, float2 ho) -> float2 {
+    const int idx = six<N>(p, cg);
+    float2 hv = fv;
+    if (q.read_h) hv = make_float2(fv.x + q.beta * ho.x, fv.y + q.beta * ho.y);
+    h[idx] = hv;
+    const float kx = sm.kx[p], ky = sm.ky[cg];
+    const float lap = q.neg_four_pi_sq * (kx * kx + ky * ky);
+    const float L = q.nu * lap - q.drag;
+    const float2 lu = make_float2(L * uo.x, L * uo.y);
+    const float2 x = make_float2((uo.x + q.g * hv.x) + q.mu * lu.x, (uo.y + q.g * hv.y) + q.mu * lu.y);
+    const float inv = 1.0f / (1.0f - q.mu * L);
+    const float2 un = make_float2(inv * x.x, inv * x.y);
+    u_out[idx] = un;
+    return un;
+  };
+  // Column N/2 (tile 0 only): (u, h) read here.
+  auto update_nyquist = [&](int p, float2 fv) -> float2 {
+    const int idx = six<N>(p, H);
+    const float2 ho = q.read_h ? h[idx] : make_float2(0.f, 0.f);
+    return update_with(p, H, fv, u_in[idx], ho);
+  };
+
+  // Per element: F at (p, cg) (filter, then the sparse, time-constant forcing
+  // curl entries), then the update (or F itself for MODE 0); tile 0 also
+  // handles column N/2.  One pass, so no array of F values stays live.
+  [[maybe_unused]] WTile<N, W> wt;
 #pragma unroll
   for (int k = 0; k < PER; ++k) {
     const int e = tid + k * kThreads;
     if (e < N * W) {
       const int p = e / W, c = e % W, cg = c0 + c;
       const float2 y = sm.buf[e];
+      float2 f, fh = make_float2(0.f, 0.f);
       if (cg == 0) {
         const float2 ym = cconj(sm.buf[((N - p) & (N - 1)) * W]);
         const float2 a = make_float2(0.5f * (y.x + ym.x), 0.5f * (y.y + ym.y));
         const float2 d = csub(y, ym);
         const float2 b = make_float2(0.5f * d.y, -0.5f * d.x);
         const float m0 = ns_filter(q, p, 0), mh = ns_filter(q, p, H);
-        f[k] = make_float2(a.x * m0, a.y * m0);
-        fh[k] = make_float2(b.x * mh, b.y * mh);
+        f = make_float2(a.x * m0, a.y * m0);
+        fh = make_float2(b.x * mh, b.y * mh);
       } else {
         const float m = ns_filter(q, p, cg);
-        f[k] = make_float2(y.x * m, y.y * m);
+        f = make_float2(y.x * m, y.y * m);
       }
-    }
-  }
-  __syncthreads();  // sm.buf free from here on
-
-  // Forcing curl entries (sparse; constant in time) are added by their owners.
-#pragma unroll
-  for (int k = 0; k < PER; ++k) {
-    const int e = tid + k * kThreads;
-    if (e < N * W) {
-      const int p = e / W, c = e % W, cg = c0 + c;
       for (int t = 0; t < q.n_force; ++t) {
         const int fp = q.force_idx[2 * t], fc = q.force_idx[2 * t + 1];
-        if (fp == p && fc == cg) f[k] = cadd(f[k], q.force_val[t]);
-        if (cg == 0 && fp == p && fc == H) fh[k] = cadd(fh[k], q.force_val[t]);
+        if (fp == p && fc == cg) f = cadd(f, q.force_val[t]);
+        if (cg == 0 && fp == p && fc == H) fh = cadd(fh, q.force_val[t]);
       }
-    }
-  }
-
-  if constexpr (MODE == 0) {
-#pragma unroll
-    for (int k = 0; k < PER; ++k) {
-      const int e = tid + k * kThreads;
-      if (e < N * W) {
-        const int p = e / W, c = e % W, cg = c0 + c;
-        u_out[p * H1 + cg] = f[k];
-        if (cg == 0) u_out[p * H1 + H] = fh[k];
-      }
-    }
-    return;
-  } else {
-    // h = F + beta h;  u = (u + g h + mu L u) / (1 - mu L), with the old
-    // (u, h) at (p, cg) given.
-    auto update_with = [&](int p, int cg, float2 fv, float2 uo, float2 ho) -> float2 {
-      const size_t idx = static_cast<size_t>(p) * H1 + cg;
-      float2 hv = fv;
-      if (q.read_h) hv = make_float2(fv.x + q.beta * ho.x, fv.y + q.beta * ho.y);
-      h[idx] = hv;
-      const float kx = sm.kx[p], ky = sm.ky[cg];
-      const float lap = q.neg_four_pi_sq * (kx * kx + ky * ky);
-      const float L = q.nu * lap - q.drag;
-      const float2 lu = make_float2(L * uo.x, L * uo.y);
-      const float2 x = make_float2((uo.x + q.g * hv.x) + q.mu * lu.x, (uo.y + q.g * hv.y) + q.mu * lu.y);
-      const float inv = 1.0f / (1.0f - q.mu * L);
-      const float2 un = make_float2(inv * x.x, inv * x.y);
-      u_out[idx] = un;
-      return un;
-    };
-    // Column N/2 (tile 0 only): (u, h) read here.
-    auto update_nyquist = [&](int p, float2 fv) -> float2 {
-      const size_t idx = static_cast<size_t>(p) * H1 + H;
-      const float2 ho = q.read_h ? h[idx] : make_float2(0.f, 0.f);
-      return update_with(p, H, fv, u_in[idx], ho);
-    };
-    [[maybe_unused]] WTile<N, W> wt;
-#pragma unroll
-    for (int k = 0; k < PER; ++k) {
-      const int e = tid + k * kThreads;
-      if (e < N * W) {
-        const int p = e / W, c = e % W, cg = c0 + c;
-        const float2 un = update_with(p, cg, f[k], u_pre[k], h_pre[k]);
+      if constexpr (MODE == 0) {
+        u_out[p * H1 + cg] = f;
+        if (cg == 0) u_out[p * H1 + H] = fh;
+      } else {
+        const float2 un = update_with(p, cg, f, u_pre[k], h_pre[k]);
         if constexpr (MODE == 2) {
           wt.w[k] = un;
           if (cg == 0) {
             sm.w0[p] = un;
-            sm.wn[p] = update_nyquist(p, fh[k]);
+            sm.wn[p] = update_nyquist(p, fh);
           }
         } else {
-          if (cg == 0) update_nyquist(p, fh[k]);
+          if (cg == 0) update_nyquist(p, fh);
         }
       }
     }
-    if constexpr (MODE == 2) {
-      __syncthreads();
-      cols_inverse_body<N, W, 4, R>(sm, wt, q, g, c0, tid);
-    }
+  }
+  if constexpr (MODE == 2) {
+    __syncthreads();  // sm.buf read by all; sm.w0 / sm.wn written
+    cols_inverse_body<N, W, 4, R>(sm, wt, q, g, c0, tid);
   }
 }
 
@@ -769,6 +808,31 @@ int ns_launch_irfft2(int n, const NsParams& q, const float2* in, float* x, float
     case 128: return NsOps<128>::irfft2(q, in, x, pbuf, batch, s);
     case 256: return NsOps<256>::irfft2(q, in, x, pbuf, batch, s);
     case 512: return NsOps<512>::irfft2(q, in, x, pbuf, batch, s);
+    default: return fail(JCFD_ERR_UNSUPPORTED, "spectral grid size not built");
+  }
+}
+
+namespace {
+template <int N>
+int ns_convert_n(bool to_internal, const float2* src, float2* dst, const int* slots, int count,
+                 hipStream_t s) {
+  const dim3 grid((N * (N / 2 + 1) + kThreads - 1) / kThreads, count);
+  if (to_internal)
+    hipLaunchKernelGGL((ns_state_convert<N, true>), grid, dim3(kThreads), 0, s, src, dst, slots);
+  else
+    hipLaunchKernelGGL((ns_state_convert<N, false>), grid, dim3(kThreads), 0, s, src, dst, slots);
+  return check_hip(hipGetLastError(), "ns_state_convert");
+}
+}  // namespace
+
+int ns_launch_state_convert(int n, bool to_internal, const float2* src, float2* dst, const int* slots,
+                            int count, hipStream_t s) {
+  switch (n) {
+    case 32: return ns_convert_n<32>(to_internal, src, dst, slots, count, s);
+    case 64: return ns_convert_n<64>(to_internal, src, dst, slots, count, s);
+    case 128: return ns_convert_n<128>(to_internal, src, dst, slots, count, s);
+    case 256: return ns_convert_n<256>(to_internal, src, dst, slots, count, s);
+    case 512: return ns_convert_n<512>(to_internal, src, dst, slots, count, s);
     default: return fail(JCFD_ERR_UNSUPPORTED, "spectral grid size not built");
   }
 }
