@@ -355,7 +355,9 @@ __global__ __launch_bounds__(kThreads) void ns_rows(NsParams q, const float2* __
   pout += inst * gstride;
   // Load the four spectra's rows in chunks (all loads of a chunk issued
   // together); the twiddle tables go to LDS while the first chunk is in
-  // flight.
+  // flight.  Element e walks the tiled layout in memory order: word w of
+  // the tile, then row r (the RB rows of a tile are contiguous), then the
+  // tile, then the spectrum m.
   constexpr int TOT = 4 * RB * H;
   constexpr int PER = (TOT + kThreads - 1) / kThreads;
   constexpr int C = PER < 8 ? PER : 8;
@@ -370,9 +372,8 @@ __global__ __launch_bounds__(kThreads) void ns_rows(NsParams q, const float2* __
     for (int k = 0; k < C; ++k) {
       const int e = tid + (k0 + k) * kThreads;
       if (e < TOT) {
-        const int t = e / H, kk = e % H;
-        const int m = t / RB, r = t % RB;
-        const float2* gm = g + m * gstride;
+        const int r = (e / kTw) % RB, kk = ((e / (kTw * RB)) % (H / kTw)) * kTw + e % kTw;
+        const float2* gm = g + (e / (RB * H)) * gstride;
         a[k] = gm[tix<N>(i0 + r, kk)];
         b[k] = gm[tix<N>(i0 + r, (H - kk) & (H - 1))];
       }
@@ -386,8 +387,8 @@ __global__ __launch_bounds__(kThreads) void ns_rows(NsParams q, const float2* __
     for (int k = 0; k < C; ++k) {
       const int e = tid + (k0 + k) * kThreads;
       if (e < TOT) {
-        const int kk = e % H;
-        rows[e] = c2r_combine(a[k], b[k], kk, twn[kk]);
+        const int r = (e / kTw) % RB, kk = ((e / (kTw * RB)) % (H / kTw)) * kTw + e % kTw;
+        rows[((e / (RB * H)) * RB + r) * H + kk] = c2r_combine(a[k], b[k], kk, twn[kk]);
       }
     }
   }
@@ -404,9 +405,12 @@ __global__ __launch_bounds__(kThreads) void ns_rows(NsParams q, const float2* __
       [&](int e, float v) { pr[e] = v; });
   __syncthreads();
   lds_fft<H, RB, kThreads, 1, H, false, false, R>(prod, twh, tid);
+  // element e in the tiled layout's memory order, as for the loads
+  auto row_of = [](int e) { return (e / kTw) % RB; };
+  auto word_of = [](int e) { return (e / (kTw * RB)) * kTw + e % kTw; };
   staged<RB * H, 8, kThreads>(
-      tid, [&](int e) { return r2c_combine(prod + (e / H) * H, e % H, H, twn[e % H]); },
-      [&](int e, float2 v) { pout[tix<N>(i0 + e / H, e % H)] = v; });
+      tid, [&](int e) { return r2c_combine(prod + row_of(e) * H, word_of(e), H, twn[word_of(e)]); },
+      [&](int e, float2 v) { pout[tix<N>(i0 + row_of(e), word_of(e))] = v; });
 }
 
 // External rfft2 layout <-> internal state layout of one instance per
