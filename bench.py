@@ -406,8 +406,8 @@ def main():
     return
   if args.gpus != 1:
     print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported here "
-          "(independent instances shard across GPUs with no communication; "
-          "see the multi-GPU section of DESIGN.md)", file=sys.stderr)
+          "(independent instances shard across GPUs with no communication)",
+          file=sys.stderr)
     sys.exit(2)
   import numpy as np
   import torch
