@@ -1027,15 +1027,33 @@ template <int N> struct ProjOp {
 // across the XCDs and cost more than they balance.  Each item runs the same
 // tile body as the per-size kernels, so an instance's arithmetic -- and its
 // result, bit for bit -- does not depend on what it is grouped with.
+// Tile shapes of the group's items, sized so that no item is much longer
+// than a workgroup's share of a mixed group: the marching stencil for
+// n >= 256 (as a tuned batch of those sizes uses) with 4096 cells per item
+// (16 rows of 256, 8 rows of 512), the default row blocks below that, and
+// Poisson column tiles of about 2048 words (n * W).  Every shape gives the
+// same results bit for bit.
+template <int N>
+struct GroupTiles {
+  static constexpr bool kMarch = N >= 256;
+  static constexpr int RB_FWD = kMarch ? 4096 / N : Tiles<N>::RB;
+  static constexpr int W = 2048 / N < N / 2 ? 2048 / N : N / 2;
+  static constexpr int RB_PROJ = Tiles<N>::RB;
+};
+
 template <int N, int PHASE>
 __device__ __forceinline__ void group_tile(const FvmGroupDesc& d, int tile, int tid, char* smem) {
-  constexpr int RB = Tiles<N>::RB, W = Tiles<N>::W;
-  if constexpr (PHASE == 0)
-    rows_forward_body<N, RB, 0>(d.p, d.b, tile * RB, tid, smem);
-  else if constexpr (PHASE == 1)
-    poisson_cols_body<N, W>(d.b, 1.0f, tile * W, tid, smem);
-  else
-    rows_project_body<N, RB, 0>(d.p, d.b, tile * RB, tid, smem);
+  using G = GroupTiles<N>;
+  if constexpr (PHASE == 0) {
+    if constexpr (G::kMarch)
+      rows_forward_march_body<N, G::RB_FWD>(d.p, d.b, tile * G::RB_FWD, tid, smem);
+    else
+      rows_forward_body<N, G::RB_FWD, 0>(d.p, d.b, tile * G::RB_FWD, tid, smem);
+  } else if constexpr (PHASE == 1) {
+    poisson_cols_body<N, G::W>(d.b, 1.0f, tile * G::W, tid, smem);
+  } else {
+    rows_project_body<N, G::RB_PROJ, 0>(d.p, d.b, tile * G::RB_PROJ, tid, smem);
+  }
 }
 
 template <int PHASE>
@@ -1090,9 +1108,10 @@ __global__ __launch_bounds__(kThreads) void fvm_group_phase(FvmGroupQueue q) {
 
 template <int N>
 constexpr int group_lds_bytes(int phase) {
-  return phase == 0   ? RowsForwardLds<N, Tiles<N>::RB, 0>::bytes
-         : phase == 1 ? PoissonLds<N, Tiles<N>::W>::bytes
-                      : RowsProjectLds<N, Tiles<N>::RB, 0>::bytes;
+  using G = GroupTiles<N>;
+  if (phase == 0)
+    return G::kMarch ? MarchLds<N, G::RB_FWD>::bytes : RowsForwardLds<N, G::RB_FWD, 0>::bytes;
+  return phase == 1 ? PoissonLds<N, G::W>::bytes : RowsProjectLds<N, G::RB_PROJ, 0>::bytes;
 }
 
 int group_lds(int phase, int max_n) {
@@ -1107,7 +1126,8 @@ int group_lds(int phase, int max_n) {
 
 template <int N>
 int group_tiles_n(int phase) {
-  return phase == 1 ? N / 2 / Tiles<N>::W : N / Tiles<N>::RB;
+  using G = GroupTiles<N>;
+  return phase == 0 ? N / G::RB_FWD : phase == 1 ? N / 2 / G::W : N / G::RB_PROJ;
 }
 
 
