@@ -379,7 +379,7 @@ __device__ __forceinline__ void rows_forward_march_body(const FvmParams& p, cons
     __syncthreads();
     if (vrow && (s - 1) % FB == FB - 1) {
       const int d0 = s - FB;  // first div row of this batch
-      lds_fft<H, FB, kThreads, 1, H, false, false, R>(zb, twh, tid);
+      lds_fft_waves<H, FB, kThreads, 1, H, false, R>(zb, twh, tid);
       for (int e = tid; e < FB * H; e += kThreads) {
         const int r = e / H, k = e % H;
         b.spec[(i0 + d0 + r) * H + k] = r2c_combine(zb + r * H, k, H, twn[k]);
@@ -583,7 +583,7 @@ __device__ __forceinline__ void rows_forward_body(const FvmParams& p, const FvmB
     if constexpr (MODE == 4) return;
     __syncthreads();
   }
-  lds_fft<H, RB, kThreads, 1, H, false, false, R>(zb, twh, tid);
+  lds_fft_waves<H, RB, kThreads, 1, H, false, R>(zb, twh, tid);
   for (int e = tid; e < RB * H; e += kThreads) {
     const int r = e / H, k = e % H;
     b.spec[(i0 + r) * H + k] = r2c_combine(zb + r * H, k, H, twn[k]);
@@ -717,7 +717,7 @@ __device__ __forceinline__ void rows_project_body(const FvmParams& p, const FvmB
     prefetch();
     tsh.store(twh, tid);
     __syncthreads();
-    lds_fft<H, T, kThreads, 1, H, false, true, R>(qb, twh, tid);
+    lds_fft_waves<H, T, kThreads, 1, H, true, R>(qb, twh, tid);
   }
   const float* q = reinterpret_cast<const float*>(qb);
 #pragma unroll

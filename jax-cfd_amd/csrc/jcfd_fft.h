@@ -163,6 +163,87 @@ __device__ __forceinline__ void lds_fft(float2* buf, const float2* tw, int tid) 
   fft_passes<N, 1, T, NT, ES, TS, TFAST, INV, MAXR>(buf, tw, tid);
 }
 
+// ---- one transform per wavefront -------------------------------------------
+// The same Stockham passes, with wavefront w running transforms w, w + NW, ...
+// (its lanes take their (transform, butterfly) pairs): a pass depends only on
+// its own wavefront's
+// earlier LDS writes, so passes are separated by a wavefront-level fence
+// instead of a workgroup barrier.  Arithmetic per transform is unchanged, so
+// results are bit-identical to lds_fft.  Best when elements of a transform
+// are contiguous (ES = 1), so a wavefront's lanes touch consecutive words.
+
+// Orders this wavefront's LDS accesses (all of its lanes) before the ones
+// that follow.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <int N, int R, int NS, int T, int NW, int ES, int TS, bool INV>
+__device__ __forceinline__ void stockham_pass_wave(float2* buf, const float2* tw, int wave, int lane) {
+  constexpr int NB = N / R;
+  constexpr int TPW = (T + NW - 1) / NW;  // transforms per wavefront: t = wave + a * NW
+  constexpr int ITEMS = TPW * NB;         // (transform, butterfly) pairs of the wavefront
+  constexpr int IPL = (ITEMS + 63) / 64;  // pairs per lane
+  constexpr int TWS = N / (NS * R);
+  float2 v[IPL][R];
+#pragma unroll
+  for (int i = 0; i < IPL; ++i) {
+    const int g = lane + 64 * i, a = g / NB, j = g % NB, t = wave + a * NW;
+    if ((ITEMS % 64 == 0 || g < ITEMS) && (T % NW == 0 || t < T)) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[i][r] = buf[(j + r * NB) * ES + t * TS];
+    }
+  }
+  wave_lds_sync();
+#pragma unroll
+  for (int i = 0; i < IPL; ++i) {
+    const int g = lane + 64 * i, a = g / NB, j = g % NB, t = wave + a * NW;
+    if ((ITEMS % 64 == 0 || g < ITEMS) && (T % NW == 0 || t < T)) {
+      const int k = j % NS;
+      if constexpr (NS > 1) {
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          float2 w = tw[r * k * TWS];
+          if (INV) w.y = -w.y;
+          v[i][r] = cmul(v[i][r], w);
+        }
+      }
+      float2 x[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) x[r] = v[i][r];
+      dft_radix<R, INV>(x);
+      const int base = (j / NS) * NS * R + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) buf[(base + r * NS) * ES + t * TS] = x[r];
+    }
+  }
+  wave_lds_sync();
+}
+
+template <int N, int NS, int T, int NW, int ES, int TS, bool INV, int MAXR>
+__device__ __forceinline__ void fft_passes_wave(float2* buf, const float2* tw, int wave, int lane) {
+  if constexpr (NS < N) {
+    constexpr int REM = N / NS;
+    constexpr int R = (MAXR >= 8 && REM % 8 == 0) ? 8 : ((REM % 4 == 0) ? 4 : 2);
+    stockham_pass_wave<N, R, NS, T, NW, ES, TS, INV>(buf, tw, wave, lane);
+    fft_passes_wave<N, NS * R, T, NW, ES, TS, INV, MAXR>(buf, tw, wave, lane);
+  }
+}
+
+// T in-place complex transforms of length N in LDS by NT / 64 wavefronts,
+// same element placement and results as lds_fft.  Callers must have a
+// barrier between writing `buf` and calling this; it ends with a barrier.
+template <int N, int T, int NT, int ES, int TS, bool INV, int MAXR = 8>
+__device__ __forceinline__ void lds_fft_waves(float2* buf, const float2* tw, int tid) {
+  static_assert((N & (N - 1)) == 0 && N >= 2, "power-of-two length");
+  static_assert(MAXR == 8 || MAXR == 4, "radix schedule");
+  static_assert(NT % 64 == 0, "whole wavefronts");
+  fft_passes_wave<N, 1, T, NT / 64, ES, TS, INV, MAXR>(buf, tw, tid / 64, tid % 64);
+  __syncthreads();
+}
+
 // Copy a length-N twiddle table to LDS (stride 1 in the table of length
 // `table_n`; N must divide table_n).  No barrier.
 __device__ __forceinline__ void load_twiddles(float2* dst, const float2* __restrict__ src, int n,

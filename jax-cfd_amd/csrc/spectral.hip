@@ -305,7 +305,7 @@ __global__ __launch_bounds__(kThreads) void rows_r2c(NsParams q, const float* __
   ta.store(twn, tid);
   tb.store(twh, tid);
   __syncthreads();
-  lds_fft<H, RB, kThreads, 1, H, false, false>(zb, twh, tid);
+  lds_fft_waves<H, RB, kThreads, 1, H, false>(zb, twh, tid);
   for (int e = tid; e < RB * H; e += kThreads) {
     const int r = e / H, k = e % H;
     pout[tix<N>(i0 + r, k)] = r2c_combine(zb + r * H, k, H, twn[k]);
@@ -334,7 +334,7 @@ __global__ __launch_bounds__(kThreads) void rows_c2r(NsParams q, const float2* _
       [&](int e, F2Triple v) { zb[e] = c2r_combine(v.a, v.b, e % H, v.t); });
   tb.store(twh, tid);
   __syncthreads();
-  lds_fft<H, RB, kThreads, 1, H, false, true>(zb, twh, tid);
+  lds_fft_waves<H, RB, kThreads, 1, H, true>(zb, twh, tid);
   const float* zf = reinterpret_cast<const float*>(zb);
   for (int e = tid; e < RB * N; e += kThreads) x[static_cast<size_t>(i0) * N + e] = zf[e];
 }
@@ -393,7 +393,7 @@ __global__ __launch_bounds__(kThreads) void ns_rows(NsParams q, const float2* __
     }
   }
   __syncthreads();
-  lds_fft<H, 4 * RB, kThreads, 1, H, false, true, R>(rows, twh, tid);
+  lds_fft_waves<H, 4 * RB, kThreads, 1, H, true, R>(rows, twh, tid);
   // advection = -(grad_x * vx + grad_y * vy)  (spectral/equations.py:170)
   const float* gx = reinterpret_cast<const float*>(rows);
   const float* gy = gx + RB * N;
@@ -404,7 +404,7 @@ __global__ __launch_bounds__(kThreads) void ns_rows(NsParams q, const float2* __
       tid, [&](int e) { return -(gx[e] * vx[e] + gy[e] * vy[e]); },
       [&](int e, float v) { pr[e] = v; });
   __syncthreads();
-  lds_fft<H, RB, kThreads, 1, H, false, false, R>(prod, twh, tid);
+  lds_fft_waves<H, RB, kThreads, 1, H, false, R>(prod, twh, tid);
   // element e in the tiled layout's memory order, as for the loads
   auto row_of = [](int e) { return (e / kTw) % RB; };
   auto word_of = [](int e) { return (e / (kTw * RB)) * kTw + e % kTw; };
