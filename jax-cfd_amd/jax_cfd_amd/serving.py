@@ -11,7 +11,14 @@ streams, so their kernels overlap on the device; the host loop polls each
 pool's completion event and, as soon as a pool's chunk is done, retires the
 requests that reached their step count, admits waiting requests into the
 freed slots and relaunches it.  A chunk advances every active slot by the
-smallest remaining step count (capped), so no request overshoots.
+smallest remaining step count, so no request overshoots, capped at
+`max_chunk` steps (default 16).  Short chunks matter twice: pools share the
+device's few hardware queues, so one pool's long chunk holds back the pools
+queued behind it, and a pool's active-slot count changes from chunk to
+chunk, which would make a plan re-capture its step graph (a chunk below the
+plan's graph length runs as plain launches).  On the mixed 400-request
+stream, a cap of 16 instead of 128 raised throughput 1.05e10 -> 1.27e10
+cell-steps/s and cut p99 latency 2.8 s -> 1.6 s.
 
 Admission is bounded by a device-memory budget: a pool's bytes (plan scratch
 + states) are reserved when it is created; a pool that does not fit waits
@@ -247,7 +254,7 @@ class RolloutServer:
   """Continuous-batching server (see the module docstring)."""
 
   def __init__(self, hbm_budget_bytes: float = 200e9,
-               capacity: Optional[Dict[int, int]] = None, max_chunk: int = 128,
+               capacity: Optional[Dict[int, int]] = None, max_chunk: int = 16,
                device: Optional[torch.device] = None, tune: bool = False):
     self.tune = bool(tune)
     self.device = device if device is not None else torch.device(
