@@ -121,7 +121,8 @@ class FvmFamily:
     plan.step_slots(state[0], state[1], slots, nsteps)
 
   def read_slot(self, state, slot):
-    return (state[0][slot].clone(), state[1][slot].clone())
+    """Views of the slot's state (valid until the slot is reused)."""
+    return (state[0][slot], state[1][slot])
 
 
 class SpectralFamily:
@@ -165,7 +166,8 @@ class SpectralFamily:
     plan.step_slots(state[0], slots, nsteps)
 
   def read_slot(self, state, slot):
-    return (state[0][slot].clone(),)
+    """A view of the slot's state (valid until the slot is reused)."""
+    return (state[0][slot],)
 
 
 FAMILIES = {'fvm': FvmFamily, 'spectral': SpectralFamily}
@@ -189,12 +191,19 @@ class Pool:
         autotune.tune_ns2d(self.plan, radices=(8,))
     self.state = family.alloc_state(capacity)
     self.stream = torch.cuda.Stream(device=family.device)  # default priority
-    # active slot list; a fixed buffer so captured graphs stay valid
+    # active slot list; a fixed buffer so captured graphs stay valid, filled
+    # from a pinned host copy without blocking the host (a copy from pageable
+    # memory would wait for the pool stream, which may be waiting for
+    # initial fields being built on the server-wide stream)
     self.slot_buf = torch.zeros(capacity, dtype=torch.int32, device=family.device)
+    self.slot_host = torch.zeros(capacity, dtype=torch.int32, pin_memory=True)
     self.free: List[int] = list(range(capacity))[::-1]
     self.active: Dict[int, List] = {}  # slot -> [request, remaining, admitted]
     self.event: Optional[torch.cuda.Event] = None
     self.chunk_done: List[int] = []
+    # finished requests whose states are being copied to the host:
+    # (slot, request, admitted, pinned host tensors, copy-done event)
+    self.retiring: List[tuple] = []
     self.bytes = (self.plan.device_bytes + family.state_bytes(capacity)
                   + 4 * capacity)
 
@@ -222,9 +231,11 @@ class Pool:
       return
     chunk = min(max_chunk, min(a[1] for a in self.active.values()))
     slots = sorted(self.active)
+    # the previous chunk's copy out of slot_host has run: the pool is idle
+    self.slot_host[:len(slots)] = torch.as_tensor(slots, dtype=torch.int32)
     with torch.cuda.stream(self.stream):
       idx = self.slot_buf[:len(slots)]
-      idx.copy_(torch.tensor(slots, dtype=torch.int32))
+      idx.copy_(self.slot_host[:len(slots)], non_blocking=True)
       self.family.advance(self.plan, self.state, idx, chunk)
       self.event = torch.cuda.Event()
       self.event.record(self.stream)
@@ -232,18 +243,37 @@ class Pool:
       self.active[s][1] -= chunk
     self.chunk_done = [s for s in slots if self.active[s][1] == 0]
 
-  def harvest(self, now_fn) -> List[RolloutResult]:
-    out = []
+  def harvest(self):
+    """Start copying the finished slots' states to pinned host memory on the
+    pool stream.  The host does not wait: a copy queues behind whatever
+    shares the stream's hardware queue, and the server keeps serving the
+    other pools meanwhile (collect() hands the results out)."""
     for slot in self.chunk_done:
       req, _, admitted = self.active.pop(slot)
       with torch.cuda.stream(self.stream):
         data = self.family.read_slot(self.state, slot)
-        host = tuple(d.cpu() for d in data)  # synchronises the pool stream
+        host = tuple(torch.empty(d.shape, dtype=d.dtype, pin_memory=True) for d in data)
+        for h, d in zip(host, data):
+          h.copy_(d, non_blocking=True)
+        done = torch.cuda.Event()
+        done.record(self.stream)
+      self.retiring.append((slot, req, admitted, host, done))
+    self.chunk_done = []
+
+  def collect(self, now_fn) -> List[RolloutResult]:
+    """Results whose host copies have completed; their slots become free
+    only now, so an admission cannot overwrite a state still being read."""
+    out, keep = [], []
+    for item in self.retiring:
+      slot, req, admitted, host, done = item
+      if not done.query():
+        keep.append(item)
+        continue
       out.append(RolloutResult(req.request_id, req.solver, req.n, req.steps, 'done',
                                tuple(h.numpy() for h in host), req.arrival, admitted,
                                now_fn()))
       self.free.append(slot)
-    self.chunk_done = []
+    self.retiring = keep
     return out
 
 
@@ -306,7 +336,7 @@ class RolloutServer:
   def _release_idle_pools(self, waiting_keys):
     for key in list(self.pools):
       pool = self.pools[key]
-      if not pool.active and not pool.busy and key not in waiting_keys:
+      if not pool.active and not pool.busy and not pool.retiring and key not in waiting_keys:
         self.reserved -= pool.bytes
         pool.plan.close()
         del self.pools[key]
@@ -320,17 +350,22 @@ class RolloutServer:
     results: List[RolloutResult] = []
     t0 = time.perf_counter()
     now = lambda: time.perf_counter() - t0  # noqa: E731
-    while pending or waiting or any(p.active for p in self.pools.values()):
+    while pending or waiting or any(p.active or p.retiring for p in self.pools.values()):
       t = now()
       while pending and (not realtime or pending[0].arrival <= t):
         waiting.append(pending.popleft())
       progressed = False
-      # 1) retire finished chunks, admit into free slots, relaunch
+      # 1) hand out copied results, retire finished chunks, admit into free
+      #    slots, relaunch
       for pool in list(self.pools.values()):
+        done = pool.collect(now)
+        if done:
+          results.extend(done)
+          progressed = True
         if pool.busy:
           continue
         if pool.chunk_done:
-          results.extend(pool.harvest(now))
+          pool.harvest()
           progressed = True
       still: Deque[RolloutRequest] = collections.deque()
       for req in waiting:
@@ -355,7 +390,7 @@ class RolloutServer:
         self._release_idle_pools({(r.solver, r.n) for r in waiting})
       if not progressed:
         if pending and realtime and not waiting and not any(
-            p.active for p in self.pools.values()):
+            p.active or p.retiring for p in self.pools.values()):
           time.sleep(max(0.0, min(pending[0].arrival - now(), 0.01)))
         else:
           time.sleep(50e-6)
