@@ -9,7 +9,9 @@ integer ``rng_key`` (an int, or a sequence whose last entry is the seed).
 """
 from __future__ import annotations
 
-from typing import Optional, Sequence, Tuple
+import concurrent.futures
+import threading
+from typing import Dict, Optional, Sequence, Tuple
 
 import numpy as np
 import torch
@@ -29,6 +31,54 @@ def noise(seed: int, shape, components: int) -> np.ndarray:
   rng = np.random.Generator(np.random.PCG64(seed))
   return np.stack([rng.standard_normal(shape, dtype=np.float32)
                    for _ in range(components)])
+
+
+# Noise drawn ahead of use on worker threads (NumPy releases the GIL while it
+# fills an array), straight into pinned host memory so the copy to the
+# device does not block the caller.  Keyed by (seed, shape, components);
+# the same values as noise().
+_prefetched: Dict[tuple, concurrent.futures.Future] = {}
+_prefetch_lock = threading.Lock()
+_executor: Optional[concurrent.futures.ThreadPoolExecutor] = None
+
+
+def _pinned_noise(seed: int, shape, components: int) -> torch.Tensor:
+  out = torch.empty((components,) + tuple(shape), dtype=torch.float32, pin_memory=True)
+  out.numpy()[...] = noise(seed, shape, components)
+  return out
+
+
+def prefetch_noise(seed: int, shape, components: int) -> None:
+  """Start drawing noise(seed, shape, components) on a worker thread; the
+  next filtered_velocity_field with that seed and shape uses it."""
+  global _executor
+  key = (int(seed), tuple(shape), int(components))
+  with _prefetch_lock:
+    if key in _prefetched:
+      return
+    if _executor is None:
+      _executor = concurrent.futures.ThreadPoolExecutor(max_workers=4)
+    _prefetched[key] = _executor.submit(_pinned_noise, *key)
+
+
+def _noise_on_device(seed: int, shape, components: int, device) -> torch.Tensor:
+  key = (int(seed), tuple(shape), int(components))
+  with _prefetch_lock:
+    fut = _prefetched.pop(key, None)
+  host = fut.result() if fut is not None else _pinned_noise(*key)
+  return host.to(device, non_blocking=True)
+
+
+_multipliers: Dict[tuple, torch.Tensor] = {}
+
+
+def _multiplier_on_device(grid: grids.Grid, peak_wavenumber: float, device) -> torch.Tensor:
+  """spectral_density_multiplier on the device, built once per grid."""
+  key = (grid.shape, grid.step, float(peak_wavenumber), str(device))
+  if key not in _multipliers:
+    _multipliers[key] = torch.as_tensor(spectral_density_multiplier(grid, peak_wavenumber),
+                                        device=device)
+  return _multipliers[key]
 
 
 def _log_normal_pdf(x, mode, variance=.25):
@@ -75,9 +125,8 @@ def filtered_velocity_field(rng_key, grid: grids.Grid, maximum_velocity: float =
   if device is None:
     device = torch.device('cuda', torch.cuda.current_device())
   plan = _fvm.plan_for(_fvm.spec_from_grid(grid), device)
-  nz = torch.as_tensor(noise(_seed(rng_key), grid.shape, grid.ndim), device=device)
-  mult = torch.as_tensor(spectral_density_multiplier(grid, peak_wavenumber),
-                         device=device)
+  nz = _noise_on_device(_seed(rng_key), grid.shape, grid.ndim, device)
+  mult = _multiplier_on_device(grid, peak_wavenumber, device)
   comps = [plan.spectral_filter(nz[c], mult) for c in range(grid.ndim)]
   bcs = [boundaries.periodic_boundary_conditions(grid.ndim)] * grid.ndim
   v = wrap_variables(comps, grid, bcs)

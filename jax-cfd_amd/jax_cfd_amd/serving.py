@@ -113,6 +113,10 @@ class FvmFamily:
                                                    self.PEAK, device=self.device)
     return (v[0].data, v[1].data)
 
+  def prefetch(self, seed: int):
+    """Start drawing the request's initial-field noise on a worker thread."""
+    initial_conditions.prefetch_noise(seed, self.grid.shape, self.grid.ndim)
+
   def write_slot(self, state, slot, init):
     state[0][slot].copy_(init[0])
     state[1][slot].copy_(init[1])
@@ -158,6 +162,10 @@ class SpectralFamily:
     if self._fft is None:
       self._fft = _ns2d.Ns2dPlan(self.spec, 1, self.device)
     return (self._fft.rfft2(w),)
+
+  def prefetch(self, seed: int):
+    """Start drawing the request's initial-field noise on a worker thread."""
+    initial_conditions.prefetch_noise(seed, self.grid.shape, self.grid.ndim)
 
   def write_slot(self, state, slot, init):
     state[0][slot].copy_(init[0])
@@ -353,7 +361,10 @@ class RolloutServer:
     while pending or waiting or any(p.active or p.retiring for p in self.pools.values()):
       t = now()
       while pending and (not realtime or pending[0].arrival <= t):
-        waiting.append(pending.popleft())
+        req = pending.popleft()
+        if req.solver in FAMILIES:
+          self._family(req.solver, req.n).prefetch(req.seed)  # overlaps the loop
+        waiting.append(req)
       progressed = False
       # 1) hand out copied results, retire finished chunks, admit into free
       #    slots, relaunch
