@@ -159,31 +159,73 @@ std::vector<float> diffusion_multiplier(int n, double dx, double dy, double nu, 
 // the caller's prior work and the caller's later work after it.  The graph
 // bakes in its buffer addresses, so it is rebuilt when `key` (the output
 // state pointer) or the chunk length changes.
+// Captured multi-step launch sequences, replayed with hipGraphLaunch.  Up to
+// kMaxGraphs are kept per plan (least recently used dropped), keyed by the
+// buffers and slot list a sequence was captured with and its step count, so
+// a plan that alternates between a few buffer sets captures each once.
 struct GraphCache {
+  static constexpr int kMaxGraphs = 32;
+  struct Key {
+    const void* k;
+    const void* k2;
+    const void* k4;
+    int k3;
+    int chunk;
+    bool operator==(const Key& o) const {
+      return k == o.k && k2 == o.k2 && k4 == o.k4 && k3 == o.k3 && chunk == o.chunk;
+    }
+  };
+  struct Entry {
+    Key key;
+    hipGraphExec_t exec;
+    uint64_t used;
+  };
   bool enabled = true;
   hipStream_t stream = nullptr;
   hipEvent_t ev_in = nullptr, ev_out = nullptr;
-  hipGraphExec_t exec = nullptr;
-  const void* key = nullptr;
-  const void* key2 = nullptr;
-  int key3 = 0;
-  const void* key4 = nullptr;
-  int chunk = 0;
+  std::vector<Entry> entries;
+  uint64_t tick = 0;
 
   void release() {
-    if (exec) (void)hipGraphExecDestroy(exec);
+    for (Entry& e : entries) (void)hipGraphExecDestroy(e.exec);
+    entries.clear();
     if (ev_in) (void)hipEventDestroy(ev_in);
     if (ev_out) (void)hipEventDestroy(ev_out);
     if (stream) (void)hipStreamDestroy(stream);
-    exec = nullptr;
     ev_in = ev_out = nullptr;
     stream = nullptr;
   }
 
-  // launch(s) enqueues one chunk of steps on stream s.
+  // Replays `reps` times the chunk (chunk_len steps) that launch(s) enqueues
+  // on stream s, capturing it first if its key is not cached.
   template <typename F>
   int run(const void* k, const void* k2, const void* k4, int k3, int chunk_len, int64_t reps,
           hipStream_t caller, F&& launch) {
+    const Key key{k, k2, k4, k3, chunk_len};
+    Entry* hit = nullptr;
+    for (Entry& e : entries)
+      if (e.key == key) hit = &e;
+    if (hit == nullptr) {
+      int rc = capture(key, launch);
+      if (rc) return rc;
+      hit = &entries.back();
+    }
+    hit->used = ++tick;
+    int rc;
+    if ((rc = check_hip(hipEventRecord(ev_in, caller), "event record")) ||
+        (rc = check_hip(hipStreamWaitEvent(stream, ev_in, 0), "stream wait")))
+      return rc;
+    for (int64_t r = 0; r < reps; ++r)
+      if ((rc = check_hip(hipGraphLaunch(hit->exec, stream), "graph launch"))) return rc;
+    if ((rc = check_hip(hipEventRecord(ev_out, stream), "event record")) ||
+        (rc = check_hip(hipStreamWaitEvent(caller, ev_out, 0), "stream wait")))
+      return rc;
+    return JCFD_OK;
+  }
+
+ private:
+  template <typename F>
+  int capture(const Key& key, F&& launch) {
     int rc = JCFD_OK;
     if (stream == nullptr) {
       if ((rc = check_hip(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "graph stream")) ||
@@ -191,46 +233,63 @@ struct GraphCache {
           (rc = check_hip(hipEventCreateWithFlags(&ev_out, hipEventDisableTiming), "graph event")))
         return rc;
     }
-    if (exec == nullptr || key != k || key2 != k2 || key3 != k3 || key4 != k4 || chunk != chunk_len) {
-      if (exec) {
-        (void)hipGraphExecDestroy(exec);
-        exec = nullptr;
-      }
-      hipGraph_t graph = nullptr;
-      if ((rc = check_hip(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "begin capture")))
-        return rc;
-      const int lrc = launch(stream);
-      const hipError_t ec = hipStreamEndCapture(stream, &graph);
-      if (lrc) {
-        if (graph) (void)hipGraphDestroy(graph);
-        return lrc;
-      }
-      if ((rc = check_hip(ec, "end capture"))) return rc;
-      rc = check_hip(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "graph instantiate");
-      (void)hipGraphDestroy(graph);
-      if (rc) {
-        exec = nullptr;
-        return rc;
-      }
-      key = k;
-      key2 = k2;
-      key3 = k3;
-      key4 = k4;
-      chunk = chunk_len;
+    if (entries.size() >= static_cast<size_t>(kMaxGraphs)) {  // drop the least recently used
+      size_t lru = 0;
+      for (size_t i = 1; i < entries.size(); ++i)
+        if (entries[i].used < entries[lru].used) lru = i;
+      (void)hipGraphExecDestroy(entries[lru].exec);
+      entries.erase(entries.begin() + static_cast<std::ptrdiff_t>(lru));
     }
-    if ((rc = check_hip(hipEventRecord(ev_in, caller), "event record")) ||
-        (rc = check_hip(hipStreamWaitEvent(stream, ev_in, 0), "stream wait")))
+    hipGraph_t graph = nullptr;
+    if ((rc = check_hip(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "begin capture")))
       return rc;
-    for (int64_t r = 0; r < reps; ++r)
-      if ((rc = check_hip(hipGraphLaunch(exec, stream), "graph launch"))) return rc;
-    if ((rc = check_hip(hipEventRecord(ev_out, stream), "event record")) ||
-        (rc = check_hip(hipStreamWaitEvent(caller, ev_out, 0), "stream wait")))
-      return rc;
+    const int lrc = launch(stream);
+    const hipError_t ec = hipStreamEndCapture(stream, &graph);
+    if (lrc) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return lrc;
+    }
+    if ((rc = check_hip(ec, "end capture"))) return rc;
+    hipGraphExec_t exec = nullptr;
+    rc = check_hip(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "graph instantiate");
+    (void)hipGraphDestroy(graph);
+    if (rc) return rc;
+    entries.push_back(Entry{key, exec, 0});
     return JCFD_OK;
   }
 };
 
-constexpr int kGraphChunk = 32;  // steps per captured graph
+// Steps per captured graph.  Shorter runs (and a long run's remainder) launch
+// directly: replaying a graph goes through the cache's own stream and two
+// event hand-offs, which cost a server of many pools more device time than
+// the host saves on a short chunk (measured on the mixed request stream).
+constexpr int kGraphChunk = 32;
+
+// nsteps repetitions of step(stream): whole chunks of kGraphChunk through the
+// graph cache when it is enabled (keyed by k..k4), the rest directly on s.
+template <typename Step>
+int run_steps(GraphCache& graphs, const void* k, const void* k2, const void* k4, int k3,
+              int64_t nsteps, hipStream_t s, Step&& step) {
+  auto chunk_of = [&](int len) {
+    return [&step, len](hipStream_t gs) {
+      for (int i = 0; i < len; ++i) {
+        const int r = step(gs);
+        if (r) return r;
+      }
+      return static_cast<int>(JCFD_OK);
+    };
+  };
+  int64_t remaining = nsteps;
+  int rc;
+  if (graphs.enabled && remaining >= kGraphChunk) {
+    if ((rc = graphs.run(k, k2, k4, k3, kGraphChunk, remaining / kGraphChunk, s, chunk_of(kGraphChunk))))
+      return rc;
+    remaining %= kGraphChunk;
+  }
+  for (int64_t i = 0; i < remaining; ++i)
+    if ((rc = step(s))) return rc;
+  return JCFD_OK;
+}
 
 }  // namespace
 }  // namespace jcfd
@@ -427,23 +486,8 @@ int fvm_run(jcfd_fvm_plan* p, FvmBuffers b, int64_t nsteps, hipStream_t s) {
   if ((rc = fvm_full_step(p, b, s))) return rc;
   b.u = b.u_out;
   b.v = b.v_out;
-  int64_t remaining = nsteps - 1;
-  if (p->graphs.enabled && remaining >= kGraphChunk) {
-    auto launch = [&](hipStream_t gs) {
-      for (int i = 0; i < kGraphChunk; ++i) {
-        const int r = fvm_full_step(p, b, gs);
-        if (r) return r;
-      }
-      return static_cast<int>(JCFD_OK);
-    };
-    if ((rc = p->graphs.run(b.u_out, b.v_out, b.slots, b.count, kGraphChunk,
-                            remaining / kGraphChunk, s, launch)))
-      return rc;
-    remaining %= kGraphChunk;
-  }
-  for (int64_t step = 0; step < remaining; ++step)
-    if ((rc = fvm_full_step(p, b, s))) return rc;
-  return JCFD_OK;
+  return run_steps(p->graphs, b.u_out, b.v_out, b.slots, b.count, nsteps - 1, s,
+                   [&](hipStream_t gs) { return fvm_full_step(p, b, gs); });
 }
 }  // namespace
 
@@ -1176,23 +1220,7 @@ int jcfd_fvm_group_step(jcfd_fvm_group* g, const jcfd_fvm_group_member* members,
     }
     return static_cast<int>(JCFD_OK);
   };
-  int64_t remaining = nsteps;
-  if (g->graphs.enabled && remaining >= kGraphChunk) {
-    auto launch = [&](hipStream_t gs) {
-      for (int i = 0; i < kGraphChunk; ++i) {
-        const int r = one_step(gs);
-        if (r) return r;
-      }
-      return static_cast<int>(JCFD_OK);
-    };
-    if ((rc = g->graphs.run(g->descs, g->items, nullptr, max_n, kGraphChunk, remaining / kGraphChunk, s,
-                            launch)))
-      return rc;
-    remaining %= kGraphChunk;
-  }
-  for (int64_t step = 0; step < remaining; ++step)
-    if ((rc = one_step(s))) return rc;
-  return JCFD_OK;
+  return run_steps(g->graphs, g->descs, g->items, nullptr, max_n, nsteps, s, one_step);
 }
 
 int jcfd_fvm_group_time_kernels(jcfd_fvm_group* g, int32_t reps, double* ms_per_launch, void* stream) {
@@ -1260,22 +1288,9 @@ int ns_run(jcfd_ns2d_plan* p, const float2* w_in, float2* w_out, const int* slot
   int rc;
   if ((rc = ns_launch_state_convert(p->n, true, w_in, p->ws, slots, count, s))) return rc;
   if ((rc = ns_launch_step(p->n, stages, b, count, s))) return rc;
-  int64_t remaining = nsteps - 1;
-  if (p->graphs.enabled && remaining >= kGraphChunk) {
-    auto launch = [&](hipStream_t gs) {
-      for (int i = 0; i < kGraphChunk; ++i) {
-        const int r = ns_launch_step(p->n, stages, b, count, gs);
-        if (r) return r;
-      }
-      return static_cast<int>(JCFD_OK);
-    };
-    if ((rc = p->graphs.run(b.u_out, nullptr, slots, count, kGraphChunk, remaining / kGraphChunk, s,
-                            launch)))
-      return rc;
-    remaining %= kGraphChunk;
-  }
-  for (int64_t step = 0; step < remaining; ++step)
-    if ((rc = ns_launch_step(p->n, stages, b, count, s))) return rc;
+  if ((rc = run_steps(p->graphs, b.u_out, nullptr, slots, count, nsteps - 1, s,
+                      [&](hipStream_t gs) { return ns_launch_step(p->n, stages, b, count, gs); })))
+    return rc;
   return ns_launch_state_convert(p->n, false, p->ws, w_out, slots, count, s);
 }
 }  // namespace
