@@ -89,3 +89,30 @@ def test_poisson_stream_realtime_small():
   summ = serving.summarize(res, time.perf_counter() - t)
   assert summ['done'] == 12 and summ['p99_latency_s'] >= summ['p50_latency_s'] > 0
   assert all(r.finished >= r.arrival for r in res)
+
+
+def test_slot_reuse_keeps_every_result():
+  """One slot per pool, so each request reuses the slot its predecessor just
+  left: a result is copied out (asynchronously) before the slot is written
+  again.  Every result equals its plain one-instance run bitwise."""
+  import jax_cfd_amd.base as cfd
+  from jax_cfd_amd import serving
+  from jax_cfd_amd.spectral import time_stepping
+  dev = torch.device('cuda', 0)
+  reqs = [_req(i, 'fvm' if i % 2 == 0 else 'spectral', 64, 30 + 7 * i, 40 + i) for i in range(6)]
+  res = _by_id(_srv(capacity={64: 1}).run(reqs, realtime=False))
+  fam, sfam = serving.FvmFamily(64, dev), serving.SpectralFamily(64, dev)
+  plan = cfd._fvm.FvmPlan(fam.spec, dev)
+  step = time_stepping.crank_nicolson_rk4(sfam.equation, sfam.DT)
+  for r in reqs:
+    got = res[r.request_id]
+    assert got.status == 'done'
+    print(f"request {r.request_id}: {r.solver} {r.n} steps {r.steps}", flush=True)
+    if r.solver == 'fvm':
+      u, v = fam.initial_state(r.seed)
+      want = [t.cpu().numpy() for t in plan.step(u, v, r.steps)]
+    else:
+      (w,) = sfam.initial_state(r.seed)
+      want = [step.multistep(w, r.steps).cpu().numpy()]
+    for a, b in zip(got.state, want):
+      np.testing.assert_array_equal(a, b)
