@@ -244,13 +244,6 @@ __device__ __forceinline__ void lds_fft_waves(float2* buf, const float2* tw, int
   __syncthreads();
 }
 
-// Copy a length-N twiddle table to LDS (stride 1 in the table of length
-// `table_n`; N must divide table_n).  No barrier.
-__device__ __forceinline__ void load_twiddles(float2* dst, const float2* __restrict__ src, int n,
-                                              int table_stride, int tid, int nt) {
-  for (int k = tid; k < n; k += nt) dst[k] = src[k * table_stride];
-}
-
 // A length-N twiddle table (entries k * STRIDE of the source table) staged
 // through registers: load() issues all of a thread's global loads, store()
 // writes them to LDS.  Splitting the two lets a kernel keep the loads in
@@ -309,14 +302,6 @@ struct F2Triple {
   float2 a, b, t;
 };
 
-// Table loads of compile-time length: loads first, then the LDS writes.  No
-// barrier.
-template <int N, int STRIDE, int NT>
-__device__ __forceinline__ void load_twiddles_n(float2* dst, const float2* __restrict__ src, int tid) {
-  TwiddleStage<N, STRIDE, NT> t;
-  t.load(src, tid);
-  t.store(dst, tid);
-}
 
 // After an in-place length-n/2 forward transform of z (row in LDS), the
 // packed half-spectrum word k of the real row: needs Z[k] and Z[n/2 - k].

@@ -252,12 +252,6 @@ struct WavenumberStage {
   }
 };
 
-template <int N, int W>
-__device__ __forceinline__ void load_wavenumbers(ColShared<N, W>& sm, const NsParams& q, int tid) {
-  WavenumberStage<N> st;
-  st.load(q, tid);
-  st.store(sm.kx, sm.ky, tid);
-}
 
 template <int N, int W, int NSPEC, int R = 8>
 __global__ __launch_bounds__(kThreads) void ns_cols_inverse(NsParams q, const float2* __restrict__ w,
