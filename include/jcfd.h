@@ -243,6 +243,8 @@ int jcfd_fvm_group_step(jcfd_fvm_group* group, const jcfd_fvm_group_member* memb
                         int32_t n_members, int64_t nsteps, void* stream);
 
 /* ------------------------------------------------------- spectral path */
+#define JCFD_MAX_RK_STAGES 8
+
 typedef struct {
   int32_t n;               /* cells per side; square periodic grid */
   double dx, dy;           /* grid steps */
@@ -254,13 +256,24 @@ typedef struct {
   int32_t n_forcing;       /* non-zero entries of the constant forcing curl */
   const int32_t* forcing_index; /* host, 2 * n_forcing ints: (p, c), 0 <= c <= n/2 */
   const float* forcing_value;   /* host, 2 * n_forcing floats: (re, im) */
+  /* Low-storage Runge-Kutta / Crank-Nicolson tableau
+   * (spectral/time_stepping.py:117-170): rk_stages in 1..JCFD_MAX_RK_STAGES
+   * with rk_stages + 1 alphas, rk_stages betas and rk_stages gammas; 0
+   * selects crank_nicolson_rk4's Carpenter-Kennedy coefficients
+   * (spectral/time_stepping.py:186-197) and ignores the arrays. */
+  int32_t rk_stages;
+  double rk_alpha[JCFD_MAX_RK_STAGES + 1];
+  double rk_beta[JCFD_MAX_RK_STAGES];
+  double rk_gamma[JCFD_MAX_RK_STAGES];
 } jcfd_ns2d_config;
 
 typedef struct jcfd_ns2d_plan jcfd_ns2d_plan;
 
-/* Plan for crank_nicolson_rk4(NavierStokes2D(...), dt)
- * (spectral/time_stepping.py:186-197, spectral/equations.py:129-220): the
- * wavenumber tables, 2/3 filter bounds and the five stage scalars. */
+/* Plan for low_storage_runge_kutta_crank_nicolson(..., NavierStokes2D(...),
+ * dt) -- crank_nicolson_rk4 by default, crank_nicolson_rk3 or any other
+ * tableau of up to JCFD_MAX_RK_STAGES stages
+ * (spectral/time_stepping.py:117-197, spectral/equations.py:129-220): the
+ * wavenumber tables, 2/3 filter bounds and the per-stage scalars. */
 int jcfd_ns2d_plan_create(const jcfd_ns2d_config* config, jcfd_ns2d_plan** plan);
 int jcfd_ns2d_plan_destroy(jcfd_ns2d_plan* plan);
 int64_t jcfd_ns2d_plan_device_bytes(const jcfd_ns2d_plan* plan);
@@ -278,7 +291,7 @@ typedef struct {
 int jcfd_ns2d_plan_get_tiles(const jcfd_ns2d_plan* plan, jcfd_ns2d_tiles* tiles);
 int jcfd_ns2d_plan_set_tiles(jcfd_ns2d_plan* plan, const jcfd_ns2d_tiles* tiles);
 
-/* nsteps CN-RK4 steps of every instance; w_hat/w_out complex64 of shape
+/* nsteps steps (of the plan's tableau) of every instance; w_hat/w_out complex64 of shape
  * (batch, n, n/2 + 1), or (n, n/2 + 1) when batch == 1.  w_out may alias
  * w_hat.  funcutils.repeated(step_fn, nsteps) (funcutils.py:82-88). */
 int jcfd_ns2d_step(jcfd_ns2d_plan* plan, const jcfd_tensor* w_hat, jcfd_tensor* w_out,

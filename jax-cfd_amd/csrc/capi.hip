@@ -328,7 +328,8 @@ struct jcfd_ns2d_plan {
   jcfd_ns2d_config cfg;
   int device = 0;
   int n = 0, batch = 1;
-  NsParams stages[5];
+  NsParams stages[JCFD_MAX_RK_STAGES];
+  int n_stages = 0;
   float* kx = nullptr;
   float* ky = nullptr;
   float2* tw = nullptr;
@@ -1274,11 +1275,12 @@ namespace {
 // enabled) and moved back.
 int ns_run(jcfd_ns2d_plan* p, const float2* w_in, float2* w_out, const int* slots, int count,
            int64_t nsteps, hipStream_t s) {
-  NsParams stages[5];
-  for (int k = 0; k < 5; ++k) {
+  NsParams stages[JCFD_MAX_RK_STAGES];
+  for (int k = 0; k < p->n_stages; ++k) {
     stages[k] = p->stages[k];
     stages[k].slots = slots;
   }
+  const int ns = p->n_stages;
   NsBuffers b{};
   b.u_in = p->ws;
   b.u_out = p->ws;
@@ -1287,9 +1289,9 @@ int ns_run(jcfd_ns2d_plan* p, const float2* w_in, float2* w_out, const int* slot
   b.pbuf = p->pbuf;
   int rc;
   if ((rc = ns_launch_state_convert(p->n, true, w_in, p->ws, slots, count, s))) return rc;
-  if ((rc = ns_launch_step(p->n, stages, b, count, s))) return rc;
+  if ((rc = ns_launch_step(p->n, stages, ns, b, count, s))) return rc;
   if ((rc = run_steps(p->graphs, b.u_out, nullptr, slots, count, nsteps - 1, s,
-                      [&](hipStream_t gs) { return ns_launch_step(p->n, stages, b, count, gs); })))
+                      [&](hipStream_t gs) { return ns_launch_step(p->n, stages, ns, b, count, gs); })))
     return rc;
   return ns_launch_state_convert(p->n, false, p->ws, w_out, slots, count, s);
 }
@@ -1329,6 +1331,13 @@ int jcfd_ns2d_plan_create(const jcfd_ns2d_config* config, jcfd_ns2d_plan** plan)
   if (c.batch < 1) return fail(JCFD_ERR_INVALID_ARGUMENT, "batch must be >= 1");
   if (c.n_forcing < 0 || (c.n_forcing > 0 && (c.forcing_index == nullptr || c.forcing_value == nullptr)))
     return fail(JCFD_ERR_INVALID_ARGUMENT, "forcing entries missing");
+  if (c.rk_stages < 0 || c.rk_stages > JCFD_MAX_RK_STAGES)
+    return fail(JCFD_ERR_UNSUPPORTED, "low-storage Runge-Kutta tableaux of 1 to " +
+                                          std::to_string(JCFD_MAX_RK_STAGES) + " stages are built");
+  for (int k = 0; k < c.rk_stages; ++k)
+    if (!std::isfinite(c.rk_alpha[k]) || !std::isfinite(c.rk_alpha[k + 1]) || !std::isfinite(c.rk_beta[k]) ||
+        !std::isfinite(c.rk_gamma[k]))
+      return fail(JCFD_ERR_INVALID_ARGUMENT, "Runge-Kutta coefficients must be finite");
   const int n = c.n, h1 = n / 2 + 1;
   for (int t = 0; t < c.n_forcing; ++t) {
     const int fp = c.forcing_index[2 * t], fc = c.forcing_index[2 * t + 1];
@@ -1385,7 +1394,12 @@ int jcfd_ns2d_plan_create(const jcfd_ns2d_config* config, jcfd_ns2d_plan** plan)
   // 2/3 rule bounds with Python int / floor-division semantics (spectral/utils.py:128-134)
   const int a = static_cast<int>(2.0 / 3.0 * n);
   const int cols = static_cast<int>(2.0 / 3.0 * h1);
-  for (int k = 0; k < 5; ++k) {
+  const bool cnrk4 = c.rk_stages == 0;
+  p->n_stages = cnrk4 ? 5 : c.rk_stages;
+  const double* alpha = cnrk4 ? kCnAlpha : c.rk_alpha;
+  const double* beta = cnrk4 ? kCnBeta : c.rk_beta;
+  const double* gamma = cnrk4 ? kCnGamma : c.rk_gamma;
+  for (int k = 0; k < p->n_stages; ++k) {
     NsParams& q = p->stages[k];
     q.kx = p->kx;
     q.ky = p->ky;
@@ -1402,9 +1416,9 @@ int jcfd_ns2d_plan_create(const jcfd_ns2d_config* config, jcfd_ns2d_plan** plan)
     q.filt_row_lo = a / 2;
     q.filt_row_hi = n - (a + 1) / 2;
     q.filt_cols = cols;
-    q.beta = static_cast<float>(kCnBeta[k]);
-    q.g = static_cast<float>(kCnGamma[k] * c.dt);
-    q.mu = static_cast<float>(0.5 * c.dt * (kCnAlpha[k + 1] - kCnAlpha[k]));
+    q.beta = static_cast<float>(beta[k]);
+    q.g = static_cast<float>(gamma[k] * c.dt);
+    q.mu = static_cast<float>(0.5 * c.dt * (alpha[k + 1] - alpha[k]));
     q.read_h = k > 0 ? 1 : 0;
     q.slots = nullptr;
     (void)ns_default_tiles(&q, n);
@@ -1512,10 +1526,10 @@ int jcfd_ns2d_time_kernels(jcfd_ns2d_plan* p, const jcfd_tensor* w_hat, int32_t 
   rc = ns_launch_state_convert(p->n, true, static_cast<const float2*>(pin), p->ws, nullptr, p->batch, s);
   if (!rc) rc = check_hip(hipEventCreate(&e0), "event");
   if (!rc) rc = check_hip(hipEventCreate(&e1), "event");
-  if (!rc) rc = ns_launch_step(p->n, p->stages, b, p->batch, s);  // fills every scratch buffer
+  if (!rc) rc = ns_launch_step(p->n, p->stages, p->n_stages, b, p->batch, s);  // fills every scratch buffer
   for (int k = 0; k < JCFD_NS2D_KERNEL_COUNT && rc == JCFD_OK; ++k) {
     rc = check_hip(hipEventRecord(e0, s), "event record");
-    for (int r = 0; r < reps && rc == JCFD_OK; ++r) rc = ns_launch_kernel(p->n, k, p->stages, b, p->batch, s);
+    for (int r = 0; r < reps && rc == JCFD_OK; ++r) rc = ns_launch_kernel(p->n, k, p->stages, p->n_stages, b, p->batch, s);
     if (!rc) rc = check_hip(hipEventRecord(e1, s), "event record");
     if (!rc) rc = check_hip(hipEventSynchronize(e1), "event sync");
     float ms = 0.f;

@@ -173,7 +173,7 @@ int fvm_group_launch(int phase, int max_n, const FvmGroupQueue& q, int grid, hip
 
 // Device-side parameters of a pseudospectral stage (spectral.hip).  The
 // wavenumber tables and multipliers follow the reference's fp32 values;
-// beta, g = gamma dt and mu are the stage's CN-RK4 scalars.
+// beta, g = gamma dt and mu are the stage's low-storage CN-RK scalars.
 struct NsParams {
   const float* kx;        // fftfreq(n, dx), n values
   const float* ky;        // rfftfreq(n, dy), n/2+1 values
@@ -215,8 +215,10 @@ bool ns_tile_supported(int n, int which, int tile);
 // `count` instances (slots[i], or i) between the two.
 int ns_launch_state_convert(int n, bool to_internal, const float2* src, float2* dst, const int* slots,
                             int count, hipStream_t s);
-// One CN-RK4 step; b.u_in / b.u_out / b.h in the internal state layout.
-int ns_launch_step(int n, const NsParams* stages, const NsBuffers& b, int batch, hipStream_t s);
+// One step of an nstages-stage low-storage CN-RK tableau (stages[k] holds
+// stage k's scalars); b.u_in / b.u_out / b.h in the internal state layout.
+int ns_launch_step(int n, const NsParams* stages, int nstages, const NsBuffers& b, int batch,
+                   hipStream_t s);
 // Explicit terms of w (internal state layout) into out (external layout).
 int ns_launch_explicit_terms(int n, const NsParams& q, const float2* w, float2* out, float2* g,
                              float2* pbuf, int batch, hipStream_t s);
@@ -224,8 +226,8 @@ int ns_launch_rfft2(int n, const NsParams& q, const float* x, float2* out, float
                     hipStream_t s);
 // One kernel class of the step: 0 cols-inverse, 1 rows, 2 cols-forward with
 // update + next inverse, 3 cols-forward with the final update.
-int ns_launch_kernel(int n, int which, const NsParams* stages, const NsBuffers& b, int batch,
-                     hipStream_t s);
+int ns_launch_kernel(int n, int which, const NsParams* stages, int nstages, const NsBuffers& b,
+                     int batch, hipStream_t s);
 int ns_launch_irfft2(int n, const NsParams& q, const float2* in, float* x, float2* pbuf, int batch,
                      hipStream_t s);
 

@@ -1,8 +1,9 @@
-// Pseudospectral vorticity step (NavierStokes2D + crank_nicolson_rk4) for gfx950.
+// Pseudospectral vorticity step (NavierStokes2D + a low-storage Runge-Kutta /
+// Crank-Nicolson stepper: crank_nicolson_rk4, _rk3 or any tableau) for gfx950.
 //
 // State: the complex64 rfft2 of the vorticity, (n, n/2+1) per instance
-// (jax_cfd/spectral/equations.py:129-190).  One CN-RK4 step
-// (spectral/time_stepping.py:117-170, 186-197) is five stages of
+// (jax_cfd/spectral/equations.py:129-190).  One step
+// (spectral/time_stepping.py:117-170; CN-RK4, 186-197, has five) is stages of
 //   F = explicit_terms(u);  h = F + beta_k h;
 //   u = (u + gamma_k dt h + mu_k L u) / (1 - mu_k L),  L = nu lap - drag,
 // where explicit_terms = filter * rfftn(-(d_x w * v_x + d_y w * v_y)) + forcing
@@ -685,13 +686,13 @@ int ns_launch(int which, const NsParams& q, const float2* a, const float2* u_in,
 }
 
 template <int N>
-int ns_step_impl(const NsParams* stages, const NsBuffers& b, int batch, hipStream_t s) {
+int ns_step_impl(const NsParams* stages, int nstages, const NsBuffers& b, int batch, hipStream_t s) {
   int rc = ns_launch<N>(0, stages[0], b.u_in, nullptr, nullptr, nullptr, b.g, batch, s);
-  for (int k = 0; k < 5 && rc == JCFD_OK; ++k) {
+  for (int k = 0; k < nstages && rc == JCFD_OK; ++k) {
     rc = ns_launch<N>(1, stages[k], b.g, nullptr, nullptr, nullptr, b.pbuf, batch, s);
     if (rc) break;
     const float2* u_src = (k == 0) ? b.u_in : b.u_out;
-    rc = ns_launch<N>(k < 4 ? 2 : 3, stages[k], b.pbuf, u_src, b.u_out, b.h, b.g, batch, s);
+    rc = ns_launch<N>(k + 1 < nstages ? 2 : 3, stages[k], b.pbuf, u_src, b.u_out, b.h, b.g, batch, s);
   }
   return rc;
 }
@@ -723,12 +724,14 @@ bool ns_size_supported(int n) { return n == 32 || n == 64 || n == 128 || n == 25
 
 namespace {
 template <int N>
-int ns_one_impl(int which, const NsParams* stages, const NsBuffers& b, int batch, hipStream_t s) {
+int ns_one_impl(int which, const NsParams* stages, int nstages, const NsBuffers& b, int batch,
+                hipStream_t s) {
   switch (which) {
     case 0: return ns_launch<N>(0, stages[0], b.u_in, nullptr, nullptr, nullptr, b.g, batch, s);
     case 1: return ns_launch<N>(1, stages[0], b.g, nullptr, nullptr, nullptr, b.pbuf, batch, s);
-    case 2: return ns_launch<N>(2, stages[1], b.pbuf, b.u_out, b.u_out, b.h, b.g, batch, s);
-    default: return ns_launch<N>(3, stages[4], b.pbuf, b.u_out, b.u_out, b.h, b.g, batch, s);
+    case 2: return ns_launch<N>(2, stages[nstages > 1 ? 1 : 0], b.pbuf, b.u_out, b.u_out, b.h, b.g,
+                                batch, s);
+    default: return ns_launch<N>(3, stages[nstages - 1], b.pbuf, b.u_out, b.u_out, b.h, b.g, batch, s);
   }
 }
 
@@ -763,25 +766,28 @@ bool ns_tile_supported(int n, int which, int tile) {
   }
 }
 
-int ns_launch_kernel(int n, int which, const NsParams* stages, const NsBuffers& b, int batch,
-                     hipStream_t s) {
+int ns_launch_kernel(int n, int which, const NsParams* stages, int nstages, const NsBuffers& b,
+                     int batch, hipStream_t s) {
+  if (nstages < 1) return fail(JCFD_ERR_INVALID_ARGUMENT, "at least one stage");
   switch (n) {
-    case 32: return ns_one_impl<32>(which, stages, b, batch, s);
-    case 64: return ns_one_impl<64>(which, stages, b, batch, s);
-    case 128: return ns_one_impl<128>(which, stages, b, batch, s);
-    case 256: return ns_one_impl<256>(which, stages, b, batch, s);
-    case 512: return ns_one_impl<512>(which, stages, b, batch, s);
+    case 32: return ns_one_impl<32>(which, stages, nstages, b, batch, s);
+    case 64: return ns_one_impl<64>(which, stages, nstages, b, batch, s);
+    case 128: return ns_one_impl<128>(which, stages, nstages, b, batch, s);
+    case 256: return ns_one_impl<256>(which, stages, nstages, b, batch, s);
+    case 512: return ns_one_impl<512>(which, stages, nstages, b, batch, s);
     default: return fail(JCFD_ERR_UNSUPPORTED, "spectral grid size not built");
   }
 }
 
-int ns_launch_step(int n, const NsParams* stages, const NsBuffers& b, int batch, hipStream_t s) {
+int ns_launch_step(int n, const NsParams* stages, int nstages, const NsBuffers& b, int batch,
+                   hipStream_t s) {
+  if (nstages < 1) return fail(JCFD_ERR_INVALID_ARGUMENT, "at least one stage");
   switch (n) {
-    case 32: return ns_step_impl<32>(stages, b, batch, s);
-    case 64: return ns_step_impl<64>(stages, b, batch, s);
-    case 128: return ns_step_impl<128>(stages, b, batch, s);
-    case 256: return ns_step_impl<256>(stages, b, batch, s);
-    case 512: return ns_step_impl<512>(stages, b, batch, s);
+    case 32: return ns_step_impl<32>(stages, nstages, b, batch, s);
+    case 64: return ns_step_impl<64>(stages, nstages, b, batch, s);
+    case 128: return ns_step_impl<128>(stages, nstages, b, batch, s);
+    case 256: return ns_step_impl<256>(stages, nstages, b, batch, s);
+    case 512: return ns_step_impl<512>(stages, nstages, b, batch, s);
     default: return fail(JCFD_ERR_UNSUPPORTED, "spectral grid size not built");
   }
 }

@@ -30,6 +30,7 @@ PRESSURE_FAST_DIAG = 0
 PRESSURE_CG = 1
 DIFFUSION_EXPLICIT = 0
 DIFFUSION_IMPLICIT_FAST_DIAG = 1
+MAX_RK_STAGES = 8  # JCFD_MAX_RK_STAGES
 
 
 class BudgetExceededError(RuntimeError):
@@ -74,6 +75,10 @@ class Ns2dConfig(ctypes.Structure):
       ("n_forcing", ctypes.c_int32),
       ("forcing_index", ctypes.POINTER(ctypes.c_int32)),
       ("forcing_value", ctypes.POINTER(ctypes.c_float)),
+      ("rk_stages", ctypes.c_int32),
+      ("rk_alpha", ctypes.c_double * (MAX_RK_STAGES + 1)),
+      ("rk_beta", ctypes.c_double * MAX_RK_STAGES),
+      ("rk_gamma", ctypes.c_double * MAX_RK_STAGES),
   ]
 
 

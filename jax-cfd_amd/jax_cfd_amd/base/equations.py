@@ -13,6 +13,7 @@ import functools
 from typing import Callable, Optional
 
 import numpy as np
+import torch
 
 from jax_cfd_amd import _lib, _symbolic
 from jax_cfd_amd.base import (_fvm, advection, boundaries, diffusion, grids,
@@ -119,7 +120,6 @@ class FvmStepFn:
     self._plans = {}
 
   def plan(self, device=None) -> _fvm.FvmPlan:
-    import torch
     if device is None:
       device = torch.device('cuda', torch.cuda.current_device())
     key = device.index if device.index is not None else 0
@@ -127,7 +127,7 @@ class FvmStepFn:
       self._plans[key] = _fvm.FvmPlan(self.spec, torch.device('cuda', key))
     return self._plans[key]
 
-  def _run(self, state, nsteps: int):
+  def _checked(self, state):
     if len(state) != 2:
       raise ValueError('FVM state must be a tuple of two GridVariables')
     u, v = state
@@ -138,10 +138,17 @@ class FvmStepFn:
       raise NotImplementedError('non-periodic boundary conditions')
     data = u.data
     dev = data.device if getattr(data, 'is_cuda', False) else None
-    plan = self.plan(dev)
-    uo, vo = plan.step(u.data, v.data, nsteps)
+    return u, v, self.plan(dev)
+
+  def _wrap(self, state, uo, vo):
+    u, v = state
     return (grids.GridVariable(grids.GridArray(uo, u.offset, self.grid), u.bc),
             grids.GridVariable(grids.GridArray(vo, v.offset, self.grid), v.bc))
+
+  def _run(self, state, nsteps: int):
+    u, v, plan = self._checked(state)
+    uo, vo = plan.step(u.data, v.data, nsteps)
+    return self._wrap(state, uo, vo)
 
   def __call__(self, state):
     return self._run(state, 1)
@@ -149,6 +156,37 @@ class FvmStepFn:
   def multistep(self, state, steps: int):
     """funcutils.repeated(self, steps)(state) as one library call."""
     return self._run(state, int(steps))
+
+  def rollout_frames(self, state, inner: int, count: int,
+                     start_with_input: bool = False):
+    """funcutils.trajectory(repeated(self, inner), count) on the device.
+
+    Frames live in one (count, n, n) tensor per component; each library
+    call reads the previous frame and its last kernel writes the next frame
+    in place, so frames cost no copies.  Returns (final state, frames).
+    """
+    u, v, plan = self._checked(state)
+    inner, count = int(inner), int(count)
+    if inner < 0 or count < 0:
+      raise ValueError('step and frame counts must be >= 0')
+    uc, vc = plan._field(u.data), plan._field(v.data)
+    fu = torch.empty((count,) + tuple(uc.shape), dtype=uc.dtype, device=uc.device)
+    fv = torch.empty_like(fu)
+    if count == 0:
+      final = (uc.clone(), vc.clone())
+    elif start_with_input:
+      fu[0].copy_(uc)
+      fv[0].copy_(vc)
+      for t in range(1, count):
+        plan.step(fu[t - 1], fv[t - 1], inner, out=(fu[t], fv[t]))
+      final = plan.step(fu[count - 1], fv[count - 1], inner)
+    else:
+      prev = (uc, vc)
+      for t in range(count):
+        plan.step(prev[0], prev[1], inner, out=(fu[t], fv[t]))
+        prev = (fu[t], fv[t])
+      final = (fu[count - 1].clone(), fv[count - 1].clone())
+    return self._wrap(state, *final), self._wrap(state, fu, fv)
 
 
 def semi_implicit_navier_stokes(

@@ -1,13 +1,17 @@
 """Rollout drivers (jax_cfd/base/funcutils.py:82-126).
 
 ``repeated`` hands a GPU step function the whole step count in one library
-call; ``trajectory`` stacks post-processed frames along a new leading axis,
-as ``jax.lax.scan`` stacks pytree leaves.
+call.  ``trajectory`` stacks post-processed frames along a new leading axis,
+as ``jax.lax.scan`` stacks pytree leaves, into one tensor per leaf that is
+allocated once for the whole rollout.  For a GPU step function with the
+identity ``post_process`` the step kernels write each frame straight into its
+slot of that tensor (``rollout_frames``), so a frame costs no copy.
 """
 from __future__ import annotations
 
 from typing import Callable
 
+import numpy as np
 import torch
 
 from jax_cfd_amd.base import grids
@@ -20,6 +24,7 @@ def _identity(x):
 def repeated(f: Callable, steps: int) -> Callable:
   """Returns a repeatedly applied version of f() (funcutils.py:82-88)."""
   multistep = getattr(f, 'multistep', None)
+  rollout = getattr(f, 'rollout_frames', None)
 
   def f_repeated(x_initial):
     if multistep is not None:
@@ -30,34 +35,76 @@ def repeated(f: Callable, steps: int) -> Callable:
     return x
   f_repeated.multistep = (
       (lambda x, k: multistep(x, k * steps)) if multistep is not None else None)
+  f_repeated.rollout_frames = (
+      (lambda x, inner, count, start_with_input:
+       rollout(x, inner * steps, count, start_with_input))
+      if rollout is not None else None)
   return f_repeated
 
 
-def _stack(frames):
-  first = frames[0]
-  if isinstance(first, grids.GridVariable):
-    arr = _stack([fr.array for fr in frames])
-    return grids.GridVariable(arr, first.bc)
-  if isinstance(first, grids.GridArray):
-    return grids.GridArray(torch.stack([fr.data for fr in frames]), first.offset,
-                           first.grid)
-  if isinstance(first, (tuple, list)):
-    return type(first)(_stack([fr[i] for fr in frames]) for i in range(len(first)))
-  if isinstance(first, torch.Tensor):
-    return torch.stack(frames)
-  raise TypeError(f'cannot stack frames of type {type(first)}')
+def _allocate(frame, count: int):
+  """Empty stacked storage for `count` frames shaped like `frame`."""
+  if isinstance(frame, grids.GridVariable):
+    return grids.GridVariable(_allocate(frame.array, count), frame.bc)
+  if isinstance(frame, grids.GridArray):
+    return grids.GridArray(_allocate(frame.data, count), frame.offset, frame.grid)
+  if isinstance(frame, (tuple, list)):
+    return type(frame)(_allocate(x, count) for x in frame)
+  if isinstance(frame, torch.Tensor):
+    return torch.empty((count,) + tuple(frame.shape), dtype=frame.dtype,
+                       device=frame.device)
+  if isinstance(frame, (np.ndarray, np.generic, float, int)):
+    a = np.asarray(frame)
+    return np.empty((count,) + a.shape, dtype=a.dtype)
+  raise TypeError(f'cannot stack frames of type {type(frame)}')
+
+
+def _write(frames, index: int, frame):
+  """frames[index] = frame, leaf by leaf."""
+  if isinstance(frames, grids.GridVariable):
+    _write(frames.array, index, frame.array)
+  elif isinstance(frames, grids.GridArray):
+    if tuple(frame.offset) != tuple(frames.offset):
+      raise grids.InconsistentOffsetError(
+          f'frame offset {frame.offset} != {frames.offset}')
+    _write(frames.data, index, frame.data)
+  elif isinstance(frames, (tuple, list)):
+    if len(frames) != len(frame):
+      raise ValueError('frames change structure along the trajectory')
+    for f, x in zip(frames, frame):
+      _write(f, index, x)
+  elif isinstance(frames, torch.Tensor):
+    frames[index].copy_(frame)
+  else:
+    frames[index] = np.asarray(frame)
 
 
 def trajectory(step_fn: Callable, steps: int, post_process: Callable = _identity,
                *, start_with_input: bool = False) -> Callable:
-  """funcutils.py:95-126: returns (final state, stacked frames)."""
+  """funcutils.py:95-126: returns (final state, stacked frames).
+
+  Frame t is post_process(state after t+1 steps), or after t steps when
+  `start_with_input`; the final state is the one after `steps` steps, before
+  post_process.  `steps == 0` gives empty frames and the input state.
+  """
+  steps = int(steps)
+  if steps < 0:
+    raise ValueError(f'steps must be >= 0, got {steps}')
+  rollout = getattr(step_fn, 'rollout_frames', None)
 
   def multistep(values):
-    frames = []
+    if rollout is not None and post_process is _identity:
+      return rollout(values, 1, steps, start_with_input)
+    if steps == 0:
+      return values, _allocate(post_process(values), 0)
+    frames = None
     state = values
-    for _ in range(steps):
+    for t in range(steps):
       new_state = step_fn(state)
-      frames.append(post_process(state if start_with_input else new_state))
+      frame = post_process(state if start_with_input else new_state)
+      if frames is None:
+        frames = _allocate(frame, steps)
+      _write(frames, t, frame)
       state = new_state
-    return state, _stack(frames)
+    return state, frames
   return multistep

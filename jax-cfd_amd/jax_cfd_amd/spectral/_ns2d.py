@@ -22,6 +22,12 @@ class Ns2dSpec:
   dt: float
   # sparse constant forcing curl: ((p, c, re, im), ...)
   forcing: Tuple[Tuple[int, int, float, float], ...] = ()
+  # low-storage CN-RK tableau (alphas, betas, gammas); None: crank_nicolson_rk4
+  tableau: Optional[Tuple[Tuple[float, ...], Tuple[float, ...], Tuple[float, ...]]] = None
+
+  @property
+  def stages(self) -> int:
+    return 5 if self.tableau is None else len(self.tableau[1])
 
 
 class Ns2dPlan:
@@ -48,6 +54,17 @@ class Ns2dPlan:
         np.array([(re, im) for _, _, re, im in spec.forcing] or [(0, 0)], np.float32))
     cfg.forcing_index = self._fi.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
     cfg.forcing_value = self._fv.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+    if spec.tableau is not None:
+      alphas, betas, gammas = spec.tableau
+      cfg.rk_stages = len(betas)
+      if not (len(alphas) == len(betas) + 1 == len(gammas) + 1) or \
+          len(betas) > _lib.MAX_RK_STAGES:
+        raise ValueError('a tableau needs k + 1 alphas, k betas and k gammas, '
+                         f'k <= {_lib.MAX_RK_STAGES}')
+      for i, a in enumerate(alphas):
+        cfg.rk_alpha[i] = float(a)
+      for i, (b, g) in enumerate(zip(betas, gammas)):
+        cfg.rk_beta[i], cfg.rk_gamma[i] = float(b), float(g)
     handle = ctypes.c_void_p()
     with torch.cuda.device(self.device):
       _lib.check(lib.jcfd_ns2d_plan_create(ctypes.byref(cfg), ctypes.byref(handle)),
@@ -109,8 +126,12 @@ class Ns2dPlan:
     return o
 
   KERNELS = ('ns_cols_inverse', 'ns_rows', 'ns_cols_update_inverse', 'ns_cols_update')
-  LAUNCHES_PER_STEP = {'ns_cols_inverse': 1, 'ns_rows': 5, 'ns_cols_update_inverse': 4,
-                       'ns_cols_update': 1}
+
+  @property
+  def LAUNCHES_PER_STEP(self):  # pylint: disable=invalid-name
+    k = self.spec.stages
+    return {'ns_cols_inverse': 1, 'ns_rows': k, 'ns_cols_update_inverse': k - 1,
+            'ns_cols_update': 1}
 
   def time_kernels(self, w, reps: int = 50):
     w = self._field(w)

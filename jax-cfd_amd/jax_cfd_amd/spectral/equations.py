@@ -82,19 +82,21 @@ class NavierStokes2D:
                                                   self.grid)
     self._plans = {}
 
-  def spec(self, dt: float) -> _ns2d.Ns2dSpec:
+  def spec(self, dt: float, tableau=None) -> _ns2d.Ns2dSpec:
     n = self.grid.shape[0]
     return _ns2d.Ns2dSpec(n=n, dx=self.grid.step[0], dy=self.grid.step[1],
                           viscosity=float(self.viscosity), drag=float(self.drag),
                           smooth=self.smooth is not None, dt=float(dt),
-                          forcing=self._forcing_entries)
+                          forcing=self._forcing_entries, tableau=tableau)
 
-  def plan(self, dt: float, batch: int = 1, device=None) -> _ns2d.Ns2dPlan:
+  def plan(self, dt: float, batch: int = 1, device=None, tableau=None) -> _ns2d.Ns2dPlan:
+    """The GPU plan of `batch` instances stepped with the low-storage CN-RK
+    `tableau` ((alphas, betas, gammas); None: crank_nicolson_rk4)."""
     if device is None:
       device = torch.device('cuda', torch.cuda.current_device())
-    key = (float(dt), int(batch), device.index or 0)
+    key = (float(dt), int(batch), device.index or 0, tableau)
     if key not in self._plans:
-      self._plans[key] = _ns2d.Ns2dPlan(self.spec(dt), batch, device)
+      self._plans[key] = _ns2d.Ns2dPlan(self.spec(dt, tableau), batch, device)
     return self._plans[key]
 
   def explicit_terms(self, vorticity_hat):

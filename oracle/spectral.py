@@ -3,13 +3,14 @@
 TEST INFRASTRUCTURE ONLY (see oracle/__init__.py).
 
 ``NavierStokes2D`` mirrors jax_cfd/spectral/equations.py:129-190 (and
-``ForcedNavierStokes2D`` 194-220); ``crank_nicolson_rk4`` mirrors
-spectral/time_stepping.py:117-170, 186-197; ``brick_wall_filter_2d`` and
+``ForcedNavierStokes2D`` 194-220); ``crank_nicolson_rk4`` and
+``crank_nicolson_rk3`` mirror spectral/time_stepping.py:117-170, 173-197; ``brick_wall_filter_2d`` and
 ``vorticity_to_velocity`` mirror spectral/utils.py:128-134, 154-189.  The
 state is the complex64 rfft2 of the vorticity, shape (n, n//2 + 1).
 
-Pinning (tests/test_oracle_spectral.py): CN-RK4 against closed-form linear
-ODE solutions (spectral/time_stepping_test.py:42-187 pattern); the
+Pinning (tests/test_oracle_spectral.py): CN-RK4 and CN-RK3 against
+closed-form linear ODE solutions with the reference's per-method tolerances
+(spectral/time_stepping_test.py:42-187); the
 Taylor-Green vortex decay in closed form (spectral/equations_test.py:263-303
 pattern); the forcing curl closed form (-2 n^2 at [0, 4] for sin(4y));
 the 2/3 filter index sets counted by hand for n in {64, 128, 256, 512};
@@ -32,6 +33,10 @@ CNRK4_BETAS = (0, -0.4178904745, -1.192151694643, -1.697784692471,
                -1.514183444257)
 CNRK4_GAMMAS = (0.1496590219993, 0.3792103129999, 0.8229550293869,
                 0.6994504559488, 0.1530572479681)
+# Williamson coefficients, spectral/time_stepping.py:178-180.
+CNRK3_ALPHAS = (0, 1 / 3, 3 / 4, 1)
+CNRK3_BETAS = (0, -5 / 9, -153 / 128)
+CNRK3_GAMMAS = (1 / 3, 15 / 16, 8 / 15)
 
 
 def brick_wall_filter_2d(grid: Grid, dtype=np.float32) -> np.ndarray:
@@ -144,6 +149,12 @@ def low_storage_rk_crank_nicolson(alphas, betas, gammas, equation, time_step):
 def crank_nicolson_rk4(equation, time_step):
   """spectral/time_stepping.py:186-197."""
   return low_storage_rk_crank_nicolson(CNRK4_ALPHAS, CNRK4_BETAS, CNRK4_GAMMAS,
+                                       equation, time_step)
+
+
+def crank_nicolson_rk3(equation, time_step):
+  """spectral/time_stepping.py:173-183 (Williamson's coefficients)."""
+  return low_storage_rk_crank_nicolson(CNRK3_ALPHAS, CNRK3_BETAS, CNRK3_GAMMAS,
                                        equation, time_step)
 
 

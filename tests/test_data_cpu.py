@@ -1,0 +1,194 @@
+"""Trajectory stacking and the netCDF trajectory writer, on the CPU.
+
+The reference has no test of `velocity_trajectory_to_xarray`,
+`construct_coords` or `grid_from_attrs` (jax_cfd/data/xarray_utils.py:63-153);
+they are pinned here by what fixes them: cell-centre coordinates in closed
+form (lo + (i + 1/2) h), the offsets of the staggered faces, the branch rules
+of `grid_from_attrs`, and the netCDF-3 header layout read byte by byte
+(independently of the library that writes the file).  `trajectory`'s frame
+order is pinned against `jax.lax.scan` semantics (funcutils.py:95-126) with a
+step whose states are known in closed form (x -> x + 1).
+"""
+import struct
+
+import numpy as np
+import pytest
+import torch
+
+import jax_cfd_amd.base as cfd
+from jax_cfd_amd.data import xarray_utils as xu
+
+TWO_PI = 2 * np.pi
+
+
+def _grid(n):
+  return cfd.grids.Grid((n, n), domain=((0, TWO_PI), (0, TWO_PI)))
+
+
+def test_construct_coords_closed_form():
+  n = 16
+  coords = xu.construct_coords(_grid(n), np.linspace(0, 1, 5), np.arange(3))
+  h = np.float32(TWO_PI / n)
+  expect = (np.arange(n, dtype=np.float32) + np.float32(0.5)) * h
+  for dim in ('x', 'y'):
+    assert coords[dim].dtype == np.float32
+    np.testing.assert_array_equal(coords[dim], expect)
+  np.testing.assert_array_equal(coords['time'], np.linspace(0, 1, 5))
+  np.testing.assert_array_equal(coords['sample'], np.arange(3))
+  assert xu.construct_coords() == {}
+
+
+def test_velocity_dataset_layout():
+  n, t = 8, 3
+  g = _grid(n)
+  u = np.random.default_rng(0).random((t, n, n), np.float32)
+  v = u + 1
+  ds = xu.velocity_trajectory_to_xarray((u, v), g, time=np.arange(t) * 0.5)
+  assert list(ds.data_vars) == ['u', 'v']
+  assert ds['u'].dims == ('time', 'x', 'y')
+  assert ds['u'].attrs['offset'] == (1.0, 0.5)
+  assert ds['v'].attrs['offset'] == (0.5, 1.0)
+  assert ds.sizes == {'x': n, 'y': n, 'time': t}
+  np.testing.assert_array_equal(ds['v'].data, v)
+  assert 'sample' not in ds
+
+  # leading sample axis, prefixed names, a scalar component with its offset
+  c = cfd.grids.GridArray(torch.zeros(2, t, n, n), (0.5, 0.5), g)
+  ds = xu.velocity_trajectory_to_xarray(
+      (c, np.stack([u, u]), np.stack([v, v])), g, samples=True, prefix_name='p_')
+  assert list(ds.data_vars) == ['p_c', 'p_u', 'p_v']
+  assert ds['p_c'].attrs == {'offset': (0.5, 0.5)}
+  assert ds['p_u'].dims == ('sample', 'time', 'x', 'y')
+  np.testing.assert_array_equal(ds['sample'].data, [0, 1])
+  assert 'time' not in ds.coords
+
+  # without a grid: dimension = number of components, no offsets, no coords
+  ds = xu.velocity_trajectory_to_xarray((u, v))
+  assert ds['u'].attrs == {} and ds.coords == {}
+
+
+def test_dimension_mismatch_raises():
+  g = _grid(8)
+  with pytest.raises(ValueError):
+    xu.velocity_trajectory_to_xarray((np.zeros((2, 8, 8)), np.zeros((3, 8, 8))), g)
+
+
+def _netcdf_dimensions(raw: bytes):
+  """Dimension names and lengths from a netCDF-3 header (classic spec:
+  magic, numrecs, NC_DIMENSION tag 0x0A, count, then name/length pairs)."""
+  assert raw[:3] == b'CDF'
+  pos = 8
+  tag, count = struct.unpack('>ii', raw[pos:pos + 8])
+  pos += 8
+  assert tag == 10
+  dims = []
+  for _ in range(count):
+    (nlen,) = struct.unpack('>i', raw[pos:pos + 4])
+    pos += 4
+    name = raw[pos:pos + nlen].decode()
+    pos += (nlen + 3) // 4 * 4
+    (length,) = struct.unpack('>i', raw[pos:pos + 4])
+    pos += 4
+    dims.append((name, length))
+  return raw[3], dims
+
+
+def test_netcdf_round_trip(tmp_path):
+  n, t = 8, 4
+  g = _grid(n)
+  rng = np.random.default_rng(1)
+  u = rng.standard_normal((2, t, n, n)).astype(np.float32)
+  v = rng.standard_normal((2, t, n, n)).astype(np.float32)
+  attrs = {'ndim': 2, 'save_grid_size': n, 'domain_size': TWO_PI,
+           'stable_time_step': 0.0017531209004407328, 'smooth': True,
+           'equation': 'navier_stokes', 'viscosity': np.float32(1e-3)}
+  times = np.arange(t) * 0.0017531209004407328
+  ds = xu.velocity_trajectory_to_xarray((u, v), g, time=times, attrs=attrs,
+                                        samples=True)
+  path = tmp_path / 'traj.nc'
+  ds.to_netcdf(path)
+  raw = path.read_bytes()
+  version, dims = _netcdf_dimensions(raw)
+  assert version == 2  # 64-bit offset
+  assert dict(dims) == {'x': n, 'y': n, 'time': t, 'sample': 2}
+
+  back = xu.open_dataset(path)
+  for name in ('u', 'v'):
+    assert back[name].dims == ('sample', 'time', 'x', 'y')
+    assert back[name].data.dtype == np.float32
+    assert back[name].data.dtype.byteorder in '=|<'
+    np.testing.assert_array_equal(back[name].data, ds[name].data)
+  np.testing.assert_array_equal(back['u'].attrs['offset'], [1.0, 0.5])
+  np.testing.assert_array_equal(back['v'].attrs['offset'], [0.5, 1.0])
+  np.testing.assert_array_equal(back['x'].data, ds['x'].data)
+  np.testing.assert_array_equal(back['time'].data, times)
+  assert back['sample'].data.dtype == np.int32
+  np.testing.assert_array_equal(back['sample'].data, [0, 1])
+  assert back.attrs['equation'] == 'navier_stokes'
+  assert back.attrs['domain_size'] == TWO_PI            # stored as float64
+  assert back.attrs['stable_time_step'] == 0.0017531209004407328
+  assert back.attrs['viscosity'] == np.float32(1e-3)    # float32 stays float32
+  assert int(back.attrs['smooth']) == 1 and int(back.attrs['ndim']) == 2
+  grid = xu.grid_from_attrs(back.attrs)
+  assert grid.shape == (n, n) and grid.domain == ((0, TWO_PI), (0, TWO_PI))
+
+
+def test_netcdf_rejects_unstorable(tmp_path):
+  ds = xu.Dataset({'z': xu.Variable(('x',), np.zeros(3, np.complex64))})
+  with pytest.raises(TypeError):
+    ds.to_netcdf(tmp_path / 'bad.nc')
+  ds = xu.Dataset({'i': xu.Variable(('x',), np.array([2 ** 40], np.int64))})
+  with pytest.raises(ValueError):
+    ds.to_netcdf(tmp_path / 'bad2.nc')
+
+
+def test_grid_from_attrs_branches():
+  g = xu.grid_from_attrs({'ndim': 2, 'save_grid_size': 32, 'domain_size': 3.0})
+  assert g.shape == (32, 32) and g.domain == ((0, 3.0), (0, 3.0))
+  g = xu.grid_from_attrs({'ndim': 2, 'save_grid_size': 16,
+                          'domain_size_multiple': 2})
+  assert g.domain == ((0, 4 * np.pi), (0, 4 * np.pi))
+  g = xu.grid_from_attrs({'ndim': 2, 'save_grid_size_x': 64,
+                          'save_grid_size_y': 16, 'aspect_ratio': 4.0})
+  assert g.shape == (64, 16) and g.domain == ((0, 4.0), (0, 1))
+  assert g.step == (4.0 / 64, 1.0 / 16)
+  with pytest.raises(ValueError):
+    xu.grid_from_attrs({'ndim': 2, 'save_grid_size': 16})
+
+
+def _count_step(x):
+  return x + 1
+
+
+@pytest.mark.parametrize('start_with_input', [False, True])
+def test_trajectory_frame_order(start_with_input):
+  x0 = torch.zeros(3)
+  final, frames = cfd.funcutils.trajectory(
+      _count_step, 4, start_with_input=start_with_input)(x0)
+  first = 0 if start_with_input else 1
+  expect = torch.arange(first, first + 4, dtype=torch.float32)[:, None].expand(4, 3)
+  assert torch.equal(frames, expect)
+  assert torch.equal(final, torch.full((3,), 4.0))
+
+
+def test_trajectory_post_process_structure_and_empty():
+  g = _grid(4)
+  bc = cfd.boundaries.periodic_boundary_conditions(2)
+
+  def step(state):
+    return tuple(cfd.grids.GridVariable(
+        cfd.grids.GridArray(c.data + 1, c.offset, g), c.bc) for c in state)
+
+  v0 = tuple(cfd.grids.GridVariable(cfd.grids.GridArray(torch.zeros(4, 4), o, g), bc)
+             for o in g.cell_faces)
+  post = lambda v: (v[0].array, v[1].data.sum())  # noqa: E731
+  final, (fu, fs) = cfd.funcutils.trajectory(step, 3, post)(v0)
+  assert fu.offset == (1.0, 0.5) and fu.data.shape == (3, 4, 4)
+  assert torch.equal(fu.data[:, 0, 0], torch.tensor([1.0, 2.0, 3.0]))
+  assert torch.equal(fs, torch.tensor([16.0, 32.0, 48.0]))
+  assert torch.equal(final[1].data, torch.full((4, 4), 3.0))
+  final, (fu, fs) = cfd.funcutils.trajectory(step, 0, post)(v0)
+  assert fu.data.shape == (0, 4, 4) and fs.shape == (0,)
+  assert final is v0
+  with pytest.raises(ValueError):
+    cfd.funcutils.trajectory(step, -1)

@@ -61,6 +61,45 @@ def test_cnrk4_closed_forms(name, f, g, ginv, x0, closed, tol):
     np.testing.assert_allclose(x, closed(x0, t), atol=tol, rtol=0)
 
 
+# the crank_nicolson_rk3 column of the same table (time_stepping_test.py:42-122,
+# tolerances index 2 of ALL_TIME_STEPPERS, 124-130)
+CNRK3_TOL = {"zero": 1e-12, "constant": 1e-12, "linear_explicit": 1e-6,
+             "linear_implicit": 1e-5, "linear_semi_implicit": 2e-6,
+             "harmonic_explicit": 6e-8, "harmonic_implicit": 2e-6}
+
+
+def _closed_form_errors(step, x0, closed, name, dt=1e-2):
+  x = x0
+  inner = 10 if name in ("zero", "constant") else 20
+  errs = []
+  for outer in range(5):
+    for _ in range(inner):
+      x = step(x)
+    errs.append(float(np.abs(x - closed(x0, dt * inner * (outer + 1))).max()))
+  return max(errs)
+
+
+@pytest.mark.parametrize("name,f,g,ginv,x0,closed,tol", CNRK4_PROBLEMS,
+                         ids=[p[0] for p in CNRK4_PROBLEMS])
+def test_cnrk3_closed_forms(name, f, g, ginv, x0, closed, tol):
+  step = sp.crank_nicolson_rk3(_ODE(f, g, ginv), 1e-2)
+  assert _closed_form_errors(step, x0, closed, name) <= CNRK3_TOL[name]
+
+
+def test_cnrk3_closed_form_detects_wrong_coefficients():
+  """A tableau off by 1e-3 in one coefficient fails the harmonic-oscillator
+  bound that the Williamson coefficients meet."""
+  name, f, g, ginv, x0, closed, _ = CNRK4_PROBLEMS[5]
+  ode = _ODE(f, g, ginv)
+  good = sp.low_storage_rk_crank_nicolson(sp.CNRK3_ALPHAS, sp.CNRK3_BETAS,
+                                          sp.CNRK3_GAMMAS, ode, 1e-2)
+  bad = sp.low_storage_rk_crank_nicolson(
+      sp.CNRK3_ALPHAS, sp.CNRK3_BETAS[:2] + (sp.CNRK3_BETAS[2] + 1e-3,),
+      sp.CNRK3_GAMMAS, ode, 1e-2)
+  assert _closed_form_errors(good, x0, closed, name) <= CNRK3_TOL[name]
+  assert _closed_form_errors(bad, x0, closed, name) > 10 * CNRK3_TOL[name]
+
+
 @pytest.mark.parametrize("n,rows_pos,rows_neg,cols", [
     (64, 21, 21, 22), (128, 42, 43, 43), (256, 85, 85, 86),
     (512, 170, 171, 171)])
