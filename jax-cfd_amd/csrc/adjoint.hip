@@ -98,10 +98,10 @@ template <int N>
 __device__ __forceinline__ float flux_bar(int t, int a, int b, const float* lu, const float* lv,
                                           const FvmParams& p) {
   switch (t) {
-    case 0: return (at<N>(lu, a, b) - at<N>(lu, a - 1, b)) / p.dx;
-    case 1: return (at<N>(lu, a - 1, b + 1) - at<N>(lu, a - 1, b)) / p.dy;
-    case 2: return (at<N>(lv, a + 1, b - 1) - at<N>(lv, a, b - 1)) / p.dx;
-    default: return (at<N>(lv, a, b) - at<N>(lv, a, b - 1)) / p.dy;
+    case 0: return over_step(at<N>(lu, a, b) - at<N>(lu, a - 1, b), p.dx, p.rcp_dx);
+    case 1: return over_step(at<N>(lu, a - 1, b + 1) - at<N>(lu, a - 1, b), p.dy, p.rcp_dy);
+    case 2: return over_step(at<N>(lv, a + 1, b - 1) - at<N>(lv, a, b - 1), p.dx, p.rcp_dx);
+    default: return over_step(at<N>(lv, a, b) - at<N>(lv, a, b - 1), p.dy, p.rcp_dy);
   }
 }
 
@@ -169,8 +169,8 @@ __global__ __launch_bounds__(kThreads) void fvm_curl_vjp(FvmParams p, const floa
   const int e = blockIdx.x * kThreads + threadIdx.x;
   if (e >= N * N) return;
   const int i = e / N, j = e % N;
-  ub[e] = (wb[e] - at<N>(wb, i, j - 1)) / p.dy;
-  vb[e] = (at<N>(wb, i - 1, j) - wb[e]) / p.dx;
+  ub[e] = over_step(wb[e] - at<N>(wb, i, j - 1), p.dy, p.rcp_dy);
+  vb[e] = over_step(at<N>(wb, i - 1, j) - wb[e], p.dx, p.rcp_dx);
 }
 
 template <int N>

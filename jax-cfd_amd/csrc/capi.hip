@@ -551,6 +551,8 @@ int jcfd_fvm_plan_create(const jcfd_fvm_config* config, jcfd_fvm_plan** plan) {
   const float ix = 1.0f / static_cast<float>(c.dx), iy = 1.0f / static_cast<float>(c.dy);
   q.inv_dx_sq = ix * ix;
   q.inv_dy_sq = iy * iy;
+  q.rcp_dx = ix;  // x / dx as over_step(x, dx, rcp_dx)
+  q.rcp_dy = iy;
   q.lap_sum = q.inv_dx_sq + q.inv_dy_sq;
   q.dt = static_cast<float>(c.dt);
   q.dt_over_dx = static_cast<float>(c.dt / c.dx);

@@ -338,13 +338,15 @@ __device__ __forceinline__ void rows_forward_march_body(const FvmParams& p, cons
       const int j = tid + c * kThreads;
       const int jm = (j - 1) & (N - 1);
       const float fx_next = face_fx_u(w, a + 1, j, p);
-      const float adv_u = -((fx_next - fx_prev[c]) / p.dx + (fy[j] - fy[jm]) / p.dy);
+      const float adv_u = -(over_step(fx_next - fx_prev[c], p.dx, p.rcp_dx) +
+                            over_step(fy[j] - fy[jm], p.dy, p.rcp_dy));
       const float ku = add_diffusion_forcing<false>(w, a, j, p, fxj[c], adv_u);
       us_cur[c] = w.U(a, j) + p.dt * ku;
       fx_prev[c] = fx_next;
       if (vrow) {
         const float gx_next = face_fx_v(w, a + 1, j, p);
-        const float adv_v = -((gx_next - gx_prev[c]) / p.dx + (gy[j] - gy[jm]) / p.dy);
+        const float adv_v = -(over_step(gx_next - gx_prev[c], p.dx, p.rcp_dx) +
+                              over_step(gy[j] - gy[jm], p.dy, p.rcp_dy));
         const float kv = add_diffusion_forcing<true>(w, a, j, p, 0.f, adv_v);
         const float vst = w.V(a, j) + p.dt * kv;
         gx_prev[c] = gx_next;
@@ -361,8 +363,8 @@ __device__ __forceinline__ void rows_forward_march_body(const FvmParams& p, cons
       for (int c = 0; c < CPT; ++c) {
         const int j = tid + c * kThreads;
         // div u* (finite_differences.py:136-143)
-        const float ddx = (us_cur[c] - us_prev[c]) / p.dx;
-        const float ddy = (vs[j] - vs[(j - 1) & (N - 1)]) / p.dy;
+        const float ddx = over_step(us_cur[c] - us_prev[c], p.dx, p.rcp_dx);
+        const float ddy = over_step(vs[j] - vs[(j - 1) & (N - 1)], p.dy, p.rcp_dy);
         zf[(d % FB) * N + j] = ddx + ddy;
       }
     }
@@ -513,8 +515,8 @@ __device__ __forceinline__ void rows_forward_body(const FvmParams& p, const FvmB
       if (e < NU * N) {
         const int r = e / N, j = e % N, lr = LO_U + r;
         const float* fy = flux + (NU + 1) * N + r * N;
-        const float adv = -((flux[(r + 1) * N + j] - flux[r * N + j]) / p.dx +
-                            (fy[j] - fy[(j - 1) & (N - 1)]) / p.dy);
+        const float adv = -(over_step(flux[(r + 1) * N + j] - flux[r * N + j], p.dx, p.rcp_dx) +
+                            over_step(fy[j] - fy[(j - 1) & (N - 1)], p.dy, p.rcp_dy));
         const float ku = add_diffusion_forcing<false>(w, lr, j, p, fxr[k], adv);
         ru[k] = (MODE == 2) ? ku : w.U(lr, j) + p.dt * ku;
       }
@@ -531,8 +533,8 @@ __device__ __forceinline__ void rows_forward_body(const FvmParams& p, const FvmB
       if (e < NV * N) {
         const int r = e / N, j = e % N, lr = 3 + r;
         const float* fy = flux + (NV + 1) * N + r * N;
-        const float adv = -((flux[(r + 1) * N + j] - flux[r * N + j]) / p.dx +
-                            (fy[j] - fy[(j - 1) & (N - 1)]) / p.dy);
+        const float adv = -(over_step(flux[(r + 1) * N + j] - flux[r * N + j], p.dx, p.rcp_dx) +
+                            over_step(fy[j] - fy[(j - 1) & (N - 1)], p.dy, p.rcp_dy));
         const float kv = add_diffusion_forcing<true>(w, lr, j, p, 0.f, adv);
         rv[k] = (MODE == 2) ? kv : w.V(lr, j) + p.dt * kv;
       }
@@ -575,8 +577,8 @@ __device__ __forceinline__ void rows_forward_body(const FvmParams& p, const FvmB
     float* zf = reinterpret_cast<float*>(zb);
     for (int e = tid; e < RB * N; e += kThreads) {
       const int r = e / N, j = e % N;
-      const float ddx = (sus[(r + 1) * N + j] - sus[r * N + j]) / p.dx;
-      const float ddy = (svs[r * N + j] - svs[r * N + ((j - 1) & (N - 1))]) / p.dy;
+      const float ddx = over_step(sus[(r + 1) * N + j] - sus[r * N + j], p.dx, p.rcp_dx);
+      const float ddy = over_step(svs[r * N + j] - svs[r * N + ((j - 1) & (N - 1))], p.dy, p.rcp_dy);
       zf[e] = ddx + ddy;
       if constexpr (MODE == 4) b.rhs[(i0 + r) * N + j] = ddx + ddy;
     }
@@ -733,8 +735,8 @@ __device__ __forceinline__ void rows_project_body(const FvmParams& p, const FvmB
         const int r = e / N, j = e % N;
         const int gi = i0 + r;
         const float qc = q[r * N + j];
-        const float gx = (q[(r + 1) * N + j] + (-qc)) / p.dx;
-        const float gy = (q[r * N + ((j + 1) & (N - 1))] + (-qc)) / p.dy;
+        const float gx = over_step(q[(r + 1) * N + j] + (-qc), p.dx, p.rcp_dx);
+        const float gy = over_step(q[r * N + ((j + 1) & (N - 1))] + (-qc), p.dy, p.rcp_dy);
         b.u_out[gi * N + j] = pa[k] - gx;
         b.v_out[gi * N + j] = pb[k] - gy;
       }
@@ -757,8 +759,8 @@ __global__ void fvm_curl_kernel(FvmParams p, const float* __restrict__ u,
   if (e >= n * n) return;
   const int i = e / n, j = e % n;
   const int ip = (i + 1) % n, jp = (j + 1) % n;
-  const float dvdx = (v[ip * n + j] + (-v[e])) / p.dx;
-  const float dudy = (u[i * n + jp] + (-u[e])) / p.dy;
+  const float dvdx = over_step(v[ip * n + j] + (-v[e]), p.dx, p.rcp_dx);
+  const float dudy = over_step(u[i * n + jp] + (-u[e]), p.dy, p.rcp_dy);
   out[e] = dvdx - dudy;
 }
 
@@ -769,8 +771,8 @@ __global__ void fvm_divergence_kernel(FvmParams p, const float* __restrict__ u,
   if (e >= n * n) return;
   const int i = e / n, j = e % n;
   const int im = (i - 1 + n) % n, jm = (j - 1 + n) % n;
-  const float ddx = (u[e] - u[im * n + j]) / p.dx;
-  const float ddy = (v[e] - v[i * n + jm]) / p.dy;
+  const float ddx = over_step(u[e] - u[im * n + j], p.dx, p.rcp_dx);
+  const float ddy = over_step(v[e] - v[i * n + jm], p.dy, p.rcp_dy);
   out[e] = ddx + ddy;
 }
 

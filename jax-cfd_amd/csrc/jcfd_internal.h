@@ -19,6 +19,7 @@ int check_hip(hipError_t e, const char* what);
 struct FvmParams {
   int n;             // cells per side
   float dx, dy;      // grid steps
+  float rcp_dx, rcp_dy;  // RN(1/dx), RN(1/dy) (host fp32 division), for over_step
   float inv_dx_sq, inv_dy_sq, lap_sum;  // (1/dx)^2, (1/dy)^2 and their fp32 sum
   float dt;          // time step
   float dt_over_dx, dt_over_dy;         // Courant factors dt/h (rounded from fp64)
@@ -32,6 +33,17 @@ struct FvmParams {
   // fvm_default_tiles / fvm_tile_supported).
   int rb_fwd, w_cols, rb_proj, radix;
 };
+
+// x / h correctly rounded, from r = RN(1/h): q = RN(x r) lies within one ulp
+// of x / h, and one remainder step RN(q + RN(x - q h) r) is then the
+// correctly rounded quotient (Markstein's theorem; it needs no overflow or
+// underflow in q h, which fp32 fields of grid-scale values never reach).
+// Three instructions where the IEEE division sequence takes about ten; the
+// result equals x / h (bitwise, except that x = -0 gives +0).
+__device__ __forceinline__ float over_step(float x, float h, float r) {
+  const float q = x * r;
+  return __builtin_fmaf(__builtin_fmaf(-q, h, x), r, q);
+}
 
 // One Runge-Kutta combination u* = u0 + dt * (0 + c[0] k[0] + ... ) with the
 // reference's term order (zero coefficients left out by the host); k arrays
