@@ -30,24 +30,40 @@ namespace {
 
 constexpr int kThreads = 256;
 
-__device__ __forceinline__ float block_sum(float v, float* red) {
-  const int tid = threadIdx.x;
-  red[tid] = v;
-  __syncthreads();
-  for (int s = kThreads / 2; s > 0; s >>= 1) {
-    if (tid < s) red[tid] = red[tid] + red[tid + s];
-    __syncthreads();
+constexpr int kWaves = kThreads / 64;
+static_assert(kWaves == 4, "block sums combine four wavefronts");
+
+// Workgroup sums of K values at once: a butterfly across each wavefront's 64
+// lanes (every lane ends with the same bits: each step adds the same two
+// numbers in both lanes of a pair), then the four wavefront sums combined
+// as (w0 + w1) + (w2 + w3).  One barrier pair for all K sums; the order is
+// fixed, so every workgroup gets the same result from the same inputs.
+// red holds K * kWaves floats.
+template <int K>
+__device__ __forceinline__ void block_sums(float (&v)[K], float* red) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < K; ++q) {
+    float x = v[q];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x = x + __shfl_xor(x, o, 64);
+    v[q] = x;
   }
-  const float out = red[0];
+  if (lane == 0) {
+#pragma unroll
+    for (int q = 0; q < K; ++q) red[q * kWaves + wave] = v[q];
+  }
   __syncthreads();
-  return out;
+#pragma unroll
+  for (int q = 0; q < K; ++q)
+    v[q] = (red[q * kWaves] + red[q * kWaves + 1]) + (red[q * kWaves + 2] + red[q * kWaves + 3]);
+  __syncthreads();  // red may be written again
 }
 
-// Sum of the per-block partials, same order in every block.
-__device__ __forceinline__ float sum_partials(const float* part, int nb, float* red) {
-  float v = 0.f;
-  for (int i = threadIdx.x; i < nb; i += kThreads) v = v + part[i];
-  return block_sum(v, red);
+__device__ __forceinline__ float block_sum(float v, float* red) {
+  float a[1] = {v};
+  block_sums<1>(a, red);
+  return a[0];
 }
 
 __device__ __forceinline__ float tol2(const FvmParams& p, float bs, float rtol, float atol) {
@@ -58,7 +74,7 @@ __device__ __forceinline__ float tol2(const FvmParams& p, float bs, float rtol, 
 
 template <int N, int RB>
 __global__ __launch_bounds__(kThreads) void cg_init(const float* __restrict__ rhs, CgBuffers c) {
-  __shared__ float red[kThreads];
+  __shared__ float red[kWaves];
   const int i0 = blockIdx.x * RB;
   float rr = 0.f;
   for (int e = threadIdx.x; e < RB * N; e += kThreads) {
@@ -83,13 +99,34 @@ __global__ __launch_bounds__(kThreads) void cg_init(const float* __restrict__ rh
 template <int N, int RB>
 __global__ __launch_bounds__(kThreads) void cg_direction(FvmParams p, CgBuffers c, int k, float rtol,
                                                          float atol, int maxiter) {
-  __shared__ float red[kThreads];
+  __shared__ float red[3 * kWaves];
   __shared__ float pt[(RB + 2) * N];  // p_k rows i0-1 .. i0+RB
   if (c.status[0]) return;
   constexpr int NB = N / RB;
+  constexpr int TOT = (RB + 2) * N, PT = (TOT + kThreads - 1) / kThreads;
   const int i0 = blockIdx.x * RB;
-  const float gamma = sum_partials(c.part_rr[k & 1], NB, red);
-  const float bs = sum_partials(c.part_bb, NB, red);
+  const float* pold = c.p[k & 1];
+  float* pnew = c.p[(k + 1) & 1];
+  // r and p_{k-1} over rows i0-1 .. i0+RB, loaded before the reductions so
+  // the two memory latencies overlap (indices clamped: loads unconditional)
+  float rv[PT], pv[PT];
+#pragma unroll
+  for (int q = 0; q < PT; ++q) {
+    const int e = min(static_cast<int>(threadIdx.x) + q * kThreads, TOT - 1);
+    const int gi = (i0 - 1 + e / N) & (N - 1);
+    rv[q] = c.r[gi * N + e % N];
+    pv[q] = pold[gi * N + e % N];
+  }
+  // gamma_k = <r_k, r_k>, <b, b> and gamma_{k-1}, reduced together (the
+  // last is unused, and part_rr of the other parity unset, at k = 0)
+  float sums[3] = {0.f, 0.f, 0.f};
+  for (int e = threadIdx.x; e < NB; e += kThreads) {
+    sums[0] = sums[0] + c.part_rr[k & 1][e];
+    sums[1] = sums[1] + c.part_bb[e];
+    if (k > 0) sums[2] = sums[2] + c.part_rr[(k + 1) & 1][e];
+  }
+  block_sums<3>(sums, red);
+  const float gamma = sums[0], bs = sums[1];
   if (!(gamma > tol2(p, bs, rtol, atol) && k < maxiter)) {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
       c.status[0] = 1;
@@ -97,13 +134,14 @@ __global__ __launch_bounds__(kThreads) void cg_direction(FvmParams p, CgBuffers 
     }
     return;
   }
-  const float beta = (k == 0) ? 0.f : gamma / sum_partials(c.part_rr[(k + 1) & 1], NB, red);
-  const float* pold = c.p[k & 1];
-  float* pnew = c.p[(k + 1) & 1];
-  for (int e = threadIdx.x; e < (RB + 2) * N; e += kThreads) {
+  const float beta = (k == 0) ? 0.f : gamma / sums[2];
+#pragma unroll
+  for (int q = 0; q < PT; ++q) {
+    const int e = static_cast<int>(threadIdx.x) + q * kThreads;
+    if (TOT % kThreads != 0 && e >= TOT) break;
     const int lr = e / N, j = e % N;
     const int gi = (i0 - 1 + lr) & (N - 1);
-    const float v = c.r[gi * N + j] + beta * pold[gi * N + j];
+    const float v = rv[q] + beta * pv[q];
     pt[e] = v;
     if (lr >= 1 && lr <= RB) pnew[gi * N + j] = v;
   }
@@ -124,19 +162,37 @@ __global__ __launch_bounds__(kThreads) void cg_direction(FvmParams p, CgBuffers 
 
 template <int N, int RB>
 __global__ __launch_bounds__(kThreads) void cg_update(CgBuffers c, int k) {
-  __shared__ float red[kThreads];
+  __shared__ float red[2 * kWaves];
   if (c.status[0]) return;
   constexpr int NB = N / RB;
+  constexpr int TOT = RB * N, PU = (TOT + kThreads - 1) / kThreads;
   const int i0 = blockIdx.x * RB;
-  const float gamma = sum_partials(c.part_rr[k & 1], NB, red);
-  const float pap = sum_partials(c.part_pap, NB, red);
-  const float alpha = gamma / pap;
   const float* pk = c.p[(k + 1) & 1];
+  // this block's x, p_k, r, Ap, loaded before the reductions (clamped)
+  float xv[PU], pv[PU], rv[PU], av[PU];
+#pragma unroll
+  for (int q = 0; q < PU; ++q) {
+    const int idx = i0 * N + min(static_cast<int>(threadIdx.x) + q * kThreads, TOT - 1);
+    xv[q] = c.x[idx];
+    pv[q] = pk[idx];
+    rv[q] = c.r[idx];
+    av[q] = c.ap[idx];
+  }
+  float sums[2] = {0.f, 0.f};  // gamma_k, <p, Ap>
+  for (int e = threadIdx.x; e < NB; e += kThreads) {
+    sums[0] = sums[0] + c.part_rr[k & 1][e];
+    sums[1] = sums[1] + c.part_pap[e];
+  }
+  block_sums<2>(sums, red);
+  const float alpha = sums[0] / sums[1];
   float rr = 0.f;
-  for (int e = threadIdx.x; e < RB * N; e += kThreads) {
+#pragma unroll
+  for (int q = 0; q < PU; ++q) {
+    const int e = static_cast<int>(threadIdx.x) + q * kThreads;
+    if (TOT % kThreads != 0 && e >= TOT) break;
     const int idx = i0 * N + e;
-    c.x[idx] = c.x[idx] + alpha * pk[idx];
-    const float rn = c.r[idx] - alpha * c.ap[idx];
+    c.x[idx] = xv[q] + alpha * pv[q];
+    const float rn = rv[q] - alpha * av[q];
     c.r[idx] = rn;
     rr = rr + rn * rn;
   }
