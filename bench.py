@@ -46,7 +46,7 @@ def _args():
                   help="requests of the served Poisson stream (0 = skip)")
   ap.add_argument("--group-steps", type=int, default=256,
                   help="steps of the mixed-size group line (0 skips it)")
-  ap.add_argument("--adjoint", type=int, default=1, help="time the C4 gradient (0 skips)")
+  ap.add_argument("--adjoint", type=int, default=1, help="time the adjoint gradient of a 256^2 64-step rollout (0 skips)")
   ap.add_argument("--cg-steps", type=int, default=100,
                   help="steps of the solve_cg variant of the 256^2 run (0 = skip)")
   ap.add_argument("--copy-mib", type=int, default=0,

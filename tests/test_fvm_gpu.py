@@ -67,8 +67,8 @@ def _state(v0, n):
 
 
 # configurations: (n, seed, v_max, forced)
-C0 = (64, 0, 2.0, False)
-C1 = (256, 1, 7.0, True)
+UNFORCED_64 = (64, 0, 2.0, False)   # 64^2, no forcing, seed 0, v_max 2
+FORCED_256 = (256, 1, 7.0, True)    # 256^2 Kolmogorov forcing + drag, seed 1, v_max 7
 
 
 def _config(n, seed, vmax, forced):
@@ -89,7 +89,7 @@ def _config(n, seed, vmax, forced):
   return ogrid, v0, dt, ostep, step
 
 
-@pytest.mark.parametrize("cfg", [C0, C1, (128, 3, 1.0, True), (512, 4, 3.0, True)],
+@pytest.mark.parametrize("cfg", [UNFORCED_64, FORCED_256, (128, 3, 1.0, True), (512, 4, 3.0, True)],
                          ids=["64-unforced", "256-forced", "128-forced", "512-forced"])
 def test_explicit_terms_parity(cfg):
   n, seed, vmax, forced = cfg
@@ -161,7 +161,7 @@ def test_cg_projection_matches_fast_diag_and_oracle_cg(n):
 
 
 def test_cg_step_variant():
-  """The C1 variant with solve_cg in place of solve_fast_diag: one step and
+  """The forced 256^2 workload's variant with solve_cg in place of solve_fast_diag: one step and
   20 steps against the fast-diagonal step and the oracle's CG step."""
   import functools
   n = 64
@@ -205,7 +205,7 @@ def test_projection_parity_and_divergence(n):
   _check_divergence(div, umax, ogrid.step[0], ref_div)
 
 
-@pytest.mark.parametrize("cfg", [C0, C1], ids=["C0-64", "C1-256"])
+@pytest.mark.parametrize("cfg", [UNFORCED_64, FORCED_256], ids=["unforced-64", "forced-256"])
 def test_one_step_parity(cfg):
   n = cfg[0]
   ogrid, v0, dt, ostep, step = _config(*cfg)
@@ -220,7 +220,7 @@ def test_one_step_parity(cfg):
   _check_divergence(div, umax, ogrid.step[0], go.divergence(ref, ogrid).data)
 
 
-@pytest.mark.parametrize("cfg", [C0, C1], ids=["C0-64", "C1-256"])
+@pytest.mark.parametrize("cfg", [UNFORCED_64, FORCED_256], ids=["unforced-64", "forced-256"])
 def test_100_steps_rel_l2(cfg):
   n = cfg[0]
   ogrid, v0, dt, ostep, step = _config(*cfg)
@@ -239,7 +239,7 @@ def test_100_steps_rel_l2(cfg):
 
 def test_repeated_equals_single_steps_and_trajectory():
   n = 64
-  ogrid, v0, dt, ostep, step = _config(*C0)
+  ogrid, v0, dt, ostep, step = _config(*UNFORCED_64)
   cfd = _cfd()
   s = _state(v0, n)
   a = cfd.funcutils.repeated(step, 7)(s)
@@ -262,7 +262,7 @@ def test_graph_replay_matches_individual_launches():
   """Steps replayed from the captured hipGraph equal eagerly launched steps,
   across replays separated by other eager work."""
   n = 64
-  ogrid, v0, dt, ostep, step = _config(*C0)
+  ogrid, v0, dt, ostep, step = _config(*UNFORCED_64)
   s = _state(v0, n)
   plan = step.plan()
   plan.set_graphs(False)
@@ -277,7 +277,7 @@ def test_graph_replay_matches_individual_launches():
 
 def test_output_may_alias_input():
   n = 64
-  ogrid, v0, dt, ostep, step = _config(*C0)
+  ogrid, v0, dt, ostep, step = _config(*UNFORCED_64)
   s = _state(v0, n)
   plan = step.plan()
   u, v = s[0].data.clone(), s[1].data.clone()
@@ -319,7 +319,7 @@ def test_filtered_velocity_field_parity(n, seed, vmax):
 
 def test_invalid_tensors_raise():
   n = 64
-  ogrid, v0, dt, ostep, step = _config(*C0)
+  ogrid, v0, dt, ostep, step = _config(*UNFORCED_64)
   plan = step.plan()
   good = torch.zeros((n, n), device=_dev())
   with pytest.raises(ValueError):
