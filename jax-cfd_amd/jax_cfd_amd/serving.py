@@ -26,6 +26,12 @@ until other pools drain, and a request that can never fit is rejected.
 
 Every slot gets exactly the arithmetic of a lone instance (the kernels never
 mix instances), so a request's result is bit-identical alone or in a mix.
+
+Trajectories: a request with `save_every = k > 0` also gets the frames of
+funcutils.trajectory(repeated(step_fn, k), steps // k) -- the state after
+k, 2k, ... steps.  Its pool's chunks then also stop at the request's frame
+boundaries, and each frame is copied from the slot to pinned host memory on
+the pool stream right behind the chunk that produced it.
 """
 from __future__ import annotations
 
@@ -54,6 +60,7 @@ class RolloutRequest:
   steps: int
   seed: int
   arrival: float = 0.0  # seconds after the server starts
+  save_every: int = 0   # > 0: also return the state every save_every steps
 
 
 @dataclasses.dataclass
@@ -67,6 +74,8 @@ class RolloutResult:
   arrival: float = 0.0
   admitted: float = 0.0
   finished: float = 0.0
+  # save_every > 0: per state component, the (steps // save_every, ...) frames
+  frames: Optional[Tuple[np.ndarray, ...]] = None
 
   @property
   def latency(self) -> float:
@@ -206,11 +215,12 @@ class Pool:
     self.slot_buf = torch.zeros(capacity, dtype=torch.int32, device=family.device)
     self.slot_host = torch.zeros(capacity, dtype=torch.int32, pin_memory=True)
     self.free: List[int] = list(range(capacity))[::-1]
-    self.active: Dict[int, List] = {}  # slot -> [request, remaining, admitted]
+    # slot -> [request, remaining, admitted, pinned host frames or None]
+    self.active: Dict[int, List] = {}
     self.event: Optional[torch.cuda.Event] = None
     self.chunk_done: List[int] = []
     # finished requests whose states are being copied to the host:
-    # (slot, request, admitted, pinned host tensors, copy-done event)
+    # (slot, request, admitted, pinned host tensors, host frames, copy-done event)
     self.retiring: List[tuple] = []
     self.bytes = (self.plan.device_bytes + family.state_bytes(capacity)
                   + 4 * capacity)
@@ -230,14 +240,29 @@ class Pool:
       ready = torch.cuda.Event()
       ready.record(ic_stream)
     self.stream.wait_event(ready)
-    self.active[slot] = [req, req.steps, now]
+    frames = None
+    if req.save_every > 0:
+      count = req.steps // req.save_every
+      frames = tuple(torch.empty((count,) + tuple(d.shape), dtype=d.dtype, pin_memory=True)
+                     for d in self.family.read_slot(self.state, slot))
+    self.active[slot] = [req, req.steps, now, frames]
     return slot
+
+  @staticmethod
+  def _until_stop(entry) -> int:
+    """Steps the slot may advance before it must stop: its end, or its
+    next frame."""
+    req, remaining = entry[0], entry[1]
+    if req.save_every <= 0:
+      return remaining
+    done = req.steps - remaining
+    return min(remaining, req.save_every - done % req.save_every)
 
   def launch(self, max_chunk: int):
     """Advance every active slot by the smallest remaining count (capped)."""
     if not self.active:
       return
-    chunk = min(max_chunk, min(a[1] for a in self.active.values()))
+    chunk = min(max_chunk, min(self._until_stop(a) for a in self.active.values()))
     slots = sorted(self.active)
     # the previous chunk's copy out of slot_host has run: the pool is idle
     self.slot_host[:len(slots)] = torch.as_tensor(slots, dtype=torch.int32)
@@ -245,6 +270,13 @@ class Pool:
       idx = self.slot_buf[:len(slots)]
       idx.copy_(self.slot_host[:len(slots)], non_blocking=True)
       self.family.advance(self.plan, self.state, idx, chunk)
+      for s in slots:
+        req, remaining, _, frames = self.active[s]
+        done = req.steps - remaining + chunk
+        if frames is not None and done % req.save_every == 0:
+          t = done // req.save_every - 1
+          for h, d in zip(frames, self.family.read_slot(self.state, s)):
+            h[t].copy_(d, non_blocking=True)
       self.event = torch.cuda.Event()
       self.event.record(self.stream)
     for s in slots:
@@ -257,7 +289,7 @@ class Pool:
     shares the stream's hardware queue, and the server keeps serving the
     other pools meanwhile (collect() hands the results out)."""
     for slot in self.chunk_done:
-      req, _, admitted = self.active.pop(slot)
+      req, _, admitted, frames = self.active.pop(slot)
       with torch.cuda.stream(self.stream):
         data = self.family.read_slot(self.state, slot)
         host = tuple(torch.empty(d.shape, dtype=d.dtype, pin_memory=True) for d in data)
@@ -265,7 +297,7 @@ class Pool:
           h.copy_(d, non_blocking=True)
         done = torch.cuda.Event()
         done.record(self.stream)
-      self.retiring.append((slot, req, admitted, host, done))
+      self.retiring.append((slot, req, admitted, host, frames, done))
     self.chunk_done = []
 
   def collect(self, now_fn) -> List[RolloutResult]:
@@ -273,13 +305,14 @@ class Pool:
     only now, so an admission cannot overwrite a state still being read."""
     out, keep = [], []
     for item in self.retiring:
-      slot, req, admitted, host, done = item
+      slot, req, admitted, host, frames, done = item
       if not done.query():
         keep.append(item)
         continue
-      out.append(RolloutResult(req.request_id, req.solver, req.n, req.steps, 'done',
-                               tuple(h.numpy() for h in host), req.arrival, admitted,
-                               now_fn()))
+      out.append(RolloutResult(
+          req.request_id, req.solver, req.n, req.steps, 'done', tuple(h.numpy() for h in host),
+          req.arrival, admitted, now_fn(),
+          tuple(f.numpy() for f in frames) if frames is not None else None))
       self.free.append(slot)
     self.retiring = keep
     return out

@@ -54,6 +54,45 @@ def test_server_matches_plain_path():
   np.testing.assert_array_equal(res[1].state[0], step.multistep(w, 140).cpu().numpy())
 
 
+def test_served_frames_equal_trajectory():
+  """Requests with save_every get the frames of
+  trajectory(repeated(step_fn, save_every), steps // save_every), bitwise,
+  while sharing pools with requests of other frame spacings and with
+  final-state-only requests."""
+  import jax_cfd_amd.base as cfd
+  from jax_cfd_amd import serving
+  from jax_cfd_amd.spectral import time_stepping
+  dev = torch.device('cuda', 0)
+  reqs = [serving.RolloutRequest(0, 'fvm', 64, 40, 31, save_every=8),
+          serving.RolloutRequest(1, 'fvm', 64, 37, 32, save_every=5),
+          serving.RolloutRequest(2, 'fvm', 64, 29, 33),
+          serving.RolloutRequest(3, 'spectral', 64, 30, 34, save_every=10),
+          serving.RolloutRequest(4, 'spectral', 64, 26, 35, save_every=7)]
+  res = _by_id(_srv(capacity={64: 4}).run(reqs, realtime=False))
+  assert all(r.status == 'done' for r in res.values())
+  assert res[2].frames is None
+  fam = serving.FvmFamily(64, dev)
+  plan = cfd._fvm.FvmPlan(fam.spec, dev)
+  for i, seed, every, steps in ((0, 31, 8, 40), (1, 32, 5, 37)):
+    u, v = fam.initial_state(seed)
+    assert res[i].frames[0].shape == (steps // every, 64, 64)
+    for t in range(steps // every):
+      uo, vo = plan.step(u, v, every * (t + 1))
+      np.testing.assert_array_equal(res[i].frames[0][t], uo.cpu().numpy())
+      np.testing.assert_array_equal(res[i].frames[1][t], vo.cpu().numpy())
+    uo, vo = plan.step(u, v, steps)
+    np.testing.assert_array_equal(res[i].state[0], uo.cpu().numpy())
+  sfam = serving.SpectralFamily(64, dev)
+  step = time_stepping.crank_nicolson_rk4(sfam.equation, sfam.DT)
+  for i, seed, every, steps in ((3, 34, 10, 30), (4, 35, 7, 26)):
+    (w,) = sfam.initial_state(seed)
+    assert res[i].frames[0].shape == (steps // every, 64, 33)
+    for t in range(steps // every):
+      np.testing.assert_array_equal(res[i].frames[0][t],
+                                    step.multistep(w, every * (t + 1)).cpu().numpy())
+    np.testing.assert_array_equal(res[i].state[0], step.multistep(w, steps).cpu().numpy())
+
+
 def test_capacity_limits_concurrency_and_all_complete():
   reqs = [_req(i, 'fvm', 64, 50 + 17 * i, i) for i in range(7)]
   srv = _srv(capacity={64: 2})
