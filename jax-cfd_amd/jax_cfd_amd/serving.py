@@ -384,7 +384,12 @@ class RolloutServer:
 
   def run(self, requests: List[RolloutRequest], realtime: bool = True) -> List[RolloutResult]:
     """Serve `requests` (arrival times relative to now); returns results in
-    completion order.  realtime=False admits everything at t = 0."""
+    completion order.  realtime=False admits everything at t = 0.  Frames
+    (save_every > 0) are held in pinned host memory, (steps // save_every)
+    states per request."""
+    for req in requests:
+      if req.steps < 0 or req.save_every < 0:
+        raise ValueError(f'request {req.request_id}: steps and save_every must be >= 0')
     pending: Deque[RolloutRequest] = collections.deque(
         sorted(requests, key=lambda r: (r.arrival, r.request_id)))
     waiting: Deque[RolloutRequest] = collections.deque()

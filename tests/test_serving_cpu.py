@@ -27,3 +27,12 @@ def test_summarize_percentiles():
   assert s['cell_steps'] == 100 * 64 * 64 * 100
   assert abs(s['p50_latency_s'] - 50.5) < 1e-9
   assert 99.0 <= s['p99_latency_s'] <= 100.0
+
+
+def test_invalid_requests_raise_before_any_work():
+  import pytest
+  srv = serving.RolloutServer.__new__(serving.RolloutServer)  # no device needed
+  with pytest.raises(ValueError):
+    serving.RolloutServer.run(srv, [serving.RolloutRequest(0, 'fvm', 64, -1, 0)])
+  with pytest.raises(ValueError):
+    serving.RolloutServer.run(srv, [serving.RolloutRequest(0, 'fvm', 64, 10, 0, save_every=-2)])
