@@ -273,7 +273,7 @@ class Pool:
       for s in slots:
         req, remaining, _, frames = self.active[s]
         done = req.steps - remaining + chunk
-        if frames is not None and done % req.save_every == 0:
+        if frames is not None and chunk > 0 and done % req.save_every == 0:
           t = done // req.save_every - 1
           for h, d in zip(frames, self.family.read_slot(self.state, s)):
             h[t].copy_(d, non_blocking=True)

@@ -67,10 +67,13 @@ def test_served_frames_equal_trajectory():
           serving.RolloutRequest(1, 'fvm', 64, 37, 32, save_every=5),
           serving.RolloutRequest(2, 'fvm', 64, 29, 33),
           serving.RolloutRequest(3, 'spectral', 64, 30, 34, save_every=10),
-          serving.RolloutRequest(4, 'spectral', 64, 26, 35, save_every=7)]
+          serving.RolloutRequest(4, 'spectral', 64, 26, 35, save_every=7),
+          serving.RolloutRequest(5, 'fvm', 64, 0, 36, save_every=4),      # no frames
+          serving.RolloutRequest(6, 'spectral', 64, 5, 37, save_every=9)]  # none either
   res = _by_id(_srv(capacity={64: 4}).run(reqs, realtime=False))
   assert all(r.status == 'done' for r in res.values())
   assert res[2].frames is None
+  assert res[5].frames[0].shape == (0, 64, 64) and res[6].frames[0].shape == (0, 64, 33)
   fam = serving.FvmFamily(64, dev)
   plan = cfd._fvm.FvmPlan(fam.spec, dev)
   for i, seed, every, steps in ((0, 31, 8, 40), (1, 32, 5, 37)):
