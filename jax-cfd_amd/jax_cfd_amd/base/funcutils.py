@@ -1,4 +1,6 @@
-"""Rollout drivers (jax_cfd/base/funcutils.py:82-126).
+"""Rollout drivers (jax_cfd/base/funcutils.py:40-126).
+
+``scan`` / ``init_context`` keep jax.lax.scan's contract for host-side loops.
 
 ``repeated`` hands a GPU step function the whole step count in one library
 call.  ``trajectory`` stacks post-processed frames along a new leading axis,
@@ -9,12 +11,79 @@ slot of that tensor (``rollout_frames``), so a frame costs no copy.
 """
 from __future__ import annotations
 
-from typing import Callable
+import contextlib
+from typing import Any, Callable, Optional
 
 import numpy as np
 import torch
 
 from jax_cfd_amd.base import grids
+
+_INITIALIZING = 0
+
+
+@contextlib.contextmanager
+def init_context():
+  """Within this context scan() evaluates f() once (funcutils.py:40-56)."""
+  global _INITIALIZING
+  _INITIALIZING += 1
+  try:
+    yield
+  finally:
+    _INITIALIZING -= 1
+
+
+def _index(xs, i: int):
+  """xs[i] of every leaf."""
+  if xs is None:
+    return None
+  if isinstance(xs, grids.GridVariable):
+    return grids.GridVariable(_index(xs.array, i), xs.bc)
+  if isinstance(xs, grids.GridArray):
+    return grids.GridArray(xs.data[i], xs.offset, xs.grid)
+  if isinstance(xs, (tuple, list)):
+    return type(xs)(_index(x, i) for x in xs)
+  if isinstance(xs, dict):
+    return {k: _index(v, i) for k, v in xs.items()}
+  return xs[i]
+
+
+def _leading_length(xs) -> int:
+  if isinstance(xs, (grids.GridVariable, grids.GridArray)):
+    return int(xs.data.shape[0])
+  if isinstance(xs, (tuple, list)):
+    return _leading_length(xs[0])
+  if isinstance(xs, dict):
+    return _leading_length(next(iter(xs.values())))
+  return int(xs.shape[0])
+
+
+def scan(f: Callable, init: Any, xs: Any, length: Optional[int] = None):
+  """jax.lax.scan semantics (funcutils.py:65-79): carry, y = f(carry, x)
+  over the leading axis of xs (or `length` times with xs None); returns the
+  final carry and the stacked ys (None if f returns no y, or for length 0).
+  Within init_context() f is evaluated once and its y repeated."""
+  if length is None:
+    if xs is None:
+      raise ValueError('scan needs xs or length')
+    length = _leading_length(xs)
+  length = int(length)
+  if _INITIALIZING:
+    carry, y0 = f(init, _index(xs, 0))
+    if y0 is None:
+      return carry, None
+    ys = _allocate(y0, length)
+    for t in range(length):
+      _write(ys, t, y0)
+    return carry, ys
+  carry, ys = init, None
+  for t in range(length):
+    carry, y = f(carry, _index(xs, t))
+    if y is not None:
+      if ys is None:
+        ys = _allocate(y, length)
+      _write(ys, t, y)
+  return carry, ys
 
 
 def _identity(x):
@@ -50,6 +119,8 @@ def _allocate(frame, count: int):
     return grids.GridArray(_allocate(frame.data, count), frame.offset, frame.grid)
   if isinstance(frame, (tuple, list)):
     return type(frame)(_allocate(x, count) for x in frame)
+  if isinstance(frame, dict):
+    return {k: _allocate(v, count) for k, v in frame.items()}
   if isinstance(frame, torch.Tensor):
     return torch.empty((count,) + tuple(frame.shape), dtype=frame.dtype,
                        device=frame.device)
@@ -73,6 +144,9 @@ def _write(frames, index: int, frame):
       raise ValueError('frames change structure along the trajectory')
     for f, x in zip(frames, frame):
       _write(f, index, x)
+  elif isinstance(frames, dict):
+    for k, f in frames.items():
+      _write(f, index, frame[k])
   elif isinstance(frames, torch.Tensor):
     frames[index].copy_(frame)
   else:

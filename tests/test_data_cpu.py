@@ -7,7 +7,8 @@ form (lo + (i + 1/2) h), the offsets of the staggered faces, the branch rules
 of `grid_from_attrs`, and the netCDF-3 header layout read byte by byte
 (independently of the library that writes the file).  `trajectory`'s frame
 order is pinned against `jax.lax.scan` semantics (funcutils.py:95-126) with a
-step whose states are known in closed form (x -> x + 1).
+step whose states are known in closed form (x -> x + 1), and `scan` itself
+(funcutils.py:65-79) with cumulative sums.
 """
 import struct
 
@@ -192,3 +193,27 @@ def test_trajectory_post_process_structure_and_empty():
   assert final is v0
   with pytest.raises(ValueError):
     cfd.funcutils.trajectory(step, -1)
+
+
+def test_scan_matches_lax_scan_semantics():
+  """funcutils.py:65-79: cumulative sums of xs in closed form, xs=None with
+  a length, a y-less f, and init_context() evaluating f once."""
+  xs = torch.arange(1.0, 6.0)
+  carry, ys = cfd.funcutils.scan(lambda c, x: (c + x, c * 10), torch.tensor(0.0), xs)
+  assert float(carry) == 15.0
+  assert torch.equal(ys, torch.tensor([0.0, 10.0, 30.0, 60.0, 100.0]))
+  carry, ys = cfd.funcutils.scan(lambda c, _: (c * 2, {'c': c}), torch.tensor(1.0), None, length=4)
+  assert float(carry) == 16.0 and torch.equal(ys['c'], torch.tensor([1.0, 2.0, 4.0, 8.0]))
+  carry, ys = cfd.funcutils.scan(lambda c, x: (c + x, None), 0.0, np.ones(3))
+  assert carry == 3.0 and ys is None
+  calls = []
+
+  def f(c, x):
+    calls.append(1)
+    return c + 1, torch.full((2,), float(c))
+
+  with cfd.funcutils.init_context():
+    carry, ys = cfd.funcutils.scan(f, 0, None, length=3)
+  assert len(calls) == 1 and carry == 1 and ys.shape == (3, 2)
+  with pytest.raises(ValueError):
+    cfd.funcutils.scan(f, 0, None)
