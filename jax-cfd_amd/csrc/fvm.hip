@@ -122,6 +122,10 @@ struct RingWindow {
 template <typename WIN>
 __device__ __forceinline__ float face_fx_u(const WIN& w, int a, int j, const FvmParams& p) {
   const float h = p.dt_over_dx;
+  if (p.convect == JCFD_CONVECT_LINEAR) {  // u and its velocity both at the face
+    const float Uf = 0.5f * w.U(a - 1, j) + 0.5f * w.U(a, j);
+    return Uf * Uf;
+  }
   if (p.convect == JCFD_CONVECT_VAN_LEER_LIMITERS) {
     const int c = a - 1;  // face between rows c and c + 1
     const float Uf = 0.5f * w.U(c, j) + 0.5f * w.U(c + 1, j);
@@ -134,6 +138,11 @@ __device__ __forceinline__ float face_fx_u(const WIN& w, int a, int j, const Fvm
 template <typename WIN>
 __device__ __forceinline__ float face_fy_u(const WIN& w, int lr, int b, const FvmParams& p) {
   const float h = p.dt_over_dy;
+  if (p.convect == JCFD_CONVECT_LINEAR) {  // (u at the face) * (v at the face)
+    const float cf = 0.5f * w.U(lr, b) + 0.5f * w.U(lr, b + 1);
+    const float Vf = 0.5f * w.V(lr, b) + 0.5f * w.V(lr + 1, b);
+    return cf * Vf;
+  }
   if (p.convect == JCFD_CONVECT_VAN_LEER_LIMITERS) {
     const float Vf = 0.5f * w.V(lr, b) + 0.5f * w.V(lr + 1, b);
     return vll_flux(w.U(lr, b - 1), w.U(lr, b), w.U(lr, b + 1), w.U(lr, b + 2), Vf, h);
@@ -148,6 +157,11 @@ template <typename WIN>
 __device__ __forceinline__ float face_fx_v(const WIN& w, int a, int j, const FvmParams& p) {
   const float h = p.dt_over_dx;
   const int c = a - 1;
+  if (p.convect == JCFD_CONVECT_LINEAR) {  // (v at the face) * (u at the face)
+    const float cf = 0.5f * w.V(c, j) + 0.5f * w.V(c + 1, j);
+    const float Uf = 0.5f * w.U(c, j) + 0.5f * w.U(c, j + 1);
+    return cf * Uf;
+  }
   if (p.convect == JCFD_CONVECT_VAN_LEER_LIMITERS) {
     const float Uf = 0.5f * w.U(c, j) + 0.5f * w.U(c, j + 1);
     return vll_flux(w.V(c - 1, j), w.V(c, j), w.V(c + 1, j), w.V(c + 2, j), Uf, h);
@@ -161,6 +175,10 @@ __device__ __forceinline__ float face_fx_v(const WIN& w, int a, int j, const Fvm
 template <typename WIN>
 __device__ __forceinline__ float face_fy_v(const WIN& w, int lr, int j, const FvmParams& p) {
   const float h = p.dt_over_dy;
+  if (p.convect == JCFD_CONVECT_LINEAR) {  // v and its velocity both at the face
+    const float Vf = 0.5f * w.V(lr, j) + 0.5f * w.V(lr, j + 1);
+    return Vf * Vf;
+  }
   if (p.convect == JCFD_CONVECT_VAN_LEER_LIMITERS) {
     const float Vf = 0.5f * w.V(lr, j) + 0.5f * w.V(lr, j + 1);
     return vll_flux(w.V(lr, j - 1), w.V(lr, j), w.V(lr, j + 1), w.V(lr, j + 2), Vf, h);

@@ -1,10 +1,14 @@
 """Advection operators (jax_cfd/base/advection.py).
 
-``advect_van_leer`` is the scheme the FVM kernels implement.  Inside a step
-factory it is read symbolically; called directly on a velocity component it
-runs the explicit-terms kernel with diffusion and forcing switched off.
+``advect_van_leer``, ``advect_van_leer_using_limiters`` and
+``advect_linear`` / ``convect_linear`` are the schemes the FVM kernels
+implement.  Inside a step factory they are read symbolically; called
+directly on a velocity component they run the explicit-terms kernel with
+diffusion and forcing switched off.
 """
 from __future__ import annotations
+
+from typing import Optional, Tuple
 
 from jax_cfd_amd import _lib, _symbolic
 from jax_cfd_amd.base import _fvm, boundaries, grids
@@ -32,7 +36,7 @@ def _eager(c, v, dt, convect):
   idx = _component_index(c, v)
   if idx is None or len(v) != 2:
     raise NotImplementedError(
-        'advect_van_leer on the GPU advects the components of a 2-D staggered '
+        'advection on the GPU advects the components of a 2-D staggered '
         'velocity field by that field')
   grid = grids.consistent_grid(*[u.array for u in v])
   if not boundaries.has_all_periodic_boundary_conditions(*v):
@@ -41,6 +45,21 @@ def _eager(c, v, dt, convect):
   plan = _fvm.plan_for(spec, v[0].data.device)
   du, dv = plan.explicit_terms(v[0].data, v[1].data)
   return grids.GridArray((du, dv)[idx], c.offset, grid)
+
+
+def advect_linear(c: grids.GridVariable, v: grids.GridVariableVector,
+                  dt: Optional[float] = None) -> grids.GridArray:
+  """advect_general with linear interpolation of c and v to the control
+  volume faces (advection.py:119-123)."""
+  if _symbolic.is_probe(c):
+    return _probe_term('advect_linear', c, v, dt)
+  return _eager(c, v, 0.0 if dt is None else dt, _lib.CONVECT_LINEAR)
+
+
+def convect_linear(v: grids.GridVariableVector) -> Tuple[grids.GridArray, ...]:
+  """Self-advection of v with linear interpolation (advection.py:184-207):
+  the same values as advect_linear of each component."""
+  return tuple(advect_linear(u, v) for u in v)
 
 
 def advect_van_leer(c: grids.GridVariable, v: grids.GridVariableVector, dt: float,

@@ -65,10 +65,11 @@ def _read_convection(convect, grid, dt):
   if len(out) != 2 or len(set(kinds)) != 1 or len(set(dts)) != 1:
     raise NotImplementedError('convect must treat both components alike')
   kinds_built = {'advect_van_leer': _lib.CONVECT_VAN_LEER,
-                 'advect_van_leer_using_limiters': _lib.CONVECT_VAN_LEER_LIMITERS}
+                 'advect_van_leer_using_limiters': _lib.CONVECT_VAN_LEER_LIMITERS,
+                 'advect_linear': _lib.CONVECT_LINEAR}
   if kinds[0] not in kinds_built:
     raise NotImplementedError(f'convection {kinds[0]} is not built')
-  if dts[0] != dt:
+  if kinds[0] != 'advect_linear' and dts[0] != dt:  # linear advection has no dt
     raise NotImplementedError('advection dt differing from the step dt')
   return kinds_built[kinds[0]]
 

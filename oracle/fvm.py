@@ -189,8 +189,9 @@ class FVMStep:
   """semi_implicit_navier_stokes + forward_euler, restated.
 
   convect: 'van_leer' (advect_van_leer for every component, as the
-  channel-flow notebook wires it) or 'van_leer_limiters' (the factory default,
-  advect_van_leer_using_limiters).  pressure: 'fast_diag' or 'cg'.
+  channel-flow notebook wires it), 'van_leer_limiters' (the factory default,
+  advect_van_leer_using_limiters) or 'linear' (convect_linear, the
+  validation tests' scheme).  pressure: 'fast_diag' or 'cg'.
   """
   grid: Grid
   dt: float
@@ -208,6 +209,8 @@ class FVMStep:
       fn = go.advect_van_leer
     elif self.convect == 'van_leer_limiters':
       fn = go.advect_van_leer_using_limiters
+    elif self.convect == 'linear':  # convect_linear (advection.py:184-207)
+      return tuple(go.advect_linear(u, v, self.grid).data for u in v)
     else:
       raise ValueError(self.convect)
     return tuple(fn(u, v, self.dt, self.grid).data for u in v)

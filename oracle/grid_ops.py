@@ -308,6 +308,19 @@ def tvd_van_leer_lax_wendroff(c: A, offset, v: Sequence[A], dt: float,
   return c
 
 
+def advect_linear(c: A, v: Sequence[A], grid: Grid) -> A:
+  """advect_general(c, v, linear, linear) (advection.py:34-123): both c and
+  the velocity interpolated linearly to the control-volume faces, flux
+  c_face * u_face, minus its divergence."""
+  targets = control_volume_offsets(c)
+  aligned_v = tuple(linear(u, t) for u, t in zip(v, targets))
+  aligned_c = tuple(linear(c, t) for t in targets)
+  flux = tuple(A(cc.data * u.data, u.offset)
+               for cc, u in zip(aligned_c, aligned_v))
+  adv = divergence(flux, grid)
+  return A(-adv.data, adv.offset)
+
+
 def advect_van_leer_using_limiters(c: A, v: Sequence[A], dt: float,
                                    grid: Grid) -> A:
   """advect_general(c, v, linear, tvd(lax_wendroff)) (advection.py:34-116, 387-395)."""

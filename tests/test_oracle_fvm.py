@@ -514,3 +514,60 @@ def test_rk4_step_invariants():
   assert np.abs(go.divergence(v, grid).data).max() < 1e-10
   for a_, b_ in zip(v0, v):
     assert abs(a_.data.sum() - b_.data.sum()) < 1e-9
+
+
+# ------------------------------------------------------ linear advection
+def test_linear_advection_translates_gaussian():
+  """advection_test.py:84-98 (linear_1D): 1000 Euler steps at Courant 0.01
+  with unit velocity move a smooth profile by 10 cells (atol 5e-2)."""
+  shape = (101,)
+  grid = Grid.from_step(shape, 1. / 101)
+  c = A(_gaussian(grid, grid.cell_center), grid.cell_center)
+  v = _unit_velocity(grid)
+  dt = 0.01 * grid.step[0]
+  c0 = c
+  for _ in range(1000):
+    c = A(c.data + dt * go.advect_linear(c, v, grid).data, c.offset)
+  np.testing.assert_allclose(c.data, np.roll(c0.data, 10), atol=5e-2)
+
+
+def test_convect_linear_matches_hand_written_stencil():
+  """convect_linear (advection.py:184-207) written out per cell from the
+  face values: u-face (u[i]+u[i+1])/2 squared, the corner product
+  (u[i,j]+u[i,j+1])/2 * (v[i,j]+v[i+1,j])/2, their backward differences."""
+  rng = np.random.default_rng(3)
+  nx, ny = 6, 5
+  grid = Grid.from_domain((nx, ny), ((0, 1.5), (0, 2.5)))
+  dx, dy = grid.step
+  u = rng.standard_normal((nx, ny))
+  v = rng.standard_normal((nx, ny))
+  vel = (A(u, grid.cell_faces[0]), A(v, grid.cell_faces[1]))
+  got_u = go.advect_linear(vel[0], vel, grid).data
+  got_v = go.advect_linear(vel[1], vel, grid).data
+  U = lambda i, j: u[i % nx, j % ny]  # noqa: E731
+  V = lambda i, j: v[i % nx, j % ny]  # noqa: E731
+  uu = lambda i, j: (0.5 * U(i, j) + 0.5 * U(i + 1, j)) ** 2  # noqa: E731  u-face of CV(u)
+  vv = lambda i, j: (0.5 * V(i, j) + 0.5 * V(i, j + 1)) ** 2  # noqa: E731  v-face of CV(v)
+  uv = lambda i, j: (0.5 * U(i, j) + 0.5 * U(i, j + 1)) * (0.5 * V(i, j) + 0.5 * V(i + 1, j))  # noqa: E731
+  for i in range(nx):
+    for j in range(ny):
+      eu = -((uu(i, j) - uu(i - 1, j)) / dx + (uv(i, j) - uv(i, j - 1)) / dy)
+      ev = -((uv(i, j) - uv(i - 1, j)) / dx + (vv(i, j) - vv(i, j - 1)) / dy)
+      assert abs(got_u[i, j] - eu) <= 1e-12 * max(1.0, abs(eu))
+      assert abs(got_v[i, j] - ev) <= 1e-12 * max(1.0, abs(ev))
+
+
+def test_taylor_green_linear_second_order():
+  """validation_test.py:34-169 with convect_linear: the error against the
+  closed form falls at second order (ratio ~4 per halving of h)."""
+  nu, t_end = 1e-2, 0.5
+  errs = []
+  for n in (16, 32):
+    grid = Grid.from_domain((n, n), ((0, 2 * np.pi), (0, 2 * np.pi)))
+    dt = 0.05 * grid.step[0]
+    steps = int(round(t_end / dt))
+    step = fvm.FVMStep(grid, dt=dt, viscosity=nu, convect='linear')
+    v = step.rollout(_taylor_green(grid, 0, nu), steps)
+    ref = _taylor_green(grid, steps * dt, nu)
+    errs.append(max(np.abs(a.data - b.data).max() for a, b in zip(v, ref)))
+  assert errs[0] / errs[1] > 3.5

@@ -75,3 +75,41 @@ def test_taylor_green_converges(kind):
   print(f"  max errors {['%.2e' % e for e in errs]}, ratios {['%.2f' % r for r in ratios]}")
   assert all(r > 2.0 for r in ratios)
   assert errs[-1] < 1.5e-4
+
+
+# The reference's own validation cases (validation_test.py:34-131), with its
+# solvers (convect_linear), time steps (dynamic_time_step at its Courant
+# numbers), end times and tolerances (np.testing.assert_allclose with the
+# given atol and the default rtol 1e-7, computed vs analytic).
+REFERENCE_CASES = [
+    # name, n, viscosity, time stepper, implicit diffusion, courant, time, atol
+    ('semi_implicit', 1024, 1e-3, 'forward_euler', False, 0.1, 10.0, 1e-5),
+    ('semi_implicit_rk1', 512, 1e-2, 'forward_euler', False, 0.1, 40.0, 6e-6),
+    ('semi_implicit_rk4', 512, 1e-2, 'classic_rk4', False, 0.1, 40.0, 8e-7),
+    ('implicit_diffusion_rfft', 1024, 1e-3, None, True, 0.1, 10.0, 4e-5),
+    ('implicit_diffusion_viscous', 1024, 0.5, None, True, 0.5, 1.0, 6e-4),
+]
+
+
+@pytest.mark.parametrize('case', REFERENCE_CASES, ids=[c[0] for c in REFERENCE_CASES])
+def test_reference_taylor_green_cases(case):
+  name, n, nu, stepper, implicit, courant, time, atol = case
+  cfd = _cfd()
+  grid = cfd.grids.Grid((n, n), domain=((0, TWO_PI), (0, TWO_PI)))
+  v0, _ = _taylor_green(grid, 0.0, nu)
+  dt = cfd.equations.dynamic_time_step(v0, courant, nu, grid, implicit)
+  steps = int(np.ceil(time / dt))
+  print(f"reference case {name}: n={n} dt={dt:.4e} steps={steps}", flush=True)
+  if implicit:
+    step = cfd.equations.implicit_diffusion_navier_stokes(
+        1.0, nu, dt, grid, convect=cfd.advection.convect_linear)
+  else:
+    step = cfd.equations.semi_implicit_navier_stokes(
+        1.0, nu, dt, grid, convect=cfd.advection.convect_linear,
+        time_stepper=getattr(cfd.time_stepping, stepper))
+  v = cfd.funcutils.repeated(step, steps)(v0)
+  _, exact = _taylor_green(grid, time, nu)
+  for c, e in zip(v, exact):
+    got = c.data.cpu().numpy().astype(np.float64)
+    print(f"  max |error| {np.abs(got - e).max():.2e} (atol {atol})")
+    np.testing.assert_allclose(got, e, atol=atol, rtol=1e-7)
