@@ -119,14 +119,14 @@ struct RingWindow {
 // (advection.py:387-395): advect_general with linear face velocities and
 // the TVD Lax-Wendroff / van_leer_limiter face value.  The operation order
 // of each face and of the differences is the reference's.
-template <typename WIN>
+template <int CONV, typename WIN>
 __device__ __forceinline__ float face_fx_u(const WIN& w, int a, int j, const FvmParams& p) {
   const float h = p.dt_over_dx;
-  if (p.convect == JCFD_CONVECT_LINEAR) {  // u and its velocity both at the face
+  if constexpr (CONV == JCFD_CONVECT_LINEAR) {  // u and its velocity both at the face
     const float Uf = 0.5f * w.U(a - 1, j) + 0.5f * w.U(a, j);
     return Uf * Uf;
   }
-  if (p.convect == JCFD_CONVECT_VAN_LEER_LIMITERS) {
+  if constexpr (CONV == JCFD_CONVECT_VAN_LEER_LIMITERS) {
     const int c = a - 1;  // face between rows c and c + 1
     const float Uf = 0.5f * w.U(c, j) + 0.5f * w.U(c + 1, j);
     return vll_flux(w.U(c - 1, j), w.U(c, j), w.U(c + 1, j), w.U(c + 2, j), Uf, h);
@@ -135,15 +135,15 @@ __device__ __forceinline__ float face_fx_u(const WIN& w, int a, int j, const Fvm
   return vl_flux(Uf, w.U(a - 1, j), w.U(a, j), w.U(a - 2, j), w.U(a - 1, j), w.U(a, j), w.U(a + 1, j), h);
 }
 
-template <typename WIN>
+template <int CONV, typename WIN>
 __device__ __forceinline__ float face_fy_u(const WIN& w, int lr, int b, const FvmParams& p) {
   const float h = p.dt_over_dy;
-  if (p.convect == JCFD_CONVECT_LINEAR) {  // (u at the face) * (v at the face)
+  if constexpr (CONV == JCFD_CONVECT_LINEAR) {  // (u at the face) * (v at the face)
     const float cf = 0.5f * w.U(lr, b) + 0.5f * w.U(lr, b + 1);
     const float Vf = 0.5f * w.V(lr, b) + 0.5f * w.V(lr + 1, b);
     return cf * Vf;
   }
-  if (p.convect == JCFD_CONVECT_VAN_LEER_LIMITERS) {
+  if constexpr (CONV == JCFD_CONVECT_VAN_LEER_LIMITERS) {
     const float Vf = 0.5f * w.V(lr, b) + 0.5f * w.V(lr + 1, b);
     return vll_flux(w.U(lr, b - 1), w.U(lr, b), w.U(lr, b + 1), w.U(lr, b + 2), Vf, h);
   }
@@ -153,16 +153,16 @@ __device__ __forceinline__ float face_fy_u(const WIN& w, int lr, int b, const Fv
                  w.U(a, b + 2), h);
 }
 
-template <typename WIN>
+template <int CONV, typename WIN>
 __device__ __forceinline__ float face_fx_v(const WIN& w, int a, int j, const FvmParams& p) {
   const float h = p.dt_over_dx;
   const int c = a - 1;
-  if (p.convect == JCFD_CONVECT_LINEAR) {  // (v at the face) * (u at the face)
+  if constexpr (CONV == JCFD_CONVECT_LINEAR) {  // (v at the face) * (u at the face)
     const float cf = 0.5f * w.V(c, j) + 0.5f * w.V(c + 1, j);
     const float Uf = 0.5f * w.U(c, j) + 0.5f * w.U(c, j + 1);
     return cf * Uf;
   }
-  if (p.convect == JCFD_CONVECT_VAN_LEER_LIMITERS) {
+  if constexpr (CONV == JCFD_CONVECT_VAN_LEER_LIMITERS) {
     const float Uf = 0.5f * w.U(c, j) + 0.5f * w.U(c, j + 1);
     return vll_flux(w.V(c - 1, j), w.V(c, j), w.V(c + 1, j), w.V(c + 2, j), Uf, h);
   }
@@ -172,14 +172,14 @@ __device__ __forceinline__ float face_fx_v(const WIN& w, int a, int j, const Fvm
                  w.V(c + 2, b), h);
 }
 
-template <typename WIN>
+template <int CONV, typename WIN>
 __device__ __forceinline__ float face_fy_v(const WIN& w, int lr, int j, const FvmParams& p) {
   const float h = p.dt_over_dy;
-  if (p.convect == JCFD_CONVECT_LINEAR) {  // v and its velocity both at the face
+  if constexpr (CONV == JCFD_CONVECT_LINEAR) {  // v and its velocity both at the face
     const float Vf = 0.5f * w.V(lr, j) + 0.5f * w.V(lr, j + 1);
     return Vf * Vf;
   }
-  if (p.convect == JCFD_CONVECT_VAN_LEER_LIMITERS) {
+  if constexpr (CONV == JCFD_CONVECT_VAN_LEER_LIMITERS) {
     const float Vf = 0.5f * w.V(lr, j) + 0.5f * w.V(lr, j + 1);
     return vll_flux(w.V(lr, j - 1), w.V(lr, j), w.V(lr, j + 1), w.V(lr, j + 2), Vf, h);
   }
@@ -282,7 +282,7 @@ struct MarchLds {
   static constexpr int bytes = vs + N * 4;
 };
 
-template <int N, int RB, int R = 8>
+template <int N, int RB, int R = 8, int CONV = JCFD_CONVECT_VAN_LEER>
 __device__ __forceinline__ void rows_forward_march_body(const FvmParams& p, const FvmBuffers& b,
                                                         int i0, int tid, char* smem) {
   using L = MarchLds<N, RB>;
@@ -324,8 +324,8 @@ __device__ __forceinline__ void rows_forward_march_body(const FvmParams& p, cons
 #pragma unroll
   for (int c = 0; c < CPT; ++c) {
     const int j = tid + c * kThreads;
-    fx_prev[c] = face_fx_u(w, 2, j, p);  // x face below the first u row (i0 - 1)
-    gx_prev[c] = face_fx_v(w, 3, j, p);  // x face below the first v row (i0)
+    fx_prev[c] = face_fx_u<CONV>(w, 2, j, p);  // x face below the first u row (i0 - 1)
+    gx_prev[c] = face_fx_v<CONV>(w, 3, j, p);  // x face below the first v row (i0)
     us_prev[c] = 0.f;
   }
   // step s: cells of ring row a = s + 2 (global row i0 - 1 + s); u for every
@@ -346,8 +346,8 @@ __device__ __forceinline__ void rows_forward_march_body(const FvmParams& p, cons
 #pragma unroll
     for (int c = 0; c < CPT; ++c) {
       const int j = tid + c * kThreads;
-      fy[j] = face_fy_u(w, a, j, p);
-      if (vrow) gy[j] = face_fy_v(w, a, j, p);
+      fy[j] = face_fy_u<CONV>(w, a, j, p);
+      if (vrow) gy[j] = face_fy_v<CONV>(w, a, j, p);
     }
     __syncthreads();
     float us_cur[CPT];
@@ -355,14 +355,14 @@ __device__ __forceinline__ void rows_forward_march_body(const FvmParams& p, cons
     for (int c = 0; c < CPT; ++c) {
       const int j = tid + c * kThreads;
       const int jm = (j - 1) & (N - 1);
-      const float fx_next = face_fx_u(w, a + 1, j, p);
+      const float fx_next = face_fx_u<CONV>(w, a + 1, j, p);
       const float adv_u = -(over_step(fx_next - fx_prev[c], p.dx, p.rcp_dx) +
                             over_step(fy[j] - fy[jm], p.dy, p.rcp_dy));
       const float ku = add_diffusion_forcing<false>(w, a, j, p, fxj[c], adv_u);
       us_cur[c] = w.U(a, j) + p.dt * ku;
       fx_prev[c] = fx_next;
       if (vrow) {
-        const float gx_next = face_fx_v(w, a + 1, j, p);
+        const float gx_next = face_fx_v<CONV>(w, a + 1, j, p);
         const float adv_v = -(over_step(gx_next - gx_prev[c], p.dx, p.rcp_dx) +
                               over_step(gy[j] - gy[jm], p.dy, p.rcp_dy));
         const float kv = add_diffusion_forcing<true>(w, a, j, p, 0.f, adv_v);
@@ -409,11 +409,24 @@ __device__ __forceinline__ void rows_forward_march_body(const FvmParams& p, cons
   }
 }
 
+// The convection scheme is a template argument of the stencil bodies, chosen
+// once per workgroup here (a run-time choice per face costs the VALU-bound
+// stencil ~15 %: the extra branches split the face computations apart).
+#define JCFD_CONVECT_DISPATCH(CALL)                                   \
+  switch (p.convect) {                                               \
+    case JCFD_CONVECT_VAN_LEER_LIMITERS: CALL(JCFD_CONVECT_VAN_LEER_LIMITERS); break; \
+    case JCFD_CONVECT_LINEAR: CALL(JCFD_CONVECT_LINEAR); break;      \
+    default: CALL(JCFD_CONVECT_VAN_LEER); break;                     \
+  }
+
 template <int N, int RB, int R = 8>
 __global__ __launch_bounds__(kThreads) void fvm_rows_forward_march(FvmParams p, FvmBuffers b) {
   __shared__ __align__(16) char smem[MarchLds<N, RB>::bytes];
-  rows_forward_march_body<N, RB, R>(p, fvm_instance(b, N), xcd_tile(blockIdx.x, gridDim.x) * RB,
-                                     threadIdx.x, smem);
+  const FvmBuffers bi = fvm_instance(b, N);
+  const int i0 = xcd_tile(blockIdx.x, gridDim.x) * RB;
+#define JCFD_MARCH(C) rows_forward_march_body<N, RB, R, C>(p, bi, i0, threadIdx.x, smem)
+  JCFD_CONVECT_DISPATCH(JCFD_MARCH)
+#undef JCFD_MARCH
 }
 
 template <int N, int W>
@@ -441,7 +454,7 @@ struct RowsProjectLds {
 // MODE 5: Runge-Kutta combination u* = u0 + dt * (0 + sum c_t k_t) with
 //         u0 = (b.u, b.v) and the terms of *b.rk (time_stepping.py:91-101);
 //         write u*, v*, spectrum of div u*.
-template <int N, int RB, int MODE, int R = 8>
+template <int N, int RB, int MODE, int R = 8, int CONV = JCFD_CONVECT_VAN_LEER>
 __device__ __forceinline__ void rows_forward_body(const FvmParams& p, const FvmBuffers& b, int i0,
                                                   int tid, char* smem) {
   constexpr int H = N / 2;
@@ -523,9 +536,9 @@ __device__ __forceinline__ void rows_forward_body(const FvmParams& p, const FvmB
     constexpr int PV = (NV * N + kThreads - 1) / kThreads;
     float ru[PU], rv[PV];  // explicit terms (MODE 2) or u*, v* of this thread's cells
     // ---- u: x faces of rows LO_U .. HI + 1, y faces of rows LO_U .. HI
-    for (int e = tid; e < (NU + 1) * N; e += kThreads) flux[e] = face_fx_u(w, LO_U + e / N, e % N, p);
+    for (int e = tid; e < (NU + 1) * N; e += kThreads) flux[e] = face_fx_u<CONV>(w, LO_U + e / N, e % N, p);
     for (int e = tid; e < NU * N; e += kThreads)
-      flux[(NU + 1) * N + e] = face_fy_u(w, LO_U + e / N, e % N, p);
+      flux[(NU + 1) * N + e] = face_fy_u<CONV>(w, LO_U + e / N, e % N, p);
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < PU; ++k) {
@@ -541,9 +554,9 @@ __device__ __forceinline__ void rows_forward_body(const FvmParams& p, const FvmB
     }
     __syncthreads();
     // ---- v: x faces of rows 3 .. HI + 1, y faces of rows 3 .. HI
-    for (int e = tid; e < (NV + 1) * N; e += kThreads) flux[e] = face_fx_v(w, 3 + e / N, e % N, p);
+    for (int e = tid; e < (NV + 1) * N; e += kThreads) flux[e] = face_fx_v<CONV>(w, 3 + e / N, e % N, p);
     for (int e = tid; e < NV * N; e += kThreads)
-      flux[(NV + 1) * N + e] = face_fy_v(w, 3 + e / N, e % N, p);
+      flux[(NV + 1) * N + e] = face_fy_v<CONV>(w, 3 + e / N, e % N, p);
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < PV; ++k) {
@@ -613,8 +626,15 @@ __device__ __forceinline__ void rows_forward_body(const FvmParams& p, const FvmB
 template <int N, int RB, int MODE, int R = 8>
 __global__ __launch_bounds__(kThreads) void fvm_rows_forward(FvmParams p, FvmBuffers b) {
   __shared__ __align__(16) char smem[RowsForwardLds<N, RB, MODE>::bytes];
-  rows_forward_body<N, RB, MODE, R>(p, fvm_instance(b, N), xcd_tile(blockIdx.x, gridDim.x) * RB,
-                                    threadIdx.x, smem);
+  const FvmBuffers bi = fvm_instance(b, N);
+  const int i0 = xcd_tile(blockIdx.x, gridDim.x) * RB;
+  if constexpr (RowsForwardLds<N, RB, MODE>::kWin) {
+#define JCFD_ROWS(C) rows_forward_body<N, RB, MODE, R, C>(p, bi, i0, threadIdx.x, smem)
+    JCFD_CONVECT_DISPATCH(JCFD_ROWS)
+#undef JCFD_ROWS
+  } else {
+    rows_forward_body<N, RB, MODE, R>(p, bi, i0, threadIdx.x, smem);
+  }
 }
 
 // Column pass of the Poisson solve on the tile of W packed-spectrum columns
@@ -1065,10 +1085,16 @@ template <int N, int PHASE>
 __device__ __forceinline__ void group_tile(const FvmGroupDesc& d, int tile, int tid, char* smem) {
   using G = GroupTiles<N>;
   if constexpr (PHASE == 0) {
-    if constexpr (G::kMarch)
-      rows_forward_march_body<N, G::RB_FWD>(d.p, d.b, tile * G::RB_FWD, tid, smem);
-    else
-      rows_forward_body<N, G::RB_FWD, 0>(d.p, d.b, tile * G::RB_FWD, tid, smem);
+    const FvmParams& p = d.p;
+    if constexpr (G::kMarch) {
+#define JCFD_GMARCH(C) rows_forward_march_body<N, G::RB_FWD, 8, C>(d.p, d.b, tile * G::RB_FWD, tid, smem)
+      JCFD_CONVECT_DISPATCH(JCFD_GMARCH)
+#undef JCFD_GMARCH
+    } else {
+#define JCFD_GROWS(C) rows_forward_body<N, G::RB_FWD, 0, 8, C>(d.p, d.b, tile * G::RB_FWD, tid, smem)
+      JCFD_CONVECT_DISPATCH(JCFD_GROWS)
+#undef JCFD_GROWS
+    }
   } else if constexpr (PHASE == 1) {
     poisson_cols_body<N, G::W>(d.b, 1.0f, tile * G::W, tid, smem);
   } else {
