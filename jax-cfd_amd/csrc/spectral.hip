@@ -444,6 +444,8 @@ __global__ __launch_bounds__(kThreads) void ns_state_convert(const float2* __res
     dst[base + static_cast<size_t>(p) * H1 + c] = src[base + t];
 }
 
+constexpr int kForceRegs = 4;  // forcing-curl entries kept in registers
+
 // 2/3 rule mask (spectral/utils.py:128-134).
 __device__ __forceinline__ float ns_filter(const NsParams& q, int p, int c) {
   if (!q.smooth) return 1.0f;
@@ -535,6 +537,19 @@ __global__ __launch_bounds__(kThreads, (NsColWaves<N, W>::value)) void ns_cols_f
     return update_with(p, H, fv, u_in[idx], ho);
   };
 
+  // The forcing-curl entries, read once (the element loop's stores could
+  // alias them as far as the compiler knows, so it would reload them for
+  // every element); longer lists are read from memory in the loop.
+  int fpr[kForceRegs], fcr[kForceRegs];
+  float2 fvr[kForceRegs];
+#pragma unroll
+  for (int t = 0; t < kForceRegs; ++t) {
+    const int tt = t < q.n_force ? t : 0;
+    fpr[t] = q.n_force > 0 ? q.force_idx[2 * tt] : -1;
+    fcr[t] = q.n_force > 0 ? q.force_idx[2 * tt + 1] : -1;
+    fvr[t] = q.n_force > 0 ? q.force_val[tt] : make_float2(0.f, 0.f);
+  }
+
   // Per element: F at (p, cg) (filter, then the sparse, time-constant forcing
   // curl entries), then the update (or F itself for MODE 0); tile 0 also
   // handles column N/2.  One pass, so no array of F values stays live.
@@ -558,10 +573,18 @@ __global__ __launch_bounds__(kThreads, (NsColWaves<N, W>::value)) void ns_cols_f
         const float m = ns_filter(q, p, cg);
         f = make_float2(y.x * m, y.y * m);
       }
-      for (int t = 0; t < q.n_force; ++t) {
-        const int fp = q.force_idx[2 * t], fc = q.force_idx[2 * t + 1];
-        if (fp == p && fc == cg) f = cadd(f, q.force_val[t]);
-        if (cg == 0 && fp == p && fc == H) fh = cadd(fh, q.force_val[t]);
+      if (q.n_force <= kForceRegs) {
+#pragma unroll
+        for (int t = 0; t < kForceRegs; ++t) {
+          if (t < q.n_force && fpr[t] == p && fcr[t] == cg) f = cadd(f, fvr[t]);
+          if (t < q.n_force && cg == 0 && fpr[t] == p && fcr[t] == H) fh = cadd(fh, fvr[t]);
+        }
+      } else {
+        for (int t = 0; t < q.n_force; ++t) {
+          const int fp = q.force_idx[2 * t], fc = q.force_idx[2 * t + 1];
+          if (fp == p && fc == cg) f = cadd(f, q.force_val[t]);
+          if (cg == 0 && fp == p && fc == H) fh = cadd(fh, q.force_val[t]);
+        }
       }
       if constexpr (MODE == 0) {
         u_out[p * H1 + cg] = f;
