@@ -2,7 +2,9 @@
 //
 // One forward-Euler step of semi_implicit_navier_stokes
 // (jax_cfd/base/equations.py:120-151, time_stepping.py:59-118) with
-// advect_van_leer convection (advection.py:210-342), explicit diffusion
+// advect_van_leer convection (advection.py:210-342; or
+// advect_van_leer_using_limiters, 387-395, or convect_linear, 184-207, chosen
+// by a template argument of the stencil bodies), explicit diffusion
 // (diffusion.py:35-37), Kolmogorov + linear forcing (forcings.py:63-178) and
 // the periodic solve_fast_diag projection (pressure.py:115-198) runs as three
 // kernels over one instance:
@@ -23,7 +25,8 @@
 // Layout: u, v are (n, n) fp32, C-contiguous, axis 1 (y) fastest, offsets
 // (1, .5) and (.5, 1).  Threads run along y so every HBM access is
 // coalesced.  The arithmetic keeps the reference's operation order (no FMA
-// contraction in this file) so fp32 results track jax-cfd's to round-off.
+// contraction in this file; quotients by the grid steps are the correctly
+// rounded ones, see over_step) so fp32 results track jax-cfd's to round-off.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
