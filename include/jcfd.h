@@ -77,6 +77,7 @@ const char* jcfd_version(void);
 #define JCFD_CONVECT_VAN_LEER 0          /* advection.py:210-342 */
 #define JCFD_CONVECT_VAN_LEER_LIMITERS 1 /* advection.py:387-395 */
 #define JCFD_CONVECT_LINEAR 2            /* convect_linear, advection.py:119-123, 184-207 */
+#define JCFD_CONVECT_UPWIND 3            /* advect_upwind, advection.py:126-130 */
 #define JCFD_PRESSURE_FAST_DIAG 0        /* pressure.py:115-157 */
 #define JCFD_PRESSURE_CG 1               /* pressure.py:67-112 */
 #define JCFD_DIFFUSION_EXPLICIT 0        /* semi_implicit_navier_stokes: diffuse (diffusion.py:35-37) */

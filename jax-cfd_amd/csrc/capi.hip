@@ -510,7 +510,7 @@ int jcfd_fvm_plan_create(const jcfd_fvm_config* config, jcfd_fvm_plan** plan) {
   if (!std::isfinite(c.dt) || !std::isfinite(c.density) || c.density == 0)
     return fail(JCFD_ERR_INVALID_ARGUMENT, "dt and density must be finite, density non-zero");
   if (c.convect != JCFD_CONVECT_VAN_LEER && c.convect != JCFD_CONVECT_VAN_LEER_LIMITERS &&
-      c.convect != JCFD_CONVECT_LINEAR)
+      c.convect != JCFD_CONVECT_LINEAR && c.convect != JCFD_CONVECT_UPWIND)
     return fail(JCFD_ERR_UNSUPPORTED,
                 "convection must be advect_van_leer or advect_van_leer_using_limiters");
   if (c.pressure != JCFD_PRESSURE_FAST_DIAG && c.pressure != JCFD_PRESSURE_CG)

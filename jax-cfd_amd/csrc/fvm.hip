@@ -3,8 +3,9 @@
 // One forward-Euler step of semi_implicit_navier_stokes
 // (jax_cfd/base/equations.py:120-151, time_stepping.py:59-118) with
 // advect_van_leer convection (advection.py:210-342; or
-// advect_van_leer_using_limiters, 387-395, or convect_linear, 184-207, chosen
-// by a template argument of the stencil bodies), explicit diffusion
+// advect_van_leer_using_limiters, 387-395, convect_linear, 184-207, or
+// advect_upwind, 126-130, chosen by a template argument of the stencil
+// bodies), explicit diffusion
 // (diffusion.py:35-37), Kolmogorov + linear forcing (forcings.py:63-178) and
 // the periodic solve_fast_diag projection (pressure.py:115-198) runs as three
 // kernels over one instance:
@@ -129,6 +130,10 @@ __device__ __forceinline__ float face_fx_u(const WIN& w, int a, int j, const Fvm
     const float Uf = 0.5f * w.U(a - 1, j) + 0.5f * w.U(a, j);
     return Uf * Uf;
   }
+  if constexpr (CONV == JCFD_CONVECT_UPWIND) {  // upwind u times the face velocity
+    const float Uf = 0.5f * w.U(a - 1, j) + 0.5f * w.U(a, j);
+    return ((Uf > 0.f) ? w.U(a - 1, j) : w.U(a, j)) * Uf;
+  }
   if constexpr (CONV == JCFD_CONVECT_VAN_LEER_LIMITERS) {
     const int c = a - 1;  // face between rows c and c + 1
     const float Uf = 0.5f * w.U(c, j) + 0.5f * w.U(c + 1, j);
@@ -145,6 +150,10 @@ __device__ __forceinline__ float face_fy_u(const WIN& w, int lr, int b, const Fv
     const float cf = 0.5f * w.U(lr, b) + 0.5f * w.U(lr, b + 1);
     const float Vf = 0.5f * w.V(lr, b) + 0.5f * w.V(lr + 1, b);
     return cf * Vf;
+  }
+  if constexpr (CONV == JCFD_CONVECT_UPWIND) {
+    const float Vf = 0.5f * w.V(lr, b) + 0.5f * w.V(lr + 1, b);
+    return ((Vf > 0.f) ? w.U(lr, b) : w.U(lr, b + 1)) * Vf;
   }
   if constexpr (CONV == JCFD_CONVECT_VAN_LEER_LIMITERS) {
     const float Vf = 0.5f * w.V(lr, b) + 0.5f * w.V(lr + 1, b);
@@ -165,6 +174,10 @@ __device__ __forceinline__ float face_fx_v(const WIN& w, int a, int j, const Fvm
     const float Uf = 0.5f * w.U(c, j) + 0.5f * w.U(c, j + 1);
     return cf * Uf;
   }
+  if constexpr (CONV == JCFD_CONVECT_UPWIND) {
+    const float Uf = 0.5f * w.U(c, j) + 0.5f * w.U(c, j + 1);
+    return ((Uf > 0.f) ? w.V(c, j) : w.V(c + 1, j)) * Uf;
+  }
   if constexpr (CONV == JCFD_CONVECT_VAN_LEER_LIMITERS) {
     const float Uf = 0.5f * w.U(c, j) + 0.5f * w.U(c, j + 1);
     return vll_flux(w.V(c - 1, j), w.V(c, j), w.V(c + 1, j), w.V(c + 2, j), Uf, h);
@@ -181,6 +194,10 @@ __device__ __forceinline__ float face_fy_v(const WIN& w, int lr, int j, const Fv
   if constexpr (CONV == JCFD_CONVECT_LINEAR) {  // v and its velocity both at the face
     const float Vf = 0.5f * w.V(lr, j) + 0.5f * w.V(lr, j + 1);
     return Vf * Vf;
+  }
+  if constexpr (CONV == JCFD_CONVECT_UPWIND) {
+    const float Vf = 0.5f * w.V(lr, j) + 0.5f * w.V(lr, j + 1);
+    return ((Vf > 0.f) ? w.V(lr, j) : w.V(lr, j + 1)) * Vf;
   }
   if constexpr (CONV == JCFD_CONVECT_VAN_LEER_LIMITERS) {
     const float Vf = 0.5f * w.V(lr, j) + 0.5f * w.V(lr, j + 1);
@@ -419,6 +436,7 @@ __device__ __forceinline__ void rows_forward_march_body(const FvmParams& p, cons
   switch (p.convect) {                                               \
     case JCFD_CONVECT_VAN_LEER_LIMITERS: CALL(JCFD_CONVECT_VAN_LEER_LIMITERS); break; \
     case JCFD_CONVECT_LINEAR: CALL(JCFD_CONVECT_LINEAR); break;      \
+    case JCFD_CONVECT_UPWIND: CALL(JCFD_CONVECT_UPWIND); break;      \
     default: CALL(JCFD_CONVECT_VAN_LEER); break;                     \
   }
 

@@ -27,6 +27,7 @@ JCFD_ERR_BUDGET = -5
 CONVECT_VAN_LEER = 0
 CONVECT_VAN_LEER_LIMITERS = 1
 CONVECT_LINEAR = 2
+CONVECT_UPWIND = 3
 PRESSURE_FAST_DIAG = 0
 PRESSURE_CG = 1
 DIFFUSION_EXPLICIT = 0

@@ -1,7 +1,7 @@
 """Advection operators (jax_cfd/base/advection.py).
 
-``advect_van_leer``, ``advect_van_leer_using_limiters`` and
-``advect_linear`` / ``convect_linear`` are the schemes the FVM kernels
+``advect_van_leer``, ``advect_van_leer_using_limiters``, ``advect_upwind``
+and ``advect_linear`` / ``convect_linear`` are the schemes the FVM kernels
 implement.  Inside a step factory they are read symbolically; called
 directly on a velocity component they run the explicit-terms kernel with
 diffusion and forcing switched off.
@@ -54,6 +54,15 @@ def advect_linear(c: grids.GridVariable, v: grids.GridVariableVector,
   if _symbolic.is_probe(c):
     return _probe_term('advect_linear', c, v, dt)
   return _eager(c, v, 0.0 if dt is None else dt, _lib.CONVECT_LINEAR)
+
+
+def advect_upwind(c: grids.GridVariable, v: grids.GridVariableVector,
+                  dt: Optional[float] = None) -> grids.GridArray:
+  """advect_general with linear face velocities and first-order upwind
+  values of c (advection.py:126-130)."""
+  if _symbolic.is_probe(c):
+    return _probe_term('advect_upwind', c, v, dt)
+  return _eager(c, v, 0.0 if dt is None else dt, _lib.CONVECT_UPWIND)
 
 
 def convect_linear(v: grids.GridVariableVector) -> Tuple[grids.GridArray, ...]:

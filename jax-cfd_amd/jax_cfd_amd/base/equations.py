@@ -66,10 +66,12 @@ def _read_convection(convect, grid, dt):
     raise NotImplementedError('convect must treat both components alike')
   kinds_built = {'advect_van_leer': _lib.CONVECT_VAN_LEER,
                  'advect_van_leer_using_limiters': _lib.CONVECT_VAN_LEER_LIMITERS,
-                 'advect_linear': _lib.CONVECT_LINEAR}
+                 'advect_linear': _lib.CONVECT_LINEAR,
+                 'advect_upwind': _lib.CONVECT_UPWIND}
   if kinds[0] not in kinds_built:
     raise NotImplementedError(f'convection {kinds[0]} is not built')
-  if kinds[0] != 'advect_linear' and dts[0] != dt:  # linear advection has no dt
+  # linear and upwind advection take no dt
+  if kinds[0] not in ('advect_linear', 'advect_upwind') and dts[0] != dt:
     raise NotImplementedError('advection dt differing from the step dt')
   return kinds_built[kinds[0]]
 

@@ -190,8 +190,8 @@ class FVMStep:
 
   convect: 'van_leer' (advect_van_leer for every component, as the
   channel-flow notebook wires it), 'van_leer_limiters' (the factory default,
-  advect_van_leer_using_limiters) or 'linear' (convect_linear, the
-  validation tests' scheme).  pressure: 'fast_diag' or 'cg'.
+  advect_van_leer_using_limiters), 'linear' (convect_linear, the
+  validation tests' scheme) or 'upwind' (advect_upwind).  pressure: 'fast_diag' or 'cg'.
   """
   grid: Grid
   dt: float
@@ -211,6 +211,8 @@ class FVMStep:
       fn = go.advect_van_leer_using_limiters
     elif self.convect == 'linear':  # convect_linear (advection.py:184-207)
       return tuple(go.advect_linear(u, v, self.grid).data for u in v)
+    elif self.convect == 'upwind':  # advect_upwind (advection.py:126-130)
+      return tuple(go.advect_upwind(u, v, self.grid).data for u in v)
     else:
       raise ValueError(self.convect)
     return tuple(fn(u, v, self.dt, self.grid).data for u in v)

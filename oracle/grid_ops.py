@@ -321,6 +321,17 @@ def advect_linear(c: A, v: Sequence[A], grid: Grid) -> A:
   return A(-adv.data, adv.offset)
 
 
+def advect_upwind(c: A, v: Sequence[A], grid: Grid) -> A:
+  """advect_general(c, v, linear, upwind) (advection.py:34-116, 126-130)."""
+  targets = control_volume_offsets(c)
+  aligned_v = tuple(linear(u, t) for u, t in zip(v, targets))
+  aligned_c = tuple(upwind(c, t, aligned_v) for t in targets)
+  flux = tuple(A(cc.data * u.data, u.offset)
+               for cc, u in zip(aligned_c, aligned_v))
+  adv = divergence(flux, grid)
+  return A(-adv.data, adv.offset)
+
+
 def advect_van_leer_using_limiters(c: A, v: Sequence[A], dt: float,
                                    grid: Grid) -> A:
   """advect_general(c, v, linear, tvd(lax_wendroff)) (advection.py:34-116, 387-395)."""

@@ -138,32 +138,35 @@ def test_van_leer_using_limiters_parity(n, forced):
       assert err <= 1e-3
 
 
-@pytest.mark.parametrize("n", [64, 256])
-def test_convect_linear_parity(n):
-  """convect_linear (advection.py:184-207) in the explicit terms and one step
-  against the oracle: explicit terms to 1e-6 (same operation order), one
-  step max relative error <= 1e-5."""
+@pytest.mark.parametrize("n,scheme", [(64, 'linear'), (256, 'linear'), (64, 'upwind'),
+                                      (256, 'upwind')])
+def test_linear_and_upwind_convection_parity(n, scheme):
+  """convect_linear (advection.py:184-207) and advect_upwind (126-130) in the
+  explicit terms and one step against the oracle: explicit terms to 1e-6
+  (same operation order), one step max relative error <= 1e-5."""
   cfd = _cfd()
   ogrid, v0, dt, ostep, _ = _config(n, 5, 2.0, True)
   forcing = cfd.forcings.simple_turbulence_forcing(_grid(n), 1.0, 4, -0.1, 'kolmogorov')
-  ostep = ofvm.FVMStep(ogrid, dt=dt, viscosity=1e-3, forcing=ostep.forcing, convect='linear')
+  ostep = ofvm.FVMStep(ogrid, dt=dt, viscosity=1e-3, forcing=ostep.forcing, convect=scheme)
+  convect = (cfd.advection.convect_linear if scheme == 'linear' else
+             (lambda v: tuple(cfd.advection.advect_upwind(u, v, dt) for u in v)))
   et = cfd.equations.navier_stokes_explicit_terms(
-      1.0, 1e-3, dt, _grid(n), convect=cfd.advection.convect_linear, forcing=forcing)
+      1.0, 1e-3, dt, _grid(n), convect=convect, forcing=forcing)
   for o, r in zip(et(_state(v0, n)), ostep.explicit_terms(v0)):
     err = _maxrel(o.data.cpu().numpy(), r.data)
-    print(f"convect_linear explicit terms n={n}: {err:.2e}")
+    print(f"{scheme} explicit terms n={n}: {err:.2e}")
     assert err <= 1e-6
   step = cfd.equations.semi_implicit_navier_stokes(
-      1.0, 1e-3, dt, _grid(n), convect=cfd.advection.convect_linear, forcing=forcing)
+      1.0, 1e-3, dt, _grid(n), convect=convect, forcing=forcing)
   for o, r in zip(step(_state(v0, n)), ostep(v0)):
     err = _maxrel(o.data.cpu().numpy(), r.data)
-    print(f"convect_linear one step n={n}: {err:.2e}")
+    print(f"{scheme} one step n={n}: {err:.2e}")
     assert err <= 1e-5
-  # the per-component form gives the same plan
-  step2 = cfd.equations.semi_implicit_navier_stokes(
-      1.0, 1e-3, dt, _grid(n), forcing=forcing,
-      convect=lambda v: tuple(cfd.advection.advect_linear(u, v) for u in v))
-  assert step2.spec == step.spec
+  if scheme == 'linear':  # the per-component form gives the same plan
+    step2 = cfd.equations.semi_implicit_navier_stokes(
+        1.0, 1e-3, dt, _grid(n), forcing=forcing,
+        convect=lambda v: tuple(cfd.advection.advect_linear(u, v) for u in v))
+    assert step2.spec == step.spec
 
 
 @pytest.mark.parametrize("n", [32, 64, 256])

@@ -571,3 +571,49 @@ def test_taylor_green_linear_second_order():
     ref = _taylor_green(grid, steps * dt, nu)
     errs.append(max(np.abs(a.data - b.data).max() for a, b in zip(v, ref)))
   assert errs[0] / errs[1] > 3.5
+
+
+# ------------------------------------------------------ upwind advection
+def test_upwind_advection_translates_gaussian():
+  """advection_test.py:99-104 (upwind_1D): 100 Euler steps at Courant 0.5
+  with unit velocity move the profile by 50 cells (atol 7e-2)."""
+  shape = (101,)
+  grid = Grid.from_step(shape, 1. / 101)
+  x = (np.arange(101) - 51) / 101.  # advection_test.py:30-34 profile
+  c = A(np.exp(-30. * x ** 2), grid.cell_center)
+  v = _unit_velocity(grid)
+  dt = 0.5 * grid.step[0]
+  c0 = c
+  for _ in range(100):
+    c = A(c.data + dt * go.advect_upwind(c, v, grid).data, c.offset)
+  np.testing.assert_allclose(c.data, np.roll(c0.data, 50), atol=7e-2)
+
+
+def test_upwind_velocity_advection_matches_hand_written_stencil():
+  """advect_upwind of each velocity component, written out per cell: face
+  velocities averaged, the upwind neighbour chosen by their sign."""
+  rng = np.random.default_rng(4)
+  nx, ny = 6, 5
+  grid = Grid.from_domain((nx, ny), ((0, 1.5), (0, 2.5)))
+  dx, dy = grid.step
+  u = rng.standard_normal((nx, ny))
+  v = rng.standard_normal((nx, ny))
+  vel = (A(u, grid.cell_faces[0]), A(v, grid.cell_faces[1]))
+  got_u = go.advect_upwind(vel[0], vel, grid).data
+  got_v = go.advect_upwind(vel[1], vel, grid).data
+  U = lambda i, j: u[i % nx, j % ny]  # noqa: E731
+  V = lambda i, j: v[i % nx, j % ny]  # noqa: E731
+
+  def up(vel_face, lo, hi):
+    return (lo if vel_face > 0 else hi) * vel_face
+
+  fxu = lambda i, j: up(0.5 * U(i, j) + 0.5 * U(i + 1, j), U(i, j), U(i + 1, j))  # noqa: E731
+  fyu = lambda i, j: up(0.5 * V(i, j) + 0.5 * V(i + 1, j), U(i, j), U(i, j + 1))  # noqa: E731
+  fxv = lambda i, j: up(0.5 * U(i, j) + 0.5 * U(i, j + 1), V(i, j), V(i + 1, j))  # noqa: E731
+  fyv = lambda i, j: up(0.5 * V(i, j) + 0.5 * V(i, j + 1), V(i, j), V(i, j + 1))  # noqa: E731
+  for i in range(nx):
+    for j in range(ny):
+      eu = -((fxu(i, j) - fxu(i - 1, j)) / dx + (fyu(i, j) - fyu(i, j - 1)) / dy)
+      ev = -((fxv(i, j) - fxv(i - 1, j)) / dx + (fyv(i, j) - fyv(i, j - 1)) / dy)
+      assert abs(got_u[i, j] - eu) <= 1e-12 * max(1.0, abs(eu))
+      assert abs(got_v[i, j] - ev) <= 1e-12 * max(1.0, abs(ev))
