@@ -1,4 +1,4 @@
-"""Initial velocity fields (jax_cfd/base/initial_conditions.py:40-121).
+"""Initial velocity fields (jax_cfd/base/initial_conditions.py:40-166).
 
 ``filtered_velocity_field`` follows the reference: N(0, 1) noise per
 component, spectral filter with a log-normal density, then three rounds of
@@ -11,7 +11,7 @@ from __future__ import annotations
 
 import concurrent.futures
 import threading
-from typing import Dict, Optional, Sequence, Tuple
+from typing import Callable, Dict, Optional, Sequence, Tuple
 
 import numpy as np
 import torch
@@ -134,4 +134,27 @@ def filtered_velocity_field(rng_key, grid: grids.Grid, maximum_velocity: float =
     v = pressure.projection(v)
     vmax = _max_speed(v)
     v = tuple(grids.GridVariable(maximum_velocity * u.array / vmax, u.bc) for u in v)
+  return v
+
+
+def initial_velocity_field(velocity_fns: Sequence[Callable], grid: grids.Grid,
+                           velocity_bc: Optional[Sequence] = None,
+                           pressure_solve: Callable = pressure.solve_fast_diag,
+                           iterations: Optional[int] = None,
+                           device: Optional[torch.device] = None):
+  """initial_conditions.py:124-166: each component evaluated on its face
+  mesh (fp32, as grid.eval_on_mesh), placed on the GPU, then projected
+  `iterations` times (if given) with `pressure_solve` on the GPU."""
+  if device is None:
+    device = torch.device('cuda', torch.cuda.current_device())
+  if velocity_bc is None:
+    velocity_bc = (boundaries.periodic_boundary_conditions(grid.ndim),) * grid.ndim
+  v = []
+  for fn, offset, bc in zip(velocity_fns, grid.cell_faces, velocity_bc):
+    arr = grid.eval_on_mesh(fn, offset)
+    data = torch.as_tensor(np.asarray(arr.data), dtype=torch.float32).to(device)
+    v.append(grids.GridVariable(grids.GridArray(data, offset, grid), bc))
+  v = tuple(v)
+  for _ in range(iterations or 0):
+    v = pressure.projection(v, pressure_solve)
   return v
