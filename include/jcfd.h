@@ -83,6 +83,8 @@ const char* jcfd_version(void);
 #define JCFD_DIFFUSION_EXPLICIT 0        /* semi_implicit_navier_stokes: diffuse (diffusion.py:35-37) */
 #define JCFD_DIFFUSION_IMPLICIT_FAST_DIAG 1 /* implicit_diffusion_navier_stokes (equations.py:154-195)
                                                 with diffusion.solve_fast_diag (diffusion.py:170-212) */
+#define JCFD_DIFFUSION_IMPLICIT_CG 2     /* ... with diffusion.solve_cg (diffusion.py:131-167):
+                                            CG on (1 - viscosity dt L) x = u per component, x0 = u */
 
 typedef struct {
   int32_t n;               /* cells per side; square periodic grid */
@@ -112,6 +114,8 @@ typedef struct {
                               kernels); up to 4 stages otherwise */
   double rk_a[6];          /* a[i-1][j], i = 1 .. stages-1, j < i, packed row by row */
   double rk_b[4];
+  double diff_cg_rtol, diff_cg_atol; /* diffusion.solve_cg tolerances (JCFD_DIFFUSION_IMPLICIT_CG) */
+  int32_t diff_cg_maxiter;           /* <= 0: 10 * n * n */
 } jcfd_fvm_config;
 
 typedef struct jcfd_fvm_plan jcfd_fvm_plan;
@@ -171,6 +175,11 @@ int jcfd_fvm_explicit_terms(jcfd_fvm_plan* plan, const jcfd_tensor* u, const jcf
  * iterations to read its convergence flag. */
 int32_t jcfd_fvm_cg_iterations(const jcfd_fvm_plan* plan);
 
+/* Iterations of the plan's last CG diffusion solve of velocity component
+ * `component` (0: u, 1: v; diffusion == JCFD_DIFFUSION_IMPLICIT_CG).  The same
+ * kind of diagnostic; the first convergence check comes after 8 iterations. */
+int32_t jcfd_fvm_diffusion_cg_iterations(const jcfd_fvm_plan* plan, int32_t component);
+
 /* pressure.projection(v, solve) (pressure.py:181-198) with the plan's solve
  * (solve_fast_diag or solve_cg). */
 int jcfd_fvm_projection(jcfd_fvm_plan* plan, const jcfd_tensor* u, const jcfd_tensor* v,
@@ -178,8 +187,10 @@ int jcfd_fvm_projection(jcfd_fvm_plan* plan, const jcfd_tensor* u, const jcfd_te
 
 /* diffusion.solve_fast_diag(v, viscosity, dt) (diffusion.py:170-212),
  * periodic: each component becomes (1 - viscosity dt L)^-1 u, computed as
- * u + irfft2(D rfft2(u)).  Implicit-diffusion plans only (the plan's
- * viscosity and dt).  u_out/v_out may alias u/v. */
+ * u + irfft2(D rfft2(u)); or, for JCFD_DIFFUSION_IMPLICIT_CG plans,
+ * diffusion.solve_cg (diffusion.py:137-167), CG from x0 = u with the plan's
+ * diff_cg_* tolerances.  Implicit-diffusion plans only (the plan's viscosity
+ * and dt).  u_out/v_out may alias u/v. */
 int jcfd_fvm_diffuse(jcfd_fvm_plan* plan, const jcfd_tensor* u, const jcfd_tensor* v,
                      jcfd_tensor* u_out, jcfd_tensor* v_out, void* stream);
 

@@ -307,6 +307,10 @@ struct jcfd_fvm_plan {
   int* status_host = nullptr;  // pinned
   int cg_maxiter = 0;
   int cg_last_iterations = 0;
+  // CG implicit diffusion (diffusion == JCFD_DIFFUSION_IMPLICIT_CG), same buffers
+  float diff_coef = 0.f;  // fp32(dt * viscosity)
+  int diff_maxiter = 0;
+  int diff_last_iterations[2] = {0, 0};
   float* us = nullptr;
   float* vs = nullptr;
   float2* spec = nullptr;
@@ -404,29 +408,74 @@ int cg_alloc(jcfd_fvm_plan* p) {
 
 constexpr int kCgChunk = 64;  // iterations enqueued between convergence checks
 
-// CG on p->rhs into p->cg.x (solve_cg); reads the convergence flag between
-// chunks of iterations, so this synchronises the stream.
-int cg_solve(jcfd_fvm_plan* p, hipStream_t s) {
-  const float rtol = static_cast<float>(p->cfg.cg_rtol), atol = static_cast<float>(p->cfg.cg_atol);
-  int rc = cg_launch_init(p->params, p->rhs, p->cg, rtol, atol, p->cg_maxiter, s);
-  for (int k = 0; rc == JCFD_OK; k += kCgChunk) {
-    for (int i = 0; i < kCgChunk && rc == JCFD_OK; ++i)
-      rc = cg_launch_iteration(p->params, p->cg, k + i, rtol, atol, p->cg_maxiter, s);
+// Enqueues CG iterations (iterate(k, s)) in chunks and reads the convergence
+// flag between chunks, so this synchronises the stream.  The first chunk is
+// `first` iterations long, the rest kCgChunk.  Returns the iteration count in
+// *iterations.
+template <typename Iterate>
+int cg_iterate(jcfd_fvm_plan* p, int maxiter, int first, int* iterations, hipStream_t s,
+               Iterate iterate) {
+  int rc = JCFD_OK;
+  for (int k = 0, chunk = first; rc == JCFD_OK; k += chunk, chunk = kCgChunk) {
+    for (int i = 0; i < chunk && rc == JCFD_OK; ++i) rc = iterate(k + i, s);
     if (rc) break;
     if ((rc = check_hip(hipMemcpyAsync(p->status_host, p->cg.status, 2 * sizeof(int),
                                        hipMemcpyDeviceToHost, s), "cg status")) ||
         (rc = check_hip(hipStreamSynchronize(s), "cg status sync")))
       break;
     if (p->status_host[0]) {
-      p->cg_last_iterations = p->status_host[1];
+      *iterations = p->status_host[1];
       break;
     }
-    if (k + kCgChunk > p->cg_maxiter) {
-      p->cg_last_iterations = p->cg_maxiter;
+    if (k + chunk > maxiter) {
+      *iterations = maxiter;
       break;
     }
   }
   return rc;
+}
+
+// CG on p->rhs into p->cg.x (solve_cg).
+int cg_solve(jcfd_fvm_plan* p, hipStream_t s) {
+  const float rtol = static_cast<float>(p->cfg.cg_rtol), atol = static_cast<float>(p->cfg.cg_atol);
+  int rc = cg_launch_init(p->params, p->rhs, p->cg, rtol, atol, p->cg_maxiter, s);
+  if (rc) return rc;
+  return cg_iterate(p, p->cg_maxiter, kCgChunk, &p->cg_last_iterations, s, [&](int k, hipStream_t st) {
+    return cg_launch_iteration(p->params, p->cg, k, rtol, atol, p->cg_maxiter, st);
+  });
+}
+
+// diffusion.solve_cg (diffusion.py:137-167) in place on both components of
+// an instance.  The system is close to the identity for stable time steps,
+// so the first chunk is short.
+constexpr int kDiffusionFirstChunk = 8;
+
+int cg_diffuse(jcfd_fvm_plan* p, float* u, float* v, hipStream_t s) {
+  const float rtol = static_cast<float>(p->cfg.diff_cg_rtol);
+  const float atol = static_cast<float>(p->cfg.diff_cg_atol);
+  const size_t bytes = sizeof(float) * static_cast<size_t>(p->params.n) * p->params.n;
+  float* comps[2] = {u, v};
+  for (int m = 0; m < 2; ++m) {
+    int rc = cg_launch_init_helmholtz(p->params, comps[m], p->cg, p->diff_coef, s);
+    if (rc == JCFD_OK)
+      rc = cg_iterate(p, p->diff_maxiter, kDiffusionFirstChunk, &p->diff_last_iterations[m], s,
+                      [&](int k, hipStream_t st) {
+                        return cg_launch_iteration_helmholtz(p->params, p->cg, k, rtol, atol,
+                                                             p->diff_maxiter, p->diff_coef, st);
+                      });
+    if (rc == JCFD_OK)
+      rc = check_hip(hipMemcpyAsync(comps[m], p->cg.x, bytes, hipMemcpyDeviceToDevice, s),
+                     "copy diffused component");
+    if (rc) return rc;
+  }
+  return JCFD_OK;
+}
+
+// The implicit diffusion solve of the plan in place on the step's output.
+int fvm_diffuse_output(jcfd_fvm_plan* p, const FvmBuffers& b, hipStream_t s) {
+  if (p->dlam != nullptr) return fvm_launch_diffuse(p->params, b, p->dlam, s);
+  if (p->cfg.diffusion == JCFD_DIFFUSION_IMPLICIT_CG) return cg_diffuse(p, b.u_out, b.v_out, s);
+  return JCFD_OK;
 }
 
 // nsteps steps: the first reads b.u/b.v, the rest run in place on
@@ -463,9 +512,9 @@ int fvm_rk_step(const jcfd_fvm_plan* p, const FvmBuffers& b, hipStream_t s) {
 
 // One step: the fast-diagonal step kernels (or the Runge-Kutta stages), then
 // (implicit diffusion plans) the diffusion solve in place on the output.
-int fvm_full_step(const jcfd_fvm_plan* p, const FvmBuffers& b, hipStream_t s) {
+int fvm_full_step(jcfd_fvm_plan* p, const FvmBuffers& b, hipStream_t s) {
   int rc = p->rk_terms != nullptr ? fvm_rk_step(p, b, s) : fvm_launch_step(p->params, b, s);
-  if (rc == JCFD_OK && p->dlam != nullptr) rc = fvm_launch_diffuse(p->params, b, p->dlam, s);
+  if (rc == JCFD_OK) rc = fvm_diffuse_output(p, b, s);
   return rc;
 }
 
@@ -478,7 +527,16 @@ int fvm_run(jcfd_fvm_plan* p, FvmBuffers b, int64_t nsteps, hipStream_t s) {
       if ((rc = fvm_launch_forward_rhs(p->params, b, s)) || (rc = cg_solve(p, s)) ||
           (rc = fvm_launch_project_q(p->params, b, s)))
         return rc;
-      if (p->dlam != nullptr && (rc = fvm_launch_diffuse(p->params, b, p->dlam, s))) return rc;
+      if ((rc = fvm_diffuse_output(p, b, s))) return rc;
+      b.u = b.u_out;
+      b.v = b.v_out;
+    }
+    return JCFD_OK;
+  }
+  if (p->cfg.diffusion == JCFD_DIFFUSION_IMPLICIT_CG) {
+    // the CG diffusion solve reads its convergence flag on the host: never captured
+    for (int64_t step = 0; step < nsteps; ++step) {
+      if ((rc = fvm_full_step(p, b, s))) return rc;
       b.u = b.u_out;
       b.v = b.v_out;
     }
@@ -519,10 +577,16 @@ int jcfd_fvm_plan_create(const jcfd_fvm_config* config, jcfd_fvm_plan** plan) {
     return fail(JCFD_ERR_UNSUPPORTED, "the CG pressure solve is built for one instance per plan");
   if (c.pressure == JCFD_PRESSURE_CG && (!std::isfinite(c.cg_rtol) || !std::isfinite(c.cg_atol)))
     return fail(JCFD_ERR_INVALID_ARGUMENT, "CG tolerances must be finite");
-  if (c.diffusion != JCFD_DIFFUSION_EXPLICIT && c.diffusion != JCFD_DIFFUSION_IMPLICIT_FAST_DIAG)
-    return fail(JCFD_ERR_UNSUPPORTED, "diffusion must be explicit or implicit fast-diagonal");
-  if (c.diffusion == JCFD_DIFFUSION_IMPLICIT_FAST_DIAG && (!c.has_viscosity || !std::isfinite(c.viscosity)))
+  if (c.diffusion != JCFD_DIFFUSION_EXPLICIT && c.diffusion != JCFD_DIFFUSION_IMPLICIT_FAST_DIAG &&
+      c.diffusion != JCFD_DIFFUSION_IMPLICIT_CG)
+    return fail(JCFD_ERR_UNSUPPORTED, "diffusion must be explicit, implicit fast-diagonal or implicit CG");
+  if (c.diffusion != JCFD_DIFFUSION_EXPLICIT && (!c.has_viscosity || !std::isfinite(c.viscosity)))
     return fail(JCFD_ERR_INVALID_ARGUMENT, "implicit diffusion needs a finite viscosity");
+  if (c.diffusion == JCFD_DIFFUSION_IMPLICIT_CG && c.batch > 1)
+    return fail(JCFD_ERR_UNSUPPORTED, "the CG diffusion solve is built for one instance per plan");
+  if (c.diffusion == JCFD_DIFFUSION_IMPLICIT_CG &&
+      (!std::isfinite(c.diff_cg_rtol) || !std::isfinite(c.diff_cg_atol)))
+    return fail(JCFD_ERR_INVALID_ARGUMENT, "CG diffusion tolerances must be finite");
   if (c.rk_stages < 0 || c.rk_stages > 4)
     return fail(JCFD_ERR_UNSUPPORTED, "Runge-Kutta tableaux of 1 to 4 stages are built");
   for (int k = 0; k < 6; ++k)
@@ -576,7 +640,8 @@ int jcfd_fvm_plan_create(const jcfd_fvm_config* config, jcfd_fvm_plan** plan) {
       (rc = device_alloc(&p->lam, static_cast<size_t>(n) * (n / 2 + 1), &p->bytes)) ||
       (rc = device_alloc(&p->tw, static_cast<size_t>(n), &p->bytes)) ||
       (rc = device_alloc(&p->fx, static_cast<size_t>(n), &p->bytes)) ||
-      (c.pressure == JCFD_PRESSURE_CG && (rc = cg_alloc(p))) ||
+      ((c.pressure == JCFD_PRESSURE_CG || c.diffusion == JCFD_DIFFUSION_IMPLICIT_CG) &&
+       (rc = cg_alloc(p))) ||
       (c.diffusion == JCFD_DIFFUSION_IMPLICIT_FAST_DIAG &&
        (rc = device_alloc(&p->dlam, static_cast<size_t>(n) * (n / 2 + 1), &p->bytes))) ||
       (generic_rk && ((rc = device_alloc(&p->kbuf, 2 * c.rk_stages * nn, &p->bytes)) ||
@@ -626,6 +691,12 @@ int jcfd_fvm_plan_create(const jcfd_fvm_config* config, jcfd_fvm_plan** plan) {
       return rc;
     }
   }
+  if (c.diffusion == JCFD_DIFFUSION_IMPLICIT_CG) {
+    // solve_cg's linear_op: u - dt * nu * laplacian(u), the Python product
+    // dt * nu rounded once to float32
+    p->diff_coef = static_cast<float>(c.dt * c.viscosity);
+    p->diff_maxiter = c.diff_cg_maxiter > 0 ? c.diff_cg_maxiter : static_cast<int>(10 * n * n);
+  }
   if (p->dlam != nullptr) {
     const std::vector<float> dm = diffusion_multiplier(n, c.dx, c.dy, c.viscosity, c.dt);
     if ((rc = check_hip(hipMemcpy(p->dlam, dm.data(), dm.size() * sizeof(float), hipMemcpyHostToDevice),
@@ -659,6 +730,10 @@ int jcfd_fvm_plan_destroy(jcfd_fvm_plan* p) {
 }
 
 int32_t jcfd_fvm_cg_iterations(const jcfd_fvm_plan* p) { return p ? p->cg_last_iterations : 0; }
+
+int32_t jcfd_fvm_diffusion_cg_iterations(const jcfd_fvm_plan* p, int32_t component) {
+  return (p && (component == 0 || component == 1)) ? p->diff_last_iterations[component] : 0;
+}
 
 int jcfd_fvm_plan_set_graphs(jcfd_fvm_plan* p, int32_t enabled) {
   if (p == nullptr) return fail(JCFD_ERR_INVALID_ARGUMENT, "null plan");
@@ -800,7 +875,7 @@ int jcfd_fvm_projection(jcfd_fvm_plan* p, const jcfd_tensor* u, const jcfd_tenso
 int jcfd_fvm_diffuse(jcfd_fvm_plan* p, const jcfd_tensor* u, const jcfd_tensor* v, jcfd_tensor* u_out,
                      jcfd_tensor* v_out, void* stream) {
   if (p == nullptr) return fail(JCFD_ERR_INVALID_ARGUMENT, "null plan");
-  if (p->dlam == nullptr)
+  if (p->dlam == nullptr && p->cfg.diffusion != JCFD_DIFFUSION_IMPLICIT_CG)
     return fail(JCFD_ERR_UNSUPPORTED, "jcfd_fvm_diffuse needs an implicit-diffusion plan");
   const int n = p->params.n;
   void *pu, *pv, *pu2, *pv2;
@@ -819,7 +894,7 @@ int jcfd_fvm_diffuse(jcfd_fvm_plan* p, const jcfd_tensor* u, const jcfd_tensor* 
   FvmBuffers b = fvm_buffers(p);
   b.u_out = static_cast<float*>(pu2);
   b.v_out = static_cast<float*>(pv2);
-  return fvm_launch_diffuse(p->params, b, p->dlam, s);
+  return fvm_diffuse_output(p, b, s);
 }
 
 int jcfd_fvm_divergence(jcfd_fvm_plan* p, const jcfd_tensor* u, const jcfd_tensor* v,

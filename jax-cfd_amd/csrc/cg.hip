@@ -14,6 +14,10 @@
 //                 LDS), Ap = L p_k, per-block partial <p, Ap>; convergence is
 //                 tested here against the reduced <r_k, r_k>.
 //   cg_update     x += alpha p_k, r -= alpha Ap, per-block partial <r, r>.
+// The same two kernels solve the implicit diffusion system of
+// diffusion.solve_cg (jax_cfd/base/diffusion.py:137-167), one velocity
+// component at a time: A x = x - coef L x with coef = fp32(dt viscosity),
+// x0 = b = the component, so r0 = b - A b (cg_init_helmholtz).
 // Dot products are reduced in a fixed order (block tree, then every block
 // reduces the partials array itself), so runs are deterministic.  Once the
 // test fails, block 0 raises a flag and every later kernel returns at once,
@@ -96,11 +100,49 @@ __global__ __launch_bounds__(kThreads) void cg_init(const float* __restrict__ rh
   }
 }
 
+// periodic 5-point Laplacian at (i, j) of an N x N field, summed as
+// finite_differences.laplacian does (finite_differences.py:127-133)
+__device__ __forceinline__ float lap5(const FvmParams& p, float centre, float up, float down, float left,
+                                      float right) {
+  return ((-2.f * centre) * p.lap_sum + (up + down) * p.inv_dx_sq) + (left + right) * p.inv_dy_sq;
+}
+
+// x0 = b, r0 = b - (b - coef L b), p_{-1} = 0, partial <r0, r0> and <b, b>
 template <int N, int RB>
-__global__ __launch_bounds__(kThreads) void cg_direction(FvmParams p, CgBuffers c, int k, float rtol,
-                                                         float atol, int maxiter) {
-  __shared__ float red[3 * kWaves];
-  __shared__ float pt[(RB + 2) * N];  // p_k rows i0-1 .. i0+RB
+__global__ __launch_bounds__(kThreads) void cg_init_helmholtz(FvmParams p, const float* __restrict__ b,
+                                                              CgBuffers c, float coef) {
+  __shared__ float red[2 * kWaves];
+  const int i0 = blockIdx.x * RB;
+  float sums[2] = {0.f, 0.f};  // <r, r>, <b, b>
+  for (int e = threadIdx.x; e < RB * N; e += kThreads) {
+    const int i = i0 + e / N, j = e % N;
+    const int idx = i * N + j;
+    const float bc = b[idx];
+    const float lap = lap5(p, bc, b[((i - 1) & (N - 1)) * N + j], b[((i + 1) & (N - 1)) * N + j],
+                           b[i * N + ((j - 1) & (N - 1))], b[i * N + ((j + 1) & (N - 1))]);
+    const float r = bc - (bc - coef * lap);
+    c.x[idx] = bc;
+    c.r[idx] = r;
+    c.p[0][idx] = 0.f;
+    sums[0] = sums[0] + r * r;
+    sums[1] = sums[1] + bc * bc;
+  }
+  block_sums<2>(sums, red);
+  if (threadIdx.x == 0) {
+    c.part_rr[0][blockIdx.x] = sums[0];
+    c.part_bb[blockIdx.x] = sums[1];
+    if (blockIdx.x == 0) {
+      c.status[0] = 0;
+      c.status[1] = 0;
+    }
+  }
+}
+
+// One direction step; HELM selects A = I - coef L over A = L.
+template <int N, int RB, bool HELM>
+__device__ __forceinline__ void direction_body(const FvmParams& p, const CgBuffers& c, int k, float rtol,
+                                               float atol, int maxiter, float coef, float* red,
+                                               float* pt) {
   if (c.status[0]) return;
   constexpr int NB = N / RB;
   constexpr int TOT = (RB + 2) * N, PT = (TOT + kThreads - 1) / kThreads;
@@ -151,13 +193,30 @@ __global__ __launch_bounds__(kThreads) void cg_direction(FvmParams p, CgBuffers 
     const int r = e / N, j = e % N;
     const float* row = pt + (r + 1) * N;
     const float pc = row[j];
-    const float lap = ((-2.f * pc) * p.lap_sum + (row[j - N] + row[j + N]) * p.inv_dx_sq) +
-                      (row[(j - 1) & (N - 1)] + row[(j + 1) & (N - 1)]) * p.inv_dy_sq;
-    c.ap[(i0 + r) * N + j] = lap;
-    pap = pap + pc * lap;
+    const float lap = lap5(p, pc, row[j - N], row[j + N], row[(j - 1) & (N - 1)], row[(j + 1) & (N - 1)]);
+    const float ap = HELM ? pc - coef * lap : lap;
+    c.ap[(i0 + r) * N + j] = ap;
+    pap = pap + pc * ap;
   }
   const float s = block_sum(pap, red);
   if (threadIdx.x == 0) c.part_pap[blockIdx.x] = s;
+}
+
+template <int N, int RB>
+__global__ __launch_bounds__(kThreads) void cg_direction(FvmParams p, CgBuffers c, int k, float rtol,
+                                                         float atol, int maxiter) {
+  __shared__ float red[3 * kWaves];
+  __shared__ float pt[(RB + 2) * N];  // p_k rows i0-1 .. i0+RB
+  direction_body<N, RB, false>(p, c, k, rtol, atol, maxiter, 0.f, red, pt);
+}
+
+template <int N, int RB>
+__global__ __launch_bounds__(kThreads) void cg_direction_helmholtz(FvmParams p, CgBuffers c, int k,
+                                                                   float rtol, float atol, int maxiter,
+                                                                   float coef) {
+  __shared__ float red[3 * kWaves];
+  __shared__ float pt[(RB + 2) * N];
+  direction_body<N, RB, true>(p, c, k, rtol, atol, maxiter, coef, red, pt);
 }
 
 template <int N, int RB>
@@ -216,6 +275,18 @@ struct CgOps {
     hipLaunchKernelGGL((cg_update<N, RB>), dim3(N / RB), dim3(kThreads), 0, s, c, k);
     return check_hip(hipGetLastError(), "cg iteration");
   }
+  static int init_helmholtz(const FvmParams& p, const float* b, const CgBuffers& c, float coef,
+                            hipStream_t s) {
+    hipLaunchKernelGGL((cg_init_helmholtz<N, RB>), dim3(N / RB), dim3(kThreads), 0, s, p, b, c, coef);
+    return check_hip(hipGetLastError(), "cg_init_helmholtz");
+  }
+  static int iteration_helmholtz(const FvmParams& p, const CgBuffers& c, int k, float rtol, float atol,
+                                 int maxiter, float coef, hipStream_t s) {
+    hipLaunchKernelGGL((cg_direction_helmholtz<N, RB>), dim3(N / RB), dim3(kThreads), 0, s, p, c, k,
+                       rtol, atol, maxiter, coef);
+    hipLaunchKernelGGL((cg_update<N, RB>), dim3(N / RB), dim3(kThreads), 0, s, c, k);
+    return check_hip(hipGetLastError(), "cg helmholtz iteration");
+  }
 };
 
 }  // namespace
@@ -244,6 +315,32 @@ int cg_launch_iteration(const FvmParams& p, const CgBuffers& c, int k, float rto
     case 256: return CgOps<256>::iteration(p, c, k, rtol, atol, maxiter, s);
     case 512: return CgOps<512>::iteration(p, c, k, rtol, atol, maxiter, s);
     case 1024: return CgOps<1024>::iteration(p, c, k, rtol, atol, maxiter, s);
+    default: return fail(JCFD_ERR_UNSUPPORTED, "CG grid size not built");
+  }
+}
+
+int cg_launch_init_helmholtz(const FvmParams& p, const float* b, const CgBuffers& c, float coef,
+                             hipStream_t s) {
+  switch (p.n) {
+    case 32: return CgOps<32>::init_helmholtz(p, b, c, coef, s);
+    case 64: return CgOps<64>::init_helmholtz(p, b, c, coef, s);
+    case 128: return CgOps<128>::init_helmholtz(p, b, c, coef, s);
+    case 256: return CgOps<256>::init_helmholtz(p, b, c, coef, s);
+    case 512: return CgOps<512>::init_helmholtz(p, b, c, coef, s);
+    case 1024: return CgOps<1024>::init_helmholtz(p, b, c, coef, s);
+    default: return fail(JCFD_ERR_UNSUPPORTED, "CG grid size not built");
+  }
+}
+
+int cg_launch_iteration_helmholtz(const FvmParams& p, const CgBuffers& c, int k, float rtol, float atol,
+                                  int maxiter, float coef, hipStream_t s) {
+  switch (p.n) {
+    case 32: return CgOps<32>::iteration_helmholtz(p, c, k, rtol, atol, maxiter, coef, s);
+    case 64: return CgOps<64>::iteration_helmholtz(p, c, k, rtol, atol, maxiter, coef, s);
+    case 128: return CgOps<128>::iteration_helmholtz(p, c, k, rtol, atol, maxiter, coef, s);
+    case 256: return CgOps<256>::iteration_helmholtz(p, c, k, rtol, atol, maxiter, coef, s);
+    case 512: return CgOps<512>::iteration_helmholtz(p, c, k, rtol, atol, maxiter, coef, s);
+    case 1024: return CgOps<1024>::iteration_helmholtz(p, c, k, rtol, atol, maxiter, coef, s);
     default: return fail(JCFD_ERR_UNSUPPORTED, "CG grid size not built");
   }
 }

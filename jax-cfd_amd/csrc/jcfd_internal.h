@@ -95,6 +95,12 @@ int cg_launch_init(const FvmParams& p, const float* rhs, const CgBuffers& c, flo
 // One CG iteration k (two kernels); a no-op once converged.
 int cg_launch_iteration(const FvmParams& p, const CgBuffers& c, int k, float rtol, float atol,
                         int maxiter, hipStream_t s);
+// The implicit diffusion system (I - coef L) x = b on the same buffers:
+// x0 = b, then iterations as above.
+int cg_launch_init_helmholtz(const FvmParams& p, const float* b, const CgBuffers& c, float coef,
+                             hipStream_t s);
+int cg_launch_iteration_helmholtz(const FvmParams& p, const CgBuffers& c, int k, float rtol, float atol,
+                                  int maxiter, float coef, hipStream_t s);
 int fvm_launch_adjoint_explicit(const FvmParams& p, const float* u, const float* v, const float* lu,
                                 const float* lv, float* ou, float* ov, hipStream_t s);
 int fvm_launch_curl_vjp(const FvmParams& p, const float* wb, float* ub, float* vb, hipStream_t s);

@@ -32,6 +32,7 @@ PRESSURE_FAST_DIAG = 0
 PRESSURE_CG = 1
 DIFFUSION_EXPLICIT = 0
 DIFFUSION_IMPLICIT_FAST_DIAG = 1
+DIFFUSION_IMPLICIT_CG = 2
 MAX_RK_STAGES = 8  # JCFD_MAX_RK_STAGES
 
 
@@ -63,6 +64,8 @@ class FvmConfig(ctypes.Structure):
       ("rk_stages", ctypes.c_int32),
       ("rk_a", ctypes.c_double * 6),
       ("rk_b", ctypes.c_double * 4),
+      ("diff_cg_rtol", ctypes.c_double), ("diff_cg_atol", ctypes.c_double),
+      ("diff_cg_maxiter", ctypes.c_int32),
   ]
 
 
@@ -106,6 +109,7 @@ _SIGNATURES = {
     "jcfd_fvm_step": (ctypes.c_int, [_P, _P, _P, _P, _P, ctypes.c_int64, _P]),
     "jcfd_fvm_step_slots": (ctypes.c_int, [_P, _P, _P, _P, ctypes.c_int64, _P]),
     "jcfd_fvm_cg_iterations": (ctypes.c_int32, [_P]),
+    "jcfd_fvm_diffusion_cg_iterations": (ctypes.c_int32, [_P, ctypes.c_int32]),
     "jcfd_ns2d_step_slots": (ctypes.c_int, [_P, _P, _P, ctypes.c_int64, _P]),
     "jcfd_fvm_explicit_terms": (ctypes.c_int, [_P, _P, _P, _P, _P, _P]),
     "jcfd_fvm_projection": (ctypes.c_int, [_P, _P, _P, _P, _P, _P]),

@@ -32,6 +32,9 @@ class FvmSpec:
   rk_stages: int = 0  # 0: forward Euler (fused); else the Butcher tableau below
   rk_a: Tuple[float, ...] = ()  # a[i-1][j] packed row by row
   rk_b: Tuple[float, ...] = ()
+  diff_cg_rtol: float = 1e-6  # diffusion.solve_cg (diffusion == IMPLICIT_CG)
+  diff_cg_atol: float = 1e-6
+  diff_cg_maxiter: int = 0
 
 
 def spec_from_grid(grid, **kw) -> FvmSpec:
@@ -77,6 +80,8 @@ class FvmPlan:
       cfg.rk_a[k] = x
     for k, x in enumerate(spec.rk_b):
       cfg.rk_b[k] = x
+    cfg.diff_cg_rtol, cfg.diff_cg_atol = spec.diff_cg_rtol, spec.diff_cg_atol
+    cfg.diff_cg_maxiter = spec.diff_cg_maxiter
     handle = ctypes.c_void_p()
     with torch.cuda.device(self.device):
       _lib.check(lib.jcfd_fvm_plan_create(ctypes.byref(cfg), ctypes.byref(handle)),
@@ -92,6 +97,10 @@ class FvmPlan:
   def cg_iterations(self) -> int:
     """Iterations of the last CG pressure solve (CG plans only)."""
     return int(self._lib.jcfd_fvm_cg_iterations(self._handle))
+
+  def diffusion_cg_iterations(self, component: int) -> int:
+    """Iterations of the last CG diffusion solve of component 0 (u) or 1 (v)."""
+    return int(self._lib.jcfd_fvm_diffusion_cg_iterations(self._handle, int(component)))
 
   def set_graphs(self, enabled: bool):
     _lib.check(self._lib.jcfd_fvm_plan_set_graphs(self._handle, int(bool(enabled))),
@@ -261,6 +270,10 @@ class FvmGroup:
       _lib.check(lib.jcfd_fvm_group_create(ctypes.byref(handle)), 'jcfd_fvm_group_create')
     self._handle = handle
     self._lib = lib
+
+  def diffusion_cg_iterations(self, component: int) -> int:
+    """Iterations of the last CG diffusion solve of component 0 (u) or 1 (v)."""
+    return int(self._lib.jcfd_fvm_diffusion_cg_iterations(self._handle, int(component)))
 
   def set_graphs(self, enabled: bool):
     _lib.check(self._lib.jcfd_fvm_group_set_graphs(self._handle, int(bool(enabled))),

@@ -1,4 +1,4 @@
-"""Diffusion (jax_cfd/base/diffusion.py:35-57, 170-212)."""
+"""Diffusion (jax_cfd/base/diffusion.py:35-57, 137-212)."""
 from __future__ import annotations
 
 from jax_cfd_amd import _symbolic
@@ -41,6 +41,43 @@ def solve_fast_diag(v, nu: float, dt: float, implementation=None):
   uo, vo = plan.diffuse(v[0].data, v[1].data)
   return tuple(grids.GridVariable(grids.GridArray(d, u.offset, grid), u.bc)
                for d, u in zip((uo, vo), v))
+
+
+def solve_cg(v, nu: float, dt: float, rtol: float = 1e-6, atol: float = 1e-6,
+             maxiter=None):
+  """diffusion.py:137-167: each velocity component u becomes the CG solution
+  of (1 - nu dt L) x = u from x0 = u, with jax's cg tolerances (stop once
+  <r, r> <= max(rtol^2 <u, u>, atol^2) or after maxiter iterations, default
+  10 n^2), on the GPU.  Also a marker implicit_diffusion_navier_stokes reads
+  as its `diffusion_solve` (with functools.partial keywords)."""
+  from jax_cfd_amd import _lib
+  from jax_cfd_amd.base import _fvm, boundaries
+  if not boundaries.has_all_periodic_boundary_conditions(*v):
+    raise ValueError('solve_cg() expects periodic BC')
+  if len(v) != 2:
+    raise NotImplementedError('the diffusion kernels are built for 2-D velocity')
+  grid = grids.consistent_grid(*[u.array for u in v])
+  spec = _fvm.spec_from_grid(grid, dt=float(dt), viscosity=float(nu),
+                             diffusion=_lib.DIFFUSION_IMPLICIT_CG,
+                             **cg_spec_fields(rtol=rtol, atol=atol, maxiter=maxiter))
+  plan = _fvm.plan_for(spec, v[0].data.device)
+  uo, vo = plan.diffuse(v[0].data, v[1].data)
+  return tuple(grids.GridVariable(grids.GridArray(d, u.offset, grid), u.bc)
+               for d, u in zip((uo, vo), v))
+
+
+def cg_spec_fields(rtol: float = 1e-6, atol: float = 1e-6, maxiter=None) -> dict:
+  """solve_cg's keywords as FvmSpec fields (maxiter None: 0, read as 10 n^2)."""
+  return dict(diff_cg_rtol=float(rtol), diff_cg_atol=float(atol),
+              diff_cg_maxiter=0 if maxiter is None else int(maxiter))
+
+
+def is_solve_cg(solve) -> bool:
+  import functools
+  if isinstance(solve, functools.partial):
+    return solve.func is solve_cg and not solve.args and set(solve.keywords) <= {
+        'rtol', 'atol', 'maxiter'}
+  return solve is solve_cg
 
 
 def is_solve_fast_diag(solve) -> bool:

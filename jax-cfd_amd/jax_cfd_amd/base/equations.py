@@ -242,9 +242,14 @@ def implicit_diffusion_navier_stokes(
   """Returns step_fn of Navier-Stokes with implicit diffusion
   (equations.py:154-195): convection (+ forcing / density), forward Euler,
   projection, then diffusion_solve(v, viscosity, dt)."""
-  if not diffusion.is_solve_fast_diag(diffusion_solve):
+  if diffusion.is_solve_fast_diag(diffusion_solve):
+    diff_kw = dict(diffusion=_lib.DIFFUSION_IMPLICIT_FAST_DIAG)
+  elif diffusion.is_solve_cg(diffusion_solve):
+    diff_kw = dict(diffusion=_lib.DIFFUSION_IMPLICIT_CG, **diffusion.cg_spec_fields(
+        **getattr(diffusion_solve, 'keywords', {})))
+  else:
     raise NotImplementedError(f'diffusion_solve {diffusion_solve!r} is not built '
-                              '(diffusion.solve_fast_diag is)')
+                              '(diffusion.solve_fast_diag and solve_cg are)')
   if pressure_solve is pressure.solve_fast_diag:
     solve_kw = {}
   elif pressure.is_solve_cg(pressure_solve):
@@ -255,8 +260,7 @@ def implicit_diffusion_navier_stokes(
   lin, fx = _read_forcing(forcing, grid)
   spec = _fvm.spec_from_grid(
       grid, dt=float(dt), density=float(density), viscosity=float(viscosity),
-      convect=conv, linear_coefficient=lin, forcing_x=fx,
-      diffusion=_lib.DIFFUSION_IMPLICIT_FAST_DIAG, **solve_kw)
+      convect=conv, linear_coefficient=lin, forcing_x=fx, **diff_kw, **solve_kw)
   return FvmStepFn(grid, spec)
 
 

@@ -10,7 +10,8 @@ Kolmogorov + linear forcing, and ``projection`` (pressure.py:181-198) with
 either ``solve_fast_diag`` (pressure.py:115-157 via fast_diagonalization.py:
 199-266) or ``solve_cg`` (pressure.py:67-112).  ``ImplicitDiffusionStep``
 mirrors ``implicit_diffusion_navier_stokes`` (equations.py:154-195) with
-``diffusion.solve_fast_diag`` (diffusion.py:170-212).
+``diffusion.solve_fast_diag`` (diffusion.py:170-212) or ``diffusion.solve_cg``
+(diffusion.py:137-167).
 
 Pinning (tests/test_oracle_fvm.py): the 4x4 periodic Poisson KAT
 ``x = b - mean(b)`` (pressure_test.py:197-213); circulant pseudo-inverse vs a
@@ -27,7 +28,11 @@ follows the
 published conjugate-gradient algorithm of jax.scipy.sparse.linalg.cg (no
 preconditioner, stop when |r|^2 <= max(tol^2 |b|^2, atol^2)); it is checked
 against the fast-diagonalisation solve, but its individual iterates are
-"parity unpinned" (no JAX here to compare against).
+"parity unpinned" (no JAX here to compare against).  ``solve_cg_diffusion``
+runs the same recurrence on 1 - nu dt L from x0 = b; it is pinned by the
+reference's constant-field case (diffusion_test.py:58-76: no iteration, x = b
+exactly), against the dense brute force and the fast-diagonal solve, and by a
+residual bound; its iterates are likewise unpinned.
 """
 from __future__ import annotations
 
@@ -289,6 +294,45 @@ def solve_fast_diag_diffusion(v: Sequence[A], nu: float, dt: float,
   return tuple(out)
 
 
+def solve_cg_diffusion(v: Sequence[A], nu: float, dt: float, grid: Grid,
+                       rtol: float = 1e-6, atol: float = 1e-6,
+                       maxiter: Optional[int] = None, return_iterations: bool = False):
+  """diffusion.solve_cg (diffusion.py:137-167), periodic: each component u
+  becomes the CG solution of (1 - nu dt L) x = u from x0 = u, with the
+  operator evaluated as x - (dt * nu) * laplacian(x) (diffusion.py:150-153;
+  the Python product dt * nu rounded to the field's dtype) and the same CG
+  recurrence and stopping rule as solve_cg above."""
+  out, iters = [], []
+  for u in v:
+    b = u.data
+    dtype = b.dtype.type
+    coef = dtype(dt * nu)
+
+    def op(x):
+      return x - coef * _periodic_laplacian(x, grid).astype(dtype)
+
+    n_max = 10 * b.size if maxiter is None else maxiter
+    bs = np.vdot(b, b).real.astype(dtype)
+    atol2 = max(np.square(dtype(rtol)) * bs, np.square(dtype(atol)))
+    x = b.copy()
+    r = b - op(x)
+    p = r
+    gamma = np.vdot(r, r).real.astype(dtype)
+    k = 0
+    while gamma > atol2 and k < n_max:
+      ap = op(p)
+      alpha = gamma / np.vdot(p, ap).real.astype(dtype)
+      x = x + alpha * p
+      r = r - alpha * ap
+      gamma_new = np.vdot(r, r).real.astype(dtype)
+      p = r + (gamma_new / gamma) * p
+      gamma = gamma_new
+      k += 1
+    out.append(A(x.astype(dtype), u.offset))
+    iters.append(k)
+  return (tuple(out), iters) if return_iterations else tuple(out)
+
+
 @dataclasses.dataclass
 class ImplicitDiffusionStep:
   """implicit_diffusion_navier_stokes (equations.py:154-195), restated:
@@ -304,6 +348,10 @@ class ImplicitDiffusionStep:
   pressure: str = 'fast_diag'
   cg_rtol: float = 1e-6
   cg_atol: float = 1e-6
+  diffusion: str = 'fast_diag'  # or 'cg' (diffusion.solve_cg)
+  diff_cg_rtol: float = 1e-6
+  diff_cg_atol: float = 1e-6
+  diff_cg_maxiter: Optional[int] = None
 
   def accelerations(self, v: Sequence[A]):
     conv = FVMStep(self.grid, self.dt, convect=self.convect).convection(v)
@@ -319,6 +367,9 @@ class ImplicitDiffusionStep:
     solve = FVMStep(self.grid, self.dt, pressure=self.pressure, cg_rtol=self.cg_rtol,
                     cg_atol=self.cg_atol).solve
     v = projection(v, self.grid, solve)
+    if self.diffusion == 'cg':
+      return solve_cg_diffusion(v, self.viscosity, self.dt, self.grid, self.diff_cg_rtol,
+                                self.diff_cg_atol, self.diff_cg_maxiter)
     return solve_fast_diag_diffusion(v, self.viscosity, self.dt, self.grid)
 
   def rollout(self, v: Sequence[A], steps: int) -> Tuple[A, ...]:

@@ -133,6 +133,25 @@ def test_implicit_diffusion_factory_spec():
                                           implementation='matmul'))
 
 
+def test_implicit_diffusion_cg_factory_spec():
+  """diffusion_solve=solve_cg (diffusion.py:137-167) maps to the CG
+  diffusion plan with its keywords; maxiter None is the 10 n^2 default."""
+  grid = _grid(64)
+  step = cfd.equations.implicit_diffusion_navier_stokes(
+      1.0, 1e-3, 0.01, grid,
+      diffusion_solve=functools.partial(cfd.diffusion.solve_cg, atol=1e-6, maxiter=512))
+  s = step.spec
+  assert s.diffusion == _lib.DIFFUSION_IMPLICIT_CG
+  assert (s.diff_cg_rtol, s.diff_cg_atol, s.diff_cg_maxiter) == (1e-6, 1e-6, 512)
+  s = cfd.equations.implicit_diffusion_navier_stokes(
+      1.0, 1e-3, 0.01, grid, diffusion_solve=cfd.diffusion.solve_cg).spec
+  assert s.diffusion == _lib.DIFFUSION_IMPLICIT_CG and s.diff_cg_maxiter == 0
+  with pytest.raises(NotImplementedError):
+    cfd.equations.implicit_diffusion_navier_stokes(
+        1.0, 1e-3, 0.01, grid,
+        diffusion_solve=functools.partial(cfd.diffusion.solve_cg, tol=1e-3))
+
+
 @pytest.mark.parametrize("stepper,stages,a,b", [
     ('midpoint_rk2', 2, (0.5,), (0.0, 1.0)),
     ('heun_rk2', 2, (1.0,), (0.5, 0.5)),

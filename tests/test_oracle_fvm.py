@@ -444,6 +444,54 @@ def test_implicit_diffusion_fourier_mode():
   np.testing.assert_allclose(x.data, mode / (1 - nu * dt * lam), atol=1e-12)
 
 
+def test_cg_diffusion_keeps_constants():
+  """diffusion_test.py:58-76 with solve_cg: r0 = 0 for a constant field, so
+  no iteration runs and x = x0 = b exactly."""
+  grid = Grid.from_step((100, 100), (1., 1.))
+  v = tuple(A(np.ones((100, 100), np.float32), o) for o in grid.cell_faces)
+  out, iters = fvm.solve_cg_diffusion(v, 1.0, 0.1, grid, atol=1e-6, return_iterations=True)
+  assert iters == [0, 0]
+  for a, b in zip(out, v):
+    np.testing.assert_array_equal(a.data, b.data)
+
+
+def test_cg_diffusion_dense_brute_force_and_fast_diag():
+  """The CG solution of (1 - nu dt L) x = b against a dense solve (float64,
+  tight tolerance) and, in float32, against the fast-diagonal solve; a wrong
+  sign of the nu dt L term or a dropped x0 would miss both."""
+  grid = Grid.from_domain((8, 6), ((0, 2.), (0, 3.)))
+  b = np.random.RandomState(1).randn(8, 6)
+  nu, dt = 0.3, 0.7
+  dense = np.linalg.solve(np.eye(48) - nu * dt * _dense_laplacian(grid), b.ravel())
+  ((x,), iters) = fvm.solve_cg_diffusion((A(b, (.5, 1)),), nu, dt, grid, rtol=1e-13,
+                                         atol=0.0, return_iterations=True)
+  np.testing.assert_allclose(x.data.ravel(), dense, atol=1e-10)
+  assert 0 < iters[0] <= 48
+  grid = Grid.from_domain((32, 32), ((0, 2 * np.pi), (0, 2 * np.pi)))
+  u = np.random.RandomState(2).randn(32, 32).astype(np.float32)
+  (xc,) = fvm.solve_cg_diffusion((A(u, (1, .5)),), 0.05, 0.2, grid, atol=1e-6)
+  (xf,) = fvm.solve_fast_diag_diffusion((A(u, (1, .5)),), 0.05, 0.2, grid)
+  assert xc.data.dtype == np.float32
+  assert np.abs(xc.data - xf.data).max() < 1e-5
+  # true residual (float64) of the float32 iterate: the recursive one is
+  # <= atol, the float32 rounding of 1024 entries adds ~1e-6 each
+  x64 = A(xc.data.astype(np.float64), xc.offset)
+  resid = x64.data - (0.05 * 0.2) * go.laplacian(x64, grid).data - u
+  assert np.linalg.norm(resid) <= 1e-4 * np.linalg.norm(u)
+
+
+def test_cg_diffusion_step_matches_fast_diag_step():
+  """ImplicitDiffusionStep with diffusion='cg' tracks the fast-diagonal
+  step (the same system, solved to atol 1e-6)."""
+  grid = Grid.from_domain((32, 32), ((0, 2 * np.pi), (0, 2 * np.pi)))
+  v0 = fvm.projection(_sinusoidal_velocity(grid), grid)
+  v0 = tuple(A(a.data.astype(np.float32), a.offset) for a in v0)
+  a = fvm.ImplicitDiffusionStep(grid, dt=0.05, viscosity=1e-2).rollout(v0, 5)
+  b = fvm.ImplicitDiffusionStep(grid, dt=0.05, viscosity=1e-2, diffusion='cg').rollout(v0, 5)
+  for x, y in zip(a, b):
+    assert np.abs(x.data - y.data).max() < 1e-5
+
+
 def test_implicit_diffusion_step_invariants():
   """equations_test.py:198-245 (the implicit_diffusion_navier_stokes cases):
   divergence stays ~0 and momentum is conserved without forcing."""

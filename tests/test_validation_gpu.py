@@ -14,6 +14,8 @@ size-independent property at sizes the fp32 oracle is too slow to run.  A
 dropped term or a wrong sign stops the convergence.
 """
 
+import functools
+
 import numpy as np
 import pytest
 import torch
@@ -88,6 +90,8 @@ REFERENCE_CASES = [
     ('semi_implicit_rk4', 512, 1e-2, 'classic_rk4', False, 0.1, 40.0, 8e-7),
     ('implicit_diffusion_rfft', 1024, 1e-3, None, True, 0.1, 10.0, 4e-5),
     ('implicit_diffusion_viscous', 1024, 0.5, None, True, 0.5, 1.0, 6e-4),
+    # diffusion_solve=partial(solve_cg, atol=1e-6, maxiter=512)
+    ('implicit_diffusion_cg', 1024, 1e-3, None, 'cg', 0.1, 10.0, 3e-5),
 ]
 
 
@@ -97,12 +101,14 @@ def test_reference_taylor_green_cases(case):
   cfd = _cfd()
   grid = cfd.grids.Grid((n, n), domain=((0, TWO_PI), (0, TWO_PI)))
   v0, _ = _taylor_green(grid, 0.0, nu)
-  dt = cfd.equations.dynamic_time_step(v0, courant, nu, grid, implicit)
+  dt = cfd.equations.dynamic_time_step(v0, courant, nu, grid, bool(implicit))
   steps = int(np.ceil(time / dt))
   print(f"reference case {name}: n={n} dt={dt:.4e} steps={steps}", flush=True)
   if implicit:
+    solve = (functools.partial(cfd.diffusion.solve_cg, atol=1e-6, maxiter=512)
+             if implicit == 'cg' else cfd.diffusion.solve_fast_diag)
     step = cfd.equations.implicit_diffusion_navier_stokes(
-        1.0, nu, dt, grid, convect=cfd.advection.convect_linear)
+        1.0, nu, dt, grid, convect=cfd.advection.convect_linear, diffusion_solve=solve)
   else:
     step = cfd.equations.semi_implicit_navier_stokes(
         1.0, nu, dt, grid, convect=cfd.advection.convect_linear,
