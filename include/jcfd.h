@@ -214,6 +214,16 @@ int jcfd_fvm_step_vjp(jcfd_fvm_plan* plan, const jcfd_tensor* u, const jcfd_tens
 int jcfd_fvm_curl_vjp(jcfd_fvm_plan* plan, const jcfd_tensor* w_bar, jcfd_tensor* u_bar,
                       jcfd_tensor* v_bar, void* stream);
 
+/* advection.advect_step_semilagrangian(c, v, dt) (advection.py:345-382) on
+ * a periodic grid whose domain starts at zero: c (offset offset_x, offset_y)
+ * sampled at the departure points x - dt v(x) by bilinear interpolation with
+ * periodic wrap (map_coordinates order 1, mode 'wrap').  Returns the advected
+ * field itself, not a time derivative.  u, v: the staggered velocity at
+ * offsets (1, 0.5) and (0.5, 1); out must not alias an input. */
+int jcfd_fvm_advect_semilagrangian(jcfd_fvm_plan* plan, const jcfd_tensor* c, double offset_x,
+                                   double offset_y, const jcfd_tensor* u, const jcfd_tensor* v,
+                                   double dt, jcfd_tensor* out, void* stream);
+
 /* fd.curl_2d of a staggered velocity (finite_differences.py:186-193),
  * offset (1, 1). */
 int jcfd_fvm_curl(jcfd_fvm_plan* plan, const jcfd_tensor* u, const jcfd_tensor* v,

@@ -1026,6 +1026,56 @@ int jcfd_fvm_curl(jcfd_fvm_plan* p, const jcfd_tensor* u, const jcfd_tensor* v, 
                          static_cast<float*>(po), static_cast<hipStream_t>(stream));
 }
 
+namespace {
+// interpolation.py:36-64 for offset difference `delta` (Python floats there)
+SlAxis sl_axis(double delta) {
+  SlAxis a{};
+  if (delta == std::floor(delta)) {
+    a.f = static_cast<int>(delta);
+    return a;
+  }
+  a.blend = 1;
+  a.f = static_cast<int>(std::floor(delta));
+  a.c = static_cast<int>(std::ceil(delta));
+  const double fw = a.c - delta;
+  a.fw = static_cast<float>(fw);
+  a.cw = static_cast<float>(1.0 - fw);
+  return a;
+}
+}  // namespace
+
+int jcfd_fvm_advect_semilagrangian(jcfd_fvm_plan* p, const jcfd_tensor* c, double offset_x,
+                                   double offset_y, const jcfd_tensor* u, const jcfd_tensor* v,
+                                   double dt, jcfd_tensor* out, void* stream) {
+  if (p == nullptr) return fail(JCFD_ERR_INVALID_ARGUMENT, "null plan");
+  const int n = p->params.n;
+  if (!std::isfinite(offset_x) || !std::isfinite(offset_y) || !std::isfinite(dt) ||
+      std::fabs(offset_x) > 1e6 || std::fabs(offset_y) > 1e6)
+    return fail(JCFD_ERR_INVALID_ARGUMENT, "offsets and dt must be finite");
+  void *pc, *pu, *pv, *po;
+  int rc;
+  if ((rc = check_tensor(c, "c", kDLFloat, 32, n, n, p->device, &pc)) ||
+      (rc = check_tensor(u, "u", kDLFloat, 32, n, n, p->device, &pu)) ||
+      (rc = check_tensor(v, "v", kDLFloat, 32, n, n, p->device, &pv)) ||
+      (rc = check_tensor(out, "out", kDLFloat, 32, n, n, p->device, &po)))
+    return rc;
+  if (po == pc || po == pu || po == pv)
+    return fail(JCFD_ERR_INVALID_ARGUMENT, "out must not alias c, u or v");
+  SlParams q{};
+  q.u0 = sl_axis(offset_x - 1.0);  // u sits at (1, 0.5)
+  q.u1 = sl_axis(offset_y - 0.5);
+  q.v0 = sl_axis(offset_x - 0.5);  // v sits at (0.5, 1)
+  q.v1 = sl_axis(offset_y - 1.0);
+  q.ox = static_cast<float>(offset_x);
+  q.oy = static_cast<float>(offset_y);
+  q.hx = p->params.dx;
+  q.hy = p->params.dy;
+  q.dt = static_cast<float>(dt);
+  return fvm_launch_semilagrangian(n, static_cast<const float*>(pc), static_cast<const float*>(pu),
+                                   static_cast<const float*>(pv), static_cast<float*>(po), q,
+                                   static_cast<hipStream_t>(stream));
+}
+
 int jcfd_fvm_spectral_filter(jcfd_fvm_plan* p, const jcfd_tensor* x, const jcfd_tensor* multiplier,
                              jcfd_tensor* out, void* stream) {
   if (p == nullptr) return fail(JCFD_ERR_INVALID_ARGUMENT, "null plan");

@@ -1343,6 +1343,66 @@ int fvm_launch_filter(const FvmParams& p, const float* x, const float* multiplie
   return dispatch_n<FilterOp>(p.n, p, b, s);
 }
 
+// advect_step_semilagrangian (advection.py:345-382) of a periodic N x N field
+// c at offset (ox, oy): u and v linearly interpolated to c's offset
+// (interpolation.py:36-93, axis 0 then axis 1), departure points
+// x - dt u on c's mesh, their fractional indices x / h - o, then bilinear
+// interpolation with periodic wrap in the term order of
+// jax.scipy.ndimage.map_coordinates(order=1, mode='wrap'): weights
+// (1 - f, f) per axis, contributions (lo,lo), (lo,up), (up,lo), (up,up)
+// summed left to right, each weight product formed before the value.
+__device__ __forceinline__ int wrap_index(int k, int n) { return ((k % n) + n) % n; }
+
+// one axis of interpolation.linear at (i, j) of a field whose value at
+// (i, j) along the other axis is given by `at`
+template <typename At>
+__device__ __forceinline__ float lin_axis(const SlAxis& a, At at) {
+  if (!a.blend) return at(a.f);
+  return a.fw * at(a.f) + a.cw * at(a.c);
+}
+
+__device__ __forceinline__ float lin_interp(const float* f, int n, int i, int j, const SlAxis& a0,
+                                            const SlAxis& a1) {
+  auto along0 = [&](int dj) {
+    const int jj = wrap_index(j + dj, n);
+    return lin_axis(a0, [&](int di) { return f[wrap_index(i + di, n) * n + jj]; });
+  };
+  return lin_axis(a1, along0);
+}
+
+__global__ __launch_bounds__(256) void fvm_semilagrangian_kernel(int n, const float* __restrict__ c,
+                                                                 const float* __restrict__ u,
+                                                                 const float* __restrict__ v,
+                                                                 float* __restrict__ out, SlParams q) {
+  const int e = blockIdx.x * 256 + threadIdx.x;
+  if (e >= n * n) return;
+  const int i = e / n, j = e % n;
+  const float ui = lin_interp(u, n, i, j, q.u0, q.u1);
+  const float vi = lin_interp(v, n, i, j, q.v0, q.v1);
+  const float x = 0.f + (static_cast<float>(i) + q.ox) * q.hx;  // grid.mesh(offset), lower 0
+  const float y = 0.f + (static_cast<float>(j) + q.oy) * q.hy;
+  const float xi = (x - q.dt * ui) / q.hx - q.ox;
+  const float yi = (y - q.dt * vi) / q.hy - q.oy;
+  const float fx = floorf(xi), fy = floorf(yi);
+  const float wx1 = xi - fx, wy1 = yi - fy;
+  const float wx0 = 1.f - wx1, wy0 = 1.f - wy1;
+  const int i0 = static_cast<int>(fx), j0 = static_cast<int>(fy);
+  const int a0 = wrap_index(i0, n), a1 = wrap_index(i0 + 1, n);
+  const int b0 = wrap_index(j0, n), b1 = wrap_index(j0 + 1, n);
+  const float r = (((wx0 * wy0) * c[a0 * n + b0] + (wx0 * wy1) * c[a0 * n + b1]) +
+                   (wx1 * wy0) * c[a1 * n + b0]) +
+                  (wx1 * wy1) * c[a1 * n + b1];
+  out[e] = r;
+}
+
+int fvm_launch_semilagrangian(int n, const float* c, const float* u, const float* v, float* out,
+                              const SlParams& q, hipStream_t s) {
+  const int total = n * n;
+  hipLaunchKernelGGL(fvm_semilagrangian_kernel, dim3((total + 255) / 256), dim3(256), 0, s, n, c, u, v,
+                     out, q);
+  return check_hip(hipGetLastError(), "semilagrangian launch");
+}
+
 int fvm_launch_curl(const FvmParams& p, const float* u, const float* v, float* out, hipStream_t s) {
   const int total = p.n * p.n;
   hipLaunchKernelGGL(fvm_curl_kernel, dim3((total + 255) / 256), dim3(256), 0, s, p, u, v, out);

@@ -155,6 +155,18 @@ int fvm_launch_projection(const FvmParams& p, const float* u, const float* v, fl
 int fvm_launch_divergence(const FvmParams& p, const float* u, const float* v, float* out,
                           hipStream_t s);
 int fvm_launch_curl(const FvmParams& p, const float* u, const float* v, float* out, hipStream_t s);
+// interpolation.linear along one axis (interpolation.py:36-64): a shift by f
+// (offset difference an integer, blend = 0) or fw * shift(f) + cw * shift(c)
+struct SlAxis {
+  int blend, f, c;
+  float fw, cw;
+};
+struct SlParams {
+  SlAxis u0, u1, v0, v1;  // u and v to c's offset, axis 0 then axis 1
+  float ox, oy, hx, hy, dt;
+};
+int fvm_launch_semilagrangian(int n, const float* c, const float* u, const float* v, float* out,
+                              const SlParams& q, hipStream_t s);
 int fvm_launch_filter(const FvmParams& p, const float* x, const float* multiplier, float* out,
                       float2* spec, const float2* tw, hipStream_t s);
 bool fvm_size_supported(int n);

@@ -110,6 +110,8 @@ _SIGNATURES = {
     "jcfd_fvm_step_slots": (ctypes.c_int, [_P, _P, _P, _P, ctypes.c_int64, _P]),
     "jcfd_fvm_cg_iterations": (ctypes.c_int32, [_P]),
     "jcfd_fvm_diffusion_cg_iterations": (ctypes.c_int32, [_P, ctypes.c_int32]),
+    "jcfd_fvm_advect_semilagrangian": (ctypes.c_int, [_P, _P, ctypes.c_double, ctypes.c_double,
+                                                      _P, _P, ctypes.c_double, _P, _P]),
     "jcfd_ns2d_step_slots": (ctypes.c_int, [_P, _P, _P, ctypes.c_int64, _P]),
     "jcfd_fvm_explicit_terms": (ctypes.c_int, [_P, _P, _P, _P, _P, _P]),
     "jcfd_fvm_projection": (ctypes.c_int, [_P, _P, _P, _P, _P, _P]),

@@ -236,6 +236,17 @@ class FvmPlan:
     _lib.check(rc, 'jcfd_fvm_curl_vjp')
     return ub, vb
 
+  def advect_semilagrangian(self, c, offset, u, v, dt: float):
+    """Semi-Lagrangian step of c (at `offset`) by the staggered (u, v)."""
+    c, u, v = self._field(c), self._field(u), self._field(v)
+    out = self._empty()
+    with _lib.dlpack_args() as dl:
+      rc = self._lib.jcfd_fvm_advect_semilagrangian(
+          self._handle, dl(c), float(offset[0]), float(offset[1]), dl(u), dl(v),
+          float(dt), dl(out), _lib.stream_handle(self.device))
+    _lib.check(rc, 'jcfd_fvm_advect_semilagrangian')
+    return out
+
   def curl(self, u, v):
     u, v = self._field(u), self._field(v)
     out = self._empty()

@@ -2,7 +2,7 @@
 
 ``advect_van_leer``, ``advect_van_leer_using_limiters``, ``advect_upwind``
 and ``advect_linear`` / ``convect_linear`` are the schemes the FVM kernels
-implement.  Inside a step factory they are read symbolically; called
+implement; ``advect_step_semilagrangian`` has a kernel of its own.  Inside a step factory they are read symbolically; called
 directly on a velocity component they run the explicit-terms kernel with
 diffusion and forcing switched off.
 """
@@ -88,6 +88,28 @@ def advect_van_leer_using_limiters(c: grids.GridVariable, v: grids.GridVariableV
   if _symbolic.is_probe(c):
     return _probe_term('advect_van_leer_using_limiters', c, v, dt)
   return _eager(c, v, dt, _lib.CONVECT_VAN_LEER_LIMITERS)
+
+
+def advect_step_semilagrangian(c: grids.GridVariable, v: grids.GridVariableVector,
+                               dt: float) -> grids.GridVariable:
+  """Semi-Lagrangian advection of c by v (advection.py:345-382): c sampled
+  at the departure points x - dt v(x) by periodic bilinear interpolation, on
+  the GPU.  Returns c at the next time step, not a time derivative."""
+  if _symbolic.is_probe(c):
+    raise NotImplementedError(
+        'advect_step_semilagrangian returns the advected field, not a time '
+        'derivative: it is not a convection term of the step factories')
+  grid = grids.consistent_grid(c.array, *[u.array for u in v])
+  if not all(d[0] == 0 for d in grid.domain):
+    raise ValueError(f'Grid domains currently must start at zero. Found {grid.domain}')
+  if not boundaries.has_all_periodic_boundary_conditions(c):
+    raise NotImplementedError('non-periodic BCs not yet supported')
+  if len(v) != 2 or tuple(v[0].offset) != (1.0, 0.5) or tuple(v[1].offset) != (0.5, 1.0):
+    raise NotImplementedError('the semi-Lagrangian kernel takes a 2-D staggered velocity')
+  spec = _fvm.spec_from_grid(grid)
+  plan = _fvm.plan_for(spec, c.data.device)
+  out = plan.advect_semilagrangian(c.data, c.offset, v[0].data, v[1].data, float(dt))
+  return grids.GridVariable(grids.GridArray(out, c.offset, grid), c.bc)
 
 
 def stable_time_step(max_velocity: float, max_courant_number: float,

@@ -345,6 +345,51 @@ def advect_van_leer_using_limiters(c: A, v: Sequence[A], dt: float,
   return A(-adv.data, adv.offset)
 
 
+def map_coordinates_wrap_linear(a: np.ndarray, indices: Sequence[np.ndarray]) -> np.ndarray:
+  """jax.scipy.ndimage.map_coordinates(a, indices, order=1, mode='wrap')
+  (JAX's published implementation): per axis lower = floor(x), weights
+  (1 - (x - lower), x - lower) at indices (lower, lower + 1) mod size; the
+  contributions in itertools.product order over the axes, each the product
+  of its weights (left to right) times the value, summed left to right."""
+  import itertools
+  dtype = a.dtype.type
+  nodes = []
+  for x, size in zip(indices, a.shape):
+    x = np.asarray(x, dtype)
+    lower = np.floor(x)
+    w_up = (x - lower).astype(dtype)
+    w_lo = (dtype(1) - w_up).astype(dtype)
+    k = lower.astype(np.int64)
+    nodes.append([(np.mod(k, size), w_lo), (np.mod(k + 1, size), w_up)])
+  out = None
+  for items in itertools.product(*nodes):
+    w = items[0][1]
+    for t in items[1:]:
+      w = w * t[1]
+    term = w * a[tuple(t[0] for t in items)]
+    out = term if out is None else out + term
+  return out
+
+
+def advect_step_semilagrangian(c: A, v: Sequence[A], dt: float, grid: Grid) -> A:
+  """advection.py:345-382: departure points x - dt * linear(u, c.offset) on
+  c's mesh (grids.py:584-602, float32), fractional indices x / step - offset,
+  then map_coordinates_wrap_linear.  Returns the advected field."""
+  if not all(lo == 0 for lo, _ in grid.domain):
+    raise ValueError(f'Grid domains currently must start at zero. Found {grid.domain}')
+  dtype = c.data.dtype.type
+  indices = []
+  for axis, u in enumerate(v):
+    n, h, o = grid.shape[axis], grid.step[axis], c.offset[axis]
+    x1 = (np.arange(n, dtype=np.int32) + dtype(o)).astype(dtype)
+    x1 = (dtype(grid.domain[axis][0]) + x1 * dtype(h)).astype(dtype)
+    x = np.meshgrid(*[x1 if a == axis else np.zeros(grid.shape[a], dtype)
+                      for a in range(grid.ndim)], indexing='ij')[axis]
+    coord = x - dtype(dt) * linear(u, c.offset).data.astype(dtype)
+    indices.append((coord / dtype(h) - dtype(o)).astype(dtype))
+  return A(map_coordinates_wrap_linear(c.data, indices), c.offset)
+
+
 def stable_time_step_advection(max_velocity: float, max_courant_number: float,
                                grid: Grid) -> float:
   """advection.py:398-416."""

@@ -665,3 +665,65 @@ def test_upwind_velocity_advection_matches_hand_written_stencil():
       ev = -((fxv(i, j) - fxv(i - 1, j)) / dx + (fyv(i, j) - fyv(i, j - 1)) / dy)
       assert abs(got_u[i, j] - eu) <= 1e-12 * max(1.0, abs(eu))
       assert abs(got_v[i, j] - ev) <= 1e-12 * max(1.0, abs(ev))
+
+
+# ------------------------------------------------- semi-Lagrangian advection
+def test_map_coordinates_wrap_linear_vs_scipy():
+  """The oracle's map_coordinates(order=1, mode='wrap') against SciPy's
+  independent implementation (mode 'grid-wrap', the periodic one) at random
+  points far outside the array, 1-D to 3-D."""
+  from scipy import ndimage
+  rng = np.random.default_rng(7)
+  for shape in [(11,), (6, 9), (4, 5, 3)]:
+    a = rng.standard_normal(shape)
+    pts = [rng.uniform(-3 * n, 3 * n, size=50) for n in shape]
+    want = ndimage.map_coordinates(a, pts, order=1, mode='grid-wrap')
+    np.testing.assert_allclose(go.map_coordinates_wrap_linear(a, pts), want, atol=1e-12)
+
+
+def test_semilagrangian_integer_displacement_is_a_roll():
+  """dt v = 2 h along axis 0 and -3 h along axis 1: every departure point
+  is a grid point, so the step is an exact periodic roll."""
+  grid = Grid.from_domain((16, 12), ((0, 2.), (0, 3.)))
+  rng = np.random.default_rng(8)
+  c = A(rng.standard_normal((16, 12)), grid.cell_center)
+  hx, hy = grid.step
+  dt = 0.5
+  v = (A(np.full((16, 12), 2 * hx / dt), grid.cell_faces[0]),
+       A(np.full((16, 12), -3 * hy / dt), grid.cell_faces[1]))
+  out = go.advect_step_semilagrangian(c, v, dt, grid)
+  np.testing.assert_allclose(out.data, np.roll(np.roll(c.data, 2, 0), -3, 1), atol=1e-12)
+
+
+def test_semilagrangian_half_cell_is_two_point_average():
+  """Courant 0.5 with unit velocity along axis 0: out[i] = (c[i-1] + c[i]) / 2,
+  with u interpolated to the cell centre from its faces."""
+  grid = Grid.from_step((10, 6), (0.1, 0.2))
+  c = A(np.random.default_rng(9).standard_normal((10, 6)), grid.cell_center)
+  v = (A(np.ones((10, 6)), grid.cell_faces[0]), A(np.zeros((10, 6)), grid.cell_faces[1]))
+  out = go.advect_step_semilagrangian(c, v, 0.05, grid)
+  np.testing.assert_allclose(out.data, 0.5 * (np.roll(c.data, 1, 0) + c.data), atol=1e-12)
+
+
+@pytest.mark.parametrize("shape", [(101,), (128, 128)])
+def test_semilagrangian_translates_gaussian(shape):
+  """advection_test.py:140-170 (semilagrangian_1D, and 2-D with the
+  velocity along axis 0): 100 steps at Courant 0.5 move the profile by 50
+  cells (atol 7e-2)."""
+  grid = Grid.from_step(shape, tuple(1. / s for s in shape))
+  n = shape[0]
+  x = (np.arange(n) - (n // 2 + 1)) / n  # advection_test.py:30-34 profile
+  prof = np.exp(-30. * x ** 2).astype(np.float32)
+  c = A(np.broadcast_to(prof.reshape((n,) + (1,) * (len(shape) - 1)), shape).copy(),
+        grid.cell_center)
+  v = tuple(A(np.full(shape, 1.0 if ax == 0 else 0.0, np.float32), off)
+            for ax, off in enumerate(grid.cell_faces))
+  dt = 0.5 * min(grid.step)
+  c0 = c
+  for _ in range(100):
+    c = go.advect_step_semilagrangian(c, v, dt, grid)
+  assert c.data.dtype == np.float32
+  np.testing.assert_allclose(c.data, np.roll(c0.data, 50, axis=0), atol=7e-2)
+  with pytest.raises(ValueError):
+    go.advect_step_semilagrangian(
+        c, v, dt, Grid.from_domain(shape, tuple((1., 2.) for _ in shape)))
