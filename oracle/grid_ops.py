@@ -15,6 +15,11 @@ Van-Leer-using-limiters on cell-centred scalars, TVD and mass conservation,
 Laplacian of constants/quadratics.  The velocity-component form of
 ``advect_van_leer`` (the stale-loop-variable quirk) is pinned by a brute-force
 per-cell evaluation of the hand-derived stencil in the tests.
+``map_coordinates_wrap_linear`` is pinned against SciPy's independent
+``ndimage.map_coordinates(order=1, mode='grid-wrap')``, and
+``advect_step_semilagrangian`` by exact periodic rolls (whole-cell
+displacements), the two-point average at Courant 0.5 and the reference's
+Gaussian translation case (advection_test.py:140-170).
 """
 from __future__ import annotations
 
