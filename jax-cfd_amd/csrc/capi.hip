@@ -1251,7 +1251,10 @@ int jcfd_fvm_group_step(jcfd_fvm_group* g, const jcfd_fvm_group_member* members,
       return fail(JCFD_ERR_UNSUPPORTED, who + ": groups step fast-diagonal plans only");
     if (p->params.radix != 8)
       return fail(JCFD_ERR_UNSUPPORTED, who + ": groups run the radix-8 FFT schedule only");
-    if (p->dlam != nullptr || p->rk_terms != nullptr)
+    // implicit-diffusion plans carry has_diffusion = 0 in their params (the
+    // solve runs after the projection), so the group would silently step
+    // them inviscid: reject every diffusion mode but the explicit one
+    if (p->cfg.diffusion != JCFD_DIFFUSION_EXPLICIT || p->dlam != nullptr || p->rk_terms != nullptr)
       return fail(JCFD_ERR_UNSUPPORTED,
                   who + ": groups step forward-Euler explicit-diffusion plans only");
     const int n = p->params.n;

@@ -36,10 +36,15 @@ struct FvmParams {
 
 // x / h correctly rounded, from r = RN(1/h): q = RN(x r) lies within one ulp
 // of x / h, and one remainder step RN(q + RN(x - q h) r) is then the
-// correctly rounded quotient (Markstein's theorem; it needs no overflow or
-// underflow in q h, which fp32 fields of grid-scale values never reach).
-// Three instructions where the IEEE division sequence takes about ten; the
-// result equals x / h (bitwise, except that x = -0 gives +0).
+// correctly rounded quotient (Markstein's theorem).  The theorem needs the
+// remainder x - q h to be exact; it is a multiple of about 2^(e_x - 47), so
+// it can fall under the subnormal grid once |x| < 2^-102.  Then the
+// correction is lost and the result is RN(x r), at most one ulp of the
+// quotient from x / h; and x = -0 gives +0.  So the result is bitwise x / h
+// for |x| >= 2^-100 (checked against IEEE division on the CPU,
+// tests/test_over_step_cpu.py), not unconditionally: flow differences that
+// small without being zero are the exception.  Three instructions where the
+// IEEE division sequence takes about ten.
 __device__ __forceinline__ float over_step(float x, float h, float r) {
   const float q = x * r;
   return __builtin_fmaf(__builtin_fmaf(-q, h, x), r, q);

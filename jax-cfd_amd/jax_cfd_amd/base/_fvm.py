@@ -282,10 +282,6 @@ class FvmGroup:
     self._handle = handle
     self._lib = lib
 
-  def diffusion_cg_iterations(self, component: int) -> int:
-    """Iterations of the last CG diffusion solve of component 0 (u) or 1 (v)."""
-    return int(self._lib.jcfd_fvm_diffusion_cg_iterations(self._handle, int(component)))
-
   def set_graphs(self, enabled: bool):
     _lib.check(self._lib.jcfd_fvm_group_set_graphs(self._handle, int(bool(enabled))),
                'jcfd_fvm_group_set_graphs')

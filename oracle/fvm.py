@@ -23,16 +23,27 @@ diffusion solve, constant fields unchanged (diffusion_test.py:58-76), the
 residual of (1 - nu dt L) x = b (diffusion_test.py:31-42), a dense
 ``numpy.linalg.solve`` brute force and the single-Fourier-mode closed form.
 The Runge-Kutta driver is pinned by the reference's harmonic-oscillator test
-(time_stepping_test.py:27-90, its per-stepper tolerances).  ``solve_cg``
-follows the
-published conjugate-gradient algorithm of jax.scipy.sparse.linalg.cg (no
-preconditioner, stop when |r|^2 <= max(tol^2 |b|^2, atol^2)); it is checked
-against the fast-diagonalisation solve, but its individual iterates are
-"parity unpinned" (no JAX here to compare against).  ``solve_cg_diffusion``
-runs the same recurrence on 1 - nu dt L from x0 = b; it is pinned by the
-reference's constant-field case (diffusion_test.py:58-76: no iteration, x = b
-exactly), against the dense brute force and the fast-diagonal solve, and by a
-residual bound; its iterates are likewise unpinned.
+(time_stepping_test.py:27-90, its per-stepper tolerances).
+
+``cg`` restates jax.scipy.sparse.linalg.cg, the third-party solver that both
+``pressure.solve_cg`` (pressure.py:105-111) and ``diffusion.solve_cg``
+(diffusion.py:156-162) call (jax is unpinned in setup.py:18 and absent here;
+the algorithm is the public unpreconditioned CG recurrence with SciPy's
+non-legacy stopping rule, as JAX publishes it in
+jax/_src/scipy/sparse/linalg.py `_cg_solve`).  It is pinned by what the
+mathematics fixes (tests/test_oracle_fvm.py): in fp64, every iterate x_k
+equals the Galerkin solution over the Krylov space K_k(A, r0) computed
+independently (orthonormal Krylov basis, projected solve), for a random SPD
+matrix, a negative-definite periodic Laplacian and the Helmholtz operator of
+the diffusion solve from x0 = b; a right-hand side that is one eigenvector
+converges in exactly one iteration with x_1 = (r.r / r.Ar) r; one with k
+distinct eigenvalues in exactly k; and the stopping rule -- continue while
+|r_k|^2 > max(tol^2 |b|^2, atol^2) and k < maxiter -- is pinned by placing
+tol, then atol, strictly between consecutive Galerkin residuals |r_{k-1}|^2
+and |r_k|^2 and requiring exactly k iterations, by ties (|r_k|^2 equal to
+the bound stops), and by maxiter.  The fp32 iterates are the same recurrence
+in float32; their rounding (the order of the dot-product sums, which XLA
+chooses) is the part only JAX could pin.
 """
 from __future__ import annotations
 
@@ -95,38 +106,61 @@ def _periodic_laplacian(x: np.ndarray, grid: Grid) -> np.ndarray:
   return go.laplacian(A(x, grid.cell_center), grid).data
 
 
-def solve_cg(v: Sequence[A], grid: Grid, rtol: float = 1e-6,
-             atol: float = 1e-6, maxiter: Optional[int] = None,
-             return_iterations: bool = False):
-  """pressure.py:67-112: unpreconditioned CG on the periodic Laplacian, x0 = 0.
+def _vdot(a: np.ndarray, b: np.ndarray, dtype) -> np.ndarray:
+  """_vdot_real_tree of one real leaf: sum(a * b) in the leaves' dtype."""
+  return np.vdot(a, b).real.astype(dtype)
 
-  Algorithm of jax.scipy.sparse.linalg.cg (the public CG recurrence with the
-  SciPy-style stopping rule |r|^2 > max(tol^2 |b|^2, atol^2) as loop
-  condition, maxiter default 10 * size).
-  """
-  rhs = go.divergence(v, grid)
-  b = rhs.data
+
+def cg(op: Callable[[np.ndarray], np.ndarray], b: np.ndarray, x0: np.ndarray,
+       tol: float, atol: float, maxiter: Optional[int] = None,
+       callback: Optional[Callable[[int, np.ndarray, np.ndarray], None]] = None):
+  """jax.scipy.sparse.linalg.cg(op, b, x0, tol=tol, atol=atol, maxiter=maxiter)
+  with the identity preconditioner, restated; returns (x, iterations).
+
+  bs = <b, b>; bound = max(tol^2 bs, atol^2); r0 = b - op(x0), p0 = r0,
+  gamma0 = <r0, r0>; while gamma > bound and k < maxiter:
+  alpha = gamma / <p, Ap>, x += alpha p, r -= alpha Ap, gamma' = <r, r>,
+  p = r + (gamma' / gamma) p.  maxiter defaults to 10 * b.size.  Scalars
+  and vectors stay in b's dtype (tol, atol are weakly typed Python floats).
+  `callback(k, x_k, r_k)` sees every iterate, k = 0 included."""
   dtype = b.dtype.type
   if maxiter is None:
     maxiter = 10 * b.size
-  bs = np.vdot(b, b).real.astype(dtype)
-  atol2 = max(np.square(dtype(rtol)) * bs, np.square(dtype(atol)))
-  x = np.zeros_like(b)
-  r = b - _periodic_laplacian(x, grid)
+  bs = _vdot(b, b, dtype)
+  bound = max(np.square(dtype(tol)) * bs, np.square(dtype(atol)))
+  x = x0.astype(dtype)
+  r = (b - op(x)).astype(dtype)
   p = r
-  gamma = np.vdot(r, r).real.astype(dtype)
+  gamma = _vdot(r, r, dtype)
   k = 0
-  while gamma > atol2 and k < maxiter:
-    ap = _periodic_laplacian(p, grid)
-    alpha = gamma / np.vdot(p, ap).real.astype(dtype)
-    x = x + alpha * p
-    r = r - alpha * ap
-    gamma_new = np.vdot(r, r).real.astype(dtype)
+  if callback is not None:
+    callback(0, x, r)
+  while gamma > bound and k < maxiter:
+    ap = op(p).astype(dtype)
+    alpha = gamma / _vdot(p, ap, dtype)
+    x = (x + alpha * p).astype(dtype)
+    r = (r - alpha * ap).astype(dtype)
+    gamma_new = _vdot(r, r, dtype)
     beta = gamma_new / gamma
-    p = r + beta * p
+    p = (r + beta * p).astype(dtype)
     gamma = gamma_new
     k += 1
-  out = A(x.astype(dtype), rhs.offset)
+    if callback is not None:
+      callback(k, x, r)
+  return x, k
+
+
+def solve_cg(v: Sequence[A], grid: Grid, rtol: float = 1e-6,
+             atol: float = 1e-6, maxiter: Optional[int] = None,
+             return_iterations: bool = False):
+  """pressure.py:67-112: cg on the periodic Laplacian (laplacian_with_bcs,
+  periodic impose_bc a no-op) with b = div v and x0 = q0 = 0
+  (pressure.py:189-192)."""
+  rhs = go.divergence(v, grid)
+  b = rhs.data
+  x, k = cg(lambda x: _periodic_laplacian(x, grid), b, np.zeros_like(b), rtol, atol,
+            maxiter)
+  out = A(x.astype(b.dtype), rhs.offset)
   return (out, k) if return_iterations else out
 
 
@@ -208,6 +242,8 @@ class FVMStep:
   cg_rtol: float = 1e-6
   cg_atol: float = 1e-6
   time_stepper: str = 'forward_euler'
+  # iterations of the last solve_cg call (pressure == 'cg'); a diagnostic
+  cg_iterations: int = dataclasses.field(default=0, compare=False)
 
   def convection(self, v: Sequence[A]):
     if self.convect == 'van_leer':
@@ -237,7 +273,9 @@ class FVMStep:
     if self.pressure == 'fast_diag':
       return solve_fast_diag(v, grid)
     if self.pressure == 'cg':
-      return solve_cg(v, grid, self.cg_rtol, self.cg_atol)
+      q, self.cg_iterations = solve_cg(v, grid, self.cg_rtol, self.cg_atol,
+                                       return_iterations=True)
+      return q
     raise ValueError(self.pressure)
 
   def intermediate(self, v: Sequence[A]):
@@ -294,41 +332,26 @@ def solve_fast_diag_diffusion(v: Sequence[A], nu: float, dt: float,
   return tuple(out)
 
 
+def helmholtz_op(grid: Grid, nu: float, dt: float, dtype):
+  """linear_op of diffusion.solve_cg (diffusion.py:150-153): x - dt * nu *
+  laplacian(x), the Python product dt * nu rounded to the field's dtype."""
+  coef = dtype(dt * nu)
+
+  def op(x):
+    return x - coef * _periodic_laplacian(x, grid).astype(dtype)
+  return op
+
+
 def solve_cg_diffusion(v: Sequence[A], nu: float, dt: float, grid: Grid,
                        rtol: float = 1e-6, atol: float = 1e-6,
                        maxiter: Optional[int] = None, return_iterations: bool = False):
   """diffusion.solve_cg (diffusion.py:137-167), periodic: each component u
-  becomes the CG solution of (1 - nu dt L) x = u from x0 = u, with the
-  operator evaluated as x - (dt * nu) * laplacian(x) (diffusion.py:150-153;
-  the Python product dt * nu rounded to the field's dtype) and the same CG
-  recurrence and stopping rule as solve_cg above."""
+  becomes cg(1 - nu dt L, b = u, x0 = u) (diffusion.py:156-164)."""
   out, iters = [], []
   for u in v:
     b = u.data
-    dtype = b.dtype.type
-    coef = dtype(dt * nu)
-
-    def op(x):
-      return x - coef * _periodic_laplacian(x, grid).astype(dtype)
-
-    n_max = 10 * b.size if maxiter is None else maxiter
-    bs = np.vdot(b, b).real.astype(dtype)
-    atol2 = max(np.square(dtype(rtol)) * bs, np.square(dtype(atol)))
-    x = b.copy()
-    r = b - op(x)
-    p = r
-    gamma = np.vdot(r, r).real.astype(dtype)
-    k = 0
-    while gamma > atol2 and k < n_max:
-      ap = op(p)
-      alpha = gamma / np.vdot(p, ap).real.astype(dtype)
-      x = x + alpha * p
-      r = r - alpha * ap
-      gamma_new = np.vdot(r, r).real.astype(dtype)
-      p = r + (gamma_new / gamma) * p
-      gamma = gamma_new
-      k += 1
-    out.append(A(x.astype(dtype), u.offset))
+    x, k = cg(helmholtz_op(grid, nu, dt, b.dtype.type), b, b, rtol, atol, maxiter)
+    out.append(A(x.astype(b.dtype), u.offset))
     iters.append(k)
   return (tuple(out), iters) if return_iterations else tuple(out)
 

@@ -140,3 +140,81 @@ def test_gradient_c4_config():
   print(f"256^2 64-step gradient, central differences: relative L2 over 16 directions {agg:.2e}; "
         f"per-direction median {np.median(np.abs(fds - ads) / np.abs(ads)):.2e}")
   assert agg <= 2e-2
+
+
+def _kink_fields(n, kind):
+  """Fields in exact arithmetic (small integers and halves: every face
+  average and difference is exact in fp32 and fp64, so both make the same
+  branch choices) that put faces on every kink of advect_van_leer: limiter
+  denominators and numerators exactly 0 (flat runs), face velocities exactly
+  0 with nonzero limiter values (+1/-1 alternations next to ramps), and
+  (kind 'bands') whole bands of zero velocity."""
+  pa = np.array([0, 0, 0, 1, 1, 2, 3, 4, 4, 4, -1, 1, -1, 1, 0, 0,
+                 2, 2, 2, 1, 0, -1, -2, -2, .5, .5, 0, 0, 3, 3, 3, 0], np.float64)
+  pb = np.array([1, 1, 1, 1, 2, 3, 4, -1, 1, -1, 1, -1, 0, 0, 0, .5,
+                 1, 1.5, 2, 2, 2, 0, 0, -3, -2, -1, 0, 1, 1, 1, 0, 0], np.float64)
+  reps = n // pa.size
+  a, b = np.tile(pa, reps), np.tile(pb, reps)
+  u = a[:, None] + b[None, :]
+  v = b[:, None] - a[None, ::-1]
+  if kind == 'bands':
+    u[n // 4: n // 2, :] = 0.0
+    v[:, n // 2: 3 * n // 4] = 0.0
+  return u.astype(np.float32), v.astype(np.float32)
+
+
+def _kink_counts(u, v):
+  """Faces of the x-direction flux of u at each kink (the other three flux
+  families have the same structure): U = 0, den = 0, num = 0, and U = 0 with
+  num and den of the same sign (a nonzero limiter, where d|U|/dU matters)."""
+  um1 = np.roll(u, 1, 0)
+  U = 0.5 * um1 + 0.5 * u
+  den = u - um1
+  num = np.where(U > 0, um1 - np.roll(u, 2, 0), np.roll(u, -1, 0) - u)
+  live = (U == 0) & (num * den > 0)
+  return dict(U0=int((U == 0).sum()), den0=int((den == 0).sum()),
+              num0=int((num == 0).sum()), U0_phi=int(live.sum()))
+
+
+@pytest.mark.parametrize("kind", ["stripes", "bands"])
+def test_step_vjp_at_limiter_kinks(kind):
+  """jcfd_fvm_step_vjp where the Van Leer limiter is not differentiable:
+  the GPU adjoint must apply jax's rules at the kinks (where routes the
+  cotangent to the taken branch, sign' = 0, the guarded quotient at den = 0,
+  d|x|/dx = +1 at 0) and match torch.autograd through the fp64 stand-in,
+  which uses the same rules, to <= 1e-4.  The same stand-in with
+  d|x|/dx = 0 at 0 (torch.abs) must differ from it by far more than that,
+  so the comparison decides the rule at U = 0."""
+  n = 64
+  grid, ogrid, step, tstep, _ = _setup(n)
+  u0, v0 = _kink_fields(n, kind)
+  counts = _kink_counts(u0, v0)
+  print(f"kinks ({kind}): {counts}", flush=True)
+  assert all(c > 0 for c in counts.values())
+  rng = np.random.default_rng(7)
+  lu, lv = (rng.standard_normal((n, n)).astype(np.float32) for _ in range(2))
+  plan = step.plan(_dev())
+  d = lambda x: torch.as_tensor(np.ascontiguousarray(x), device=_dev())  # noqa: E731
+  gu, gv = plan.step_vjp(d(u0), d(v0), d(lu), d(lv))
+  gu, gv = gu.cpu().numpy(), gv.cpu().numpy()
+
+  def stand_in():
+    u = torch.tensor(u0.astype(np.float64), requires_grad=True)
+    v = torch.tensor(v0.astype(np.float64), requires_grad=True)
+    uo, vo = tstep(u, v)
+    return torch.autograd.grad((uo * torch.tensor(lu, dtype=torch.float64)).sum()
+                               + (vo * torch.tensor(lv, dtype=torch.float64)).sum(), (u, v))
+  ru, rv = (x.numpy() for x in stand_in())
+  assert np.isfinite(gu).all() and np.isfinite(gv).all()
+  eu, ev = _rel(gu, ru), _rel(gv, rv)
+  jabs = tf.jabs
+  try:
+    tf.jabs = torch.abs  # the other rule at 0
+    au, av = (x.numpy() for x in stand_in())
+  finally:
+    tf.jabs = jabs
+  alt = max(_rel(au, ru), _rel(av, rv))
+  print(f"step vjp at kinks ({kind}): rel err u {eu:.2e} v {ev:.2e}; "
+        f"stand-in with d|x|/dx(0) = 0 differs by {alt:.2e}")
+  assert eu <= 1e-4 and ev <= 1e-4
+  assert alt >= 1e-3

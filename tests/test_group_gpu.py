@@ -114,3 +114,13 @@ def test_group_rejects_bad_members():
   ub = torch.zeros((1, 1024, 1024), device=_dev())
   with pytest.raises(NotImplementedError):
     group.step([(plan_big, ub, ub.clone(), [0])], 1)  # grid not built for groups
+  # implicit-diffusion plans (diffusion.solve_cg, diffusion.solve_fast_diag)
+  # would be stepped without their diffusion solve: rejected, not run
+  import dataclasses
+  from jax_cfd_amd import _lib
+  for mode in (_lib.DIFFUSION_IMPLICIT_CG, _lib.DIFFUSION_IMPLICIT_FAST_DIAG):
+    spec = dataclasses.replace(fam.spec, diffusion=mode, viscosity=1e-3)
+    plan_impl = cfd._fvm.FvmPlan(spec, _dev(), batch=1)
+    ui, vi = _fields(fam, 1, 7)
+    with pytest.raises(NotImplementedError):
+      group.step([(plan_impl, ui, vi, [0])], 1)

@@ -1,7 +1,9 @@
 """advect_step_semilagrangian on the GPU (advection.py:345-382) against the
 oracle's restatement: seeded fields at the four staggered offsets, with
-departure points up to several cells away (max relative error <= 1e-6; the
-kernel keeps the oracle's operation order, so most values are identical),
+departure points up to several cells away, offsets whose velocity
+interpolation is a nonzero whole shift, and a domain with dx != dy (max
+relative error <= 1e-6; the kernel keeps the oracle's operation order, so
+most values are identical),
 and the reference's Gaussian translation case in 2-D (100 steps at Courant
 0.5 move the profile 50 cells, atol 7e-2; advection_test.py:140-170)."""
 import numpy as np
@@ -31,12 +33,24 @@ def _var(x, offset, grid):
       cfd.boundaries.periodic_boundary_conditions(2))
 
 
-@pytest.mark.parametrize("n", [64, 256])
-@pytest.mark.parametrize("offset", [(0.5, 0.5), (1.0, 0.5), (0.5, 1.0), (1.0, 1.0)])
-def test_semilagrangian_parity(n, offset):
+# The staggered offsets, plus offsets whose difference to u's (1, 0.5) or
+# v's (0.5, 1) offset is a nonzero whole number on some axis, so the
+# velocity interpolation takes its pure-shift branch with a nonzero shift
+# (interpolation.py:52-56): (0, 0.5) and (0, 0) shift u by -1 along x,
+# (1.5, 0.5) shifts v by +1 along x, (0.5, 2) shifts v by +1 along y.
+OFFSETS = [(0.5, 0.5), (1.0, 0.5), (0.5, 1.0), (1.0, 1.0),
+           (0.0, 0.5), (0.0, 0.0), (1.5, 0.5), (0.5, 2.0)]
+# square cells, and a domain of lengths (2 pi, pi) so dx != dy
+DOMAINS = {'square': ((0, TWO_PI), (0, TWO_PI)), 'rect': ((0, TWO_PI), (0, np.pi))}
+
+
+@pytest.mark.parametrize("n,domain", [(64, 'square'), (256, 'square'), (64, 'rect')])
+@pytest.mark.parametrize("offset", OFFSETS)
+def test_semilagrangian_parity(n, domain, offset):
   cfd = _cfd()
-  grid = cfd.grids.Grid((n, n), domain=((0, TWO_PI), (0, TWO_PI)))
-  og = OGrid.from_domain((n, n), ((0, TWO_PI), (0, TWO_PI)))
+  dom = DOMAINS[domain]
+  grid = cfd.grids.Grid((n, n), domain=dom)
+  og = OGrid.from_domain((n, n), dom)
   rng = np.random.default_rng(n + int(4 * offset[0] + 2 * offset[1]))
   c = rng.standard_normal((n, n)).astype(np.float32)
   u = (3 * rng.standard_normal((n, n))).astype(np.float32)
@@ -44,7 +58,7 @@ def test_semilagrangian_parity(n, offset):
   dt = 0.7 * og.step[0]  # departure points up to ~10 cells away
   ref = go.advect_step_semilagrangian(
       A(c, offset), (A(u, og.cell_faces[0]), A(v, og.cell_faces[1])), dt, og).data
-  print(f"semilagrangian n={n} offset={offset}", flush=True)
+  print(f"semilagrangian n={n} domain={dom} offset={offset}", flush=True)
   out = cfd.advection.advect_step_semilagrangian(
       _var(c, offset, grid), (_var(u, (1.0, 0.5), grid), _var(v, (0.5, 1.0), grid)), dt)
   assert out.offset == offset
