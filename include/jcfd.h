@@ -164,6 +164,21 @@ int jcfd_fvm_step(jcfd_fvm_plan* plan, const jcfd_tensor* u, const jcfd_tensor* 
 int jcfd_fvm_step_slots(jcfd_fvm_plan* plan, jcfd_tensor* u, jcfd_tensor* v,
                         const jcfd_tensor* slots, int64_t nsteps, void* stream);
 
+/* jcfd_fvm_step without any allocation, for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  No counterpart in the reference.  The
+ * step's scratch (u*, v*, the packed divergence spectrum: 12 * batch * n^2
+ * bytes) lives in a static device array of 16 MiB that each GPU holds until
+ * the process ends, instead of the plan's scratch; larger plans are rejected
+ * (JCFD_ERR_INVALID_ARGUMENT), as are plans other than forward Euler with
+ * solve_fast_diag (JCFD_ERR_UNSUPPORTED).  Steps launch directly (never a
+ * graph).  Results equal jcfd_fvm_step's bit for bit.  All calls on one GPU
+ * share that GPU's array: calls that overlap in time on one GPU (two
+ * streams, two threads) race and must be serialised by the caller.  The
+ * first call on a device looks the array up (hipGetSymbolAddress). */
+int jcfd_fvm_step_noalloc(jcfd_fvm_plan* plan, const jcfd_tensor* u, const jcfd_tensor* v,
+                          jcfd_tensor* u_out, jcfd_tensor* v_out, int64_t nsteps, void* stream);
+
 /* Explicit terms dv/dt = convection + nu/rho * laplacian + forcing / rho
  * (equations.py:77-116, the explicit_terms_with_same_bcs seam). */
 int jcfd_fvm_explicit_terms(jcfd_fvm_plan* plan, const jcfd_tensor* u, const jcfd_tensor* v,

@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -822,6 +823,108 @@ int jcfd_fvm_step_slots(jcfd_fvm_plan* p, jcfd_tensor* u, jcfd_tensor* v, const 
   b.slots = static_cast<const int*>(ps);
   b.count = count;
   return fvm_run(p, b, nsteps, static_cast<hipStream_t>(stream));
+}
+
+}  // extern "C"
+
+// ------------------------------------------------- allocation-free stepping
+// Scratch of jcfd_fvm_step_noalloc: u*, v* and the packed divergence
+// spectrum of every instance of one call (12 * batch * n^2 bytes), in a
+// file-scope device array that every GPU the library runs on holds until the
+// process ends.  Host code never uses the array's name as an address (on the
+// host it is a stand-in symbol, not device memory): the device address comes
+// from hipGetSymbolAddress, once per device, with that device current.
+namespace {
+constexpr size_t kNoallocScratchBytes = size_t(16) << 20;
+constexpr int kNoallocMaxDevices = 64;
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kNoallocScratchBytes];
+
+namespace {
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kNoallocMaxDevices] = {};
+
+// Device address of `scratch` on `device` (looked up on the first call for
+// that device, with it made current for the lookup).
+int noalloc_scratch(int device, float** out) {
+  if (device < 0 || device >= kNoallocMaxDevices)
+    return fail(JCFD_ERR_UNSUPPORTED, "device id " + std::to_string(device) + " out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[device] == nullptr) {
+    int current = -1;
+    int rc;
+    if ((rc = check_hip(hipGetDevice(&current), "hipGetDevice"))) return rc;
+    if (current != device && (rc = check_hip(hipSetDevice(device), "hipSetDevice"))) return rc;
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (current != device) (void)hipSetDevice(current);
+    if ((rc = check_hip(e, "hipGetSymbolAddress(scratch)"))) return rc;
+    if (ptr == nullptr) return fail(JCFD_ERR_DEVICE, "hipGetSymbolAddress(scratch) gave a null address");
+    g_scratch_ptr[device] = ptr;
+  }
+  *out = static_cast<float*>(g_scratch_ptr[device]);
+  return JCFD_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int jcfd_fvm_step_noalloc(jcfd_fvm_plan* p, const jcfd_tensor* u, const jcfd_tensor* v,
+                          jcfd_tensor* u_out, jcfd_tensor* v_out, int64_t nsteps, void* stream) {
+  if (p == nullptr) return fail(JCFD_ERR_INVALID_ARGUMENT, "null plan");
+  if (nsteps < 0) return fail(JCFD_ERR_INVALID_ARGUMENT, "nsteps must be >= 0");
+  if (p->cfg.pressure != JCFD_PRESSURE_FAST_DIAG || p->rk_terms != nullptr ||
+      p->cfg.diffusion == JCFD_DIFFUSION_IMPLICIT_CG)
+    return fail(JCFD_ERR_UNSUPPORTED,
+                "jcfd_fvm_step_noalloc steps forward-Euler plans with solve_fast_diag "
+                "(explicit or fast-diagonal implicit diffusion)");
+  const int n = p->params.n;
+  const size_t cells = static_cast<size_t>(p->batch) * n * n;
+  const size_t need = 12 * cells;  // u*, v* (fp32) + packed spectrum (n^2 / 2 complex64)
+  if (need > kNoallocScratchBytes)
+    return fail(JCFD_ERR_INVALID_ARGUMENT,
+                "jcfd_fvm_step_noalloc: batch " + std::to_string(p->batch) + " x " +
+                    std::to_string(n) + "^2 needs " + std::to_string(need) +
+                    " bytes of scratch, above the static maximum of " +
+                    std::to_string(kNoallocScratchBytes) + " (12 * batch * n^2 <= 16 MiB)");
+  void *pu, *pv, *pu2, *pv2;
+  int rc;
+  if ((rc = check_batched(u, "u", kDLFloat, 32, p->batch, n, n, p->device, &pu)) ||
+      (rc = check_batched(v, "v", kDLFloat, 32, p->batch, n, n, p->device, &pv)) ||
+      (rc = check_batched(u_out, "u_out", kDLFloat, 32, p->batch, n, n, p->device, &pu2)) ||
+      (rc = check_batched(v_out, "v_out", kDLFloat, 32, p->batch, n, n, p->device, &pv2)))
+    return rc;
+  float* base = nullptr;
+  if ((rc = noalloc_scratch(p->device, &base))) return rc;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (nsteps == 0) {
+    const size_t bytes = sizeof(float) * cells;
+    if (pu2 != pu && (rc = check_hip(hipMemcpyAsync(pu2, pu, bytes, hipMemcpyDeviceToDevice, s), "copy")))
+      return rc;
+    if (pv2 != pv && (rc = check_hip(hipMemcpyAsync(pv2, pv, bytes, hipMemcpyDeviceToDevice, s), "copy")))
+      return rc;
+    return JCFD_OK;
+  }
+  // Every scratch word a step reads was written earlier in the same step
+  // (rows-forward writes u*, v* and the spectrum before the column and
+  // projection kernels read them), so nothing needs clearing between calls.
+  FvmBuffers b = fvm_buffers(p);
+  b.us = base;
+  b.vs = base + cells;
+  b.spec = reinterpret_cast<float2*>(base + 2 * cells);
+  b.u = static_cast<const float*>(pu);
+  b.v = static_cast<const float*>(pv);
+  b.u_out = static_cast<float*>(pu2);
+  b.v_out = static_cast<float*>(pv2);
+  b.count = p->batch;
+  // direct launches only: capturing a graph would allocate
+  for (int64_t k = 0; k < nsteps; ++k) {
+    if ((rc = fvm_full_step(p, b, s))) return rc;
+    b.u = b.u_out;
+    b.v = b.v_out;
+  }
+  return JCFD_OK;
 }
 
 int jcfd_fvm_explicit_terms(jcfd_fvm_plan* p, const jcfd_tensor* u, const jcfd_tensor* v,

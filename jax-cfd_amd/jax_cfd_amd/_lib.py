@@ -108,6 +108,7 @@ _SIGNATURES = {
     "jcfd_fvm_plan_device_bytes": (ctypes.c_int64, [_P]),
     "jcfd_fvm_step": (ctypes.c_int, [_P, _P, _P, _P, _P, ctypes.c_int64, _P]),
     "jcfd_fvm_step_slots": (ctypes.c_int, [_P, _P, _P, _P, ctypes.c_int64, _P]),
+    "jcfd_fvm_step_noalloc": (ctypes.c_int, [_P, _P, _P, _P, _P, ctypes.c_int64, _P]),
     "jcfd_fvm_cg_iterations": (ctypes.c_int32, [_P]),
     "jcfd_fvm_diffusion_cg_iterations": (ctypes.c_int32, [_P, ctypes.c_int32]),
     "jcfd_fvm_advect_semilagrangian": (ctypes.c_int, [_P, _P, ctypes.c_double, ctypes.c_double,

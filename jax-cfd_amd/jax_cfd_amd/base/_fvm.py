@@ -162,6 +162,18 @@ class FvmPlan:
     _lib.check(rc, 'jcfd_fvm_step')
     return uo, vo
 
+  def step_noalloc(self, u, v, nsteps: int = 1, out=None):
+    """As `step`, through jcfd_fvm_step_noalloc: the library allocates
+    nothing (the step's scratch is a static per-GPU array); `out` should be
+    passed by callers that cannot allocate either."""
+    u, v = self._field(u), self._field(v)
+    uo, vo = out if out is not None else (self._empty(True), self._empty(True))
+    with _lib.dlpack_args() as dl:
+      rc = self._lib.jcfd_fvm_step_noalloc(self._handle, dl(u), dl(v), dl(uo), dl(vo),
+                                           int(nsteps), _lib.stream_handle(self.device))
+    _lib.check(rc, 'jcfd_fvm_step_noalloc')
+    return uo, vo
+
   def explicit_terms(self, u, v):
     u, v = self._field(u), self._field(v)
     du, dv = self._empty(), self._empty()
