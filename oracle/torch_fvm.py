@@ -9,8 +9,13 @@ the fast-diagonal projection through torch.fft), differentiated by
 torch.autograd.  The differentiation rules match jax's for every operation
 used: `where` sends the cotangent to the selected branch, sign has zero
 derivative, division follows the quotient rule, and |x| is written as
-where(x >= 0, x, -x) so its derivative at 0 is +1 as in jax.  Runs in the
-dtype of its inputs (fp64 for the gradient reference).
+where(x >= 0, x, -x) so its derivative at 0 is +1, as jax's JVP rule for abs
+on reals (select(x >= 0, g, -g)) gives.  That rule at 0 is parity unpinned:
+no reference test reaches |U| at U = 0 with a nonzero limiter value.  Pinned
+by the reference's gradient tests (advection_test.py:298-351, restated in
+tests/test_oracle_torch.py), central differences, and equality with the
+NumPy oracle's forward step.  Runs in the dtype of its inputs (fp64 for the
+gradient reference).
 """
 from __future__ import annotations
 
