@@ -53,6 +53,7 @@ from typing import Callable, Optional, Sequence, Tuple
 
 import numpy as np
 
+from oracle import fftlib
 from oracle import grid_ops as go
 from oracle.grid_ops import A, Grid
 
@@ -97,7 +98,7 @@ def solve_fast_diag(v: Sequence[A], grid: Grid) -> A:
   rhs = go.divergence(v, grid)
   dtype = rhs.data.dtype.type
   diag = poisson_pinv_diagonal(grid, dtype)
-  q = np.fft.irfftn(diag * np.fft.rfftn(rhs.data), s=rhs.data.shape,
+  q = fftlib.irfftn(diag * fftlib.rfftn(rhs.data), s=rhs.data.shape,
                     axes=tuple(range(rhs.data.ndim)))
   return A(q.real.astype(dtype), rhs.offset)
 
@@ -326,7 +327,7 @@ def solve_fast_diag_diffusion(v: Sequence[A], nu: float, dt: float,
   for u in v:
     dtype = u.data.dtype.type
     diag = implicit_diffusion_diagonal(grid, nu, dt, dtype)
-    t = np.fft.irfftn(diag * np.fft.rfftn(u.data), s=u.data.shape,
+    t = fftlib.irfftn(diag * fftlib.rfftn(u.data), s=u.data.shape,
                       axes=tuple(range(u.data.ndim)))
     out.append(A(u.data + t.real.astype(dtype), u.offset))
   return tuple(out)

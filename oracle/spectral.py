@@ -24,6 +24,7 @@ from typing import Optional, Tuple
 
 import numpy as np
 
+from oracle import fftlib
 from oracle.grid_ops import Forcing, Grid
 
 # Carpenter-Kennedy coefficients, spectral/time_stepping.py:192-194.
@@ -96,13 +97,13 @@ class NavierStokes2D:
     """spectral/equations.py:160-184."""
     n = self.grid.shape
     vxhat, vyhat = self._v2v(vorticity_hat)
-    vx, vy = np.fft.irfftn(vxhat, s=n, axes=(0, 1)), np.fft.irfftn(vyhat, s=n, axes=(0, 1))
+    vx, vy = fftlib.irfftn(vxhat, s=n, axes=(0, 1)), fftlib.irfftn(vyhat, s=n, axes=(0, 1))
     grad_x_hat = 2j * np.pi * self.kx * vorticity_hat
     grad_y_hat = 2j * np.pi * self.ky * vorticity_hat
-    grad_x = np.fft.irfftn(grad_x_hat, s=n, axes=(0, 1))
-    grad_y = np.fft.irfftn(grad_y_hat, s=n, axes=(0, 1))
+    grad_x = fftlib.irfftn(grad_x_hat, s=n, axes=(0, 1))
+    grad_y = fftlib.irfftn(grad_y_hat, s=n, axes=(0, 1))
     advection = -(grad_x * vx + grad_y * vy)
-    advection_hat = np.fft.rfftn(advection)
+    advection_hat = fftlib.rfftn(advection)
     if self.smooth is not None:
       advection_hat = advection_hat * self.filter_
     terms = advection_hat
