@@ -1557,7 +1557,7 @@ int jcfd_ns2d_plan_create(const jcfd_ns2d_config* config, jcfd_ns2d_plan** plan)
   const jcfd_ns2d_config& c = *config;
   if (!ns_size_supported(c.n))
     return fail(JCFD_ERR_UNSUPPORTED,
-                "spectral grid size " + std::to_string(c.n) + " not built (32..512, powers of two)");
+                "spectral grid size " + std::to_string(c.n) + " not built (32..1024, powers of two)");
   if (!(c.dx > 0) || !(c.dy > 0) || !std::isfinite(c.dx) || !std::isfinite(c.dy))
     return fail(JCFD_ERR_INVALID_ARGUMENT, "grid steps must be positive and finite");
   if (!std::isfinite(c.dt) || !std::isfinite(c.viscosity) || !std::isfinite(c.drag))

@@ -615,6 +615,7 @@ template <> struct NsTiles<64> { static constexpr int W = 8, RB = 2; };
 template <> struct NsTiles<128> { static constexpr int W = 8, RB = 2; };
 template <> struct NsTiles<256> { static constexpr int W = 8, RB = 2; };
 template <> struct NsTiles<512> { static constexpr int W = 8, RB = 1; };
+template <> struct NsTiles<1024> { static constexpr int W = 4, RB = 1; };
 
 template <int N>
 struct NsOps {
@@ -752,7 +753,9 @@ int ns_explicit_impl(const NsParams& q, const float2* w, float2* out, float2* g,
 
 }  // namespace
 
-bool ns_size_supported(int n) { return n == 32 || n == 64 || n == 128 || n == 256 || n == 512; }
+bool ns_size_supported(int n) {
+  return n == 32 || n == 64 || n == 128 || n == 256 || n == 512 || n == 1024;
+}
 
 namespace {
 template <int N>
@@ -783,6 +786,7 @@ int ns_default_tiles(NsParams* q, int n) {
     case 128: NsDefaultOp<128>::run(q); return JCFD_OK;
     case 256: NsDefaultOp<256>::run(q); return JCFD_OK;
     case 512: NsDefaultOp<512>::run(q); return JCFD_OK;
+    case 1024: NsDefaultOp<1024>::run(q); return JCFD_OK;
     default: return fail(JCFD_ERR_UNSUPPORTED, "spectral grid size not built");
   }
 }
@@ -794,6 +798,7 @@ bool ns_tile_supported(int n, int which, int tile) {
     case 128: return ns_tile_on_menu<128>(which, tile);
     case 256: return ns_tile_on_menu<256>(which, tile);
     case 512: return ns_tile_on_menu<512>(which, tile);
+    case 1024: return ns_tile_on_menu<1024>(which, tile);
     default: return false;
   }
 }
@@ -807,6 +812,7 @@ int ns_launch_kernel(int n, int which, const NsParams* stages, int nstages, cons
     case 128: return ns_one_impl<128>(which, stages, nstages, b, batch, s);
     case 256: return ns_one_impl<256>(which, stages, nstages, b, batch, s);
     case 512: return ns_one_impl<512>(which, stages, nstages, b, batch, s);
+    case 1024: return ns_one_impl<1024>(which, stages, nstages, b, batch, s);
     default: return fail(JCFD_ERR_UNSUPPORTED, "spectral grid size not built");
   }
 }
@@ -820,6 +826,7 @@ int ns_launch_step(int n, const NsParams* stages, int nstages, const NsBuffers& 
     case 128: return ns_step_impl<128>(stages, nstages, b, batch, s);
     case 256: return ns_step_impl<256>(stages, nstages, b, batch, s);
     case 512: return ns_step_impl<512>(stages, nstages, b, batch, s);
+    case 1024: return ns_step_impl<1024>(stages, nstages, b, batch, s);
     default: return fail(JCFD_ERR_UNSUPPORTED, "spectral grid size not built");
   }
 }
@@ -832,6 +839,7 @@ int ns_launch_rfft2(int n, const NsParams& q, const float* x, float2* out, float
     case 128: return NsOps<128>::rfft2(q, x, out, pbuf, batch, s);
     case 256: return NsOps<256>::rfft2(q, x, out, pbuf, batch, s);
     case 512: return NsOps<512>::rfft2(q, x, out, pbuf, batch, s);
+    case 1024: return NsOps<1024>::rfft2(q, x, out, pbuf, batch, s);
     default: return fail(JCFD_ERR_UNSUPPORTED, "spectral grid size not built");
   }
 }
@@ -844,6 +852,7 @@ int ns_launch_irfft2(int n, const NsParams& q, const float2* in, float* x, float
     case 128: return NsOps<128>::irfft2(q, in, x, pbuf, batch, s);
     case 256: return NsOps<256>::irfft2(q, in, x, pbuf, batch, s);
     case 512: return NsOps<512>::irfft2(q, in, x, pbuf, batch, s);
+    case 1024: return NsOps<1024>::irfft2(q, in, x, pbuf, batch, s);
     default: return fail(JCFD_ERR_UNSUPPORTED, "spectral grid size not built");
   }
 }
@@ -869,6 +878,7 @@ int ns_launch_state_convert(int n, bool to_internal, const float2* src, float2* 
     case 128: return ns_convert_n<128>(to_internal, src, dst, slots, count, s);
     case 256: return ns_convert_n<256>(to_internal, src, dst, slots, count, s);
     case 512: return ns_convert_n<512>(to_internal, src, dst, slots, count, s);
+    case 1024: return ns_convert_n<1024>(to_internal, src, dst, slots, count, s);
     default: return fail(JCFD_ERR_UNSUPPORTED, "spectral grid size not built");
   }
 }
@@ -881,6 +891,7 @@ int ns_launch_explicit_terms(int n, const NsParams& q, const float2* w, float2* 
     case 128: return ns_explicit_impl<128>(q, w, out, g, pbuf, batch, s);
     case 256: return ns_explicit_impl<256>(q, w, out, g, pbuf, batch, s);
     case 512: return ns_explicit_impl<512>(q, w, out, g, pbuf, batch, s);
+    case 1024: return ns_explicit_impl<1024>(q, w, out, g, pbuf, batch, s);
     default: return fail(JCFD_ERR_UNSUPPORTED, "spectral grid size not built");
   }
 }

@@ -60,7 +60,7 @@ def _eqs(n, forced=True):
 
 
 @pytest.mark.parametrize("n,forced", [(32, True), (64, True), (128, False), (256, True),
-                                      (512, True)])
+                                      (512, True), (1024, True)])
 def test_explicit_terms_parity(n, forced):
   w = _w0(n, n)
   o32, _, eq = _eqs(n, forced)
@@ -85,7 +85,7 @@ def test_explicit_terms_drop_imaginary_dc_nyquist():
   assert _maxrel(out, ref) <= 1e-5
 
 
-@pytest.mark.parametrize("n", [64, 512])
+@pytest.mark.parametrize("n", [64, 512, 1024])
 def test_one_step_parity(n):
   w = _w0(n, 3)
   o32, o64, eq = _eqs(n)
@@ -201,7 +201,43 @@ def test_taylor_green_decay():
   np.testing.assert_allclose(out, w0 * np.exp(-2 * nu * 0.5), atol=1e-5)
 
 
-@pytest.mark.parametrize("n,b", [(64, 1), (256, 2)])
+def test_reference_taylor_green_accuracy_1024():
+  """spectral/equations_test.py:263-303 as written: Taylor-Green at 1024^2,
+  viscosity 1e-3, NavierStokes2D without forcing or drag, crank_nicolson_rk4
+  with dt = stable_time_step(7, 0.5, 1e-3, grid, implicit_diffusion=True),
+  ceil(0.11 / dt) steps from rfftn(curl_2d(v(0))), compared with
+  curl_2d(v(0.11)) at atol 1e-3."""
+  n, nu, time = 1024, 1e-3, 0.11
+  grid = _grid(n)
+  import jax_cfd_amd.base as cfd
+
+  def vorticity(t):
+    # staggered Taylor-Green velocity (validation_problems.py:80-103), fp32,
+    # then curl_2d (finite_differences.py:186-193) at offset (1, 1)
+    scale = np.exp(-2 * nu * t)
+    (ux, uy), (vx, vy) = [tuple(m.astype(np.float32) for m in grid.mesh(off))
+                          for off in grid.cell_faces]
+    u = (np.float32(scale) * np.cos(ux) * np.sin(uy)).astype(np.float32)
+    v = (-np.float32(scale) * np.sin(vx) * np.cos(vy)).astype(np.float32)
+    dx, dy = grid.step
+    return ((np.roll(v, -1, 0) - v) / np.float32(dx)
+            - (np.roll(u, -1, 1) - u) / np.float32(dy)).astype(np.float32)
+
+  dt = cfd.equations.stable_time_step(7, 0.5, nu, grid, implicit_diffusion=True)
+  steps = int(np.ceil(time / dt))
+  eq = _sp().equations.NavierStokes2D(nu, grid, drag=0.0)
+  step = _sp().time_stepping.crank_nicolson_rk4(eq, dt)
+  w0 = np.fft.rfftn(vorticity(0.0)).astype(np.complex64)
+  print(f"taylor-green spectral 1024^2: dt {dt:.6g}, {steps} steps", flush=True)
+  w = step.multistep(torch.as_tensor(w0, device=_dev()), steps)
+  out = np.fft.irfftn(w.cpu().numpy(), s=(n, n), axes=(0, 1))
+  want = vorticity(time)
+  err = float(np.abs(out - want).max())
+  print(f"  max |error| {err:.3e} (atol 1e-3)")
+  np.testing.assert_allclose(out, want, atol=1e-3)
+
+
+@pytest.mark.parametrize("n,b", [(64, 1), (256, 2), (1024, 1)])
 def test_rfft2_irfft2(n, b):
   from jax_cfd_amd.spectral import _ns2d
   plan = _ns2d.Ns2dPlan(_sp().equations.NavierStokes2D(1e-3, _grid(n)).spec(0.0), b, _dev())

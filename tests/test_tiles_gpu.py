@@ -105,7 +105,7 @@ def _spectral(n, batch=1):
   return fam, _ns2d.Ns2dPlan(fam.spec, batch, _dev())
 
 
-@pytest.mark.parametrize("n", [64, 512])
+@pytest.mark.parametrize("n", [64, 512, 1024])
 def test_spectral_tile_shapes_bit_identical_and_radix4_close(n):
   fam, plan = _spectral(n)
   (w0,) = fam.initial_state(7)
