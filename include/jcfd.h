@@ -280,6 +280,80 @@ int jcfd_fvm_group_set_graphs(jcfd_fvm_group* group, int32_t enabled);
 int jcfd_fvm_group_step(jcfd_fvm_group* group, const jcfd_fvm_group_member* members,
                         int32_t n_members, int64_t nsteps, void* stream);
 
+/* ------------------------------------------- learned interpolation (ml/) */
+/* The learned-interpolation FVM step: modular_navier_stokes_model
+ * (ml/equations.py:92-146) with self_advection(modular_self_advection(
+ * FusedLearnedInterpolation)) (ml/advections.py:48-85,
+ * ml/interpolations.py:31-121) as the convection, forward Euler with
+ * explicit diffusion (semi_implicit_navier_stokes) or
+ * implicit_diffusion_navier_stokes with diffusion.solve_fast_diag, and the
+ * fast-diagonal projection.  Each step runs the CNN tower
+ * (ml/towers.py:144-262: periodic 3x3 convolutions with relu between them)
+ * on (u, v) on the matrix cores, turns its output into the 8
+ * interpolations' stencil coefficients, interpolates u and v to the
+ * control-volume faces (4 x 4 stencils, ml/layers.py:586-642) and advects
+ * (base/advection.py:34-116). */
+#define JCFD_LI_MAX_LAYERS 16
+
+typedef struct {
+  jcfd_fvm_config fvm;       /* physics: n >= 32, batch, dt, density, viscosity,
+                                forcing, diffusion (explicit or implicit fast-
+                                diagonal); pressure must be fast-diagonal, the
+                                stepper forward Euler; convect is ignored */
+  int32_t n_layers;          /* convolutions of the tower (1 .. 16): 2 -> hidden,
+                                hidden -> hidden ..., hidden -> output */
+  int32_t hidden_channels;   /* 16, 32 or 64 */
+  int32_t n_logits;          /* the tower's own output channels (8 interpolations
+                                x subspace size: 120 for 4 x 4 stencils and two
+                                tags), <= 128 */
+  int32_t interp_key[8];     /* first coefficient channel (16 per interpolation,
+                                in the tower's split order) of the interpolation
+                                each face flux uses: u's x face (advected u,
+                                velocity u), u's y face (advected u, velocity
+                                v), v's x face (advected v, velocity u), v's y
+                                face (advected v, velocity v) */
+} jcfd_li_config;
+
+typedef struct jcfd_li_plan jcfd_li_plan;
+
+/* Build a plan (device scratch for the tower's activations, the 128
+ * coefficient planes per instance and the explicit terms).  Replaces the
+ * factory modular_navier_stokes_model (ml/equations.py:92-146). */
+int jcfd_li_plan_create(const jcfd_li_config* config, jcfd_li_plan** plan);
+int jcfd_li_plan_destroy(jcfd_li_plan* plan);
+int64_t jcfd_li_plan_device_bytes(const jcfd_li_plan* plan);
+/* Weights of convolution `layer` as HOST float32 arrays in haiku's Conv2D
+ * layout (3, 3, cin, cout) and its bias (cout) (ml/layers.py:24-116).  For
+ * the output layer (layer n_layers - 1) these are the coefficient layer:
+ * cout = 128, the tower's output weights times each interpolation's
+ * constraint nullspace (and bias . nullspace + the constraint's bias),
+ * which the caller forms (PolynomialConstraint, ml/layers.py:451-544). */
+int jcfd_li_set_layer(jcfd_li_plan* plan, int32_t layer, const float* w, const float* b);
+/* The tower's own output layer (3, 3, cin, n_logits), (n_logits): only
+ * jcfd_li_tower's logits output uses it. */
+int jcfd_li_set_logits_layer(jcfd_li_plan* plan, const float* w, const float* b);
+/* nsteps learned-interpolation steps of (u, v) -> (u_out, v_out), fields
+ * (batch, n, n) (or (n, n) for one instance).  Replaces
+ * funcutils.repeated(step_fn, nsteps) of the modular model. */
+int jcfd_li_step(jcfd_li_plan* plan, const jcfd_tensor* u, const jcfd_tensor* v,
+                 jcfd_tensor* u_out, jcfd_tensor* v_out, int64_t nsteps, void* stream);
+/* The tower on (u, v) into out (batch * 128, n, n) planes (instance b's
+ * channel c at plane 128 b + c): the logits (first n_logits channels; the
+ * tower network's output, interpolations.py:93-96) when coefficients == 0,
+ * else the 8 x 16 stencil coefficients (bias + logits . nullspace). */
+int jcfd_li_tower(jcfd_li_plan* plan, const jcfd_tensor* u, const jcfd_tensor* v,
+                  jcfd_tensor* out, int32_t coefficients, void* stream);
+/* The explicit terms of the step: convection by the learned interpolations
+ * plus diffusion (explicit plans) and forcing, divided by density
+ * (equations.py:77-116). */
+int jcfd_li_explicit_terms(jcfd_li_plan* plan, const jcfd_tensor* u, const jcfd_tensor* v,
+                           jcfd_tensor* du, jcfd_tensor* dv, void* stream);
+/* One interpolation (0 .. 7, the tower's split order) of field x with the
+ * coefficient planes of jcfd_li_tower(coefficients = 1): the interpolator
+ * of FusedLearnedInterpolation.__call__ (interpolations.py:98-121). */
+int jcfd_li_interpolate(jcfd_li_plan* plan, const jcfd_tensor* coefficients, const jcfd_tensor* x,
+                        int32_t interpolation, jcfd_tensor* out, void* stream);
+
 /* ------------------------------------------------------- spectral path */
 #define JCFD_MAX_RK_STAGES 8
 

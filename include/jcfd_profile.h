@@ -28,9 +28,21 @@ extern "C" {
 int jcfd_fvm_time_kernels(jcfd_fvm_plan* plan, const jcfd_tensor* u, const jcfd_tensor* v,
                           int32_t reps, double* ms_per_launch, void* stream);
 
-/* Algorithmic HBM bytes one launch of each kernel class moves (distinct
- * logical inputs read once + outputs written once). */
+/* Algorithmic HBM bytes one launch of each kernel class moves over every
+ * instance of the plan (distinct logical inputs read once + outputs written
+ * once). */
 int jcfd_fvm_kernel_bytes(const jcfd_fvm_plan* plan, int64_t* bytes_per_launch);
+
+/* Learned-interpolation step: ms_per_launch[k] for each of the tower's
+ * n_layers convolutions (the output layer is the coefficient layer), then
+ * the learned explicit-terms kernel, then the combination u + dt k with its
+ * projection (three launches, timed together); n_layers + 2 entries. */
+int jcfd_li_time_kernels(jcfd_li_plan* plan, const jcfd_tensor* u, const jcfd_tensor* v,
+                         int32_t reps, double* ms_per_launch, void* stream);
+/* Algorithmic floating-point operations of each convolution of the tower
+ * (2 * 9 * cin * cout multiply-adds per cell, cout = the tower's own output
+ * channels for the last layer), n_layers entries. */
+int jcfd_li_kernel_flops(const jcfd_li_plan* plan, double* flops_per_launch);
 
 /* Kernel classes of one CN-RK4 step of the spectral plan. */
 #define JCFD_NS2D_KERNEL_COLS_INVERSE 0       /* stage-0 inverse column FFTs */

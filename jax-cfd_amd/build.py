@@ -12,7 +12,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 INCLUDE = os.path.join(ROOT, "include")
 OUT = os.path.join(HERE, "jax_cfd_amd", "libjcfd_amd.so")
-SOURCES = ["capi.hip", "fvm.hip", "spectral.hip", "cg.hip", "adjoint.hip"]
+SOURCES = ["capi.hip", "fvm.hip", "spectral.hip", "cg.hip", "adjoint.hip", "ml.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = [

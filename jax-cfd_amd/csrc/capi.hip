@@ -1062,7 +1062,8 @@ int jcfd_fvm_time_kernels(jcfd_fvm_plan* p, const jcfd_tensor* u, const jcfd_ten
 
 int jcfd_fvm_kernel_bytes(const jcfd_fvm_plan* p, int64_t* bytes) {
   if (p == nullptr || bytes == nullptr) return fail(JCFD_ERR_INVALID_ARGUMENT, "null argument");
-  const int64_t cells = static_cast<int64_t>(p->params.n) * p->params.n;
+  // every instance of the plan: time_kernels launches them all
+  const int64_t cells = static_cast<int64_t>(p->params.n) * p->params.n * p->batch;
   bytes[JCFD_FVM_KERNEL_ROWS_FORWARD] = 20 * cells;  // read u,v; write u*,v*; write packed spectrum
   bytes[JCFD_FVM_KERNEL_POISSON_COLS] = 8 * cells;   // read + write packed spectrum
   bytes[JCFD_FVM_KERNEL_ROWS_PROJECT] = 20 * cells;  // read spectrum, u*, v*; write u, v
@@ -1191,6 +1192,384 @@ int jcfd_fvm_spectral_filter(jcfd_fvm_plan* p, const jcfd_tensor* x, const jcfd_
     return rc;
   return fvm_launch_filter(p->params, static_cast<const float*>(px), static_cast<const float*>(pm),
                            static_cast<float*>(po), p->spec, p->tw, static_cast<hipStream_t>(stream));
+}
+
+}  // extern "C"
+
+// ------------------------------------------------- learned interpolation
+struct jcfd_li_plan {
+  jcfd_fvm_plan* fvm = nullptr;  // physics, projection and diffusion (forward Euler)
+  int n = 0, batch = 1, device = 0;
+  int n_layers = 0, hidden = 0, n_logits = 0;
+  int cin[JCFD_LI_MAX_LAYERS] = {}, cout[JCFD_LI_MAX_LAYERS] = {};
+  float* w[JCFD_LI_MAX_LAYERS] = {};  // device [tap][cin (first layer: 4)][cout]
+  float* b[JCFD_LI_MAX_LAYERS] = {};
+  float* w_logits = nullptr;  // the tower's own output layer, padded to 128 channels
+  float* b_logits = nullptr;
+  bool layer_set[JCFD_LI_MAX_LAYERS] = {};
+  bool logits_set = false;
+  float* act[2] = {nullptr, nullptr};  // (batch, hidden, n, n)
+  float* coef = nullptr;               // (batch, 128, n, n)
+  float* kbuf = nullptr;               // (2, batch, n, n): k_u, k_v
+  RkTerms* rk = nullptr;               // u* = u + dt (0 + 1 k)
+  int key[8] = {};
+  int64_t bytes = 0;
+};
+
+namespace {
+// The tower over (u, v) into `out` (the coefficient layer, or the logits
+// layer when logits is set).
+int li_tower(jcfd_li_plan* p, const float* u, const float* v, float* out, bool logits, hipStream_t s) {
+  int rc = JCFD_OK;
+  const float* in = u;
+  for (int k = 0; k < p->n_layers && rc == JCFD_OK; ++k) {
+    const bool last = k + 1 == p->n_layers;
+    LiConvArgs a{};
+    a.n = p->n;
+    a.in = in;
+    a.in_v = k == 0 ? v : nullptr;
+    a.w = (last && logits) ? p->w_logits : p->w[k];
+    a.b = (last && logits) ? p->b_logits : p->b[k];
+    a.out = last ? out : p->act[k % 2];
+    rc = li_launch_conv(a, p->cin[k], p->cout[k], !last, p->batch, s);
+    in = a.out;
+  }
+  return rc;
+}
+
+int li_explicit(jcfd_li_plan* p, const float* u, const float* v, float* ku, float* kv, hipStream_t s) {
+  int rc = li_tower(p, u, v, p->coef, false, s);
+  if (rc) return rc;
+  LiExplicitArgs a{};
+  a.coef = p->coef;
+  a.u = u;
+  a.v = v;
+  a.ku = ku;
+  a.kv = kv;
+  a.fx = p->fvm->params.has_forcing ? p->fvm->fx : nullptr;
+  std::memcpy(a.key, p->key, sizeof(a.key));
+  return li_launch_explicit(p->fvm->params, a, p->batch, s);
+}
+
+// One step: tower, explicit terms, u* = u + dt k with its projection, then
+// (implicit-diffusion plans) the diffusion solve.
+int li_step_once(jcfd_li_plan* p, const float* u, const float* v, float* u_out, float* v_out,
+                 hipStream_t s) {
+  const size_t field = static_cast<size_t>(p->n) * p->n * p->batch;
+  int rc = li_explicit(p, u, v, p->kbuf, p->kbuf + field, s);
+  if (rc) return rc;
+  FvmBuffers c = fvm_buffers(p->fvm);
+  c.u = u;
+  c.v = v;
+  c.u_out = u_out;
+  c.v_out = v_out;
+  c.count = p->batch;
+  c.rk = p->rk;
+  if ((rc = fvm_launch_rk_project(p->fvm->params, c, s))) return rc;
+  return fvm_diffuse_output(p->fvm, c, s);
+}
+
+int li_ready(const jcfd_li_plan* p) {
+  if (p == nullptr) return fail(JCFD_ERR_INVALID_ARGUMENT, "null plan");
+  for (int k = 0; k < p->n_layers; ++k)
+    if (!p->layer_set[k]) return fail(JCFD_ERR_INVALID_ARGUMENT, "tower layer " + std::to_string(k) + " not set");
+  return JCFD_OK;
+}
+
+int li_upload_layer(const float* w, const float* b, int cin, int cin_dev, int cout, int cout_dev,
+                    float* wd, float* bd) {
+  // (3, 3, cin, cout) host -> [tap][cin_dev][cout_dev], zero padded
+  std::vector<float> wh(static_cast<size_t>(9) * cin_dev * cout_dev, 0.f), bh(cout_dev, 0.f);
+  for (int t = 0; t < 9; ++t)
+    for (int ci = 0; ci < cin; ++ci)
+      for (int co = 0; co < cout; ++co)
+        wh[(static_cast<size_t>(t) * cin_dev + ci) * cout_dev + co] =
+            w[(static_cast<size_t>(t) * cin + ci) * cout + co];
+  for (int co = 0; co < cout; ++co) bh[co] = b[co];
+  for (float x : wh)
+    if (!std::isfinite(x)) return fail(JCFD_ERR_INVALID_ARGUMENT, "weights must be finite");
+  for (float x : bh)
+    if (!std::isfinite(x)) return fail(JCFD_ERR_INVALID_ARGUMENT, "biases must be finite");
+  int rc;
+  if ((rc = check_hip(hipMemcpy(wd, wh.data(), wh.size() * sizeof(float), hipMemcpyHostToDevice), "copy weights")))
+    return rc;
+  return check_hip(hipMemcpy(bd, bh.data(), bh.size() * sizeof(float), hipMemcpyHostToDevice), "copy biases");
+}
+}  // namespace
+
+extern "C" {
+
+int jcfd_li_plan_destroy(jcfd_li_plan* p) {
+  if (p == nullptr) return JCFD_OK;
+  for (int k = 0; k < JCFD_LI_MAX_LAYERS; ++k) {
+    if (p->w[k]) (void)hipFree(p->w[k]);
+    if (p->b[k]) (void)hipFree(p->b[k]);
+  }
+  for (void* ptr : {static_cast<void*>(p->w_logits), static_cast<void*>(p->b_logits),
+                    static_cast<void*>(p->act[0]), static_cast<void*>(p->act[1]),
+                    static_cast<void*>(p->coef), static_cast<void*>(p->kbuf), static_cast<void*>(p->rk)})
+    if (ptr != nullptr) (void)hipFree(ptr);
+  jcfd_fvm_plan_destroy(p->fvm);
+  delete p;
+  return JCFD_OK;
+}
+
+int jcfd_li_plan_create(const jcfd_li_config* config, jcfd_li_plan** plan) {
+  if (config == nullptr || plan == nullptr) return fail(JCFD_ERR_INVALID_ARGUMENT, "null argument");
+  *plan = nullptr;
+  const jcfd_li_config& c = *config;
+  const jcfd_fvm_config& f = c.fvm;
+  if (f.n < 32) return fail(JCFD_ERR_UNSUPPORTED, "learned interpolation needs n >= 32");
+  if (f.pressure != JCFD_PRESSURE_FAST_DIAG)
+    return fail(JCFD_ERR_UNSUPPORTED, "learned interpolation is built with solve_fast_diag");
+  if (f.diffusion == JCFD_DIFFUSION_IMPLICIT_CG)
+    return fail(JCFD_ERR_UNSUPPORTED, "learned interpolation is built with explicit or fast-diagonal diffusion");
+  if (f.rk_stages > 1 || (f.rk_stages == 1 && f.rk_b[0] != 1.0))
+    return fail(JCFD_ERR_UNSUPPORTED, "learned interpolation is built with forward Euler");
+  if (c.n_layers < 1 || c.n_layers > JCFD_LI_MAX_LAYERS)
+    return fail(JCFD_ERR_UNSUPPORTED, "towers of 1 to " + std::to_string(JCFD_LI_MAX_LAYERS) +
+                                          " convolutions are built");
+  if (c.n_layers > 1 && c.hidden_channels != 16 && c.hidden_channels != 32 && c.hidden_channels != 64)
+    return fail(JCFD_ERR_UNSUPPORTED, "hidden channels must be 16, 32 or 64");
+  if (c.n_logits < 1 || c.n_logits > kLiCoefChannels)
+    return fail(JCFD_ERR_INVALID_ARGUMENT, "the tower's output channels must be 1 .. 128");
+  for (int k = 0; k < 8; ++k)
+    if (c.interp_key[k] < 0 || c.interp_key[k] > kLiCoefChannels - 16 || c.interp_key[k] % 16 != 0)
+      return fail(JCFD_ERR_INVALID_ARGUMENT, "interpolation keys must be multiples of 16 below 128");
+  jcfd_fvm_config fc = f;
+  fc.convect = JCFD_CONVECT_LINEAR;  // the convection is the learned one; the plan's kernels are unused
+  fc.rk_stages = 0;
+  auto* p = new (std::nothrow) jcfd_li_plan();
+  if (p == nullptr) return fail(JCFD_ERR_OUT_OF_MEMORY, "host allocation failed");
+  int rc = jcfd_fvm_plan_create(&fc, &p->fvm);
+  if (rc) {
+    delete p;
+    return rc;
+  }
+  p->n = f.n;
+  p->batch = p->fvm->batch;
+  p->device = p->fvm->device;
+  p->n_layers = c.n_layers;
+  p->hidden = c.hidden_channels;
+  p->n_logits = c.n_logits;
+  std::memcpy(p->key, c.interp_key, sizeof(p->key));
+  for (int k = 0; k < c.n_layers; ++k) {
+    p->cin[k] = k == 0 ? 2 : c.hidden_channels;
+    p->cout[k] = k + 1 == c.n_layers ? kLiCoefChannels : c.hidden_channels;
+  }
+  const size_t nn = static_cast<size_t>(p->n) * p->n * p->batch;
+  for (int k = 0; k < c.n_layers && rc == JCFD_OK; ++k) {
+    const int cin_dev = k == 0 ? 4 : p->cin[k];
+    if ((rc = device_alloc(&p->w[k], static_cast<size_t>(9) * cin_dev * p->cout[k], &p->bytes))) break;
+    rc = device_alloc(&p->b[k], static_cast<size_t>(p->cout[k]), &p->bytes);
+  }
+  const int cin_last = c.n_layers == 1 ? 4 : c.hidden_channels;
+  if (rc == JCFD_OK &&
+      ((rc = device_alloc(&p->w_logits, static_cast<size_t>(9) * cin_last * kLiCoefChannels, &p->bytes)) ||
+       (rc = device_alloc(&p->b_logits, static_cast<size_t>(kLiCoefChannels), &p->bytes)) ||
+       (c.n_layers > 1 && (rc = device_alloc(&p->act[0], nn * c.hidden_channels, &p->bytes))) ||
+       (c.n_layers > 2 && (rc = device_alloc(&p->act[1], nn * c.hidden_channels, &p->bytes))) ||
+       (rc = device_alloc(&p->coef, nn * kLiCoefChannels, &p->bytes)) ||
+       (rc = device_alloc(&p->kbuf, 2 * nn, &p->bytes)) ||
+       (rc = device_alloc(&p->rk, 1, &p->bytes)))) {
+  }
+  if (rc == JCFD_OK) {
+    RkTerms t{};
+    t.n = 1;
+    t.c[0] = 1.0f;
+    t.ku[0] = p->kbuf;
+    t.kv[0] = p->kbuf + nn;
+    rc = check_hip(hipMemcpy(p->rk, &t, sizeof(t), hipMemcpyHostToDevice), "copy combination");
+  }
+  if (rc) {
+    jcfd_li_plan_destroy(p);
+    return rc;
+  }
+  *plan = p;
+  return JCFD_OK;
+}
+
+int64_t jcfd_li_plan_device_bytes(const jcfd_li_plan* p) {
+  return p ? p->bytes + p->fvm->bytes : 0;
+}
+
+int jcfd_li_set_layer(jcfd_li_plan* p, int32_t layer, const float* w, const float* b) {
+  if (p == nullptr || w == nullptr || b == nullptr) return fail(JCFD_ERR_INVALID_ARGUMENT, "null argument");
+  if (layer < 0 || layer >= p->n_layers) return fail(JCFD_ERR_INVALID_ARGUMENT, "layer out of range");
+  int rc = li_upload_layer(w, b, p->cin[layer], layer == 0 ? 4 : p->cin[layer], p->cout[layer],
+                           p->cout[layer], p->w[layer], p->b[layer]);
+  if (rc == JCFD_OK) p->layer_set[layer] = true;
+  return rc;
+}
+
+int jcfd_li_set_logits_layer(jcfd_li_plan* p, const float* w, const float* b) {
+  if (p == nullptr || w == nullptr || b == nullptr) return fail(JCFD_ERR_INVALID_ARGUMENT, "null argument");
+  const int last = p->n_layers - 1;
+  int rc = li_upload_layer(w, b, p->cin[last], last == 0 ? 4 : p->cin[last], p->n_logits,
+                           kLiCoefChannels, p->w_logits, p->b_logits);
+  if (rc == JCFD_OK) p->logits_set = true;
+  return rc;
+}
+
+int jcfd_li_tower(jcfd_li_plan* p, const jcfd_tensor* u, const jcfd_tensor* v, jcfd_tensor* out,
+                  int32_t coefficients, void* stream) {
+  int rc = li_ready(p);
+  if (rc) return rc;
+  if (!coefficients && !p->logits_set) return fail(JCFD_ERR_INVALID_ARGUMENT, "logits layer not set");
+  const int n = p->n;
+  void *pu, *pv, *po;
+  if ((rc = check_batched(u, "u", kDLFloat, 32, p->batch, n, n, p->device, &pu)) ||
+      (rc = check_batched(v, "v", kDLFloat, 32, p->batch, n, n, p->device, &pv)) ||
+      (rc = check_batched(out, "out", kDLFloat, 32, p->batch * kLiCoefChannels, n, n, p->device, &po)))
+    return rc;
+  return li_tower(p, static_cast<const float*>(pu), static_cast<const float*>(pv), static_cast<float*>(po),
+                  !coefficients, static_cast<hipStream_t>(stream));
+}
+
+int jcfd_li_explicit_terms(jcfd_li_plan* p, const jcfd_tensor* u, const jcfd_tensor* v, jcfd_tensor* du,
+                           jcfd_tensor* dv, void* stream) {
+  int rc = li_ready(p);
+  if (rc) return rc;
+  const int n = p->n;
+  void *pu, *pv, *pdu, *pdv;
+  if ((rc = check_batched(u, "u", kDLFloat, 32, p->batch, n, n, p->device, &pu)) ||
+      (rc = check_batched(v, "v", kDLFloat, 32, p->batch, n, n, p->device, &pv)) ||
+      (rc = check_batched(du, "du", kDLFloat, 32, p->batch, n, n, p->device, &pdu)) ||
+      (rc = check_batched(dv, "dv", kDLFloat, 32, p->batch, n, n, p->device, &pdv)))
+    return rc;
+  if (pdu == pu || pdu == pv || pdv == pu || pdv == pv)
+    return fail(JCFD_ERR_INVALID_ARGUMENT, "explicit terms: outputs must not alias inputs");
+  return li_explicit(p, static_cast<const float*>(pu), static_cast<const float*>(pv), static_cast<float*>(pdu),
+                     static_cast<float*>(pdv), static_cast<hipStream_t>(stream));
+}
+
+int jcfd_li_interpolate(jcfd_li_plan* p, const jcfd_tensor* coefficients, const jcfd_tensor* x,
+                        int32_t interpolation, jcfd_tensor* out, void* stream) {
+  if (p == nullptr) return fail(JCFD_ERR_INVALID_ARGUMENT, "null plan");
+  if (interpolation < 0 || interpolation >= 8)
+    return fail(JCFD_ERR_INVALID_ARGUMENT, "interpolation index must be 0 .. 7");
+  const int n = p->n;
+  void *pc, *px, *po;
+  int rc;
+  if ((rc = check_batched(coefficients, "coefficients", kDLFloat, 32, p->batch * kLiCoefChannels, n, n,
+                          p->device, &pc)) ||
+      (rc = check_batched(x, "x", kDLFloat, 32, p->batch, n, n, p->device, &px)) ||
+      (rc = check_batched(out, "out", kDLFloat, 32, p->batch, n, n, p->device, &po)))
+    return rc;
+  if (po == px) return fail(JCFD_ERR_INVALID_ARGUMENT, "interpolate: out must not alias x");
+  return li_launch_interpolate(n, static_cast<const float*>(pc), static_cast<const float*>(px),
+                               16 * interpolation, static_cast<float*>(po), p->batch,
+                               static_cast<hipStream_t>(stream));
+}
+
+int jcfd_li_step(jcfd_li_plan* p, const jcfd_tensor* u, const jcfd_tensor* v, jcfd_tensor* u_out,
+                 jcfd_tensor* v_out, int64_t nsteps, void* stream) {
+  int rc = li_ready(p);
+  if (rc) return rc;
+  if (nsteps < 0) return fail(JCFD_ERR_INVALID_ARGUMENT, "nsteps must be >= 0");
+  const int n = p->n;
+  void *pu, *pv, *pu2, *pv2;
+  if ((rc = check_batched(u, "u", kDLFloat, 32, p->batch, n, n, p->device, &pu)) ||
+      (rc = check_batched(v, "v", kDLFloat, 32, p->batch, n, n, p->device, &pv)) ||
+      (rc = check_batched(u_out, "u_out", kDLFloat, 32, p->batch, n, n, p->device, &pu2)) ||
+      (rc = check_batched(v_out, "v_out", kDLFloat, 32, p->batch, n, n, p->device, &pv2)))
+    return rc;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const size_t bytes = sizeof(float) * p->batch * static_cast<size_t>(n) * n;
+  if (nsteps == 0) {
+    if (pu2 != pu && (rc = check_hip(hipMemcpyAsync(pu2, pu, bytes, hipMemcpyDeviceToDevice, s), "copy")))
+      return rc;
+    if (pv2 != pv && (rc = check_hip(hipMemcpyAsync(pv2, pv, bytes, hipMemcpyDeviceToDevice, s), "copy")))
+      return rc;
+    return JCFD_OK;
+  }
+  const float* cu = static_cast<const float*>(pu);
+  const float* cv = static_cast<const float*>(pv);
+  for (int64_t k = 0; k < nsteps; ++k) {
+    if ((rc = li_step_once(p, cu, cv, static_cast<float*>(pu2), static_cast<float*>(pv2), s))) return rc;
+    cu = static_cast<const float*>(pu2);
+    cv = static_cast<const float*>(pv2);
+  }
+  return JCFD_OK;
+}
+
+int jcfd_li_time_kernels(jcfd_li_plan* p, const jcfd_tensor* u, const jcfd_tensor* v, int32_t reps,
+                         double* ms_per_launch, void* stream) {
+  int rc = li_ready(p);
+  if (rc) return rc;
+  if (ms_per_launch == nullptr || reps <= 0) return fail(JCFD_ERR_INVALID_ARGUMENT, "bad arguments");
+  const int n = p->n;
+  void *pu, *pv;
+  if ((rc = check_batched(u, "u", kDLFloat, 32, p->batch, n, n, p->device, &pu)) ||
+      (rc = check_batched(v, "v", kDLFloat, 32, p->batch, n, n, p->device, &pv)))
+    return rc;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const float* cu = static_cast<const float*>(pu);
+  const float* cv = static_cast<const float*>(pv);
+  const size_t field = static_cast<size_t>(n) * n * p->batch;
+  hipEvent_t e0, e1;
+  if ((rc = check_hip(hipEventCreate(&e0), "event"))) return rc;
+  if ((rc = check_hip(hipEventCreate(&e1), "event"))) {
+    (void)hipEventDestroy(e0);
+    return rc;
+  }
+  // entries: each convolution, li_explicit, then the combination u + dt k
+  // with its projection (rows-forward, Poisson columns, rows-project)
+  rc = li_explicit(p, cu, cv, p->kbuf, p->kbuf + field, s);  // every buffer valid
+  FvmBuffers c = fvm_buffers(p->fvm);
+  c.u = cu;
+  c.v = cv;
+  c.u_out = p->fvm->us;  // rows-project reads and writes the same element per thread
+  c.v_out = p->fvm->vs;
+  c.count = p->batch;
+  c.rk = p->rk;
+  const int entries = p->n_layers + 2;
+  for (int e = 0; e < entries && rc == JCFD_OK; ++e) {
+    rc = check_hip(hipEventRecord(e0, s), "event record");
+    for (int r = 0; r < reps && rc == JCFD_OK; ++r) {
+      if (e < p->n_layers) {
+        const bool last = e + 1 == p->n_layers;
+        LiConvArgs a{};
+        a.n = n;
+        a.in = e == 0 ? cu : p->act[(e - 1) % 2];
+        a.in_v = e == 0 ? cv : nullptr;
+        a.w = p->w[e];
+        a.b = p->b[e];
+        a.out = last ? p->coef : p->act[e % 2];
+        rc = li_launch_conv(a, p->cin[e], p->cout[e], !last, p->batch, s);
+      } else if (e == p->n_layers) {
+        LiExplicitArgs a{};
+        a.coef = p->coef;
+        a.u = cu;
+        a.v = cv;
+        a.ku = p->kbuf;
+        a.kv = p->kbuf + field;
+        a.fx = p->fvm->params.has_forcing ? p->fvm->fx : nullptr;
+        std::memcpy(a.key, p->key, sizeof(a.key));
+        rc = li_launch_explicit(p->fvm->params, a, p->batch, s);
+      } else {
+        rc = fvm_launch_rk_project(p->fvm->params, c, s);
+      }
+    }
+    if (rc == JCFD_OK) rc = check_hip(hipEventRecord(e1, s), "event record");
+    if (rc == JCFD_OK) rc = check_hip(hipEventSynchronize(e1), "event sync");
+    float ms = 0.f;
+    if (rc == JCFD_OK) rc = check_hip(hipEventElapsedTime(&ms, e0, e1), "elapsed");
+    ms_per_launch[e] = static_cast<double>(ms) / reps;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return rc;
+}
+
+int jcfd_li_kernel_flops(const jcfd_li_plan* p, double* flops) {
+  if (p == nullptr || flops == nullptr) return fail(JCFD_ERR_INVALID_ARGUMENT, "null argument");
+  const double px = static_cast<double>(p->n) * p->n * p->batch;
+  for (int k = 0; k < p->n_layers; ++k) {
+    const int co = k + 1 == p->n_layers ? p->n_logits : p->cout[k];
+    flops[k] = 2.0 * 9.0 * p->cin[k] * co * px;  // the tower's own multiply-adds
+  }
+  return JCFD_OK;
 }
 
 }  // extern "C"

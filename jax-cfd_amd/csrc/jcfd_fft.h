@@ -284,6 +284,10 @@ template <int TOT, int CH, int NT, typename Load, typename Store>
 __device__ __forceinline__ void staged(int tid, Load&& load, Store&& store) {
   constexpr int PER = (TOT + NT - 1) / NT;
   constexpr int C = PER < CH ? PER : CH;
+  // every e of every chunk is < TOT only when the threads tile TOT exactly
+  // AND the chunks tile PER exactly; otherwise the last chunk's surplus
+  // lanes must be clamped (e.g. TOT = 51 * NT in chunks of 16)
+  constexpr bool kExact = TOT % NT == 0 && PER % C == 0;
   using V = decltype(load(0));
 #pragma unroll 1
   for (int k0 = 0; k0 < PER; k0 += C) {
@@ -291,7 +295,7 @@ __device__ __forceinline__ void staged(int tid, Load&& load, Store&& store) {
 #pragma unroll
     for (int k = 0; k < C; ++k) {
       const int e = tid + (k0 + k) * NT;
-      v[k] = load(TOT % NT == 0 || e < TOT ? e : TOT - 1);
+      v[k] = load(kExact || e < TOT ? e : TOT - 1);
     }
 #pragma unroll
     for (int k = 0; k < C; ++k) {

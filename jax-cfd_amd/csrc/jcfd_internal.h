@@ -206,6 +206,35 @@ int fvm_group_grid(int phase, int max_n, int* grid);
 // Launch one phase.
 int fvm_group_launch(int phase, int max_n, const FvmGroupQueue& q, int grid, hipStream_t s);
 
+// Learned interpolation (ml.hip).  The tower's output layer carries the
+// 8 interpolations' 16 stencil coefficients each (constraint folded in).
+constexpr int kLiCoefChannels = 128;
+struct LiConvArgs {
+  int n;
+  const float* in;    // (batch, cin, n, n); the first layer: u (batch, n, n)
+  const float* in_v;  // the first layer: v (batch, n, n); otherwise null
+  const float* w;     // [tap = 3 kh + kw][cin padded to 4 for the first layer][cout]
+  const float* b;     // [cout]
+  float* out;         // (batch, cout, n, n)
+};
+struct LiExplicitArgs {
+  const float* coef;  // (launch instances, 128, n, n)
+  const float* u;
+  const float* v;
+  float* ku;          // explicit terms out, (batch, n, n)
+  float* kv;
+  const float* fx;    // forcing column (null without forcing)
+  const int* slots;   // instance of launch row blockIdx.y, or null = blockIdx.y
+  // first coefficient channel of the interpolations each face flux uses:
+  // u: x face (advected, velocity), y face (advected, velocity); then v
+  int key[8];
+};
+bool li_conv_supported(int cin, int cout, bool relu);
+int li_launch_conv(const LiConvArgs& a, int cin, int cout, bool relu, int batch, hipStream_t s);
+int li_launch_explicit(const FvmParams& p, const LiExplicitArgs& a, int batch, hipStream_t s);
+int li_launch_interpolate(int n, const float* coef, const float* x, int key, float* out, int batch,
+                          hipStream_t s);
+
 // Device-side parameters of a pseudospectral stage (spectral.hip).  The
 // wavenumber tables and multipliers follow the reference's fp32 values;
 // beta, g = gamma dt and mu are the stage's low-storage CN-RK scalars.

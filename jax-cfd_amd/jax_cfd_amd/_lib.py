@@ -87,8 +87,36 @@ class Ns2dConfig(ctypes.Structure):
   ]
 
 
+LI_MAX_LAYERS = 16  # JCFD_LI_MAX_LAYERS
+LI_COEF_CHANNELS = 128
+
+
+class LiConfig(ctypes.Structure):
+  """jcfd_li_config."""
+  _fields_ = [
+      ("fvm", FvmConfig),
+      ("n_layers", ctypes.c_int32),
+      ("hidden_channels", ctypes.c_int32),
+      ("n_logits", ctypes.c_int32),
+      ("interp_key", ctypes.c_int32 * 8),
+  ]
+
+
 _P = ctypes.c_void_p
+_FP = ctypes.POINTER(ctypes.c_float)
 _SIGNATURES = {
+    "jcfd_li_plan_create": (ctypes.c_int, [ctypes.POINTER(LiConfig), ctypes.POINTER(_P)]),
+    "jcfd_li_plan_destroy": (ctypes.c_int, [_P]),
+    "jcfd_li_plan_device_bytes": (ctypes.c_int64, [_P]),
+    "jcfd_li_set_layer": (ctypes.c_int, [_P, ctypes.c_int32, _FP, _FP]),
+    "jcfd_li_set_logits_layer": (ctypes.c_int, [_P, _FP, _FP]),
+    "jcfd_li_step": (ctypes.c_int, [_P, _P, _P, _P, _P, ctypes.c_int64, _P]),
+    "jcfd_li_tower": (ctypes.c_int, [_P, _P, _P, _P, ctypes.c_int32, _P]),
+    "jcfd_li_explicit_terms": (ctypes.c_int, [_P, _P, _P, _P, _P, _P]),
+    "jcfd_li_interpolate": (ctypes.c_int, [_P, _P, _P, ctypes.c_int32, _P, _P]),
+    "jcfd_li_time_kernels": (ctypes.c_int, [_P, _P, _P, ctypes.c_int32,
+                                            ctypes.POINTER(ctypes.c_double), _P]),
+    "jcfd_li_kernel_flops": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_double)]),
     "jcfd_ns2d_plan_create": (ctypes.c_int, [ctypes.POINTER(Ns2dConfig), ctypes.POINTER(_P)]),
     "jcfd_ns2d_plan_destroy": (ctypes.c_int, [_P]),
     "jcfd_ns2d_plan_device_bytes": (ctypes.c_int64, [_P]),

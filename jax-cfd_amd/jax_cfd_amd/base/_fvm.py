@@ -54,6 +54,17 @@ class FvmPlan:
     self.batch = int(batch)
     self.device = device if device is not None else torch.device(
         'cuda', torch.cuda.current_device())
+    cfg, self._fx = FvmPlan.config(spec, self.batch)
+    handle = ctypes.c_void_p()
+    with torch.cuda.device(self.device):
+      _lib.check(lib.jcfd_fvm_plan_create(ctypes.byref(cfg), ctypes.byref(handle)),
+                 'jcfd_fvm_plan_create')
+    self._handle = handle
+    self._lib = lib
+
+  @staticmethod
+  def config(spec: FvmSpec, batch: int = 1):
+    """(jcfd_fvm_config of `spec`, the host forcing column it points at)."""
     cfg = _lib.FvmConfig()
     cfg.n = spec.n
     cfg.dx, cfg.dy = spec.dx, spec.dy
@@ -67,13 +78,13 @@ class FvmPlan:
     cfg.cg_maxiter = spec.cg_maxiter
     cfg.has_forcing = int(spec.linear_coefficient is not None)
     cfg.linear_coefficient = spec.linear_coefficient or 0.0
-    self._fx = None
+    fx = None
     if spec.forcing_x is not None:
-      self._fx = np.ascontiguousarray(np.asarray(spec.forcing_x, np.float32))
-      if self._fx.shape != (spec.n,):
+      fx = np.ascontiguousarray(np.asarray(spec.forcing_x, np.float32))
+      if fx.shape != (spec.n,):
         raise ValueError('forcing_x must hold n values')
-      cfg.forcing_x = self._fx.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
-    cfg.batch = self.batch
+      cfg.forcing_x = fx.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+    cfg.batch = int(batch)
     cfg.diffusion = spec.diffusion
     cfg.rk_stages = spec.rk_stages
     for k, x in enumerate(spec.rk_a):
@@ -82,12 +93,7 @@ class FvmPlan:
       cfg.rk_b[k] = x
     cfg.diff_cg_rtol, cfg.diff_cg_atol = spec.diff_cg_rtol, spec.diff_cg_atol
     cfg.diff_cg_maxiter = spec.diff_cg_maxiter
-    handle = ctypes.c_void_p()
-    with torch.cuda.device(self.device):
-      _lib.check(lib.jcfd_fvm_plan_create(ctypes.byref(cfg), ctypes.byref(handle)),
-                 'jcfd_fvm_plan_create')
-    self._handle = handle
-    self._lib = lib
+    return cfg, fx
 
   @property
   def device_bytes(self) -> int:
