@@ -53,6 +53,8 @@ def _args():
                        "0 reports the plain leg instead)")
   ap.add_argument("--group-steps", type=int, default=256,
                   help="steps of the mixed-size group line (0 skips it)")
+  ap.add_argument("--learned-interpolation", type=int, default=1,
+                  help="time the learned-interpolation step at 256^2 (0 skips)")
   ap.add_argument("--adjoint", type=int, default=1, help="time the adjoint gradient of a 256^2 64-step rollout (0 skips)")
   ap.add_argument("--cg-steps", type=int, default=100,
                   help="steps of the solve_cg variant of the 256^2 run (0 = skip)")
@@ -657,6 +659,63 @@ def _group_mixed(dev, mix, steps):
                      for k in tk}}
 
 
+def _learned_interpolation(dev, n=256, steps=20, reps=10):
+  """The learned-interpolation FVM step with the tower of the reference's
+  inference demo (7 periodic 3x3 convolutions: 2 -> 64, five 64 -> 64,
+  64 -> 120 logits for 8 interpolations of 4 x 4 stencils), seeded
+  haiku-default weights (output layer scaled by 0.1), forced 256^2 physics,
+  implicit diffusion (modular_navier_stokes_model's default solver).  The
+  tower runs on the matrix cores in fp32: its roofline is fp32 MFMA."""
+  import functools
+  import numpy as np
+  import torch
+  import jax_cfd_amd.base as cfd
+  from jax_cfd_amd import ml
+  grid = cfd.grids.Grid((n, n), domain=((0, 2 * np.pi), (0, 2 * np.pi)))
+  dt = cfd.equations.stable_time_step(VMAX, CFL, NU, grid, implicit_diffusion=True)
+  tower = functools.partial(ml.towers.forward_tower_factory, num_hidden_channels=64,
+                            num_hidden_layers=6)
+  params = tower(120, 2).init_params(0)
+  params[-1] = (params[-1][0] * 0.1, params[-1][1])
+  conv = functools.partial(
+      ml.advections.self_advection, advection_module=functools.partial(
+          ml.advections.modular_self_advection, interpolation_module=functools.partial(
+              ml.interpolations.FusedLearnedInterpolation, tags=("u", "c"),
+              tower_factory=tower, params=params)))
+  forcing = functools.partial(cfd.forcings.simple_turbulence_forcing, constant_magnitude=1.0,
+                              constant_wavenumber=4, linear_coefficient=-0.1,
+                              forcing_type="kolmogorov")
+  specs = ml.physics_specifications.NavierStokesPhysicsSpecs(1.0, NU, forcing)
+  step = ml.equations.modular_navier_stokes_model(grid, dt, specs, convection_module=conv)
+  v0 = cfd.initial_conditions.filtered_velocity_field(SEED, grid, VMAX, PEAK)
+  plan = step.plan(dev)
+  u, v = v0[0].data.clone(), v0[1].data.clone()
+  plan.step(u, v, 2, out=(u, v))
+  torch.cuda.synchronize()
+  t0 = time.perf_counter()
+  plan.step(u, v, steps, out=(u, v))
+  torch.cuda.synchronize()
+  wall = time.perf_counter() - t0
+  assert torch.isfinite(u).all() and torch.isfinite(v).all()
+  ms = plan.time_kernels(u, v, reps)
+  flops = plan.kernel_flops()
+  layers = [{"ms_per_launch": round(ms[k], 5), "flops": flops[k],
+             "TFLOPs": round(flops[k] / (ms[k] * 1e-3) / 1e12, 2)}
+            for k in range(len(flops))]
+  dom = max(range(len(flops)), key=lambda k: ms[k])
+  ach = flops[dom] / (ms[dom] * 1e-3) / 1e12
+  tower_ms = sum(ms[:len(flops)])
+  return {"workload": f"learned interpolation {n}^2, tower 2-64x6-120 (7 convolutions), "
+                      "forced physics, implicit diffusion",
+          "steps": steps, "ms_per_step": wall / steps * 1e3,
+          "value": n * n * steps / wall, "unit": "cell-steps/s",
+          "tower_TFLOPs": round(sum(flops) / (tower_ms * 1e-3) / 1e12, 2),
+          "layers": layers, "explicit_ms": round(ms[-2], 5), "projection_ms": round(ms[-1], 5),
+          "roofline": {"bound": "mfma", "kernel": f"li_conv3x3 layer {dom}",
+                       "achieved": round(ach, 2), "peak": 157.3, "unit": "TFLOP/s",
+                       "frac": round(ach / 157.3, 4)}}
+
+
 def _adjoint(dev, n=256, steps=64, every=8, reps=3):
   """Gradient of the vorticity MSE after 64 steps of the forced 256^2 run
   with respect to the initial velocity (hand-written adjoint, checkpoints
@@ -749,6 +808,9 @@ def main():
   if args.group_steps > 0:
     group = _group_mixed(dev, {64: 64, 128: 32, 256: 16, 512: 8}, args.group_steps)
     print(f"[bench] group {json.dumps(group)}", file=sys.stderr, flush=True)
+  li = _learned_interpolation(dev) if args.learned_interpolation else None
+  if li:
+    print(f"[bench] learned interpolation {json.dumps(li)}", file=sys.stderr, flush=True)
   adj = _adjoint(dev) if args.adjoint else None
   if adj:
     print(f"[bench] adjoint {json.dumps(adj)}", file=sys.stderr, flush=True)
@@ -779,7 +841,7 @@ def main():
       "roofline": roofline if roofline else plain["roofline"],
       "cpu_baseline": cpu,
       "served": served, "plain": plain, "batched": batched, "cg_variant": cg,
-      "group": group, "adjoint": adj,
+      "group": group, "adjoint": adj, "learned_interpolation": li,
   }
   print(json.dumps(out), flush=True)
 
