@@ -54,7 +54,9 @@ def _args():
   ap.add_argument("--group-steps", type=int, default=256,
                   help="steps of the mixed-size group line (0 skips it)")
   ap.add_argument("--learned-interpolation", type=int, default=1,
-                  help="time the learned-interpolation step at 256^2 (0 skips)")
+                  help="time the learned-interpolation step (0 skips)")
+  ap.add_argument("--li-sizes", type=int, nargs="+", default=[256],
+                  help="grid sizes of the learned-interpolation leg")
   ap.add_argument("--adjoint", type=int, default=1, help="time the adjoint gradient of a 256^2 64-step rollout (0 skips)")
   ap.add_argument("--cg-steps", type=int, default=100,
                   help="steps of the solve_cg variant of the 256^2 run (0 = skip)")
@@ -808,7 +810,10 @@ def main():
   if args.group_steps > 0:
     group = _group_mixed(dev, {64: 64, 128: 32, 256: 16, 512: 8}, args.group_steps)
     print(f"[bench] group {json.dumps(group)}", file=sys.stderr, flush=True)
-  li = _learned_interpolation(dev) if args.learned_interpolation else None
+  li = ([_learned_interpolation(dev, n) for n in args.li_sizes]
+        if args.learned_interpolation else None)
+  if li and len(li) == 1:
+    li = li[0]
   if li:
     print(f"[bench] learned interpolation {json.dumps(li)}", file=sys.stderr, flush=True)
   adj = _adjoint(dev) if args.adjoint else None
