@@ -427,6 +427,37 @@ int jcfd_ns2d_explicit_terms(jcfd_ns2d_plan* plan, const jcfd_tensor* w_hat, jcf
 int jcfd_ns2d_rfft2(jcfd_ns2d_plan* plan, const jcfd_tensor* x, jcfd_tensor* x_hat, void* stream);
 int jcfd_ns2d_irfft2(jcfd_ns2d_plan* plan, const jcfd_tensor* x_hat, jcfd_tensor* x, void* stream);
 
+/* Mixed-size spectral groups: one step of many NavierStokes2D instances of
+ * different grid sizes (and physics), each a slot of some plan, as ONE
+ * persistent launch per kernel class and stage, over a work list of
+ * (instance, tile) items of equal work.  No counterpart in the reference
+ * (one field per call, funcutils.repeated, funcutils.py:82-88); every
+ * instance gets exactly the arithmetic of jcfd_ns2d_step on it alone. */
+typedef struct jcfd_ns2d_group jcfd_ns2d_group;
+
+/* One plan's share: advance, in place, slots[0..n_slots) of the plan's
+ * (batch, n, n/2 + 1) complex64 state w_hat.  `slots` is HOST memory
+ * (int32, distinct, in [0, batch)).  Radix-8 plans (the default); every
+ * member's plan must step the same number of tableau stages. */
+typedef struct {
+  jcfd_ns2d_plan* plan;
+  jcfd_tensor* w_hat;
+  const int32_t* slots;
+  int32_t n_slots;
+} jcfd_ns2d_group_member;
+
+/* The group holds the device work list (re-uploaded only when the member
+ * set changes) and a captured graph of 32 steps (on by default). */
+int jcfd_ns2d_group_create(jcfd_ns2d_group** group);
+int jcfd_ns2d_group_destroy(jcfd_ns2d_group* group);
+int jcfd_ns2d_group_set_graphs(jcfd_ns2d_group* group, int32_t enabled);
+/* nsteps steps of every member instance.  No (plan, slot) and no
+ * (w_hat, slot) may appear twice.  Calls on one group must be ordered on one
+ * stream, and a plan must not be stepped on another stream meanwhile (the
+ * group steps its instances in the plan's internal buffers). */
+int jcfd_ns2d_group_step(jcfd_ns2d_group* group, const jcfd_ns2d_group_member* members,
+                         int32_t n_members, int64_t nsteps, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -67,6 +67,18 @@ int jcfd_fvm_group_time_kernels(jcfd_fvm_group* group, int32_t reps, double* ms_
 int jcfd_fvm_group_launch_shape(const jcfd_fvm_group* group, int32_t* grid, int32_t* items,
                                 int64_t* bytes_per_launch);
 
+/* Spectral groups: `reps` further steps of the member set of the last
+ * jcfd_ns2d_group_step on the plans' internal states (the members' own
+ * states are not touched), events around every launch; ms_per_launch[k] =
+ * mean duration of class k (the JCFD_NS2D_KERNEL_* classes). */
+int jcfd_ns2d_group_time_kernels(jcfd_ns2d_group* group, int32_t reps, double* ms_per_launch,
+                                 void* stream);
+/* Shape of the last spectral group step: workgroups per class launch, work
+ * items per launch, algorithmic bytes per launch (jcfd_ns2d_kernel_bytes per
+ * instance, summed over the member instances). */
+int jcfd_ns2d_group_launch_shape(const jcfd_ns2d_group* group, int32_t* grid, int32_t* items,
+                                 int64_t* bytes_per_launch);
+
 #ifdef __cplusplus
 }
 #endif

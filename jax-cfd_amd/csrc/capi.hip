@@ -2191,3 +2191,349 @@ int jcfd_ns2d_irfft2(jcfd_ns2d_plan* p, const jcfd_tensor* x_hat, jcfd_tensor* o
 }
 
 }  // extern "C"
+
+// ------------------------------------------------ mixed-size spectral groups
+struct jcfd_ns2d_group {
+  int device = 0;
+  NsGroupDesc* descs = nullptr;  // device [instance][JCFD_MAX_RK_STAGES]
+  NsGroupItem* items = nullptr;  // device work lists, list k at items + k * cap_items
+  int* d_slots = nullptr;        // device slot lists of the members (state conversion)
+  int* d_counts = nullptr;       // device int[2]: items per list
+  int cap_descs = 0, cap_items = 0, cap_slots = 0;
+  char* staging = nullptr;       // pinned upload buffer
+  size_t staging_bytes = 0;
+  hipEvent_t staged = nullptr;
+  bool staged_pending = false;
+  std::vector<char> last;
+  int max_n = 0;
+  int grid[kNsGroupClasses] = {0, 0, 0, 0};
+  int n_items[2] = {0, 0};
+  int n_stages = 0;
+  int64_t bytes[kNsGroupClasses] = {0, 0, 0, 0};
+  GraphCache graphs;
+};
+
+namespace {
+
+void ns_group_free_tables(jcfd_ns2d_group* g) {
+  for (void* ptr : {static_cast<void*>(g->descs), static_cast<void*>(g->items), static_cast<void*>(g->d_slots)})
+    if (ptr != nullptr) (void)hipFree(ptr);
+  g->descs = nullptr;
+  g->items = nullptr;
+  g->d_slots = nullptr;
+  g->cap_descs = g->cap_items = g->cap_slots = 0;
+}
+
+int ns_group_reserve(jcfd_ns2d_group* g, int n_desc, int n_item, int n_slot) {
+  if (n_desc <= g->cap_descs && n_item <= g->cap_items && n_slot <= g->cap_slots) return JCFD_OK;
+  int rc;
+  if (g->descs != nullptr && (rc = check_hip(hipDeviceSynchronize(), "group regrow sync"))) return rc;
+  const int cd = std::max({n_desc, 2 * g->cap_descs, 16});
+  const int ci = std::max({n_item, 2 * g->cap_items, 256});
+  const int cs = std::max({n_slot, 2 * g->cap_slots, 16});
+  ns_group_free_tables(g);
+  int64_t unused = 0;
+  if ((rc = device_alloc(&g->descs, static_cast<size_t>(cd) * JCFD_MAX_RK_STAGES, &unused)) ||
+      (rc = device_alloc(&g->items, static_cast<size_t>(2) * ci, &unused)) ||
+      (rc = device_alloc(&g->d_slots, static_cast<size_t>(cs), &unused))) {
+    ns_group_free_tables(g);
+    return rc;
+  }
+  g->cap_descs = cd;
+  g->cap_items = ci;
+  g->cap_slots = cs;
+  g->graphs.release();  // captured launches point at the old tables
+  return JCFD_OK;
+}
+
+NsGroupQueue ns_group_queue(const jcfd_ns2d_group* g) {
+  NsGroupQueue q{};
+  q.descs = g->descs;
+  q.items[0] = g->items;
+  q.items[1] = g->items + g->cap_items;
+  q.n_items = g->d_counts;
+  return q;
+}
+
+// Upload (counts[2] | descs | items 0 | items 1 | slots) through the pinned
+// staging buffer, stream-ordered on s.
+int ns_group_upload(jcfd_ns2d_group* g, const std::vector<char>& table, int n_desc, const int* counts,
+                    int n_slot, hipStream_t s) {
+  int rc;
+  if (g->staged_pending && (rc = check_hip(hipEventSynchronize(g->staged), "staging reuse"))) return rc;
+  g->staged_pending = false;
+  if (table.size() > g->staging_bytes) {
+    if (g->staging != nullptr) (void)hipHostFree(g->staging);
+    g->staging = nullptr;
+    const size_t bytes = std::max(table.size(), 2 * g->staging_bytes);
+    if ((rc = check_hip(hipHostMalloc(reinterpret_cast<void**>(&g->staging), bytes, hipHostMallocDefault),
+                        "pinned group staging")))
+      return rc;
+    g->staging_bytes = bytes;
+  }
+  std::memcpy(g->staging, table.data(), table.size());
+  size_t off = 2 * sizeof(int);
+  const size_t desc_bytes = static_cast<size_t>(n_desc) * JCFD_MAX_RK_STAGES * sizeof(NsGroupDesc);
+  if ((rc = check_hip(hipMemcpyAsync(g->d_counts, g->staging, 2 * sizeof(int), hipMemcpyHostToDevice, s),
+                      "group upload")) ||
+      (rc = check_hip(hipMemcpyAsync(g->descs, g->staging + off, desc_bytes, hipMemcpyHostToDevice, s),
+                      "group upload")))
+    return rc;
+  off += desc_bytes;
+  for (int k = 0; k < 2; ++k) {
+    const size_t bytes = counts[k] * sizeof(NsGroupItem);
+    if (bytes > 0 &&
+        (rc = check_hip(hipMemcpyAsync(g->items + static_cast<size_t>(k) * g->cap_items, g->staging + off, bytes,
+                                       hipMemcpyHostToDevice, s),
+                        "group upload")))
+      return rc;
+    off += bytes;
+  }
+  if (n_slot > 0 && (rc = check_hip(hipMemcpyAsync(g->d_slots, g->staging + off, n_slot * sizeof(int),
+                                                   hipMemcpyHostToDevice, s),
+                                    "group upload")))
+    return rc;
+  if ((rc = check_hip(hipEventRecord(g->staged, s), "staging event"))) return rc;
+  g->staged_pending = true;
+  return JCFD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int jcfd_ns2d_group_create(jcfd_ns2d_group** group) {
+  if (group == nullptr) return fail(JCFD_ERR_INVALID_ARGUMENT, "null argument");
+  *group = nullptr;
+  auto* g = new (std::nothrow) jcfd_ns2d_group();
+  if (g == nullptr) return fail(JCFD_ERR_OUT_OF_MEMORY, "host allocation failed");
+  int64_t unused = 0;
+  int rc;
+  if ((rc = check_hip(hipGetDevice(&g->device), "hipGetDevice")) ||
+      (rc = device_alloc(&g->d_counts, 2, &unused)) ||
+      (rc = check_hip(hipMemset(g->d_counts, 0, 2 * sizeof(int)), "zero counts")) ||
+      (rc = check_hip(hipEventCreateWithFlags(&g->staged, hipEventDisableTiming), "staging event"))) {
+    jcfd_ns2d_group_destroy(g);
+    return rc;
+  }
+  *group = g;
+  return JCFD_OK;
+}
+
+int jcfd_ns2d_group_destroy(jcfd_ns2d_group* g) {
+  if (g == nullptr) return JCFD_OK;
+  if (g->staged_pending) (void)hipEventSynchronize(g->staged);
+  g->graphs.release();
+  ns_group_free_tables(g);
+  if (g->d_counts != nullptr) (void)hipFree(g->d_counts);
+  if (g->staging != nullptr) (void)hipHostFree(g->staging);
+  if (g->staged != nullptr) (void)hipEventDestroy(g->staged);
+  delete g;
+  return JCFD_OK;
+}
+
+int jcfd_ns2d_group_set_graphs(jcfd_ns2d_group* g, int32_t enabled) {
+  if (g == nullptr) return fail(JCFD_ERR_INVALID_ARGUMENT, "null group");
+  g->graphs.enabled = enabled != 0;
+  return JCFD_OK;
+}
+
+int jcfd_ns2d_group_step(jcfd_ns2d_group* g, const jcfd_ns2d_group_member* members, int32_t n_members,
+                         int64_t nsteps, void* stream) {
+  if (g == nullptr) return fail(JCFD_ERR_INVALID_ARGUMENT, "null group");
+  if (n_members < 0 || (n_members > 0 && members == nullptr))
+    return fail(JCFD_ERR_INVALID_ARGUMENT, "members: null array or negative count");
+  if (nsteps < 0) return fail(JCFD_ERR_INVALID_ARGUMENT, "nsteps must be >= 0");
+  struct Conv {
+    int n, slot_off, count;
+    const jcfd_ns2d_plan* plan;
+    float2* state;
+  };
+  std::vector<Conv> convs;
+  std::vector<int> slots_all;
+  struct Inst {
+    int n;
+    NsGroupDesc stage[JCFD_MAX_RK_STAGES];
+  };
+  std::vector<Inst> insts;
+  std::vector<const void*> owners;
+  int n_stages = 0, rc;
+  for (int m = 0; m < n_members; ++m) {
+    const jcfd_ns2d_group_member& mb = members[m];
+    jcfd_ns2d_plan* p = mb.plan;
+    const std::string who = "member " + std::to_string(m);
+    if (p == nullptr) return fail(JCFD_ERR_INVALID_ARGUMENT, who + ": null plan");
+    if (p->device != g->device)
+      return fail(JCFD_ERR_INVALID_ARGUMENT, who + ": plan on another device than the group");
+    if (p->stages[0].radix != 8)
+      return fail(JCFD_ERR_UNSUPPORTED, who + ": groups run the radix-8 FFT schedule only");
+    if (n_stages != 0 && p->n_stages != n_stages)
+      return fail(JCFD_ERR_UNSUPPORTED, who + ": every member must step the same number of stages");
+    n_stages = p->n_stages;
+    const int n = p->n;
+    if (!ns_group_size_supported(n))
+      return fail(JCFD_ERR_UNSUPPORTED, who + ": grid size " + std::to_string(n) + " not built for groups");
+    void* pw;
+    if ((rc = check_spectral(mb.w_hat, "w_hat", p->batch, n, p->device, &pw)))
+      return fail(rc, who + ": " + jcfd_last_error());
+    if (mb.n_slots < 0 || mb.n_slots > p->batch || (mb.n_slots > 0 && mb.slots == nullptr))
+      return fail(JCFD_ERR_INVALID_ARGUMENT, who + ": slots must hold 0..batch host int32 values");
+    const size_t ext = static_cast<size_t>(n) * (n / 2 + 1), half = static_cast<size_t>(n) * (n / 2);
+    const int slot_off = static_cast<int>(slots_all.size());
+    for (int k = 0; k < mb.n_slots; ++k) {
+      const int s = mb.slots[k];
+      if (s < 0 || s >= p->batch)
+        return fail(JCFD_ERR_INVALID_ARGUMENT, who + ": slot " + std::to_string(s) + " out of range");
+      slots_all.push_back(s);
+      Inst in{};
+      in.n = n;
+      for (int st = 0; st < p->n_stages; ++st) {
+        NsGroupDesc& d = in.stage[st];
+        d.q = p->stages[st];
+        d.q.slots = nullptr;
+        d.n = n;
+        d.ws = p->ws + s * ext;
+        d.h = p->h + s * ext;
+        d.g = p->g + s * 4 * half;
+        d.pbuf = p->pbuf + s * half;
+      }
+      insts.push_back(in);
+      owners.push_back(p->ws + s * ext);
+      owners.push_back(static_cast<float2*>(pw) + s * ext);
+    }
+    if (mb.n_slots > 0) convs.push_back(Conv{n, slot_off, mb.n_slots, p, static_cast<float2*>(pw)});
+  }
+  std::vector<const void*> sorted_owners = owners;
+  std::sort(sorted_owners.begin(), sorted_owners.end());
+  if (std::adjacent_find(sorted_owners.begin(), sorted_owners.end()) != sorted_owners.end())
+    return fail(JCFD_ERR_INVALID_ARGUMENT,
+                "group members repeat an instance (same plan slot or same state slot twice)");
+  if (nsteps == 0 || insts.empty()) return JCFD_OK;
+  // largest grids first (items are of equal work; this keeps an instance's
+  // items contiguous in the order the fixed-stride walk deals them)
+  std::stable_sort(insts.begin(), insts.end(), [](const Inst& a, const Inst& b) { return a.n > b.n; });
+  const int n_desc = static_cast<int>(insts.size());
+  const int max_n = insts.front().n;
+  std::vector<NsGroupItem> items[2];
+  for (int l = 0; l < 2; ++l)
+    for (int d = 0; d < n_desc; ++d)
+      for (int t = 0, nt = ns_group_tiles(insts[d].n, l); t < nt; ++t) items[l].push_back({d, t});
+  const int counts[2] = {static_cast<int>(items[0].size()), static_cast<int>(items[1].size())};
+  const int n_slot = static_cast<int>(slots_all.size());
+  std::vector<char> table(2 * sizeof(int) + static_cast<size_t>(n_desc) * JCFD_MAX_RK_STAGES * sizeof(NsGroupDesc) +
+                          (counts[0] + counts[1]) * sizeof(NsGroupItem) + n_slot * sizeof(int));
+  {
+    char* w = table.data();
+    std::memcpy(w, counts, 2 * sizeof(int));
+    w += 2 * sizeof(int);
+    for (const Inst& in : insts) {
+      std::memcpy(w, in.stage, JCFD_MAX_RK_STAGES * sizeof(NsGroupDesc));
+      w += JCFD_MAX_RK_STAGES * sizeof(NsGroupDesc);
+    }
+    for (int l = 0; l < 2; ++l) {
+      std::memcpy(w, items[l].data(), counts[l] * sizeof(NsGroupItem));
+      w += counts[l] * sizeof(NsGroupItem);
+    }
+    std::memcpy(w, slots_all.data(), n_slot * sizeof(int));
+  }
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (n_desc > g->cap_descs || std::max(counts[0], counts[1]) > g->cap_items || n_slot > g->cap_slots) {
+    if ((rc = ns_group_reserve(g, n_desc, std::max(counts[0], counts[1]), n_slot))) return rc;
+    g->last.clear();
+  }
+  if (table != g->last) {
+    if ((rc = ns_group_upload(g, table, n_desc, counts, n_slot, s))) return rc;
+    g->last = table;
+  }
+  if (max_n != g->max_n) {
+    for (int k = 0; k < kNsGroupClasses; ++k)
+      if ((rc = ns_group_grid(k, max_n, &g->grid[k]))) return rc;
+    g->max_n = max_n;
+  }
+  g->n_items[0] = counts[0];
+  g->n_items[1] = counts[1];
+  g->n_stages = n_stages;
+  for (int k = 0; k < kNsGroupClasses; ++k) g->bytes[k] = 0;
+  for (const Inst& in : insts) {  // per instance, as jcfd_ns2d_kernel_bytes with batch 1
+    const int64_t n = in.n, ext = n * (n / 2 + 1) * 8, packed = n * (n / 2) * 8;
+    g->bytes[0] += ext + 4 * packed;
+    g->bytes[1] += 4 * packed + packed;
+    g->bytes[2] += packed + 2 * ext + 2 * ext + 4 * packed;
+    g->bytes[3] += packed + 2 * ext + 2 * ext;
+  }
+  // external states -> the plans' internal layout, the steps, and back
+  for (const Conv& c : convs)
+    if ((rc = ns_launch_state_convert(c.n, true, c.state, c.plan->ws, g->d_slots + c.slot_off, c.count, s)))
+      return rc;
+  const NsGroupQueue q = ns_group_queue(g);
+  const int S = n_stages;
+  auto one_step = [&](hipStream_t st) {
+    int r = ns_group_launch(0, max_n, q, 0, g->grid[0], st);
+    for (int k = 0; k < S && r == JCFD_OK; ++k) {
+      r = ns_group_launch(1, max_n, q, k, g->grid[1], st);
+      if (r == JCFD_OK) r = ns_group_launch(k + 1 < S ? 2 : 3, max_n, q, k, g->grid[k + 1 < S ? 2 : 3], st);
+    }
+    return r;
+  };
+  if ((rc = run_steps(g->graphs, g->descs, g->items, g->d_slots, max_n * 16 + S, nsteps, s, one_step)))
+    return rc;
+  for (const Conv& c : convs)
+    if ((rc = ns_launch_state_convert(c.n, false, c.plan->ws, c.state, g->d_slots + c.slot_off, c.count, s)))
+      return rc;
+  return JCFD_OK;
+}
+
+int jcfd_ns2d_group_time_kernels(jcfd_ns2d_group* g, int32_t reps, double* ms_per_launch, void* stream) {
+  if (g == nullptr || ms_per_launch == nullptr) return fail(JCFD_ERR_INVALID_ARGUMENT, "null argument");
+  if (reps <= 0) return fail(JCFD_ERR_INVALID_ARGUMENT, "reps must be positive");
+  if (g->max_n == 0 || g->n_items[0] == 0)
+    return fail(JCFD_ERR_INVALID_ARGUMENT, "no group step to time (call jcfd_ns2d_group_step first)");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const NsGroupQueue q = ns_group_queue(g);
+  const int S = g->n_stages;
+  // launch sequence of one step: (class, stage)
+  std::vector<std::pair<int, int>> seq{{0, 0}};
+  for (int k = 0; k < S; ++k) {
+    seq.push_back({1, k});
+    seq.push_back({k + 1 < S ? 2 : 3, k});
+  }
+  std::vector<hipEvent_t> ev(seq.size() + 1, nullptr);
+  int rc = JCFD_OK;
+  for (size_t k = 0; k < ev.size() && rc == JCFD_OK; ++k) rc = check_hip(hipEventCreate(&ev[k]), "event");
+  double total[kNsGroupClasses] = {0, 0, 0, 0};
+  int count[kNsGroupClasses] = {0, 0, 0, 0};
+  // whole steps on the group's internal states (the members' caller-side
+  // states are untouched: nothing converts back)
+  for (int r = 0; r < reps && rc == JCFD_OK; ++r) {
+    rc = check_hip(hipEventRecord(ev[0], s), "event record");
+    for (size_t k = 0; k < seq.size() && rc == JCFD_OK; ++k) {
+      const int c = seq[k].first;
+      rc = ns_group_launch(c, g->max_n, q, seq[k].second, g->grid[c], s);
+      if (rc == JCFD_OK) rc = check_hip(hipEventRecord(ev[k + 1], s), "event record");
+    }
+    if (rc == JCFD_OK) rc = check_hip(hipEventSynchronize(ev.back()), "event sync");
+    for (size_t k = 0; k < seq.size() && rc == JCFD_OK; ++k) {
+      float ms = 0.f;
+      rc = check_hip(hipEventElapsedTime(&ms, ev[k], ev[k + 1]), "elapsed");
+      total[seq[k].first] += ms;
+      ++count[seq[k].first];
+    }
+  }
+  for (int k = 0; k < kNsGroupClasses; ++k) ms_per_launch[k] = count[k] ? total[k] / count[k] : 0.0;
+  for (hipEvent_t e : ev)
+    if (e != nullptr) (void)hipEventDestroy(e);
+  return rc;
+}
+
+int jcfd_ns2d_group_launch_shape(const jcfd_ns2d_group* g, int32_t* grid, int32_t* items,
+                                 int64_t* bytes_per_launch) {
+  if (g == nullptr) return fail(JCFD_ERR_INVALID_ARGUMENT, "null group");
+  for (int k = 0; k < kNsGroupClasses; ++k) {
+    if (grid) grid[k] = g->grid[k];
+    if (items) items[k] = g->n_items[k == 1 ? 1 : 0];
+    if (bytes_per_launch) bytes_per_launch[k] = g->bytes[k];
+  }
+  return JCFD_OK;
+}
+
+}  // extern "C"

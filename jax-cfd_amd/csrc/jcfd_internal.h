@@ -267,6 +267,36 @@ struct NsBuffers {
   float2* pbuf;        // packed product spectrum, (batch, n x n/2), tiled
 };
 
+// Mixed-size spectral group (spectral.hip): one persistent launch per kernel
+// class of the step walks a work list of (instance, tile) over instances of
+// any built grid size; each tile runs the per-size kernels' tile body.
+// Descriptors are per (instance, stage): the member plan's stage parameters
+// and the instance's buffers (internal state layout).
+struct NsGroupDesc {
+  NsParams q;
+  int n;
+  float2* ws;    // state, internal layout
+  float2* h;     // low-storage RK register
+  float2* g;     // four packed spectra
+  float2* pbuf;  // packed product spectrum
+};
+struct NsGroupItem {
+  int desc;
+  int tile;  // column tile (list 0) or row block (list 1)
+};
+struct NsGroupQueue {
+  const NsGroupDesc* descs;     // [instance][stage], JCFD_MAX_RK_STAGES per instance
+  const NsGroupItem* items[2];  // 0: column tiles, 1: row blocks
+  const int* n_items;           // device int[2]
+};
+// Kernel classes of a group step (as the per-size step): 0 inverse columns,
+// 1 rows, 2 forward columns + update + next inverse, 3 forward columns +
+// final update.
+constexpr int kNsGroupClasses = 4;
+bool ns_group_size_supported(int n);
+int ns_group_tiles(int n, int list);
+int ns_group_grid(int cls, int max_n, int* grid);
+int ns_group_launch(int cls, int max_n, const NsGroupQueue& q, int stage, int grid, hipStream_t s);
 bool ns_size_supported(int n);
 // Default tile shapes and radix for size n into q.
 int ns_default_tiles(NsParams* q, int n);

@@ -29,6 +29,9 @@
 // DC and Nyquist columns after the axis-0 inverse transform.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <string>
+
 #include "jcfd_fft.h"
 #include "jcfd_internal.h"
 
@@ -263,16 +266,14 @@ struct NsColWaves {
   static constexpr int value = (N == 512 && W <= 2) ? 6 : 1;
 };
 
+// One column tile: the state's W columns from c0 -> the inverse transforms of
+// the four spectra (NSPEC = 4) or of the field (NSPEC = 1); w and g already
+// point at the instance.
 template <int N, int W, int NSPEC, int R = 8>
-__global__ __launch_bounds__(kThreads, (NsColWaves<N, W>::value)) void ns_cols_inverse(NsParams q, const float2* __restrict__ w,
-                                                            float2* __restrict__ g) {
-  __shared__ ColShared<N, W> sm;
+__device__ __forceinline__ void cols_inverse_tile(ColShared<N, W>& sm, const NsParams& q,
+                                                  const float2* __restrict__ w, float2* __restrict__ g,
+                                                  int c0, int tid) {
   WTile<N, W> wt;
-  const int tid = threadIdx.x;
-  const int c0 = xcd_tile(blockIdx.x, gridDim.x) * W;
-  const size_t inst = q.slots ? static_cast<size_t>(q.slots[blockIdx.y]) : blockIdx.y;
-  w += inst * static_cast<size_t>(N) * (N / 2 + 1);
-  g += inst * NSPEC * static_cast<size_t>(N) * (N / 2);
   TwiddleStage<N, 1, kThreads> tw;
   WavenumberStage<N> wn;
   tw.load(q.tw, tid);
@@ -284,6 +285,16 @@ __global__ __launch_bounds__(kThreads, (NsColWaves<N, W>::value)) void ns_cols_i
   wn.store(sm.kx, sm.ky, tid);
   __syncthreads();
   cols_inverse_body<N, W, NSPEC, R>(sm, wt, q, g, c0, tid);
+}
+
+template <int N, int W, int NSPEC, int R = 8>
+__global__ __launch_bounds__(kThreads, (NsColWaves<N, W>::value)) void ns_cols_inverse(NsParams q, const float2* __restrict__ w,
+                                                            float2* __restrict__ g) {
+  __shared__ ColShared<N, W> sm;
+  const int c0 = xcd_tile(blockIdx.x, gridDim.x) * W;
+  const size_t inst = q.slots ? static_cast<size_t>(q.slots[blockIdx.y]) : blockIdx.y;
+  cols_inverse_tile<N, W, NSPEC, R>(sm, q, w + inst * static_cast<size_t>(N) * (N / 2 + 1),
+                                    g + inst * NSPEC * static_cast<size_t>(N) * (N / 2), c0, threadIdx.x);
 }
 
 // Packed real row transforms of a batch of real (N, N) fields.
@@ -343,20 +354,26 @@ __global__ __launch_bounds__(kThreads) void rows_c2r(NsParams q, const float2* _
   for (int e = tid; e < RB * N; e += kThreads) x[static_cast<size_t>(i0) * N + e] = zf[e];
 }
 
+template <int N, int RB>
+struct RowsShared {
+  float2 rows[4 * RB * (N / 2)];  // transform t = m * RB + r
+  float2 prod[RB * (N / 2)];
+  float2 twn[N];
+  float2 twh[N / 2];
+};
+
+// One row block: rows i0 .. i0+RB-1 of the four spectra G -> the product
+// spectrum rows of P; g and pout already point at the instance.
 template <int N, int RB, int R = 8>
-__global__ __launch_bounds__(kThreads) void ns_rows(NsParams q, const float2* __restrict__ g,
-                                                    float2* __restrict__ pout) {
+__device__ __forceinline__ void rows_tile(RowsShared<N, RB>& sm, const NsParams& q,
+                                          const float2* __restrict__ g, float2* __restrict__ pout,
+                                          int i0, int tid) {
   constexpr int H = N / 2;
-  __shared__ float2 rows[4 * RB * H];  // transform t = m * RB + r
-  __shared__ float2 prod[RB * H];
-  __shared__ float2 twn[N];
-  __shared__ float2 twh[H];
-  const int tid = threadIdx.x;
-  const int i0 = xcd_tile(blockIdx.x, gridDim.x) * RB;
-  const size_t inst = q.slots ? static_cast<size_t>(q.slots[blockIdx.y]) : blockIdx.y;
   const size_t gstride = static_cast<size_t>(N) * H;
-  g += inst * 4 * gstride;
-  pout += inst * gstride;
+  float2* rows = sm.rows;
+  float2* prod = sm.prod;
+  float2* twn = sm.twn;
+  float2* twh = sm.twh;
   // Load the four spectra's rows in chunks (all loads of a chunk issued
   // together); the twiddle tables go to LDS while the first chunk is in
   // flight.  Element e walks the tiled layout in memory order: word w of
@@ -417,6 +434,17 @@ __global__ __launch_bounds__(kThreads) void ns_rows(NsParams q, const float2* __
       [&](int e, float2 v) { pout[tix<N>(i0 + row_of(e), word_of(e))] = v; });
 }
 
+
+template <int N, int RB, int R = 8>
+__global__ __launch_bounds__(kThreads) void ns_rows(NsParams q, const float2* __restrict__ g,
+                                                    float2* __restrict__ pout) {
+  __shared__ RowsShared<N, RB> sm;
+  const size_t inst = q.slots ? static_cast<size_t>(q.slots[blockIdx.y]) : blockIdx.y;
+  const size_t gstride = static_cast<size_t>(N) * (N / 2);
+  rows_tile<N, RB, R>(sm, q, g + inst * 4 * gstride, pout + inst * gstride,
+                      xcd_tile(blockIdx.x, gridDim.x) * RB, threadIdx.x);
+}
+
 // External rfft2 layout <-> internal state layout of one instance per
 // blockIdx.y (slots[blockIdx.y] when given); one word per thread, walked in
 // the internal order (contiguous internal side).
@@ -456,23 +484,18 @@ __device__ __forceinline__ float ns_filter(const NsParams& q, int p, int c) {
 // MODE 0: write F (explicit terms) to u_out in the external layout.
 // MODE 1: RK/CN update of (u, h) (internal state layout).
 // MODE 2: update, then the next stage's inverse column work on the new tile.
+// One column tile of the forward column pass: P's W columns from c0 ->
+// explicit terms (MODE 0), the RK/CN update (MODE 1), or the update followed
+// by the next stage's inverse work (MODE 2); all pointers already point at
+// the instance.
 template <int N, int W, int MODE, int R = 8>
-__global__ __launch_bounds__(kThreads, (NsColWaves<N, W>::value)) void ns_cols_forward(NsParams q, const float2* __restrict__ pin,
-                                                            const float2* u_in, float2* u_out,
-                                                            float2* h, float2* __restrict__ g) {
+__device__ __forceinline__ void cols_forward_tile(ColShared<N, W>& sm, const NsParams& q,
+                                                  const float2* __restrict__ pin, const float2* u_in,
+                                                  float2* u_out, float2* h, float2* __restrict__ g,
+                                                  int c0, int tid) {
   constexpr int H = N / 2;
   constexpr int H1 = H + 1;
   constexpr int PER = (N * W + kThreads - 1) / kThreads;
-  __shared__ ColShared<N, W> sm;
-  const int tid = threadIdx.x;
-  const int c0 = xcd_tile(blockIdx.x, gridDim.x) * W;
-  const size_t inst = q.slots ? static_cast<size_t>(q.slots[blockIdx.y]) : blockIdx.y;
-  const size_t ext = static_cast<size_t>(N) * H1;
-  pin += inst * static_cast<size_t>(N) * H;
-  if (u_in) u_in += inst * ext;
-  u_out += inst * ext;
-  if (h) h += inst * ext;
-  if (g) g += inst * 4 * static_cast<size_t>(N) * H;
 
   // Every global load of the kernel's first half is issued up front: the
   // tile of P, the state (u, h) at this thread's elements for the update
@@ -606,6 +629,110 @@ __global__ __launch_bounds__(kThreads, (NsColWaves<N, W>::value)) void ns_cols_f
   if constexpr (MODE == 2) {
     __syncthreads();  // sm.buf read by all; sm.w0 / sm.wn written
     cols_inverse_body<N, W, 4, R>(sm, wt, q, g, c0, tid);
+  }
+}
+
+
+template <int N, int W, int MODE, int R = 8>
+__global__ __launch_bounds__(kThreads, (NsColWaves<N, W>::value)) void ns_cols_forward(NsParams q, const float2* __restrict__ pin,
+                                                            const float2* u_in, float2* u_out,
+                                                            float2* h, float2* __restrict__ g) {
+  __shared__ ColShared<N, W> sm;
+  const size_t inst = q.slots ? static_cast<size_t>(q.slots[blockIdx.y]) : blockIdx.y;
+  const size_t ext = static_cast<size_t>(N) * (N / 2 + 1);
+  cols_forward_tile<N, W, MODE, R>(sm, q, pin + inst * static_cast<size_t>(N) * (N / 2),
+                                   u_in ? u_in + inst * ext : nullptr, u_out + inst * ext,
+                                   h ? h + inst * ext : nullptr,
+                                   g ? g + inst * 4 * static_cast<size_t>(N) * (N / 2) : nullptr,
+                                   xcd_tile(blockIdx.x, gridDim.x) * W, threadIdx.x);
+}
+
+// ---- mixed-size group step ---------------------------------------------------
+// Every item is 1024 complex words of work whatever its grid size (column
+// tiles of W = 1024 / n columns, row blocks of 1024 / n rows; capped by the
+// sizes' limits), so the fixed-stride walk of a persistent grid stays
+// balanced across sizes.  Tile shapes regroup the same per-column and
+// per-row arithmetic: results are bit-identical to the per-size kernels'.
+template <int N>
+struct NsGroupShape {
+  static constexpr int W = (1024 / N < N / 2) ? 1024 / N : N / 2;
+  static constexpr int RB = (1024 / N < N / 4) ? 1024 / N : N / 4;
+};
+
+template <int N, int CLS>
+__device__ __forceinline__ void ns_group_tile(const NsGroupDesc& d, int tile, int tid, char* smem) {
+  using S = NsGroupShape<N>;
+  if constexpr (CLS == 1) {
+    rows_tile<N, S::RB, 8>(*reinterpret_cast<RowsShared<N, S::RB>*>(smem), d.q, d.g, d.pbuf,
+                           tile * S::RB, tid);
+  } else {
+    auto& sm = *reinterpret_cast<ColShared<N, S::W>*>(smem);
+    if constexpr (CLS == 0)
+      cols_inverse_tile<N, S::W, 4, 8>(sm, d.q, d.ws, d.g, tile * S::W, tid);
+    else
+      cols_forward_tile<N, S::W, CLS == 2 ? 2 : 1, 8>(sm, d.q, d.pbuf, d.ws, d.ws, d.h, d.g,
+                                                       tile * S::W, tid);
+  }
+}
+
+template <int CLS>
+__global__ __launch_bounds__(kThreads) void ns_group_phase(NsGroupQueue q, int stage) {
+  extern __shared__ float4 ns_group_smem[];
+  char* smem = reinterpret_cast<char*>(ns_group_smem);
+  constexpr int kList = CLS == 1 ? 1 : 0;
+  const NsGroupItem* items = q.items[kList];
+  const int total = q.n_items[kList];
+  for (int it = blockIdx.x; it < total; it += gridDim.x) {
+    const NsGroupItem w = items[it];
+    const NsGroupDesc d = q.descs[static_cast<size_t>(w.desc) * JCFD_MAX_RK_STAGES + stage];
+    // a fresh copy of the thread index per item keeps per-thread addresses
+    // of every size's tile body from being hoisted out of the loop
+    int t = threadIdx.x;
+    asm volatile("" : "+v"(t));
+    switch (d.n) {
+      case 32: ns_group_tile<32, CLS>(d, w.tile, t, smem); break;
+      case 64: ns_group_tile<64, CLS>(d, w.tile, t, smem); break;
+      case 128: ns_group_tile<128, CLS>(d, w.tile, t, smem); break;
+      case 256: ns_group_tile<256, CLS>(d, w.tile, t, smem); break;
+      case 512: ns_group_tile<512, CLS>(d, w.tile, t, smem); break;
+      case 1024: ns_group_tile<1024, CLS>(d, w.tile, t, smem); break;
+      default: break;  // rejected on the host
+    }
+    __syncthreads();  // LDS is reused by the next item
+  }
+}
+
+template <int N>
+constexpr int ns_group_lds_n(int cls) {
+  using S = NsGroupShape<N>;
+  return cls == 1 ? static_cast<int>(sizeof(RowsShared<N, S::RB>))
+                  : static_cast<int>(sizeof(ColShared<N, S::W>));
+}
+
+// dynamic LDS stays within the 64 KiB a launch gets without raising the
+// kernel's limit
+static_assert(ns_group_lds_n<1024>(0) <= 65536 && ns_group_lds_n<1024>(1) <= 65536 &&
+                  ns_group_lds_n<512>(0) <= 65536 && ns_group_lds_n<512>(1) <= 65536 &&
+                  ns_group_lds_n<64>(0) <= 65536 && ns_group_lds_n<64>(1) <= 65536,
+              "group LDS");
+
+int ns_group_lds(int cls, int max_n) {
+  int bytes = 0;
+  if (max_n >= 32) bytes = std::max(bytes, ns_group_lds_n<32>(cls));
+  if (max_n >= 64) bytes = std::max(bytes, ns_group_lds_n<64>(cls));
+  if (max_n >= 128) bytes = std::max(bytes, ns_group_lds_n<128>(cls));
+  if (max_n >= 256) bytes = std::max(bytes, ns_group_lds_n<256>(cls));
+  if (max_n >= 512) bytes = std::max(bytes, ns_group_lds_n<512>(cls));
+  if (max_n >= 1024) bytes = std::max(bytes, ns_group_lds_n<1024>(cls));
+  return bytes;
+}
+
+const void* ns_group_kernel(int cls) {
+  switch (cls) {
+    case 0: return reinterpret_cast<const void*>(&ns_group_phase<0>);
+    case 1: return reinterpret_cast<const void*>(&ns_group_phase<1>);
+    case 2: return reinterpret_cast<const void*>(&ns_group_phase<2>);
+    default: return reinterpret_cast<const void*>(&ns_group_phase<3>);
   }
 }
 
@@ -752,6 +879,47 @@ int ns_explicit_impl(const NsParams& q, const float2* w, float2* out, float2* g,
 }
 
 }  // namespace
+
+bool ns_group_size_supported(int n) { return ns_size_supported(n); }
+
+int ns_group_tiles(int n, int list) {
+  switch (n) {
+    case 32: return list == 0 ? 16 / NsGroupShape<32>::W : 32 / NsGroupShape<32>::RB;
+    case 64: return list == 0 ? 32 / NsGroupShape<64>::W : 64 / NsGroupShape<64>::RB;
+    case 128: return list == 0 ? 64 / NsGroupShape<128>::W : 128 / NsGroupShape<128>::RB;
+    case 256: return list == 0 ? 128 / NsGroupShape<256>::W : 256 / NsGroupShape<256>::RB;
+    case 512: return list == 0 ? 256 / NsGroupShape<512>::W : 512 / NsGroupShape<512>::RB;
+    case 1024: return list == 0 ? 512 / NsGroupShape<1024>::W : 1024 / NsGroupShape<1024>::RB;
+    default: return 0;
+  }
+}
+
+int ns_group_grid(int cls, int max_n, int* grid) {
+  int dev = 0, cus = 0, per_cu = 0;
+  int rc;
+  if ((rc = check_hip(hipGetDevice(&dev), "hipGetDevice")) ||
+      (rc = check_hip(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev),
+                      "CU count")) ||
+      (rc = check_hip(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, ns_group_kernel(cls), kThreads,
+                                                                   ns_group_lds(cls, max_n)),
+                      "spectral group occupancy")))
+    return rc;
+  *grid = std::max(1, per_cu) * std::max(1, cus);
+  return JCFD_OK;
+}
+
+int ns_group_launch(int cls, int max_n, const NsGroupQueue& q, int stage, int grid, hipStream_t s) {
+  if (!ns_group_size_supported(max_n))
+    return fail(JCFD_ERR_UNSUPPORTED, "spectral group grid size not supported: " + std::to_string(max_n));
+  const size_t lds = static_cast<size_t>(ns_group_lds(cls, max_n));
+  switch (cls) {
+    case 0: hipLaunchKernelGGL(ns_group_phase<0>, dim3(grid), dim3(kThreads), lds, s, q, stage); break;
+    case 1: hipLaunchKernelGGL(ns_group_phase<1>, dim3(grid), dim3(kThreads), lds, s, q, stage); break;
+    case 2: hipLaunchKernelGGL(ns_group_phase<2>, dim3(grid), dim3(kThreads), lds, s, q, stage); break;
+    default: hipLaunchKernelGGL(ns_group_phase<3>, dim3(grid), dim3(kThreads), lds, s, q, stage);
+  }
+  return check_hip(hipGetLastError(), "spectral group launch");
+}
 
 bool ns_size_supported(int n) {
   return n == 32 || n == 64 || n == 128 || n == 256 || n == 512 || n == 1024;

@@ -166,6 +166,15 @@ _SIGNATURES = {
     "jcfd_fvm_group_launch_shape": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int32),
                                                    ctypes.POINTER(ctypes.c_int32),
                                                    ctypes.POINTER(ctypes.c_int64)]),
+    "jcfd_ns2d_group_create": (ctypes.c_int, [ctypes.POINTER(_P)]),
+    "jcfd_ns2d_group_destroy": (ctypes.c_int, [_P]),
+    "jcfd_ns2d_group_set_graphs": (ctypes.c_int, [_P, ctypes.c_int32]),
+    "jcfd_ns2d_group_step": (ctypes.c_int, [_P, _P, ctypes.c_int32, ctypes.c_int64, _P]),
+    "jcfd_ns2d_group_time_kernels": (ctypes.c_int, [_P, ctypes.c_int32,
+                                                    ctypes.POINTER(ctypes.c_double), _P]),
+    "jcfd_ns2d_group_launch_shape": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int32),
+                                                    ctypes.POINTER(ctypes.c_int32),
+                                                    ctypes.POINTER(ctypes.c_int64)]),
 }
 
 
@@ -183,6 +192,12 @@ class Ns2dTiles(ctypes.Structure):
 class FvmGroupMember(ctypes.Structure):
   """jcfd_fvm_group_member."""
   _fields_ = [("plan", _P), ("u", _P), ("v", _P),
+              ("slots", ctypes.POINTER(ctypes.c_int32)), ("n_slots", ctypes.c_int32)]
+
+
+class Ns2dGroupMember(ctypes.Structure):
+  """jcfd_ns2d_group_member."""
+  _fields_ = [("plan", _P), ("w_hat", _P),
               ("slots", ctypes.POINTER(ctypes.c_int32)), ("n_slots", ctypes.c_int32)]
 
 _lib = None

@@ -185,3 +185,73 @@ class Ns2dPlan:
                                               _lib.stream_handle(self.device))
     _lib.check(rc, 'jcfd_ns2d_explicit_terms')
     return o
+
+
+class Ns2dGroup:
+  """Mixed-size group stepping (jcfd_ns2d_group_*): one persistent launch per
+  kernel class and tableau stage advances every listed slot of every member
+  plan, whatever its grid size, each with exactly the arithmetic of a lone
+  step."""
+
+  def __init__(self, device: Optional[torch.device] = None):
+    lib = _lib.load()
+    self.device = device if device is not None else torch.device(
+        'cuda', torch.cuda.current_device())
+    handle = ctypes.c_void_p()
+    with torch.cuda.device(self.device):
+      _lib.check(lib.jcfd_ns2d_group_create(ctypes.byref(handle)), 'jcfd_ns2d_group_create')
+    self._handle = handle
+    self._lib = lib
+
+  def set_graphs(self, enabled: bool):
+    _lib.check(self._lib.jcfd_ns2d_group_set_graphs(self._handle, int(bool(enabled))),
+               'jcfd_ns2d_group_set_graphs')
+
+  def close(self):
+    if getattr(self, '_handle', None) is not None and self._handle.value:
+      self._lib.jcfd_ns2d_group_destroy(self._handle)
+      self._handle = None
+
+  def __del__(self):
+    try:
+      self.close()
+    except Exception:  # interpreter shutdown
+      pass
+
+  def step(self, members, nsteps: int):
+    """members: iterable of (plan, w_hat, slots) with w_hat the plan's
+    complex64 device state of shape plan.shape and slots a sequence of ints;
+    every listed slot advances nsteps steps in place."""
+    members = list(members)
+    arr = (_lib.Ns2dGroupMember * max(1, len(members)))()
+    keep = []
+    with _lib.dlpack_args() as dl:
+      for k, (plan, w, slots) in enumerate(members):
+        idx = np.ascontiguousarray(np.asarray(slots, dtype=np.int32).reshape(-1))
+        keep.append(idx)
+        arr[k].plan = plan._handle
+        arr[k].w_hat = dl(w)
+        arr[k].slots = idx.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+        arr[k].n_slots = int(idx.size)
+      rc = self._lib.jcfd_ns2d_group_step(self._handle, ctypes.cast(arr, ctypes.c_void_p),
+                                          len(members), int(nsteps),
+                                          _lib.stream_handle(self.device))
+    _lib.check(rc, 'jcfd_ns2d_group_step')
+
+  def time_kernels(self, reps: int = 20):
+    """Mean ms per launch of each kernel class over `reps` further steps of
+    the last member set (on the plans' internal states)."""
+    ms = (ctypes.c_double * 4)()
+    rc = self._lib.jcfd_ns2d_group_time_kernels(self._handle, int(reps), ms,
+                                                _lib.stream_handle(self.device))
+    _lib.check(rc, 'jcfd_ns2d_group_time_kernels')
+    return dict(zip(Ns2dPlan.KERNELS, list(ms)))
+
+  def launch_shape(self):
+    """(grid, items, algorithmic bytes) per kernel class of the last step."""
+    grid, items = (ctypes.c_int32 * 4)(), (ctypes.c_int32 * 4)()
+    nbytes = (ctypes.c_int64 * 4)()
+    _lib.check(self._lib.jcfd_ns2d_group_launch_shape(self._handle, grid, items, nbytes),
+               'jcfd_ns2d_group_launch_shape')
+    return {k: {'grid': int(g), 'items': int(i), 'bytes': int(b)}
+            for k, g, i, b in zip(Ns2dPlan.KERNELS, grid, items, nbytes)}
