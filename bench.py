@@ -51,6 +51,11 @@ def _args():
   ap.add_argument("--served-requests", type=int, default=400,
                   help="requests of the served Poisson stream (the headline value; "
                        "0 reports the plain leg instead)")
+  ap.add_argument("--served-group", nargs="*", default=["fvm"], choices=["fvm", "spectral"],
+                  help="solvers whose pools the headline server groups into one "
+                       "mixed-size group step per chunk")
+  ap.add_argument("--served-variants", type=int, default=1,
+                  help="also time the stream under the other groupings (0 skips)")
   ap.add_argument("--group-steps", type=int, default=256,
                   help="steps of the mixed-size group line (0 skips it)")
   ap.add_argument("--learned-interpolation", type=int, default=1,
@@ -425,7 +430,38 @@ def _plain_step_seconds(dev, solver, n, steps=40):
   return (time.perf_counter() - t0) / steps
 
 
-def _served(dev, requests, kernel_reps):
+def _per_solver(res):
+  """Cell-steps per second of each solver's requests over their own span
+  (first arrival to last result)."""
+  out = {}
+  for s in ("fvm", "spectral"):
+    rs = [r for r in res if r.solver == s and r.status == "done"]
+    if rs:
+      span = max(r.finished for r in rs) - min(r.arrival for r in rs)
+      out[s] = sum(r.n * r.n * r.steps for r in rs) / span
+  return out
+
+
+def _served_run(dev, requests, group=()):
+  """One timed pass of the served stream on a warmed server."""
+  import torch
+  from jax_cfd_amd import serving
+  srv = serving.RolloutServer(hbm_budget_bytes=200e9, tune=True, group=group)
+  warm = [serving.RolloutRequest(10 ** 6 + i, s, n, 40, i)
+          for i, (s, n) in enumerate((s, n) for s in ("fvm", "spectral")
+                                     for n in (64, 128, 256, 512))]
+  srv.run(warm, realtime=False)  # creates the pools (plans, tuned tiles, states)
+  torch.cuda.synchronize()
+  t0 = time.perf_counter()
+  res = srv.run(requests, realtime=True)
+  wall = time.perf_counter() - t0
+  summ = serving.summarize(res, wall)
+  summ["per_solver_cell_steps_per_s"] = _per_solver(res)
+  summ["grouped"] = list(group)
+  return srv, res, wall, summ
+
+
+def _served(dev, requests, kernel_reps, group=()):
   """The served Poisson stream: continuous batching over one pool per
   (solver, n) (tuned tiles, radix-8 schedule).  The plain figure replays the
   same mix one instance at a time on the plain path (tuned single-instance
@@ -436,16 +472,7 @@ def _served(dev, requests, kernel_reps):
   import numpy as np
   import torch
   from jax_cfd_amd import serving
-  srv = serving.RolloutServer(hbm_budget_bytes=200e9, tune=True)
-  warm = [serving.RolloutRequest(10 ** 6 + i, s, n, 40, i)
-          for i, (s, n) in enumerate((s, n) for s in ("fvm", "spectral")
-                                     for n in (64, 128, 256, 512))]
-  srv.run(warm, realtime=False)  # creates the pools (plans, tuned tiles, states)
-  torch.cuda.synchronize()
-  t0 = time.perf_counter()
-  res = srv.run(requests, realtime=True)
-  wall = time.perf_counter() - t0
-  summ = serving.summarize(res, wall)
+  srv, res, wall, summ = _served_run(dev, requests, group)
   print(f"[bench] served {summ['done']} of {len(requests)} requests in {wall:.3f} s",
         file=sys.stderr, flush=True)
   done = [r for r in res if r.status == "done"]
@@ -606,27 +633,29 @@ def _plain_fvm(dev, steps, warmup, kernel_reps, min_seconds=0.2):
   }
 
 
-def _group_mixed(dev, mix, steps):
-  """Uneven sizes in one persistent launch per kernel class (jcfd_fvm_group)
-  against the same instances as one batched launch sequence per size on
-  concurrent streams (tuned tiles).  FVM physics of the 256^2 run at every
-  size, seeds 0.. per size."""
+def _group_mixed(dev, mix, steps, solver="fvm"):
+  """Uneven sizes in one persistent launch per kernel class (jcfd_fvm_group /
+  jcfd_ns2d_group) against the same instances as one batched launch
+  sequence per size on concurrent streams (tuned tiles).  Physics of the
+  served families (FVM: the forced 256^2 run; spectral: the forced 512^2
+  run) at every size, seeds 0.. per size."""
   import torch
   from jax_cfd_amd import autotune, serving
   from jax_cfd_amd.base import _fvm
+  from jax_cfd_amd.spectral import _ns2d
+  fvm = solver == "fvm"
   members, pools = [], []
   for n, b in mix.items():
-    fam = serving.FvmFamily(n, dev)
-    u = torch.stack([fam.initial_state(s)[0] for s in range(b)])
-    v = torch.stack([fam.initial_state(s)[1] for s in range(b)])
-    gplan = fam.make_plan(b)
-    members.append((gplan, u, v, list(range(b))))
+    fam = serving.FAMILIES[solver](n, dev)
+    fields = [torch.stack([fam.initial_state(s)[k] for s in range(b)])
+              for k in range(2 if fvm else 1)]
+    members.append((fam.make_plan(b), *fields, list(range(b))))
     pplan = fam.make_plan(b)
-    autotune.tune_fvm(pplan, radices=(8,))
-    pools.append((pplan, u.clone(), v.clone(),
+    (autotune.tune_fvm if fvm else autotune.tune_ns2d)(pplan, radices=(8,))
+    pools.append((pplan, [f.clone() for f in fields],
                   torch.arange(b, dtype=torch.int32, device=dev), torch.cuda.Stream(device=dev)))
   cells = sum(n * n * b for n, b in mix.items())
-  group = _fvm.FvmGroup(dev)
+  group = _fvm.FvmGroup(dev) if fvm else _ns2d.Ns2dGroup(dev)
   group.step(members, 32)
   torch.cuda.synchronize()
   t0 = time.perf_counter()
@@ -636,10 +665,10 @@ def _group_mixed(dev, mix, steps):
   shape = group.launch_shape()
 
   def run_pools(k):
-    for plan, u, v, idx, st in pools:
+    for plan, fields, idx, st in pools:
       st.wait_stream(torch.cuda.current_stream())
       with torch.cuda.stream(st):
-        plan.step_slots(u, v, idx, k)
+        plan.step_slots(*fields, idx, k)
     for *_, st in pools:
       torch.cuda.current_stream().wait_stream(st)
   run_pools(32)
@@ -648,9 +677,10 @@ def _group_mixed(dev, mix, steps):
   run_pools(steps)
   torch.cuda.synchronize()
   p_wall = time.perf_counter() - t0
-  same = all(torch.equal(m[1], p[1]) and torch.equal(m[2], p[2]) for m, p in zip(members, pools))
-  tk = group.time_kernels(10)  # advances the group's members further
-  return {"workload": "fvm mixed sizes " + ", ".join(f"{b}x{n}^2" for n, b in mix.items()),
+  same = all(all(torch.equal(a, b) for a, b in zip(m[1:-1], p[1]))
+             for m, p in zip(members, pools))
+  tk = group.time_kernels(10)  # advances the group's instances further
+  return {"workload": f"{solver} mixed sizes " + ", ".join(f"{b}x{n}^2" for n, b in mix.items()),
           "steps": steps, "group_value": cells * steps / g_wall,
           "pools_value": cells * steps / p_wall, "unit": "cell-steps/s",
           "group_ms_per_step": g_wall / steps * 1e3,
@@ -796,9 +826,19 @@ def main():
 
   plain = _plain_fvm(dev, args.steps, args.warmup, args.kernel_reps)
   print(f"[bench] plain {json.dumps(plain)}", file=sys.stderr, flush=True)
-  served, roofline = _served(dev, requests, 20) if requests else (None, None)
+  headline_group = tuple(sorted(set(args.served_group)))
+  served, roofline = (_served(dev, requests, 20, headline_group) if requests
+                      else (None, None))
+  variants = None
   if served:
     print(f"[bench] served {json.dumps(served)}", file=sys.stderr, flush=True)
+    if args.served_variants:
+      variants = {}
+      for grp in ((), ("fvm",), ("spectral",), ("fvm", "spectral")):
+        if grp != headline_group:
+          v = _served_run(dev, requests, grp)[3]
+          variants["+".join(grp) or "none"] = v
+          print(f"[bench] served, grouped {grp}: {json.dumps(v)}", file=sys.stderr, flush=True)
   cg = _cg_variant(dev, args.cg_steps) if args.cg_steps > 0 else None
   if cg:
     print(f"[bench] cg variant {json.dumps(cg)}", file=sys.stderr, flush=True)
@@ -806,10 +846,13 @@ def main():
   if args.batched_steps > 0:
     batched = _batched_spectral(dev, 64, 512, args.batched_steps, 20)
     print(f"[bench] batched {json.dumps(batched)}", file=sys.stderr, flush=True)
-  group = None
+  group = group_spectral = None
   if args.group_steps > 0:
-    group = _group_mixed(dev, {64: 64, 128: 32, 256: 16, 512: 8}, args.group_steps)
+    mix = {64: 64, 128: 32, 256: 16, 512: 8}
+    group = _group_mixed(dev, mix, args.group_steps)
     print(f"[bench] group {json.dumps(group)}", file=sys.stderr, flush=True)
+    group_spectral = _group_mixed(dev, mix, args.group_steps, "spectral")
+    print(f"[bench] spectral group {json.dumps(group_spectral)}", file=sys.stderr, flush=True)
   li = ([_learned_interpolation(dev, n) for n in args.li_sizes]
         if args.learned_interpolation else None)
   if li and len(li) == 1:
@@ -845,8 +888,8 @@ def main():
       "max_rel_l2_vs_oracle": served["max_rel_l2_vs_oracle_fp32"] if served else None,
       "roofline": roofline if roofline else plain["roofline"],
       "cpu_baseline": cpu,
-      "served": served, "plain": plain, "batched": batched, "cg_variant": cg,
-      "group": group, "adjoint": adj, "learned_interpolation": li,
+      "served": served, "served_variants": variants, "plain": plain, "batched": batched, "cg_variant": cg,
+      "group": group, "group_spectral": group_spectral, "adjoint": adj, "learned_interpolation": li,
   }
   print(json.dumps(out), flush=True)
 

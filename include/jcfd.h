@@ -429,8 +429,8 @@ int jcfd_ns2d_irfft2(jcfd_ns2d_plan* plan, const jcfd_tensor* x_hat, jcfd_tensor
 
 /* Mixed-size spectral groups: one step of many NavierStokes2D instances of
  * different grid sizes (and physics), each a slot of some plan, as ONE
- * persistent launch per kernel class and stage, over a work list of
- * (instance, tile) items of equal work.  No counterpart in the reference
+ * launch per kernel class and stage over a work list of (instance, tile)
+ * items of equal work, one workgroup per item.  No counterpart in the reference
  * (one field per call, funcutils.repeated, funcutils.py:82-88); every
  * instance gets exactly the arithmetic of jcfd_ns2d_step on it alone. */
 typedef struct jcfd_ns2d_group jcfd_ns2d_group;

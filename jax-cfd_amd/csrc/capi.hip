@@ -2445,11 +2445,8 @@ int jcfd_ns2d_group_step(jcfd_ns2d_group* g, const jcfd_ns2d_group_member* membe
     if ((rc = ns_group_upload(g, table, n_desc, counts, n_slot, s))) return rc;
     g->last = table;
   }
-  if (max_n != g->max_n) {
-    for (int k = 0; k < kNsGroupClasses; ++k)
-      if ((rc = ns_group_grid(k, max_n, &g->grid[k]))) return rc;
-    g->max_n = max_n;
-  }
+  g->max_n = max_n;
+  for (int k = 0; k < kNsGroupClasses; ++k) g->grid[k] = counts[k == 1 ? 1 : 0];  // one workgroup per item
   g->n_items[0] = counts[0];
   g->n_items[1] = counts[1];
   g->n_stages = n_stages;
@@ -2475,7 +2472,10 @@ int jcfd_ns2d_group_step(jcfd_ns2d_group* g, const jcfd_ns2d_group_member* membe
     }
     return r;
   };
-  if ((rc = run_steps(g->graphs, g->descs, g->items, g->d_slots, max_n * 16 + S, nsteps, s, one_step)))
+  // captured launches bake in the grids (the item counts) and LDS size
+  const void* count_key = reinterpret_cast<const void*>(
+      (static_cast<uintptr_t>(static_cast<uint32_t>(counts[0])) << 32) | static_cast<uint32_t>(counts[1]));
+  if ((rc = run_steps(g->graphs, g->descs, g->items, count_key, max_n * 16 + S, nsteps, s, one_step)))
     return rc;
   for (const Conv& c : convs)
     if ((rc = ns_launch_state_convert(c.n, false, c.plan->ws, c.state, g->d_slots + c.slot_off, c.count, s)))

@@ -1120,6 +1120,21 @@ __global__ __launch_bounds__(kThreads) void fvm_group_phase(FvmGroupQueue q) {
 #pragma unroll
     for (int k = 0; k < kDescWords; ++k)
       dw[k] = static_cast<unsigned>(__builtin_amdgcn_readlane(static_cast<int>(word), k));
+    // the descriptor's pointers are all global memory
+    d.b.u = as_global(d.b.u);
+    d.b.v = as_global(d.b.v);
+    d.b.us = as_global(d.b.us);
+    d.b.vs = as_global(d.b.vs);
+    d.b.spec = as_global(d.b.spec);
+    d.b.u_out = as_global(d.b.u_out);
+    d.b.v_out = as_global(d.b.v_out);
+    d.b.fx = as_global(d.b.fx);
+    d.b.lam = as_global(d.b.lam);
+    d.b.tw = as_global(d.b.tw);
+    d.b.slots = as_global(d.b.slots);
+    d.b.rhs = as_global(d.b.rhs);
+    d.b.q = as_global(d.b.q);
+    d.b.rk = as_global(d.b.rk);
     const int cur = tile;
     const int nxt = it + static_cast<int>(gridDim.x);
     if (nxt < total) fetch(nxt, tile, word);

@@ -124,6 +124,18 @@ __device__ __forceinline__ int xcd_tile(int b, int nb) {
   return (nb % 8 != 0) ? b : (b % 8) * (nb / 8) + b / 8;
 }
 
+// p, known to point at global memory, as a generic pointer the compiler can
+// trace back to the global address space.  Pointers read from memory (work
+// list descriptors) are otherwise generic to the compiler: its flat
+// accesses count on the LDS wait counter too, so every wait on them waits
+// for all outstanding memory operations.
+template <typename T>
+__device__ __forceinline__ T* as_global(T* p) {
+  using G = __attribute__((address_space(1))) T;
+  G* gp = (G*)reinterpret_cast<uintptr_t>(p);
+  return (T*)gp;
+}
+
 // Per-instance view of FvmBuffers for the instance this block works on.
 __device__ __forceinline__ FvmBuffers fvm_instance(FvmBuffers b, int n) {
   const size_t s = b.slots ? static_cast<size_t>(b.slots[blockIdx.y]) : blockIdx.y;
@@ -267,9 +279,10 @@ struct NsBuffers {
   float2* pbuf;        // packed product spectrum, (batch, n x n/2), tiled
 };
 
-// Mixed-size spectral group (spectral.hip): one persistent launch per kernel
-// class of the step walks a work list of (instance, tile) over instances of
-// any built grid size; each tile runs the per-size kernels' tile body.
+// Mixed-size spectral group (spectral.hip): one launch per kernel class of
+// the step covers a work list of (instance, tile) items over instances of
+// any built grid size, one workgroup per item; each item runs the per-size
+// kernels' tile body.
 // Descriptors are per (instance, stage): the member plan's stage parameters
 // and the instance's buffers (internal state layout).
 struct NsGroupDesc {
@@ -295,7 +308,6 @@ struct NsGroupQueue {
 constexpr int kNsGroupClasses = 4;
 bool ns_group_size_supported(int n);
 int ns_group_tiles(int n, int list);
-int ns_group_grid(int cls, int max_n, int* grid);
 int ns_group_launch(int cls, int max_n, const NsGroupQueue& q, int stage, int grid, hipStream_t s);
 bool ns_size_supported(int n);
 // Default tile shapes and radix for size n into q.

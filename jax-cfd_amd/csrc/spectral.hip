@@ -650,8 +650,7 @@ __global__ __launch_bounds__(kThreads, (NsColWaves<N, W>::value)) void ns_cols_f
 // ---- mixed-size group step ---------------------------------------------------
 // Every item is 1024 complex words of work whatever its grid size (column
 // tiles of W = 1024 / n columns, row blocks of 1024 / n rows; capped by the
-// sizes' limits), so the fixed-stride walk of a persistent grid stays
-// balanced across sizes.  Tile shapes regroup the same per-column and
+// sizes' limits), so workgroups of every size take about the same time.  Tile shapes regroup the same per-column and
 // per-row arithmetic: results are bit-identical to the per-size kernels'.
 template <int N>
 struct NsGroupShape {
@@ -675,30 +674,38 @@ __device__ __forceinline__ void ns_group_tile(const NsGroupDesc& d, int tile, in
   }
 }
 
+// One workgroup per item (the host launches exactly the list's item count):
+// the hardware's workgroup dispatch keeps the CUs full, as for the per-size
+// kernels, with no loop state held in registers.
 template <int CLS>
-__global__ __launch_bounds__(kThreads) void ns_group_phase(NsGroupQueue q, int stage) {
+__global__ __launch_bounds__(kThreads) void ns_group_phase(const NsGroupDesc* __restrict__ descs,
+                                                           const NsGroupItem* __restrict__ items,
+                                                           const int* __restrict__ n_items, int stage) {
   extern __shared__ float4 ns_group_smem[];
   char* smem = reinterpret_cast<char*>(ns_group_smem);
-  constexpr int kList = CLS == 1 ? 1 : 0;
-  const NsGroupItem* items = q.items[kList];
-  const int total = q.n_items[kList];
-  for (int it = blockIdx.x; it < total; it += gridDim.x) {
-    const NsGroupItem w = items[it];
-    const NsGroupDesc d = q.descs[static_cast<size_t>(w.desc) * JCFD_MAX_RK_STAGES + stage];
-    // a fresh copy of the thread index per item keeps per-thread addresses
-    // of every size's tile body from being hoisted out of the loop
-    int t = threadIdx.x;
-    asm volatile("" : "+v"(t));
-    switch (d.n) {
-      case 32: ns_group_tile<32, CLS>(d, w.tile, t, smem); break;
-      case 64: ns_group_tile<64, CLS>(d, w.tile, t, smem); break;
-      case 128: ns_group_tile<128, CLS>(d, w.tile, t, smem); break;
-      case 256: ns_group_tile<256, CLS>(d, w.tile, t, smem); break;
-      case 512: ns_group_tile<512, CLS>(d, w.tile, t, smem); break;
-      case 1024: ns_group_tile<1024, CLS>(d, w.tile, t, smem); break;
-      default: break;  // rejected on the host
-    }
-    __syncthreads();  // LDS is reused by the next item
+  const int it = blockIdx.x;
+  if (it >= *n_items) return;
+  const NsGroupItem w = items[it];
+  // the descriptor's pointers are all global memory
+  NsGroupDesc d = descs[static_cast<size_t>(w.desc) * JCFD_MAX_RK_STAGES + stage];
+  d.ws = as_global(d.ws);
+  d.h = as_global(d.h);
+  d.g = as_global(d.g);
+  d.pbuf = as_global(d.pbuf);
+  d.q.kx = as_global(d.q.kx);
+  d.q.ky = as_global(d.q.ky);
+  d.q.tw = as_global(d.q.tw);
+  d.q.force_idx = as_global(d.q.force_idx);
+  d.q.force_val = as_global(d.q.force_val);
+  const int t = threadIdx.x;
+  switch (d.n) {
+    case 32: ns_group_tile<32, CLS>(d, w.tile, t, smem); break;
+    case 64: ns_group_tile<64, CLS>(d, w.tile, t, smem); break;
+    case 128: ns_group_tile<128, CLS>(d, w.tile, t, smem); break;
+    case 256: ns_group_tile<256, CLS>(d, w.tile, t, smem); break;
+    case 512: ns_group_tile<512, CLS>(d, w.tile, t, smem); break;
+    case 1024: ns_group_tile<1024, CLS>(d, w.tile, t, smem); break;
+    default: break;  // rejected on the host
   }
 }
 
@@ -727,14 +734,6 @@ int ns_group_lds(int cls, int max_n) {
   return bytes;
 }
 
-const void* ns_group_kernel(int cls) {
-  switch (cls) {
-    case 0: return reinterpret_cast<const void*>(&ns_group_phase<0>);
-    case 1: return reinterpret_cast<const void*>(&ns_group_phase<1>);
-    case 2: return reinterpret_cast<const void*>(&ns_group_phase<2>);
-    default: return reinterpret_cast<const void*>(&ns_group_phase<3>);
-  }
-}
 
 template <int N> struct NsTiles;
 template <> struct NsTiles<32> { static constexpr int W = 8, RB = 2; };
@@ -894,29 +893,15 @@ int ns_group_tiles(int n, int list) {
   }
 }
 
-int ns_group_grid(int cls, int max_n, int* grid) {
-  int dev = 0, cus = 0, per_cu = 0;
-  int rc;
-  if ((rc = check_hip(hipGetDevice(&dev), "hipGetDevice")) ||
-      (rc = check_hip(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev),
-                      "CU count")) ||
-      (rc = check_hip(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, ns_group_kernel(cls), kThreads,
-                                                                   ns_group_lds(cls, max_n)),
-                      "spectral group occupancy")))
-    return rc;
-  *grid = std::max(1, per_cu) * std::max(1, cus);
-  return JCFD_OK;
-}
-
 int ns_group_launch(int cls, int max_n, const NsGroupQueue& q, int stage, int grid, hipStream_t s) {
   if (!ns_group_size_supported(max_n))
     return fail(JCFD_ERR_UNSUPPORTED, "spectral group grid size not supported: " + std::to_string(max_n));
   const size_t lds = static_cast<size_t>(ns_group_lds(cls, max_n));
   switch (cls) {
-    case 0: hipLaunchKernelGGL(ns_group_phase<0>, dim3(grid), dim3(kThreads), lds, s, q, stage); break;
-    case 1: hipLaunchKernelGGL(ns_group_phase<1>, dim3(grid), dim3(kThreads), lds, s, q, stage); break;
-    case 2: hipLaunchKernelGGL(ns_group_phase<2>, dim3(grid), dim3(kThreads), lds, s, q, stage); break;
-    default: hipLaunchKernelGGL(ns_group_phase<3>, dim3(grid), dim3(kThreads), lds, s, q, stage);
+    case 0: hipLaunchKernelGGL(ns_group_phase<0>, dim3(grid), dim3(kThreads), lds, s, q.descs, q.items[0], q.n_items, stage); break;
+    case 1: hipLaunchKernelGGL(ns_group_phase<1>, dim3(grid), dim3(kThreads), lds, s, q.descs, q.items[1], q.n_items + 1, stage); break;
+    case 2: hipLaunchKernelGGL(ns_group_phase<2>, dim3(grid), dim3(kThreads), lds, s, q.descs, q.items[0], q.n_items, stage); break;
+    default: hipLaunchKernelGGL(ns_group_phase<3>, dim3(grid), dim3(kThreads), lds, s, q.descs, q.items[0], q.n_items, stage);
   }
   return check_hip(hipGetLastError(), "spectral group launch");
 }

@@ -188,8 +188,8 @@ class Ns2dPlan:
 
 
 class Ns2dGroup:
-  """Mixed-size group stepping (jcfd_ns2d_group_*): one persistent launch per
-  kernel class and tableau stage advances every listed slot of every member
+  """Mixed-size group stepping (jcfd_ns2d_group_*): one launch per kernel
+  class and tableau stage advances every listed slot of every member
   plan, whatever its grid size, each with exactly the arithmetic of a lone
   step."""
 

@@ -66,7 +66,10 @@ def test_batched_spectral_64x512(radix):
     assert err <= 1e-5
 
 
-def test_served_stream_all_pools():
+@pytest.mark.parametrize("group", [(), ("fvm", "spectral")])
+def test_served_stream_all_pools(group):
+  """Every request equals its plain run bitwise, with per-pool launches and
+  with each solver's pools grouped into one mixed-size group step."""
   import jax_cfd_amd.base as cfd
   from jax_cfd_amd import serving
   from jax_cfd_amd.spectral import time_stepping
@@ -78,8 +81,8 @@ def test_served_stream_all_pools():
       for k, st in enumerate(steps):
         reqs.append(serving.RolloutRequest(i, solver, n, st, 50 + i, arrival=0.002 * i))
         i += 1
-  srv = serving.RolloutServer(capacity={64: 2, 128: 2, 256: 2, 512: 2})
-  print(f"serving {len(reqs)} requests over 8 pools", flush=True)
+  srv = serving.RolloutServer(capacity={64: 2, 128: 2, 256: 2, 512: 2}, group=group)
+  print(f"serving {len(reqs)} requests over 8 pools, grouped {group}", flush=True)
   res = {r.request_id: r for r in srv.run(reqs, realtime=False)}
   assert all(r.status == 'done' for r in res.values())
   assert {(r.solver, r.n) for r in reqs} == set(k for k in srv.pools)

@@ -1,5 +1,6 @@
 """Host-side pieces of the server that need no GPU."""
 import numpy as np
+import pytest
 
 from jax_cfd_amd import serving
 
@@ -36,3 +37,9 @@ def test_invalid_requests_raise_before_any_work():
     serving.RolloutServer.run(srv, [serving.RolloutRequest(0, 'fvm', 64, -1, 0)])
   with pytest.raises(ValueError):
     serving.RolloutServer.run(srv, [serving.RolloutRequest(0, 'fvm', 64, 10, 0, save_every=-2)])
+
+
+def test_unknown_group_solver_raises_before_any_device_work():
+  from jax_cfd_amd import serving
+  with pytest.raises(ValueError):
+    serving.RolloutServer(group=('lbm',))

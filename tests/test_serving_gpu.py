@@ -158,3 +158,29 @@ def test_slot_reuse_keeps_every_result():
       want = [step.multistep(w, r.steps).cpu().numpy()]
     for a, b in zip(got.state, want):
       np.testing.assert_array_equal(a, b)
+
+
+def test_grouped_pools_equal_per_pool_launches():
+  """Grouped pools (one mixed-size group step per chunk over every pool of a
+  solver) give the per-pool results bit for bit, frames included, with slot
+  reuse (capacity 2) and requests that end and start between chunks."""
+  from jax_cfd_amd import serving
+  reqs = []
+  for i in range(14):
+    solver = ('fvm', 'spectral')[i % 2]
+    n = (64, 128, 256)[i % 3]
+    every = 6 if i % 4 == 1 else 0
+    reqs.append(serving.RolloutRequest(i, solver, n, 20 + 13 * i, 60 + i, save_every=every))
+  cap = {64: 2, 128: 2, 256: 2}
+  print("grouped vs per-pool server, 14 requests", flush=True)
+  a = _by_id(_srv(capacity=cap, group=()).run(reqs, realtime=False))
+  srv = _srv(capacity=cap, group=('fvm', 'spectral'))
+  b = _by_id(srv.run(reqs, realtime=False))
+  assert set(srv.groups) == {'fvm', 'spectral'} and len(srv.groups['spectral'].pools) == 3
+  for r in reqs:
+    assert a[r.request_id].status == b[r.request_id].status == 'done'
+    for x, y in zip(a[r.request_id].state, b[r.request_id].state):
+      np.testing.assert_array_equal(x, y)
+    if r.save_every:
+      for x, y in zip(a[r.request_id].frames, b[r.request_id].frames):
+        np.testing.assert_array_equal(x, y)
