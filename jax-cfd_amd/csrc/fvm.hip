@@ -499,14 +499,14 @@ __device__ __forceinline__ void rows_forward_body(const FvmParams& p, const FvmB
       const int lr = e / N, j = e % N;
       const int gu = ((i0 - 1 + lr) & (N - 1)) * N + j;
       float acc = 0.f;
-      for (int k = 0; k < t.n; ++k) acc = acc + t.c[k] * t.ku[k][off + gu];
+      for (int k = 0; k < t.n; ++k) acc = acc + t.c[k] * as_global(t.ku[k])[off + gu];
       const float ust = b.u[gu] + dt * acc;
       sus[e] = ust;
       if (lr >= 1) b.us[gu] = ust;
       if (lr < RB) {
         const int gv = ((i0 + lr) & (N - 1)) * N + j;
         float accv = 0.f;
-        for (int k = 0; k < t.n; ++k) accv = accv + t.c[k] * t.kv[k][off + gv];
+        for (int k = 0; k < t.n; ++k) accv = accv + t.c[k] * as_global(t.kv[k])[off + gv];
         const float vst = b.v[gv] + dt * accv;
         svs[e] = vst;
         b.vs[gv] = vst;
