@@ -93,9 +93,22 @@ __device__ __forceinline__ float2 ns_spectrum(int m, const NsMult& k, float2 w) 
   }
 }
 
+// Spectra built and inverse-transformed per round of the inverse column
+// work: a round's spectra sit side by side as NSP * W columns of one LDS
+// tile, so one set of FFT passes (and barriers) serves all of them.  NSP is
+// the largest of 4, 2, 1 whose round fits kNsRoundWords complex words.
+constexpr int kNsMaxSpectraPerRound = 4;
+constexpr int kNsRoundWords = 2048;
+template <int N, int W>
+struct NsRound {
+  static constexpr int value = (kNsMaxSpectraPerRound >= 4 && 4 * N * W <= kNsRoundWords) ? 4
+                               : (kNsMaxSpectraPerRound >= 2 && 2 * N * W <= kNsRoundWords) ? 2
+                                                                                            : 1;
+};
+
 template <int N, int W>
 struct ColShared {
-  float2 buf[N * W];  // work tile [p][c]
+  float2 buf[N * W * NsRound<N, W>::value];  // work tile [p][c] (inverse rounds: [p][spectrum][c])
   float2 w0[N];       // tile 0: vorticity column 0
   float2 wn[N];       // tile 0: vorticity column N/2
   float2 tw[N];
@@ -118,10 +131,10 @@ struct WTile {
 // -1/laplacian at this thread's element k (columns other than 0; one
 // division per element for all four spectra); 2 pi kx, 2 pi ky come from
 // the LDS wavenumber tables.
-template <int N, int W>
+template <int N, int W, int NSP = 1>
 __device__ __forceinline__ void build_spectrum(ColShared<N, W>& sm, const WTile<N, W>& wt,
                                                const float (&psi)[WTile<N, W>::PER],
-                                               const NsParams& q, int m, int c0, int tid) {
+                                               const NsParams& q, int m, int slot, int c0, int tid) {
   constexpr int H = N / 2;
 #pragma unroll
   for (int k = 0; k < WTile<N, W>::PER; ++k) {
@@ -145,7 +158,7 @@ __device__ __forceinline__ void build_spectrum(ColShared<N, W>& sm, const WTile<
       const float2 sh = make_float2(0.5f * (ah.x + bh.x), 0.5f * (ah.y - bh.y));
       out = make_float2(s0.x - sh.y, s0.y + sh.x);  // s0 + i sh
     }
-    sm.buf[e] = make_float2(out.x * q.inv_nn, out.y * q.inv_nn);
+    sm.buf[p * (NSP * W) + slot * W + c] = make_float2(out.x * q.inv_nn, out.y * q.inv_nn);
   }
 }
 
@@ -166,22 +179,29 @@ __device__ __forceinline__ void cols_inverse_body(ColShared<N, W>& sm, const WTi
     const int p = (e < N * W ? e : N * W - 1) / W, cg = c0 + e % W;
     psi[k] = (NSPEC == 4 && cg != 0) ? ns_mult(q, sm.kx[p], sm.ky[cg], false).psi : 0.f;
   }
+  constexpr int NSP = NSPEC < NsRound<N, W>::value ? NSPEC : NsRound<N, W>::value;
+  static_assert(NSPEC % NSP == 0, "whole rounds");
 #pragma unroll 1
-  for (int m = 0; m < NSPEC; ++m) {
-    build_spectrum<N, W>(sm, wt, psi, q, NSPEC == 1 ? 4 : m, c0, tid);
+  for (int m0 = 0; m0 < NSPEC; m0 += NSP) {
+#pragma unroll
+    for (int j = 0; j < NSP; ++j)
+      build_spectrum<N, W, NSP>(sm, wt, psi, q, NSPEC == 1 ? 4 : m0 + j, j, c0, tid);
     __syncthreads();
-    lds_fft<N, W, kThreads, W, 1, true, true, R>(sm.buf, sm.tw, tid);
-    float2* gm = g + m * gstride;
-    float2 o[PER];
+    lds_fft<N, NSP * W, kThreads, NSP * W, 1, true, true, R>(sm.buf, sm.tw, tid);
 #pragma unroll
-    for (int k = 0; k < PER; ++k) {
-      const int e = tid + k * kThreads;
-      if ((N * W) % kThreads == 0 || e < N * W) o[k] = sm.buf[e];
-    }
+    for (int j = 0; j < NSP; ++j) {
+      float2* gm = g + (m0 + j) * gstride;
+      float2 o[PER];
 #pragma unroll
-    for (int k = 0; k < PER; ++k) {
-      const int e = tid + k * kThreads;
-      if ((N * W) % kThreads == 0 || e < N * W) gm[tix<N>(e / W, c0 + e % W)] = o[k];
+      for (int k = 0; k < PER; ++k) {
+        const int e = tid + k * kThreads;
+        if ((N * W) % kThreads == 0 || e < N * W) o[k] = sm.buf[(e / W) * (NSP * W) + j * W + e % W];
+      }
+#pragma unroll
+      for (int k = 0; k < PER; ++k) {
+        const int e = tid + k * kThreads;
+        if ((N * W) % kThreads == 0 || e < N * W) gm[tix<N>(e / W, c0 + e % W)] = o[k];
+      }
     }
     __syncthreads();
   }
