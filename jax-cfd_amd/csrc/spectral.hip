@@ -43,15 +43,20 @@ namespace {
 constexpr int kThreads = 256;
 
 // Internal layout of packed half spectra (n rows x n/2 words: the four
-// derivative spectra G and the product spectrum P): column tiles of kTw
+// derivative spectra G and the product spectrum P): column tiles of kTw<N>
 // words, each tile one contiguous block of n rows x kTw words.  A column
 // kernel's tile is then a contiguous block, and a row kernel's rows cover
 // whole cache lines in every tile (4 rows x 4 words = 128 bytes), where the
-// row-major layout left 32-byte pieces of 2 KB rows to both.
-constexpr int kTw = 4;
+// row-major layout left 32-byte pieces of 2 KB rows to both.  From 512 on
+// the column kernels run two-column tiles: tiles of 2 words make each of
+// their rows a whole 16-byte piece of one tile (with 4-word tiles two
+// workgroups wrote the halves of every 32-byte piece; the update-and-inverse
+// kernel at 64 x 512^2 ran 149 instead of 133 us).
+template <int N>
+constexpr int kTw = N >= 512 ? 2 : 4;
 template <int N>
 __device__ __forceinline__ int tix(int p, int c) {
-  return ((c / kTw) * N + p) * kTw + (c % kTw);
+  return ((c / kTw<N>) * N + p) * kTw<N> + (c % kTw<N>);
 }
 // Internal state layout (n rows x (n/2 + 1) words, the same size as the
 // external rfft2 layout): columns 0 .. n/2 - 1 tiled as above, then column
@@ -396,16 +401,24 @@ __device__ __forceinline__ void rows_tile(RowsShared<N, RB>& sm, const NsParams&
   float2* twh = sm.twh;
   // Load the four spectra's rows in chunks (all loads of a chunk issued
   // together); the twiddle tables go to LDS while the first chunk is in
-  // flight.  Element e walks the tiled layout in memory order: word w of
+  // flight.  Words are taken in mirror pairs (kk, H - kk), kk < H/2: both
+  // outputs of a pair need both words, so each word is read once.  Element
+  // e walks the first half of the tiled layout in memory order: word w of
   // the tile, then row r (the RB rows of a tile are contiguous), then the
-  // tile, then the spectrum m.
-  constexpr int TOT = 4 * RB * H;
+  // tile, then the spectrum m; word H/2 (its own mirror) goes alone.
+  constexpr int HH = H / 2;
+  constexpr int TW = kTw<N>;
+  static_assert(HH % TW == 0, "whole tiles in each half row");
+  constexpr int TOT = 4 * RB * HH;
   constexpr int PER = (TOT + kThreads - 1) / kThreads;
   constexpr int C = PER < 8 ? PER : 8;
+  static_assert(4 * RB <= kThreads, "one thread per middle word");
   TwiddleStage<N, 1, kThreads> ta;
   TwiddleStage<H, 2, kThreads> tb;
   ta.load(q.tw, tid);
   tb.load(q.tw, tid);
+  float2 mid = make_float2(0.f, 0.f);
+  if (tid < 4 * RB) mid = g[(tid / RB) * gstride + tix<N>(i0 + tid % RB, HH)];
 #pragma unroll 1
   for (int k0 = 0; k0 < PER; k0 += C) {
     float2 a[C], b[C];
@@ -413,8 +426,8 @@ __device__ __forceinline__ void rows_tile(RowsShared<N, RB>& sm, const NsParams&
     for (int k = 0; k < C; ++k) {
       const int e = tid + (k0 + k) * kThreads;
       if (e < TOT) {
-        const int r = (e / kTw) % RB, kk = ((e / (kTw * RB)) % (H / kTw)) * kTw + e % kTw;
-        const float2* gm = g + (e / (RB * H)) * gstride;
+        const int r = (e / TW) % RB, kk = ((e / (TW * RB)) % (HH / TW)) * TW + e % TW;
+        const float2* gm = g + (e / (RB * HH)) * gstride;
         a[k] = gm[tix<N>(i0 + r, kk)];
         b[k] = gm[tix<N>(i0 + r, (H - kk) & (H - 1))];
       }
@@ -423,13 +436,16 @@ __device__ __forceinline__ void rows_tile(RowsShared<N, RB>& sm, const NsParams&
       ta.store(twn, tid);
       tb.store(twh, tid);
       __syncthreads();
+      if (tid < 4 * RB) rows[tid * H + HH] = c2r_combine(mid, mid, HH, twn[HH]);
     }
 #pragma unroll
     for (int k = 0; k < C; ++k) {
       const int e = tid + (k0 + k) * kThreads;
       if (e < TOT) {
-        const int r = (e / kTw) % RB, kk = ((e / (kTw * RB)) % (H / kTw)) * kTw + e % kTw;
-        rows[((e / (RB * H)) * RB + r) * H + kk] = c2r_combine(a[k], b[k], kk, twn[kk]);
+        const int r = (e / TW) % RB, kk = ((e / (TW * RB)) % (HH / TW)) * TW + e % TW;
+        float2* row = rows + ((e / (RB * HH)) * RB + r) * H;
+        row[kk] = c2r_combine(a[k], b[k], kk, twn[kk]);
+        if (kk != 0) row[H - kk] = c2r_combine(b[k], a[k], H - kk, twn[H - kk]);
       }
     }
   }
@@ -447,8 +463,8 @@ __device__ __forceinline__ void rows_tile(RowsShared<N, RB>& sm, const NsParams&
   __syncthreads();
   lds_fft_waves<H, RB, kThreads, 1, H, false, R>(prod, twh, tid);
   // element e in the tiled layout's memory order, as for the loads
-  auto row_of = [](int e) { return (e / kTw) % RB; };
-  auto word_of = [](int e) { return (e / (kTw * RB)) * kTw + e % kTw; };
+  auto row_of = [](int e) { return (e / TW) % RB; };
+  auto word_of = [](int e) { return (e / (TW * RB)) * TW + e % TW; };
   staged<RB * H, 8, kThreads>(
       tid, [&](int e) { return r2c_combine(prod + row_of(e) * H, word_of(e), H, twn[word_of(e)]); },
       [&](int e, float2 v) { pout[tix<N>(i0 + row_of(e), word_of(e))] = v; });
@@ -478,9 +494,10 @@ __global__ __launch_bounds__(kThreads) void ns_state_convert(const float2* __res
   const size_t inst = slots ? static_cast<size_t>(slots[blockIdx.y]) : blockIdx.y;
   int p, c;
   if (t < N * H) {
-    const int tile = t / (N * kTw), r = t % (N * kTw);
-    p = r / kTw;
-    c = tile * kTw + r % kTw;
+    constexpr int TW = kTw<N>;
+    const int tile = t / (N * TW), r = t % (N * TW);
+    p = r / TW;
+    c = tile * TW + r % TW;
   } else {
     p = t - N * H;
     c = H;
