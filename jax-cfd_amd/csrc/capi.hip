@@ -121,6 +121,8 @@ std::vector<float2> twiddles(int n) {
 // (fast_diagonalization.py:199-266 with array_utils.py:168-173): eigenvalues
 // (2 cos(2 pi p / n) - 2) / h^2 per axis in fp64, rfft layout on the last axis,
 // zero where |lambda| <= 10 * eps(float32); the irfftn 1/n^2 is folded in.
+// Stored column by column (entry (p, c) at c * n + p): the Poisson column
+// kernel's tiles then read contiguous runs of it.
 std::vector<float> poisson_multiplier(int n, double dx, double dy) {
   const int h1 = n / 2 + 1;
   const double cutoff = static_cast<double>(10.0f * 1.1920928955078125e-07f);
@@ -131,7 +133,7 @@ std::vector<float> poisson_multiplier(int n, double dx, double dy) {
       const double ly = (-2.0 / (dy * dy)) + 2.0 * std::cos(2.0 * M_PI * c / n) / (dy * dy);
       const double s = lx + ly;
       const double inv = (std::fabs(s) > cutoff) ? 1.0 / s : 0.0;
-      lam[static_cast<size_t>(p) * h1 + c] = static_cast<float>(inv / (static_cast<double>(n) * n));
+      lam[static_cast<size_t>(c) * n + p] = static_cast<float>(inv / (static_cast<double>(n) * n));
     }
   }
   return lam;
@@ -139,7 +141,8 @@ std::vector<float> poisson_multiplier(int n, double dx, double dy) {
 
 // D / n^2 for the implicit diffusion solve (diffusion.py:180-185 through
 // fast_diagonalization.py:196-225): D = (dt nu) s / (1 - (dt nu) s) with s the
-// summed periodic Laplacian eigenvalues, in fp64, rfft layout on the last axis.
+// summed periodic Laplacian eigenvalues, in fp64, rfft layout on the last axis;
+// stored column by column like Lambda^+.
 std::vector<float> diffusion_multiplier(int n, double dx, double dy, double nu, double dt) {
   const int h1 = n / 2 + 1;
   std::vector<float> m(static_cast<size_t>(n) * h1);
@@ -149,7 +152,7 @@ std::vector<float> diffusion_multiplier(int n, double dx, double dy, double nu, 
     for (int c = 0; c < h1; ++c) {
       const double ly = (-2.0 / (dy * dy)) + 2.0 * std::cos(2.0 * M_PI * c / n) / (dy * dy);
       const double x = dt_nu * (lx + ly);
-      m[static_cast<size_t>(p) * h1 + c] = static_cast<float>(x / (1.0 - x) / (static_cast<double>(n) * n));
+      m[static_cast<size_t>(c) * n + p] = static_cast<float>(x / (1.0 - x) / (static_cast<double>(n) * n));
     }
   }
   return m;
@@ -382,6 +385,7 @@ FvmBuffers fvm_buffers(const jcfd_fvm_plan* p) {
   b.spec = p->spec;
   b.fx = p->fx;
   b.lam = p->lam;
+  b.lam_cols = 1;
   b.tw = p->tw;
   return b;
 }
@@ -1037,6 +1041,7 @@ int jcfd_fvm_time_kernels(jcfd_fvm_plan* p, const jcfd_tensor* u, const jcfd_ten
   b.v_out = p->vs;
   b.fx = p->fx;
   b.lam = p->lam;
+  b.lam_cols = 1;
   b.tw = p->tw;
   hipEvent_t e0, e1;
   if ((rc = check_hip(hipEventCreate(&e0), "event")) ) return rc;

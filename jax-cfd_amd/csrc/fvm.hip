@@ -648,7 +648,7 @@ __device__ __forceinline__ void poisson_cols_body(const FvmBuffers& b, float sca
   for (int q = 0; q < PER; ++q) {
     const int e0 = tid + q * kThreads;
     const int e = e0 < N * W ? e0 : N * W - 1;
-    lam[q] = b.lam[(e / W) * (H + 1) + c0 + e % W];
+    lam[q] = b.lam_cols ? b.lam[(c0 + e % W) * N + e / W] : b.lam[(e / W) * (H + 1) + c0 + e % W];
   }
   tsn.store(twn, tid);
   __syncthreads();
@@ -667,7 +667,7 @@ __device__ __forceinline__ void poisson_cols_body(const FvmBuffers& b, float sca
         const float2 d = csub(y, ym);
         const float2 fh = make_float2(0.5f * d.y, -0.5f * d.x);
         const float l0 = lam[q] * scale;
-        const float lh = b.lam[pp * (H + 1) + H] * scale;
+        const float lh = (b.lam_cols ? b.lam[H * N + pp] : b.lam[pp * (H + 1) + H]) * scale;
         const float2 g0 = cscale(f0, l0), gh = cscale(fh, lh);
         nv[q] = make_float2(g0.x - gh.y, g0.y + gh.x);  // g0 + i gh
       } else {
@@ -967,6 +967,7 @@ int launch_diffuse(const FvmParams& p, const FvmBuffers& b, const float* mult, h
     f.u = field;
     f.u_out = field;
     f.lam = mult;
+    f.lam_cols = 1;  // the plan's multiplier table
     hipLaunchKernelGGL((fvm_rows_forward<N, RB, 3>), dim3(N / RB, nb(f)), dim3(kThreads), 0, s, p, f);
     hipLaunchKernelGGL((fvm_poisson_cols<N, W>), dim3(N / 2 / W, nb(f)), dim3(kThreads), 0, s, f, 1.0f);
     hipLaunchKernelGGL((fvm_rows_project<N, RB, 3>), dim3(N / RB, nb(f)), dim3(kThreads), 0, s, p, f);
@@ -1312,6 +1313,7 @@ int fvm_launch_projection(const FvmParams& p, const float* u, const float* v, fl
   b.v_out = v_out;
   b.spec = spec;
   b.lam = lam;
+  b.lam_cols = 1;  // the plan's Lambda^+ table
   b.tw = tw;
   return dispatch_n<ProjOp>(p.n, p, b, s);
 }

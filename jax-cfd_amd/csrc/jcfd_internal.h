@@ -70,7 +70,8 @@ struct FvmBuffers {
   float* u_out;          // state out
   float* v_out;
   const float* fx;       // forcing column f_x(y_j), n floats (may be null)
-  const float* lam;      // Lambda^+ / n^2, (n, n/2 + 1) floats
+  const float* lam;      // Lambda^+ / n^2, (n, n/2 + 1) floats (spectrum multiplier)
+  int lam_cols;          // lam stored column by column (entry (p, c) at c * n + p)
   const float2* tw;      // twiddles exp(-2 pi i k / n), n entries
   // Batches: instance k of a launch (blockIdx.y = k) is slots[k], or k when
   // slots is null; fields of instance s start at s * n * n (spectrum:
@@ -154,7 +155,8 @@ __device__ __forceinline__ FvmBuffers fvm_instance(FvmBuffers b, int n) {
 // Kernel launchers (fvm.hip).  All are stream-ordered and asynchronous.
 int fvm_launch_step(const FvmParams& p, const FvmBuffers& b, hipStream_t s);
 // Implicit diffusion solve in place on b.u_out / b.v_out (every instance of b)
-// with the multiplier table `mult` ((n, n/2 + 1) floats, 1/n^2 folded in).
+// with the multiplier table `mult` ((n, n/2 + 1) floats, 1/n^2 folded in,
+// stored column by column).
 int fvm_launch_diffuse(const FvmParams& p, const FvmBuffers& b, const float* mult, hipStream_t s);
 // Runge-Kutta pieces (every instance of b, default tile shapes):
 // explicit terms of (b.u, b.v) into (b.us, b.vs) rows of all instances, and
@@ -166,6 +168,7 @@ int fvm_launch_rk_project(const FvmParams& p, const FvmBuffers& b, hipStream_t s
 int fvm_launch_kernel(const FvmParams& p, const FvmBuffers& b, int which, hipStream_t s);
 int fvm_launch_explicit_terms(const FvmParams& p, const float* u, const float* v,
                               float* du, float* dv, const float* fx, hipStream_t s);
+// lam: the plan's Lambda^+ table (column by column).
 int fvm_launch_projection(const FvmParams& p, const float* u, const float* v, float* u_out,
                           float* v_out, float2* spec, const float* lam, const float2* tw,
                           hipStream_t s);
