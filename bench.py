@@ -392,17 +392,25 @@ def _cg_variant(dev, steps):
       pressure_solve=functools.partial(cfd.pressure.solve_cg, rtol=1e-6, atol=1e-6))
   v0 = cfd.initial_conditions.filtered_velocity_field(SEED, grid, VMAX, PEAK)
   plan = step.plan(dev)
-  u, v = v0[0].data.clone(), v0[1].data.clone()
-  plan.step(u, v, 2, out=(u, v))
-  torch.cuda.synchronize()
-  t0 = time.perf_counter()
-  plan.step(u, v, steps, out=(u, v))
-  torch.cuda.synchronize()
-  wall = time.perf_counter() - t0
+
+  def run(persistent):
+    plan.set_cg_persistent(persistent)
+    u, v = v0[0].data.clone(), v0[1].data.clone()
+    plan.step(u, v, 2, out=(u, v))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    plan.step(u, v, steps, out=(u, v))
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0, plan.cg_iterations
+  wall1, _ = run(True)  # one cooperative launch per solve
+  wall, iters = run(False)  # two launches per iteration, flag read every 64 (default)
   return {"workload": f"fvm {N}x{N} forced, solve_cg rtol=atol=1e-6", "steps": steps,
           "value": N * N * steps / wall, "unit": "cell-steps/s",
           "ms_per_step": wall / steps * 1e3,
-          "cg_iterations_last_step": plan.cg_iterations}
+          "us_per_iteration": wall / steps / (iters + 1) * 1e6,
+          "cg_iterations_last_step": iters,
+          "solve": "two launches per iteration",
+          "cooperative_form_ms_per_step": wall1 / steps * 1e3}
 
 
 def _plain_step_seconds(dev, solver, n, steps=40):

@@ -133,6 +133,15 @@ int64_t jcfd_fvm_plan_device_bytes(const jcfd_fvm_plan* plan);
  * steps when enabled (default: off for a single FVM instance, on for batched
  * spectral plans); 0 launches every kernel individually.  Same results. */
 int jcfd_fvm_plan_set_graphs(jcfd_fvm_plan* plan, int32_t enabled);
+/* CG solves (pressure solve_cg, diffusion solve_cg) as one cooperative
+ * launch per solve with grid barriers between the iteration steps (where the
+ * device keeps all the solve's workgroups resident), or as two launches per
+ * iteration with the convergence flag read every 64 iterations (default:
+ * on MI355X a grid barrier, with the cache write-back and invalidation that
+ * make other XCDs' writes visible, costs more than a launch: 21.7 against
+ * 7.5 us per iteration at 256^2).  No counterpart in the reference; same
+ * results. */
+int jcfd_fvm_plan_set_cg_persistent(jcfd_fvm_plan* plan, int32_t enabled);
 
 /* Tile shapes and FFT radix schedule of the three step kernels.  No
  * counterpart in the reference (XLA picks its own kernels); set by the

@@ -308,7 +308,8 @@ struct jcfd_fvm_plan {
   // CG pressure solve (pressure == JCFD_PRESSURE_CG)
   CgBuffers cg{};
   float* rhs = nullptr;
-  int* status_host = nullptr;  // pinned
+  int* status_host = nullptr;  // pinned, 4 ints
+  bool cg_persistent = false;  // one cooperative launch per CG solve (opt-in: slower, see the header)
   int cg_maxiter = 0;
   int cg_last_iterations = 0;
   // CG implicit diffusion (diffusion == JCFD_DIFFUSION_IMPLICIT_CG), same buffers
@@ -401,9 +402,9 @@ int cg_alloc(jcfd_fvm_plan* p) {
       (rc = device_alloc(&c.part_rr[0], nb, &p->bytes)) ||
       (rc = device_alloc(&c.part_rr[1], nb, &p->bytes)) ||
       (rc = device_alloc(&c.part_pap, nb, &p->bytes)) || (rc = device_alloc(&c.part_bb, nb, &p->bytes)) ||
-      (rc = device_alloc(&c.status, 2, &p->bytes)))
+      (rc = device_alloc(&c.status, 4, &p->bytes)))
     return rc;
-  if ((rc = check_hip(hipHostMalloc(reinterpret_cast<void**>(&p->status_host), 2 * sizeof(int),
+  if ((rc = check_hip(hipHostMalloc(reinterpret_cast<void**>(&p->status_host), 4 * sizeof(int),
                                     hipHostMallocDefault),
                       "pinned status")))
     return rc;
@@ -440,11 +441,32 @@ int cg_iterate(jcfd_fvm_plan* p, int maxiter, int first, int* iterations, hipStr
   return rc;
 }
 
+// The solve as one cooperative launch (cg_launch_persistent) when enabled
+// and the device holds it: reads the status once at the end.  Returns
+// JCFD_ERR_UNSUPPORTED, with nothing launched, when the caller should run
+// the chunked two-kernel form instead.
+int cg_persistent_solve(jcfd_fvm_plan* p, float rtol, float atol, int maxiter, bool helm, float coef,
+                        int* iterations, hipStream_t s) {
+  if (!p->cg_persistent) return JCFD_ERR_UNSUPPORTED;
+  int rc = cg_launch_persistent(p->params, p->cg, rtol, atol, maxiter, helm, coef, s);
+  if (rc) return rc;
+  if ((rc = check_hip(hipMemcpyAsync(p->status_host, p->cg.status, 4 * sizeof(int), hipMemcpyDeviceToHost,
+                                     s), "cg status")) ||
+      (rc = check_hip(hipStreamSynchronize(s), "cg status sync")))
+    return rc;
+  if (p->status_host[3] || !p->status_host[0])
+    return fail(JCFD_ERR_DEVICE, "persistent CG: a grid barrier timed out");
+  *iterations = p->status_host[1];
+  return JCFD_OK;
+}
+
 // CG on p->rhs into p->cg.x (solve_cg).
 int cg_solve(jcfd_fvm_plan* p, hipStream_t s) {
   const float rtol = static_cast<float>(p->cfg.cg_rtol), atol = static_cast<float>(p->cfg.cg_atol);
   int rc = cg_launch_init(p->params, p->rhs, p->cg, rtol, atol, p->cg_maxiter, s);
   if (rc) return rc;
+  rc = cg_persistent_solve(p, rtol, atol, p->cg_maxiter, false, 0.f, &p->cg_last_iterations, s);
+  if (rc != JCFD_ERR_UNSUPPORTED) return rc;
   return cg_iterate(p, p->cg_maxiter, kCgChunk, &p->cg_last_iterations, s, [&](int k, hipStream_t st) {
     return cg_launch_iteration(p->params, p->cg, k, rtol, atol, p->cg_maxiter, st);
   });
@@ -462,12 +484,16 @@ int cg_diffuse(jcfd_fvm_plan* p, float* u, float* v, hipStream_t s) {
   float* comps[2] = {u, v};
   for (int m = 0; m < 2; ++m) {
     int rc = cg_launch_init_helmholtz(p->params, comps[m], p->cg, p->diff_coef, s);
-    if (rc == JCFD_OK)
-      rc = cg_iterate(p, p->diff_maxiter, kDiffusionFirstChunk, &p->diff_last_iterations[m], s,
-                      [&](int k, hipStream_t st) {
-                        return cg_launch_iteration_helmholtz(p->params, p->cg, k, rtol, atol,
-                                                             p->diff_maxiter, p->diff_coef, st);
-                      });
+    if (rc == JCFD_OK) {
+      rc = cg_persistent_solve(p, rtol, atol, p->diff_maxiter, true, p->diff_coef,
+                               &p->diff_last_iterations[m], s);
+      if (rc == JCFD_ERR_UNSUPPORTED)
+        rc = cg_iterate(p, p->diff_maxiter, kDiffusionFirstChunk, &p->diff_last_iterations[m], s,
+                        [&](int k, hipStream_t st) {
+                          return cg_launch_iteration_helmholtz(p->params, p->cg, k, rtol, atol,
+                                                               p->diff_maxiter, p->diff_coef, st);
+                        });
+    }
     if (rc == JCFD_OK)
       rc = check_hip(hipMemcpyAsync(comps[m], p->cg.x, bytes, hipMemcpyDeviceToDevice, s),
                      "copy diffused component");
@@ -743,6 +769,12 @@ int32_t jcfd_fvm_diffusion_cg_iterations(const jcfd_fvm_plan* p, int32_t compone
 int jcfd_fvm_plan_set_graphs(jcfd_fvm_plan* p, int32_t enabled) {
   if (p == nullptr) return fail(JCFD_ERR_INVALID_ARGUMENT, "null plan");
   p->graphs.enabled = enabled != 0;
+  return JCFD_OK;
+}
+
+int jcfd_fvm_plan_set_cg_persistent(jcfd_fvm_plan* p, int32_t enabled) {
+  if (p == nullptr) return fail(JCFD_ERR_INVALID_ARGUMENT, "null plan");
+  p->cg_persistent = enabled != 0;
   return JCFD_OK;
 }
 

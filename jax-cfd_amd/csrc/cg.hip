@@ -9,7 +9,9 @@
 // periodic 5-point Laplacian of finite_differences.laplacian
 // (finite_differences.py:127-133).
 //
-// Each iteration is two kernels over row blocks:
+// Each iteration is two steps over row blocks, as two kernels or, where the
+// device can hold all the row blocks at once, as one cooperative launch for
+// the whole solve with a grid barrier between the steps (cg_persistent):
 //   cg_direction  p_k = r_k + beta_k p_{k-1} (rows +-1 halo recomputed in
 //                 LDS), Ap = L p_k, per-block partial <p, Ap>; convergence is
 //                 tested here against the reduced <r_k, r_k>.
@@ -96,6 +98,8 @@ __global__ __launch_bounds__(kThreads) void cg_init(const float* __restrict__ rh
     if (blockIdx.x == 0) {
       c.status[0] = 0;
       c.status[1] = 0;
+      c.status[2] = 0;  // grid-barrier arrivals of the persistent solve
+      c.status[3] = 0;  // grid-barrier timeout
     }
   }
 }
@@ -134,16 +138,18 @@ __global__ __launch_bounds__(kThreads) void cg_init_helmholtz(FvmParams p, const
     if (blockIdx.x == 0) {
       c.status[0] = 0;
       c.status[1] = 0;
+      c.status[2] = 0;
+      c.status[3] = 0;
     }
   }
 }
 
 // One direction step; HELM selects A = I - coef L over A = L.
 template <int N, int RB, bool HELM>
-__device__ __forceinline__ void direction_body(const FvmParams& p, const CgBuffers& c, int k, float rtol,
+__device__ __forceinline__ bool direction_body(const FvmParams& p, const CgBuffers& c, int k, float rtol,
                                                float atol, int maxiter, float coef, float* red,
                                                float* pt) {
-  if (c.status[0]) return;
+  if (c.status[0]) return true;
   constexpr int NB = N / RB;
   constexpr int TOT = (RB + 2) * N, PT = (TOT + kThreads - 1) / kThreads;
   const int i0 = blockIdx.x * RB;
@@ -174,7 +180,7 @@ __device__ __forceinline__ void direction_body(const FvmParams& p, const CgBuffe
       c.status[0] = 1;
       c.status[1] = k;
     }
-    return;
+    return true;
   }
   const float beta = (k == 0) ? 0.f : gamma / sums[2];
 #pragma unroll
@@ -200,6 +206,7 @@ __device__ __forceinline__ void direction_body(const FvmParams& p, const CgBuffe
   }
   const float s = block_sum(pap, red);
   if (threadIdx.x == 0) c.part_pap[blockIdx.x] = s;
+  return false;
 }
 
 template <int N, int RB>
@@ -220,9 +227,7 @@ __global__ __launch_bounds__(kThreads) void cg_direction_helmholtz(FvmParams p, 
 }
 
 template <int N, int RB>
-__global__ __launch_bounds__(kThreads) void cg_update(CgBuffers c, int k) {
-  __shared__ float red[2 * kWaves];
-  if (c.status[0]) return;
+__device__ __forceinline__ void update_body(const CgBuffers& c, int k, float* red) {
   constexpr int NB = N / RB;
   constexpr int TOT = RB * N, PU = (TOT + kThreads - 1) / kThreads;
   const int i0 = blockIdx.x * RB;
@@ -259,6 +264,70 @@ __global__ __launch_bounds__(kThreads) void cg_update(CgBuffers c, int k) {
   if (threadIdx.x == 0) c.part_rr[(k + 1) & 1][blockIdx.x] = s;
 }
 
+template <int N, int RB>
+__global__ __launch_bounds__(kThreads) void cg_update(CgBuffers c, int k) {
+  __shared__ float red[2 * kWaves];
+  if (c.status[0]) return;
+  update_body<N, RB>(c, k, red);
+}
+
+// Polls of a grid barrier before a workgroup gives up (seconds of waiting;
+// a barrier of a healthy launch completes in microseconds).
+constexpr unsigned kBarrierSpinLimit = 1u << 24;
+
+// Grid-wide barrier of a cooperative launch (every workgroup resident at
+// once): the arrival counter grows by one per workgroup and barrier, and the
+// workgroups of barrier number b wait until it reaches target = b *
+// gridDim.x.  The agent-scope release of the arrival and the acquire after
+// the wait make every workgroup's earlier global writes visible to all the
+// others (the caches of every XCD included), as a kernel boundary would.
+// Bounded: a workgroup that polls kBarrierSpinLimit times raises the timeout
+// flag and returns false, so a broken launch ends with an error, not a hang.
+__device__ __forceinline__ bool grid_barrier(unsigned* count, int* timeout, unsigned target, int* s_ok) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int ok = 1;
+    __hip_atomic_fetch_add(count, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    unsigned spins = 0;
+    // wrap-safe: the count may pass 2^32 in very long solves
+    while (static_cast<int>(target - __hip_atomic_load(count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) > 0) {
+      __builtin_amdgcn_s_sleep(2);
+      if (++spins > kBarrierSpinLimit) {
+        __hip_atomic_store(timeout, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        ok = 0;
+        break;
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    *s_ok = ok;
+  }
+  __syncthreads();
+  return *s_ok != 0;
+}
+
+// The whole solve in one cooperative launch: the iterations of cg_direction
+// and cg_update with a grid barrier in place of each kernel boundary.  Every
+// workgroup computes the same reduced gamma (same partials, same order), so
+// all of them stop at the same iteration; the per-workgroup arithmetic is
+// that of the two-kernel form, so results are bitwise the same.
+template <int N, int RB, bool HELM>
+__global__ __launch_bounds__(kThreads) void cg_persistent(FvmParams p, CgBuffers c, float rtol, float atol,
+                                                          int maxiter, float coef) {
+  __shared__ float red[3 * kWaves];
+  __shared__ float pt[(RB + 2) * N];
+  __shared__ int s_ok;
+  unsigned* count = reinterpret_cast<unsigned*>(&c.status[2]);
+  unsigned target = 0;
+  for (int k = 0;; ++k) {
+    if (direction_body<N, RB, HELM>(p, c, k, rtol, atol, maxiter, coef, red, pt)) break;
+    target += gridDim.x;
+    if (!grid_barrier(count, &c.status[3], target, &s_ok)) break;
+    update_body<N, RB>(c, k, red);
+    target += gridDim.x;
+    if (!grid_barrier(count, &c.status[3], target, &s_ok)) break;
+  }
+}
+
 template <int N> struct CgRows { static constexpr int RB = N >= 512 ? N / 256 : 1; };
 
 template <int N>
@@ -279,6 +348,26 @@ struct CgOps {
                             hipStream_t s) {
     hipLaunchKernelGGL((cg_init_helmholtz<N, RB>), dim3(N / RB), dim3(kThreads), 0, s, p, b, c, coef);
     return check_hip(hipGetLastError(), "cg_init_helmholtz");
+  }
+  // The whole solve as one cooperative launch (N / RB workgroups, all
+  // resident).  JCFD_ERR_UNSUPPORTED when the device cannot hold them.
+  static int persistent(const FvmParams& p, const CgBuffers& c, float rtol, float atol, int maxiter,
+                        bool helm, float coef, hipStream_t s) {
+    const void* fn = helm ? reinterpret_cast<const void*>(&cg_persistent<N, RB, true>)
+                          : reinterpret_cast<const void*>(&cg_persistent<N, RB, false>);
+    int dev = 0, coop = 0, cus = 0, per_cu = 0, rc;
+    if ((rc = check_hip(hipGetDevice(&dev), "hipGetDevice")) ||
+        (rc = check_hip(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev),
+                        "cooperative launch attribute")) ||
+        (rc = check_hip(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "CU count")) ||
+        (rc = check_hip(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kThreads, 0), "occupancy")))
+      return rc;
+    if (!coop || per_cu * cus < N / RB) return JCFD_ERR_UNSUPPORTED;
+    FvmParams pp = p;
+    CgBuffers cc = c;
+    void* args[] = {&pp, &cc, &rtol, &atol, &maxiter, &coef};
+    return check_hip(hipLaunchCooperativeKernel(fn, dim3(N / RB), dim3(kThreads), args, 0, s),
+                     "cg_persistent");
   }
   static int iteration_helmholtz(const FvmParams& p, const CgBuffers& c, int k, float rtol, float atol,
                                  int maxiter, float coef, hipStream_t s) {
@@ -315,6 +404,19 @@ int cg_launch_iteration(const FvmParams& p, const CgBuffers& c, int k, float rto
     case 256: return CgOps<256>::iteration(p, c, k, rtol, atol, maxiter, s);
     case 512: return CgOps<512>::iteration(p, c, k, rtol, atol, maxiter, s);
     case 1024: return CgOps<1024>::iteration(p, c, k, rtol, atol, maxiter, s);
+    default: return fail(JCFD_ERR_UNSUPPORTED, "CG grid size not built");
+  }
+}
+
+int cg_launch_persistent(const FvmParams& p, const CgBuffers& c, float rtol, float atol, int maxiter,
+                         bool helm, float coef, hipStream_t s) {
+  switch (p.n) {
+    case 32: return CgOps<32>::persistent(p, c, rtol, atol, maxiter, helm, coef, s);
+    case 64: return CgOps<64>::persistent(p, c, rtol, atol, maxiter, helm, coef, s);
+    case 128: return CgOps<128>::persistent(p, c, rtol, atol, maxiter, helm, coef, s);
+    case 256: return CgOps<256>::persistent(p, c, rtol, atol, maxiter, helm, coef, s);
+    case 512: return CgOps<512>::persistent(p, c, rtol, atol, maxiter, helm, coef, s);
+    case 1024: return CgOps<1024>::persistent(p, c, rtol, atol, maxiter, helm, coef, s);
     default: return fail(JCFD_ERR_UNSUPPORTED, "CG grid size not built");
   }
 }

@@ -93,11 +93,16 @@ struct CgBuffers {
   float* part_rr[2];   // per-block partial sums of r.r (by parity)
   float* part_pap;     // per-block partial sums of p.Ap
   float* part_bb;      // per-block partial sums of b.b
-  int* status;         // [0] done flag, [1] iterations run
+  int* status;         // [0] done flag, [1] iterations run, [2] grid-barrier arrivals, [3] barrier timeout
 };
 int cg_blocks(int n);
 int cg_launch_init(const FvmParams& p, const float* rhs, const CgBuffers& c, float rtol, float atol,
                    int maxiter, hipStream_t s);
+// The whole solve (after cg_launch_init / cg_launch_init_helmholtz) as one
+// cooperative launch; JCFD_ERR_UNSUPPORTED (no error recorded) when the
+// device cannot keep all its workgroups resident.
+int cg_launch_persistent(const FvmParams& p, const CgBuffers& c, float rtol, float atol, int maxiter,
+                         bool helm, float coef, hipStream_t s);
 // One CG iteration k (two kernels); a no-op once converged.
 int cg_launch_iteration(const FvmParams& p, const CgBuffers& c, int k, float rtol, float atol,
                         int maxiter, hipStream_t s);

@@ -282,13 +282,13 @@ struct WavenumberStage {
 };
 
 
-// Minimum waves per SIMD asked of the column kernels: 6 for the 512-point
-// tiles of one or two columns, which brings their registers from ~86 to ~78
-// without spills (6 rather than 5 workgroups per CU; ~3% faster kernels);
-// other shapes keep the compiler's choice (wider tiles would spill).
+// Minimum waves per SIMD asked of the column kernels: 5 for the 512-point
+// tiles of one or two columns, the most their LDS (inverse rounds of two or
+// four spectra) allows; other shapes keep the compiler's choice (wider
+// tiles would spill).
 template <int N, int W>
 struct NsColWaves {
-  static constexpr int value = (N == 512 && W <= 2) ? 6 : 1;
+  static constexpr int value = (N == 512 && W <= 2) ? 5 : 1;
 };
 
 // One column tile: the state's W columns from c0 -> the inverse transforms of

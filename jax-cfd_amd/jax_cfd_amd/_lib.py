@@ -122,6 +122,7 @@ _SIGNATURES = {
     "jcfd_ns2d_plan_device_bytes": (ctypes.c_int64, [_P]),
     "jcfd_ns2d_plan_set_graphs": (ctypes.c_int, [_P, ctypes.c_int32]),
     "jcfd_fvm_plan_set_graphs": (ctypes.c_int, [_P, ctypes.c_int32]),
+    "jcfd_fvm_plan_set_cg_persistent": (ctypes.c_int, [_P, ctypes.c_int32]),
     "jcfd_ns2d_step": (ctypes.c_int, [_P, _P, _P, ctypes.c_int64, _P]),
     "jcfd_ns2d_explicit_terms": (ctypes.c_int, [_P, _P, _P, _P]),
     "jcfd_ns2d_time_kernels": (ctypes.c_int, [_P, _P, ctypes.c_int32,

@@ -112,6 +112,12 @@ class FvmPlan:
     _lib.check(self._lib.jcfd_fvm_plan_set_graphs(self._handle, int(bool(enabled))),
                'jcfd_fvm_plan_set_graphs')
 
+  def set_cg_persistent(self, enabled: bool):
+    """CG solves as one cooperative launch, or two launches per iteration
+    (default, faster on MI355X); same results."""
+    _lib.check(self._lib.jcfd_fvm_plan_set_cg_persistent(self._handle, int(bool(enabled))),
+               'jcfd_fvm_plan_set_cg_persistent')
+
   @property
   def tiles(self) -> dict:
     """Tile shapes and radix schedule of the step kernels."""

@@ -553,6 +553,8 @@ def poisson_requests(count: int, rate: float = 200.0, seed: int = 0,
                      sizes=(64, 128, 256, 512), steps=(100, 2000),
                      solvers=('fvm', 'spectral')) -> List[RolloutRequest]:
   """Seeded Poisson arrivals with uniformly drawn solver, size and steps."""
+  if count <= 0:
+    return []
   rng = np.random.default_rng(seed)
   gaps = rng.exponential(1.0 / rate, count)
   arrivals = np.cumsum(gaps) - gaps[0]
