@@ -382,7 +382,6 @@ __global__ __launch_bounds__(kThreads) void rows_c2r(NsParams q, const float2* _
 template <int N, int RB>
 struct RowsShared {
   float2 rows[4 * RB * (N / 2)];  // transform t = m * RB + r
-  float2 prod[RB * (N / 2)];
   float2 twn[N];
   float2 twh[N / 2];
 };
@@ -396,7 +395,9 @@ __device__ __forceinline__ void rows_tile(RowsShared<N, RB>& sm, const NsParams&
   constexpr int H = N / 2;
   const size_t gstride = static_cast<size_t>(N) * H;
   float2* rows = sm.rows;
-  float2* prod = sm.prod;
+  // the product spectrum overwrites the first transform's rows (gx, element
+  // by element: each element is read and written by its owner only)
+  float2* prod = sm.rows;
   float2* twn = sm.twn;
   float2* twh = sm.twh;
   // Load the four spectra's rows in chunks (all loads of a chunk issued
@@ -828,7 +829,7 @@ constexpr bool ns_cols_ok() {
 }
 template <int N, int RB>
 constexpr bool ns_rows_ok() {
-  return RB <= N / 4 && (5 * RB * (N / 2) + N + N / 2) * 8 <= kNsLdsMax;
+  return RB <= N / 4 && (4 * RB * (N / 2) + N + N / 2) * 8 <= kNsLdsMax;
 }
 
 // which: 0 cols-inverse, 1 rows, 2 cols-forward + update + next inverse,
