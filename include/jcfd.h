@@ -261,7 +261,7 @@ int jcfd_fvm_spectral_filter(jcfd_fvm_plan* plan, const jcfd_tensor* x,
 
 /* Mixed-size groups: one step of many FVM instances of different grid sizes
  * (and different physics), each of them a slot of some plan, as ONE
- * persistent launch per kernel class.  The reference steps one field per
+ * launch per kernel class (one workgroup per work item).  The reference steps one field per
  * call (funcutils.repeated over step_fn, funcutils.py:82-88; a batch of
  * same-size fields would be jax.vmap of that); there is no counterpart for
  * uneven sizes.  Every instance gets exactly the arithmetic of jcfd_fvm_step

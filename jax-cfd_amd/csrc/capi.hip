@@ -1850,11 +1850,8 @@ int jcfd_fvm_group_step(jcfd_fvm_group* g, const jcfd_fvm_group_member* members,
     if ((rc = group_upload(g, table, n_desc, counts, s))) return rc;
     g->last = table;
   }
-  if (max_n != g->max_n) {
-    for (int k = 0; k < 3; ++k)
-      if ((rc = fvm_group_grid(k, max_n, &g->grid[k]))) return rc;
-    g->max_n = max_n;
-  }
+  g->max_n = max_n;
+  for (int k = 0; k < 3; ++k) g->grid[k] = counts[k];  // one workgroup per item
   for (int k = 0; k < 3; ++k) g->n_items[k] = counts[k];
   g->bytes[0] = g->bytes[1] = g->bytes[2] = 0;
   for (const FvmGroupDesc& d : descs) {
@@ -1871,7 +1868,11 @@ int jcfd_fvm_group_step(jcfd_fvm_group* g, const jcfd_fvm_group_member* members,
     }
     return static_cast<int>(JCFD_OK);
   };
-  return run_steps(g->graphs, g->descs, g->items, nullptr, max_n, nsteps, s, one_step);
+  // captured launches bake in the grids (the item counts) and LDS size
+  const void* count_key = reinterpret_cast<const void*>(
+      (static_cast<uintptr_t>(static_cast<uint32_t>(counts[0])) << 42) |
+      (static_cast<uintptr_t>(static_cast<uint32_t>(counts[1])) << 21) | static_cast<uint32_t>(counts[2]));
+  return run_steps(g->graphs, g->descs, g->items, count_key, max_n, nsteps, s, one_step);
 }
 
 int jcfd_fvm_group_time_kernels(jcfd_fvm_group* g, int32_t reps, double* ms_per_launch, void* stream) {

@@ -221,9 +221,7 @@ constexpr int kFvmGroupMaxN = 512;
 bool fvm_group_size_supported(int n);
 // Work items per instance of grid size n in `phase`.
 int fvm_group_tiles(int n, int phase);
-// Resident workgroups of `phase` on this device when the largest grid is max_n.
-int fvm_group_grid(int phase, int max_n, int* grid);
-// Launch one phase.
+// Launch one phase: grid = the phase's item count (one workgroup per item).
 int fvm_group_launch(int phase, int max_n, const FvmGroupQueue& q, int grid, hipStream_t s);
 
 // Learned interpolation (ml.hip).  The tower's output layer carries the

@@ -292,8 +292,8 @@ class FvmPlan:
 
 
 class FvmGroup:
-  """Mixed-size group stepping (jcfd_fvm_group_*): one persistent launch per
-  kernel class advances every listed slot of every member plan, whatever its
+  """Mixed-size group stepping (jcfd_fvm_group_*): one launch per kernel
+  class advances every listed slot of every member plan, whatever its
   grid size, each with exactly the arithmetic of a lone step."""
 
   def __init__(self, device: Optional[torch.device] = None):

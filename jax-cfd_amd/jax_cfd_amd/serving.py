@@ -366,8 +366,8 @@ class Pool:
 class PoolGroup:
   """The pools of one solver advanced together: each chunk is ONE group step
   (jcfd_fvm_group_step / jcfd_ns2d_group_step) over the active slots of
-  every member pool, whatever their grid sizes -- one persistent launch per
-  kernel class instead of one launch sequence per pool.  The chunk is the
+  every member pool, whatever their grid sizes -- one launch per kernel
+  class instead of one launch sequence per pool.  The chunk is the
   smallest next_chunk of the member pools, so all of them stop together;
   the pools share the group's stream and completion event.  Each slot still
   gets exactly the arithmetic of a lone instance."""
