@@ -117,26 +117,18 @@ std::vector<float2> twiddles(int n) {
   return tw;
 }
 
-// Lambda^+ / n^2 for the periodic 5-point Laplacian on an n x n grid
-// (fast_diagonalization.py:199-266 with array_utils.py:168-173): eigenvalues
-// (2 cos(2 pi p / n) - 2) / h^2 per axis in fp64, rfft layout on the last axis,
-// zero where |lambda| <= 10 * eps(float32); the irfftn 1/n^2 is folded in.
-// Stored column by column (entry (p, c) at c * n + p): the Poisson column
-// kernel's tiles then read contiguous runs of it.
-std::vector<float> poisson_multiplier(int n, double dx, double dy) {
-  const int h1 = n / 2 + 1;
-  const double cutoff = static_cast<double>(10.0f * 1.1920928955078125e-07f);
-  std::vector<float> lam(static_cast<size_t>(n) * h1);
-  for (int p = 0; p < n; ++p) {
-    const double lx = (-2.0 / (dx * dx)) + 2.0 * std::cos(2.0 * M_PI * p / n) / (dx * dx);
-    for (int c = 0; c < h1; ++c) {
-      const double ly = (-2.0 / (dy * dy)) + 2.0 * std::cos(2.0 * M_PI * c / n) / (dy * dy);
-      const double s = lx + ly;
-      const double inv = (std::fabs(s) > cutoff) ? 1.0 / s : 0.0;
-      lam[static_cast<size_t>(c) * n + p] = static_cast<float>(inv / (static_cast<double>(n) * n));
-    }
-  }
-  return lam;
+// Eigenvalues (2 cos(2 pi p / n) - 2) / h^2 of the periodic 5-point
+// Laplacian per axis, in fp64 (fast_diagonalization.py:199-266 with
+// array_utils.py:168-173): x for p in [0, n), then y for c in [0, n/2]
+// (the rfft layout of the last axis).  The Poisson column kernel forms
+// Lambda^+ / n^2 from their sums, as a table of it would hold it.
+std::vector<double> laplacian_eigenvalues(int n, double dx, double dy) {
+  std::vector<double> e(static_cast<size_t>(n) + n / 2 + 1);
+  for (int p = 0; p < n; ++p)
+    e[p] = (-2.0 / (dx * dx)) + 2.0 * std::cos(2.0 * M_PI * p / n) / (dx * dx);
+  for (int c = 0; c <= n / 2; ++c)
+    e[n + c] = (-2.0 / (dy * dy)) + 2.0 * std::cos(2.0 * M_PI * c / n) / (dy * dy);
+  return e;
 }
 
 // D / n^2 for the implicit diffusion solve (diffusion.py:180-185 through
@@ -319,7 +311,7 @@ struct jcfd_fvm_plan {
   float* us = nullptr;
   float* vs = nullptr;
   float2* spec = nullptr;
-  float* lam = nullptr;
+  double* lam_xy = nullptr;  // Laplacian eigenvalues: x (n), then y (n/2 + 1)
   float2* tw = nullptr;
   float* fx = nullptr;
   float* dlam = nullptr;  // implicit diffusion multiplier / n^2 (diffusion == implicit)
@@ -385,8 +377,8 @@ FvmBuffers fvm_buffers(const jcfd_fvm_plan* p) {
   b.vs = p->vs;
   b.spec = p->spec;
   b.fx = p->fx;
-  b.lam = p->lam;
-  b.lam_cols = 1;
+  b.lam_xy = p->lam_xy;
+  b.lam_mode = 2;
   b.tw = p->tw;
   return b;
 }
@@ -668,7 +660,7 @@ int jcfd_fvm_plan_create(const jcfd_fvm_config* config, jcfd_fvm_plan** plan) {
   int rc = JCFD_OK;
   if ((rc = device_alloc(&p->us, nn, &p->bytes)) || (rc = device_alloc(&p->vs, nn, &p->bytes)) ||
       (rc = device_alloc(&p->spec, nn / 2, &p->bytes)) ||
-      (rc = device_alloc(&p->lam, static_cast<size_t>(n) * (n / 2 + 1), &p->bytes)) ||
+      (rc = device_alloc(&p->lam_xy, static_cast<size_t>(n) + n / 2 + 1, &p->bytes)) ||
       (rc = device_alloc(&p->tw, static_cast<size_t>(n), &p->bytes)) ||
       (rc = device_alloc(&p->fx, static_cast<size_t>(n), &p->bytes)) ||
       ((c.pressure == JCFD_PRESSURE_CG || c.diffusion == JCFD_DIFFUSION_IMPLICIT_CG) &&
@@ -681,12 +673,12 @@ int jcfd_fvm_plan_create(const jcfd_fvm_config* config, jcfd_fvm_plan** plan) {
     jcfd_fvm_plan_destroy(p);
     return rc;
   }
-  const std::vector<float> lam = poisson_multiplier(n, c.dx, c.dy);
+  const std::vector<double> lam = laplacian_eigenvalues(n, c.dx, c.dy);
   const std::vector<float2> tw = twiddles(n);
   std::vector<float> fx(n, 0.0f);
   if (c.has_forcing && c.forcing_x != nullptr) std::memcpy(fx.data(), c.forcing_x, n * sizeof(float));
-  if ((rc = check_hip(hipMemcpy(p->lam, lam.data(), lam.size() * sizeof(float), hipMemcpyHostToDevice),
-                      "copy Lambda")) ||
+  if ((rc = check_hip(hipMemcpy(p->lam_xy, lam.data(), lam.size() * sizeof(double), hipMemcpyHostToDevice),
+                      "copy Laplacian eigenvalues")) ||
       (rc = check_hip(hipMemcpy(p->tw, tw.data(), tw.size() * sizeof(float2), hipMemcpyHostToDevice),
                       "copy twiddles")) ||
       (rc = check_hip(hipMemcpy(p->fx, fx.data(), fx.size() * sizeof(float), hipMemcpyHostToDevice),
@@ -743,7 +735,7 @@ int jcfd_fvm_plan_create(const jcfd_fvm_config* config, jcfd_fvm_plan** plan) {
 int jcfd_fvm_plan_destroy(jcfd_fvm_plan* p) {
   if (p == nullptr) return JCFD_OK;
   for (void* ptr : {static_cast<void*>(p->us), static_cast<void*>(p->vs), static_cast<void*>(p->spec),
-                    static_cast<void*>(p->lam), static_cast<void*>(p->tw), static_cast<void*>(p->fx),
+                    static_cast<void*>(p->lam_xy), static_cast<void*>(p->tw), static_cast<void*>(p->fx),
                     static_cast<void*>(p->dlam), static_cast<void*>(p->kbuf), static_cast<void*>(p->sbuf),
                     static_cast<void*>(p->rk_terms)})
     if (ptr != nullptr) (void)hipFree(ptr);
@@ -1007,7 +999,7 @@ int jcfd_fvm_projection(jcfd_fvm_plan* p, const jcfd_tensor* u, const jcfd_tenso
     return fvm_launch_project_q(p->params, b, s);
   }
   return fvm_launch_projection(p->params, static_cast<const float*>(pu), static_cast<const float*>(pv),
-                               static_cast<float*>(pu2), static_cast<float*>(pv2), p->spec, p->lam,
+                               static_cast<float*>(pu2), static_cast<float*>(pv2), p->spec, p->lam_xy,
                                p->tw, s);
 }
 
@@ -1072,8 +1064,8 @@ int jcfd_fvm_time_kernels(jcfd_fvm_plan* p, const jcfd_tensor* u, const jcfd_ten
   b.u_out = p->us;  // rows-project reads and writes the same element per thread
   b.v_out = p->vs;
   b.fx = p->fx;
-  b.lam = p->lam;
-  b.lam_cols = 1;
+  b.lam_xy = p->lam_xy;
+  b.lam_mode = 2;
   b.tw = p->tw;
   hipEvent_t e0, e1;
   if ((rc = check_hip(hipEventCreate(&e0), "event")) ) return rc;
@@ -1131,7 +1123,7 @@ int jcfd_fvm_step_vjp(jcfd_fvm_plan* p, const jcfd_tensor* u, const jcfd_tensor*
   hipStream_t s = static_cast<hipStream_t>(stream);
   // lam* = P lam into the plan's u*/v* scratch, then the explicit-term transpose.
   if ((rc = fvm_launch_projection(p->params, static_cast<const float*>(plu),
-                                  static_cast<const float*>(plv), p->us, p->vs, p->spec, p->lam,
+                                  static_cast<const float*>(plv), p->us, p->vs, p->spec, p->lam_xy,
                                   p->tw, s)))
     return rc;
   return fvm_launch_adjoint_explicit(p->params, static_cast<const float*>(pu),

@@ -627,6 +627,23 @@ __global__ __launch_bounds__(kThreads) void fvm_rows_forward(FvmParams p, FvmBuf
   }
 }
 
+// Entry (p, c) of the column pass's spectrum multiplier (see lam_mode).  For
+// Lambda^+ it is computed as the host tabulated it -- fp64 eigenvalue sum,
+// pseudo-inverse where |s| > 10 eps(fp32) (fast_diagonalization.py:199-266
+// with array_utils.py:168-173), / n^2, rounded to fp32; every operation is
+// IEEE, so the bits equal the table's -- which saves the kernel a table read
+// per spectrum word.
+constexpr double kLamCutoff = static_cast<double>(10.0f * 1.1920928955078125e-07f);
+template <int N>
+__device__ __forceinline__ float lam_value(const FvmBuffers& b, int p, int c) {
+  if (b.lam_mode == 2) {
+    const double s = b.lam_xy[p] + b.lam_xy[N + c];
+    const double inv = (fabs(s) > kLamCutoff) ? 1.0 / s : 0.0;
+    return static_cast<float>(inv / (static_cast<double>(N) * N));
+  }
+  return b.lam_mode == 1 ? b.lam[c * N + p] : b.lam[p * (N / 2 + 1) + c];
+}
+
 // Column pass of the Poisson solve on the tile of W packed-spectrum columns
 // starting at c0 (b already points at the instance).
 template <int N, int W, int R = 8>
@@ -648,7 +665,7 @@ __device__ __forceinline__ void poisson_cols_body(const FvmBuffers& b, float sca
   for (int q = 0; q < PER; ++q) {
     const int e0 = tid + q * kThreads;
     const int e = e0 < N * W ? e0 : N * W - 1;
-    lam[q] = b.lam_cols ? b.lam[(c0 + e % W) * N + e / W] : b.lam[(e / W) * (H + 1) + c0 + e % W];
+    lam[q] = lam_value<N>(b, e / W, c0 + e % W);
   }
   tsn.store(twn, tid);
   __syncthreads();
@@ -667,7 +684,7 @@ __device__ __forceinline__ void poisson_cols_body(const FvmBuffers& b, float sca
         const float2 d = csub(y, ym);
         const float2 fh = make_float2(0.5f * d.y, -0.5f * d.x);
         const float l0 = lam[q] * scale;
-        const float lh = (b.lam_cols ? b.lam[H * N + pp] : b.lam[pp * (H + 1) + H]) * scale;
+        const float lh = lam_value<N>(b, pp, H) * scale;
         const float2 g0 = cscale(f0, l0), gh = cscale(fh, lh);
         nv[q] = make_float2(g0.x - gh.y, g0.y + gh.x);  // g0 + i gh
       } else {
@@ -967,7 +984,7 @@ int launch_diffuse(const FvmParams& p, const FvmBuffers& b, const float* mult, h
     f.u = field;
     f.u_out = field;
     f.lam = mult;
-    f.lam_cols = 1;  // the plan's multiplier table
+    f.lam_mode = 1;  // the plan's multiplier table, column by column
     hipLaunchKernelGGL((fvm_rows_forward<N, RB, 3>), dim3(N / RB, nb(f)), dim3(kThreads), 0, s, p, f);
     hipLaunchKernelGGL((fvm_poisson_cols<N, W>), dim3(N / 2 / W, nb(f)), dim3(kThreads), 0, s, f, 1.0f);
     hipLaunchKernelGGL((fvm_rows_project<N, RB, 3>), dim3(N / RB, nb(f)), dim3(kThreads), 0, s, p, f);
@@ -1112,6 +1129,7 @@ __global__ __launch_bounds__(kThreads) void fvm_group_phase(FvmGroupQueue q) {
   d.b.v_out = as_global(d.b.v_out);
   d.b.fx = as_global(d.b.fx);
   d.b.lam = as_global(d.b.lam);
+  d.b.lam_xy = as_global(d.b.lam_xy);
   d.b.tw = as_global(d.b.tw);
   d.b.slots = as_global(d.b.slots);
   d.b.rhs = as_global(d.b.rhs);
@@ -1247,7 +1265,7 @@ int fvm_launch_explicit_terms(const FvmParams& p, const float* u, const float* v
 }
 
 int fvm_launch_projection(const FvmParams& p, const float* u, const float* v, float* u_out,
-                          float* v_out, float2* spec, const float* lam, const float2* tw,
+                          float* v_out, float2* spec, const double* lam_xy, const float2* tw,
                           hipStream_t s) {
   FvmBuffers b{};
   b.u = u;
@@ -1257,8 +1275,8 @@ int fvm_launch_projection(const FvmParams& p, const float* u, const float* v, fl
   b.u_out = u_out;
   b.v_out = v_out;
   b.spec = spec;
-  b.lam = lam;
-  b.lam_cols = 1;  // the plan's Lambda^+ table
+  b.lam_xy = lam_xy;
+  b.lam_mode = 2;  // Lambda^+ from the plan's eigenvalues
   b.tw = tw;
   return dispatch_n<ProjOp>(p.n, p, b, s);
 }

@@ -70,8 +70,13 @@ struct FvmBuffers {
   float* u_out;          // state out
   float* v_out;
   const float* fx;       // forcing column f_x(y_j), n floats (may be null)
-  const float* lam;      // Lambda^+ / n^2, (n, n/2 + 1) floats (spectrum multiplier)
-  int lam_cols;          // lam stored column by column (entry (p, c) at c * n + p)
+  // Spectrum multiplier of the Poisson column kernel, entry (p, c) of an
+  // (n, n/2 + 1) array: lam_mode 0 a table lam, row-major; 1 the table lam
+  // stored column by column (c * n + p); 2 Lambda^+ / n^2 computed from
+  // lam_xy (x eigenvalues, n doubles, then y eigenvalues, n/2 + 1).
+  const float* lam;
+  const double* lam_xy;
+  int lam_mode;
   const float2* tw;      // twiddles exp(-2 pi i k / n), n entries
   // Batches: instance k of a launch (blockIdx.y = k) is slots[k], or k when
   // slots is null; fields of instance s start at s * n * n (spectrum:
@@ -173,9 +178,9 @@ int fvm_launch_rk_project(const FvmParams& p, const FvmBuffers& b, hipStream_t s
 int fvm_launch_kernel(const FvmParams& p, const FvmBuffers& b, int which, hipStream_t s);
 int fvm_launch_explicit_terms(const FvmParams& p, const float* u, const float* v,
                               float* du, float* dv, const float* fx, hipStream_t s);
-// lam: the plan's Lambda^+ table (column by column).
+// lam_xy: the plan's Laplacian eigenvalues (x, then y), for Lambda^+.
 int fvm_launch_projection(const FvmParams& p, const float* u, const float* v, float* u_out,
-                          float* v_out, float2* spec, const float* lam, const float2* tw,
+                          float* v_out, float2* spec, const double* lam_xy, const float2* tw,
                           hipStream_t s);
 int fvm_launch_divergence(const FvmParams& p, const float* u, const float* v, float* out,
                           hipStream_t s);
