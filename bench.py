@@ -74,6 +74,9 @@ def _args():
   ap.add_argument("--trace-summary", metavar="DIR",
                   help="per-launch-configuration kernel statistics of a rocprofv3 "
                        "--kernel-trace run under DIR, then exit")
+  ap.add_argument("--trace-filter", metavar="REGEX",
+                  help="with --trace-summary: every launch configuration matching REGEX "
+                       "instead of the 40 largest")
   return ap.parse_args()
 
 
@@ -146,11 +149,14 @@ def _traffic(key):
   return None if k is None else round(k["fetch_bytes"] + k["write_bytes"])
 
 
-def _trace_summary(root, top=40):
+def _trace_summary(root, top=40, pattern=None):
   """Markdown table of kernel durations per launch configuration (kernel
   name with template arguments, grid size in work-items) from rocprofv3
-  --kernel-trace CSV files under `root`, largest total time first."""
+  --kernel-trace CSV files under `root`, largest total time first: the
+  `top` largest, or every configuration whose key matches the regular
+  expression `pattern`."""
   import csv
+  import re
   per = {}
   for dirpath, _, files in os.walk(root):
     for fn in files:
@@ -168,7 +174,9 @@ def _trace_summary(root, top=40):
   total = sum(sum(v) for v in per.values()) or 1
   lines = ["| kernel (launch) | calls | avg ns | min ns | max ns | % of GPU time |",
            "|---|---|---|---|---|---|"]
-  for key, d in sorted(per.items(), key=lambda kv: -sum(kv[1]))[:top]:
+  ranked = sorted(per.items(), key=lambda kv: -sum(kv[1]))
+  ranked = [kv for kv in ranked if re.search(pattern, kv[0])] if pattern else ranked[:top]
+  for key, d in ranked:
     lines.append(f"| `{key}` | {len(d)} | {sum(d) / len(d):.0f} | {min(d)} | {max(d)} | "
                  f"{100.0 * sum(d) / total:.2f} |")
   return "\n".join(lines)
@@ -808,7 +816,7 @@ def main():
     print(json.dumps(_pmc_summary(*args.pmc_summary, args.copy_mib), indent=1))
     return
   if args.trace_summary:
-    print(_trace_summary(args.trace_summary))
+    print(_trace_summary(args.trace_summary, pattern=args.trace_filter))
     return
   if args.gpus != 1:
     print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported here "
