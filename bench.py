@@ -51,7 +51,7 @@ def _args():
   ap.add_argument("--served-requests", type=int, default=400,
                   help="requests of the served Poisson stream (the headline value; "
                        "0 reports the plain leg instead)")
-  ap.add_argument("--served-group", nargs="*", default=["fvm"], choices=["fvm", "spectral"],
+  ap.add_argument("--served-group", nargs="*", default=["fvm", "spectral"], choices=["fvm", "spectral"],
                   help="solvers whose pools the headline server groups into one "
                        "mixed-size group step per chunk")
   ap.add_argument("--served-variants", type=int, default=1,

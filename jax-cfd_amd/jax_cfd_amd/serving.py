@@ -27,16 +27,18 @@ until other pools drain, and a request that can never fit is rejected.
 Every slot gets exactly the arithmetic of a lone instance (the kernels never
 mix instances), so a request's result is bit-identical alone or in a mix.
 
-Grouping (`group`, default ('fvm',)): the pools of a grouped solver share one
-stream and advance together, each chunk ONE mixed-size group step
+Grouping (`group`, default both solvers): the pools of a grouped solver
+share one stream and advance together, each chunk ONE mixed-size group step
 (jcfd_fvm_group_step / jcfd_ns2d_group_step) over the active slots of all of
 them -- one launch per kernel class instead of one launch sequence per pool;
-the chunk is then the smallest over the solver's pools.  Results are
-unchanged (bit for bit).  On the mixed 400-request stream, grouping the FVM
-pools raised throughput 1.30e10 -> 1.45e10 cell-steps/s and cut p99 latency
-1.27 s -> 0.80 s; grouping the spectral pools too lowered their share (their
-group kernels hold more registers than the per-size ones, and the common
-chunk shortens every pool's chunks), so they stay per-pool by default.
+the chunk is then the smallest over the solver's pools.  Pools above the
+family's GROUP_MAX_N (spectral: 256) keep their own stream.  Results are
+unchanged (bit for bit).  On the mixed 400-request stream (three runs each):
+per-pool FVM and spectral launches with FVM grouped 1.48-1.51e10
+cell-steps/s, p50 0.30-0.34 s, p99 0.74-1.0 s; with the spectral pools up
+to 256^2 grouped too 1.54e10, p50 0.20-0.22 s, p99 0.63-0.70 s, spectral
+share 6.6 -> 6.8e9 cell-steps/s.  Grouping the 512^2 spectral pool as well
+lowered the spectral share (its long launches set the grouped pools' pace).
 
 Trajectories: a request with `save_every = k > 0` also gets the frames of
 funcutils.trajectory(repeated(step_fn, k), steps // k) -- the state after
@@ -202,7 +204,10 @@ class SpectralFamily:
   def advance(self, plan, state, slots, nsteps):
     plan.step_slots(state[0], slots, nsteps)
 
-  GROUP_MAX_N = 1024  # largest grid jcfd_ns2d_group_step runs
+  # Largest pool a server groups (jcfd_ns2d_group_step runs up to 1024^2):
+  # the 512^2 pool's long launches would set every grouped spectral pool's
+  # pace, so it keeps its own stream.
+  GROUP_MAX_N = 256
 
   def make_group(self):
     return _ns2d.Ns2dGroup(self.device)
@@ -410,7 +415,7 @@ class RolloutServer:
   def __init__(self, hbm_budget_bytes: float = 200e9,
                capacity: Optional[Dict[int, int]] = None, max_chunk: int = 16,
                device: Optional[torch.device] = None, tune: bool = False,
-               group: Tuple[str, ...] = ('fvm',)):
+               group: Tuple[str, ...] = ('fvm', 'spectral')):
     self.tune = bool(tune)
     unknown = set(group) - set(FAMILIES)
     if unknown:
