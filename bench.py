@@ -657,9 +657,10 @@ def _plain_fvm(dev, steps, warmup, kernel_reps, min_seconds=0.2):
 
 
 def _group_mixed(dev, mix, steps, solver="fvm"):
-  """Uneven sizes in one persistent launch per kernel class (jcfd_fvm_group /
-  jcfd_ns2d_group) against the same instances as one batched launch
-  sequence per size on concurrent streams (tuned tiles).  Physics of the
+  """Uneven sizes in one launch per kernel class, one workgroup per work
+  item (jcfd_fvm_group / jcfd_ns2d_group), against the same instances as
+  one batched launch sequence per size on concurrent streams (tuned
+  tiles).  Physics of the
   served families (FVM: the forced 256^2 run; spectral: the forced 512^2
   run) at every size, seeds 0.. per size."""
   import torch

@@ -206,7 +206,8 @@ int fvm_default_tiles(FvmParams* p);
 // 2 rows-project) is built with tile `tile` for size n.
 bool fvm_tile_supported(int n, int which, int tile);
 
-// Mixed-size group step (fvm.hip): one persistent launch per kernel class
+// Mixed-size group step (fvm.hip): one launch per kernel class, one
+// workgroup per work item
 // (phase 0 rows-forward, 1 Poisson columns, 2 rows-project) advances every
 // instance of a device-resident work list, whatever its grid size.
 struct FvmGroupDesc {

@@ -1,5 +1,5 @@
-"""Mixed-size group stepping (jcfd_fvm_group_step): one persistent launch per
-kernel class over instances of every grid size.  Each instance must equal,
+"""Mixed-size group stepping (jcfd_fvm_group_step): one launch per kernel
+class (one workgroup per work item) over instances of every grid size.  Each instance must equal,
 bit for bit, the plain single-instance step of its own plan; slots not
 listed must stay untouched; graph replay must equal eager launches; member
 sets may change between calls."""

@@ -1,5 +1,5 @@
-"""Mixed-size spectral group stepping (jcfd_ns2d_group_step): one persistent
-launch per kernel class and tableau stage over instances of every grid size.
+"""Mixed-size spectral group stepping (jcfd_ns2d_group_step): one launch
+per kernel class and tableau stage (one workgroup per work item) over instances of every grid size.
 Each instance must equal, bit for bit, the plain single-instance step of its
 own plan; slots not listed must stay untouched; graph replay must equal eager
 launches; member sets may change between calls."""
