@@ -1231,7 +1231,7 @@ struct jcfd_li_plan {
   int n = 0, batch = 1, device = 0;
   int n_layers = 0, hidden = 0, n_logits = 0;
   int cin[JCFD_LI_MAX_LAYERS] = {}, cout[JCFD_LI_MAX_LAYERS] = {};
-  float* w[JCFD_LI_MAX_LAYERS] = {};  // device [tap][cin (first layer: 4)][cout]
+  float* w[JCFD_LI_MAX_LAYERS] = {};  // device, li_conv_weight_layout of [tap][cin (first layer: 4)][cout]
   float* b[JCFD_LI_MAX_LAYERS] = {};
   float* w_logits = nullptr;  // the tower's own output layer, padded to 128 channels
   float* b_logits = nullptr;
@@ -1307,7 +1307,8 @@ int li_ready(const jcfd_li_plan* p) {
 
 int li_upload_layer(const float* w, const float* b, int cin, int cin_dev, int cout, int cout_dev,
                     float* wd, float* bd) {
-  // (3, 3, cin, cout) host -> [tap][cin_dev][cout_dev], zero padded
+  // (3, 3, cin, cout) host -> [tap][cin_dev][cout_dev], zero padded, then
+  // the kernel's operand order
   std::vector<float> wh(static_cast<size_t>(9) * cin_dev * cout_dev, 0.f), bh(cout_dev, 0.f);
   for (int t = 0; t < 9; ++t)
     for (int ci = 0; ci < cin; ++ci)
@@ -1319,8 +1320,10 @@ int li_upload_layer(const float* w, const float* b, int cin, int cin_dev, int co
     if (!std::isfinite(x)) return fail(JCFD_ERR_INVALID_ARGUMENT, "weights must be finite");
   for (float x : bh)
     if (!std::isfinite(x)) return fail(JCFD_ERR_INVALID_ARGUMENT, "biases must be finite");
+  std::vector<float> wf(wh.size());
+  li_conv_weight_layout(wh.data(), cin_dev, cout_dev, wf.data());
   int rc;
-  if ((rc = check_hip(hipMemcpy(wd, wh.data(), wh.size() * sizeof(float), hipMemcpyHostToDevice), "copy weights")))
+  if ((rc = check_hip(hipMemcpy(wd, wf.data(), wf.size() * sizeof(float), hipMemcpyHostToDevice), "copy weights")))
     return rc;
   return check_hip(hipMemcpy(bd, bh.data(), bh.size() * sizeof(float), hipMemcpyHostToDevice), "copy biases");
 }

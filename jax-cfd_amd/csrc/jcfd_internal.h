@@ -237,7 +237,7 @@ struct LiConvArgs {
   int n;
   const float* in;    // (batch, cin, n, n); the first layer: u (batch, n, n)
   const float* in_v;  // the first layer: v (batch, n, n); otherwise null
-  const float* w;     // [tap = 3 kh + kw][cin padded to 4 for the first layer][cout]
+  const float* w;     // MFMA operand order (li_conv_weight_layout)
   const float* b;     // [cout]
   float* out;         // (batch, cout, n, n)
 };
@@ -255,6 +255,9 @@ struct LiExplicitArgs {
 };
 bool li_conv_supported(int cin, int cout, bool relu);
 int li_launch_conv(const LiConvArgs& a, int cin, int cout, bool relu, int batch, hipStream_t s);
+// Host: weights [tap = 3 kh + kw][cinp][cout] (cinp = input channels padded
+// to a multiple of 4) -> the device layout li_conv3x3 reads (same size).
+void li_conv_weight_layout(const float* w, int cinp, int cout, float* frag);
 int li_launch_explicit(const FvmParams& p, const LiExplicitArgs& a, int batch, hipStream_t s);
 int li_launch_interpolate(int n, const float* coef, const float* x, int key, float* out, int batch,
                           hipStream_t s);

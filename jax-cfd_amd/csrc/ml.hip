@@ -16,10 +16,14 @@
 //                 W[tap][ci][co] * x[ci][px + tap], + bias, relu.  A
 //                 workgroup owns 4 rows x 32 columns of pixels (one row per
 //                 wavefront) and every output channel; the input tile with
-//                 its one-cell periodic halo stays in LDS for all nine taps,
-//                 the taps' weights pass through LDS one tap at a time (the
-//                 next tap's loaded into registers while the current tap's
-//                 products run).  The tower's output layer is the
+//                 its one-cell periodic halo stays in LDS for all nine taps.
+//                 The weights are stored in MFMA operand order, so each
+//                 wavefront reads its A operands straight from global memory
+//                 (cache-resident: every workgroup reads the same few
+//                 hundred KB), one vector load per lane for four k-steps,
+//                 the next group's loads in flight during the current
+//                 group's products: no barrier after the halo, and LDS holds
+//                 the halo alone.  The tower's output layer is the
 //                 constraint folded in: its 128 channels are the 8
 //                 interpolations' 16 stencil coefficients each (weights
 //                 W . nullspace, bias b . nullspace + constraint bias, formed
@@ -35,6 +39,8 @@
 //
 // Activations are planar per instance: (batch, channels, n, n) fp32.
 #include <hip/hip_runtime.h>
+
+#include <type_traits>
 
 #include "fvm_terms.h"
 #include "jcfd_fft.h"
@@ -55,24 +61,40 @@ constexpr int kHaloW = kConvCols + 2;
 constexpr int kHaloPlane = (kConvRows + 2) * kHaloW;  // 204 words
 constexpr int kHaloPitch = 208;  // plane pitch: 4 channels of a k-step land 16 banks apart
 
+// Weight layout of a convolution layer in MFMA operand order.  K-steps of 4
+// input channels are grouped KG at a time (one KG-wide vector per lane);
+// fragment (tap, group g, channel block cb) holds, for lane l = i + 16 k
+// (the A[i][k] lane of v_mfma_f32_16x16x4_f32), the KG weights
+// W[tap][ci = 4 (g KG + kk) + k][co = 16 cb + i], kk = 0 .. KG-1, at
+// float offset (((tap GPT + g) NCB + cb) 64 + l) KG + kk.
 template <int CINP, int COUT>
-struct ConvLds {
-  static constexpr int kWPitch = COUT + 16;  // weight row pitch (same bank spreading)
-  float in[CINP * kHaloPitch];
-  float w[CINP * kWPitch];
+struct ConvFrag {
+  static constexpr int KS = CINP / 4;         // k-steps
+  static constexpr int KG = KS < 4 ? KS : 4;  // k-steps per weight vector
+  static constexpr int GPT = KS / KG;         // groups per tap
+  static constexpr int NCB = COUT / 16;       // output-channel blocks
+  static constexpr int STEPS = 9 * GPT;       // (tap, group) pairs, tap-major
 };
+template <int KG>
+using ConvAVec = std::conditional_t<KG == 4, float4, float>;
+template <int KG>
+__device__ __forceinline__ float conv_acomp(const ConvAVec<KG>& v, int kk) {
+  if constexpr (KG == 4) return kk == 0 ? v.x : (kk == 1 ? v.y : (kk == 2 ? v.z : v.w));
+  else return v;
+}
 
 // v_mfma_f32_16x16x4_f32 operand layout (checked on gfx950): A[i][k] in lane
 // i + 16 k, B[k][j] in lane j + 16 k, D[i][j] in lane j + 16 (i / 4),
 // register i % 4.  Here i = output channel, j = pixel, k = input channel.
+// Every accumulator sums its products tap by tap, input channels in order
+// within a tap.
 template <int CINP, int COUT, bool RELU, bool FIRST>
 __global__ __launch_bounds__(kThreads) void li_conv3x3(LiConvArgs a) {
   static_assert(CINP % 4 == 0 && COUT % 16 == 0, "MFMA blocking");
-  constexpr int NCB = COUT / 16;
-  constexpr int WP = ConvLds<CINP, COUT>::kWPitch;
-  constexpr int WTOT = CINP * COUT;                  // one tap's weights
-  constexpr int WPT = (WTOT + kThreads - 1) / kThreads;
-  __shared__ ConvLds<CINP, COUT> sm;
+  using F = ConvFrag<CINP, COUT>;
+  constexpr int NCB = F::NCB, KG = F::KG, GPT = F::GPT, STEPS = F::STEPS;
+  using AV = ConvAVec<KG>;
+  __shared__ float sin[CINP * kHaloPitch];
   const int n = a.n;
   const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
   const int tiles_x = n / kConvCols;
@@ -82,15 +104,17 @@ __global__ __launch_bounds__(kThreads) void li_conv3x3(LiConvArgs a) {
   const size_t inst = blockIdx.y;
   const float* in = a.in + inst * (FIRST ? 1 : CINP) * plane;
   const float* in_v = FIRST ? a.in_v + inst * plane : nullptr;
-
-  // tap 0's weights go out first, then the halo tile (rows i0-1 .. i0+4,
-  // columns j0-1 .. j0+32, periodic)
-  float wreg[WPT];
+  // fragment (step, cb) of this lane at wf[(step * NCB + cb) * 64]
+  const AV* wf = reinterpret_cast<const AV*>(a.w) + lane;
+  auto load = [&](int s, AV (&w)[NCB]) {
+    const int sc = s < STEPS ? s : STEPS - 1;  // unconditional: the last prefetch repeats
 #pragma unroll
-  for (int k = 0; k < WPT; ++k) {
-    const int e = tid + k * kThreads;
-    wreg[k] = a.w[e < WTOT ? e : WTOT - 1];
-  }
+    for (int cb = 0; cb < NCB; ++cb) w[cb] = wf[(sc * NCB + cb) * 64];
+  };
+  // the first step's weights go out first, then the halo tile (rows
+  // i0-1 .. i0+4, columns j0-1 .. j0+32, periodic)
+  AV wa[NCB], wb[NCB];
+  load(0, wa);
   staged<CINP * kHaloPlane, 16, kThreads>(
       tid,
       [&](int e) {
@@ -99,52 +123,36 @@ __global__ __launch_bounds__(kThreads) void li_conv3x3(LiConvArgs a) {
         if constexpr (FIRST) return c == 0 ? in[g] : (c == 1 ? in_v[g] : 0.f);
         return in[c * plane + g];
       },
-      [&](int e, float x) { sm.in[(e / kHaloPlane) * kHaloPitch + e % kHaloPlane] = x; });
-#pragma unroll
-  for (int k = 0; k < WPT; ++k) {
-    const int e = tid + k * kThreads;
-    if (WTOT % kThreads == 0 || e < WTOT) sm.w[(e / COUT) * WP + e % COUT] = wreg[k];
-  }
+      [&](int e, float x) { sin[(e / kHaloPlane) * kHaloPitch + e % kHaloPlane] = x; });
   __syncthreads();
 
   floatx4 acc[NCB][2];
 #pragma unroll
   for (int cb = 0; cb < NCB; ++cb) acc[cb][0] = acc[cb][1] = floatx4{0.f, 0.f, 0.f, 0.f};
   const int kq = lane / 16, lp = lane % 16;
-#pragma unroll 1
-  for (int tap = 0; tap < 9; ++tap) {
-    if (tap < 8) {
-      const float* wn = a.w + static_cast<size_t>(tap + 1) * WTOT;
+  auto step = [&](int s, const AV (&w)[NCB]) {
+    const int tap = s / GPT, g = s % GPT;
+    const float* bin = sin + (wave + tap / 3) * kHaloW + tap % 3 + lp;
 #pragma unroll
-      for (int k = 0; k < WPT; ++k) {
-        const int e = tid + k * kThreads;
-        wreg[k] = wn[e < WTOT ? e : WTOT - 1];
-      }
-    }
-    const int di = tap / 3, dj = tap % 3;
-    const float* bin = sm.in + (wave + di) * kHaloW + dj + lp;
-#pragma unroll 4
-    for (int ks = 0; ks < CINP / 4; ++ks) {
-      const int ci = ks * 4 + kq;
+    for (int kk = 0; kk < KG; ++kk) {
+      const int ci = (g * KG + kk) * 4 + kq;
       const float b0 = bin[ci * kHaloPitch];
       const float b1 = bin[ci * kHaloPitch + 16];
-      const float* wa = sm.w + ci * WP + lp;
 #pragma unroll
       for (int cb = 0; cb < NCB; ++cb) {
-        const float av = wa[cb * 16];
+        const float av = conv_acomp<KG>(w[cb], kk);
         acc[cb][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, b0, acc[cb][0], 0, 0, 0);
         acc[cb][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, b1, acc[cb][1], 0, 0, 0);
       }
     }
-    if (tap < 8) {
-      __syncthreads();  // every wavefront is done with this tap's weights
-#pragma unroll
-      for (int k = 0; k < WPT; ++k) {
-        const int e = tid + k * kThreads;
-        if (WTOT % kThreads == 0 || e < WTOT) sm.w[(e / COUT) * WP + e % COUT] = wreg[k];
-      }
-      __syncthreads();
-    }
+  };
+  // two register sets in turn, so no copy waits for a load in flight
+#pragma unroll 1
+  for (int s = 0; s < STEPS; s += 2) {
+    load(s + 1, wb);
+    step(s, wa);
+    load(s + 2, wa);
+    if (STEPS % 2 == 0 || s + 1 < STEPS) step(s + 1, wb);
   }
 
   // out[co][i0 + wave][j0 + px] = act(acc + b[co])
@@ -306,6 +314,20 @@ int interpolate_launch(const float* coef, const float* x, int key, float* out, i
 }
 
 }  // namespace
+
+void li_conv_weight_layout(const float* w, int cinp, int cout, float* frag) {
+  // the ConvFrag layout: k-step groups of up to 4, fragments tap-major
+  const int ks = cinp / 4, kg = ks < 4 ? ks : 4, gpt = ks / kg, ncb = cout / 16;
+  for (int tap = 0; tap < 9; ++tap)
+    for (int g = 0; g < gpt; ++g)
+      for (int cb = 0; cb < ncb; ++cb)
+        for (int l = 0; l < 64; ++l)
+          for (int kk = 0; kk < kg; ++kk) {
+            const int ci = (g * kg + kk) * 4 + l / 16, co = cb * 16 + l % 16;
+            frag[(((static_cast<size_t>(tap) * gpt + g) * ncb + cb) * 64 + l) * kg + kk] =
+                w[(static_cast<size_t>(tap) * cinp + ci) * cout + co];
+          }
+}
 
 bool li_conv_supported(int cin, int cout, bool relu) {
   const bool hidden_out = relu && (cout == 16 || cout == 32 || cout == 64);
