@@ -45,6 +45,16 @@ constexpr int kThreads = 256;
 
 __device__ __forceinline__ float sgnf(float x) { return (x > 0.f) ? 1.f : ((x < 0.f) ? -1.f : x); }
 
+// Radix schedule of the packed row transforms (length n/2).  A row kernel
+// transforms a handful of rows per workgroup (4 per marching batch, RB or
+// RB + 1 per row tile), often one per wavefront: radix-8 passes then keep
+// n/16 butterflies per transform busy, half a wavefront's lanes or fewer
+// for n <= 512, where radix-4 passes (n/8 butterflies) issue fewer
+// instructions.  A function of n and the plan's radix only, so every tile
+// shape (and every group shape) gives the same bits.
+template <int N, int R>
+constexpr int kRowRadix = (R == 8 && N <= 512) ? 4 : R;
+
 // Van Leer flux correction (advection.py:284-334 for one face):
 // pre = 0.5 (1 - (dt/h)|U|)|U|;  phi = num (sgn den + sgn num) den / D,
 // D = |den| + |num| if |den| > 0 else 1.
@@ -388,7 +398,7 @@ __device__ __forceinline__ void rows_forward_march_body(const FvmParams& p, cons
     __syncthreads();
     if (vrow && (s - 1) % FB == FB - 1) {
       const int d0 = s - FB;  // first div row of this batch
-      lds_fft_waves<H, FB, kThreads, 1, H, false, R>(zb, twh, tid);
+      lds_fft_waves<H, FB, kThreads, 1, H, false, kRowRadix<N, R>>(zb, twh, tid);
       for (int e = tid; e < FB * H; e += kThreads) {
         const int r = e / H, k = e % H;
         b.spec[(i0 + d0 + r) * H + k] = r2c_combine(zb + r * H, k, H, twn[k]);
@@ -606,7 +616,7 @@ __device__ __forceinline__ void rows_forward_body(const FvmParams& p, const FvmB
     if constexpr (MODE == 4) return;
     __syncthreads();
   }
-  lds_fft_waves<H, RB, kThreads, 1, H, false, R>(zb, twh, tid);
+  lds_fft_waves<H, RB, kThreads, 1, H, false, kRowRadix<N, R>>(zb, twh, tid);
   for (int e = tid; e < RB * H; e += kThreads) {
     const int r = e / H, k = e % H;
     b.spec[(i0 + r) * H + k] = r2c_combine(zb + r * H, k, H, twn[k]);
@@ -764,7 +774,7 @@ __device__ __forceinline__ void rows_project_body(const FvmParams& p, const FvmB
     prefetch();
     tsh.store(twh, tid);
     __syncthreads();
-    lds_fft_waves<H, T, kThreads, 1, H, true, R>(qb, twh, tid);
+    lds_fft_waves<H, T, kThreads, 1, H, true, kRowRadix<N, R>>(qb, twh, tid);
   }
   const float* q = reinterpret_cast<const float*>(qb);
 #pragma unroll

@@ -462,7 +462,13 @@ __device__ __forceinline__ void rows_tile(RowsShared<N, RB>& sm, const NsParams&
       tid, [&](int e) { return -(gx[e] * vx[e] + gy[e] * vy[e]); },
       [&](int e, float v) { pr[e] = v; });
   __syncthreads();
-  lds_fft_waves<H, RB, kThreads, 1, H, false, R>(prod, twh, tid);
+  // The product's RB forward transforms: with radix-8 passes a transform
+  // has H / 8 butterflies, half a wavefront's lanes or fewer for n <= 512;
+  // radix-4 passes (H / 4 butterflies) issue fewer instructions per
+  // wavefront there.  A function of n and the plan's radix only, so every
+  // row tile shape still gives the same bits.
+  constexpr int RF = (R == 8 && N <= 512) ? 4 : R;
+  lds_fft_waves<H, RB, kThreads, 1, H, false, RF>(prod, twh, tid);
   // element e in the tiled layout's memory order, as for the loads
   auto row_of = [](int e) { return (e / TW) % RB; };
   auto word_of = [](int e) { return (e / (TW * RB)) * TW + e % TW; };
