@@ -460,6 +460,27 @@ def _per_solver(res):
   return out
 
 
+def _copy_rate(dev, nbytes, reps=20):
+  """GB/s of a device tensor copy moving nbytes (half read, half written):
+  what a plain streaming kernel attains on this GPU, reported beside a
+  kernel's algorithmic rate."""
+  import torch
+  n = max(1, int(nbytes) // 8)
+  src = torch.ones(n, dtype=torch.float32, device=dev)
+  dst = torch.empty_like(src)
+  for _ in range(3):
+    dst.copy_(src)
+  e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+  e0.record()
+  for _ in range(reps):
+    dst.copy_(src)
+  e1.record()
+  e1.synchronize()
+  ms = e0.elapsed_time(e1) / reps
+  del src, dst
+  return 2 * n * 4 / (ms * 1e-3) / 1e9
+
+
 def _served_run(dev, requests, group=()):
   """One timed pass of the served stream on a warmed server."""
   import torch
@@ -523,6 +544,9 @@ def _served(dev, requests, kernel_reps, group=(), runs=3):
               "traffic": _traffic(key), "kernel": kname, "launch": key,
               "pool": f"{solver} {n}^2 x {plan.batch} slots",
               "bytes_per_launch": nbytes, "ms_per_launch": round(ms, 5)}
+  copy_gbps = _copy_rate(dev, nbytes)
+  roofline["copy_GBps"] = round(copy_gbps, 1)
+  roofline["frac_of_copy"] = round(achieved / copy_gbps, 4)
   plain_s = {}
   for s in ("fvm", "spectral"):
     for n in (64, 128, 256, 512):
