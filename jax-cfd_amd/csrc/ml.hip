@@ -16,14 +16,15 @@
 //                 W[tap][ci][co] * x[ci][px + tap], + bias, relu.  A
 //                 workgroup owns 4 rows x 32 columns of pixels (one row per
 //                 wavefront) and every output channel; the input tile with
-//                 its one-cell periodic halo stays in LDS for all nine taps.
-//                 The weights are stored in MFMA operand order, so each
-//                 wavefront reads its A operands straight from global memory
-//                 (cache-resident: every workgroup reads the same few
-//                 hundred KB), one vector load per lane for four k-steps,
-//                 the next group's loads in flight during the current
-//                 group's products: no barrier after the halo, and LDS holds
-//                 the halo alone.  The tower's output layer is the
+//                 its one-cell periodic halo goes through LDS in chunks of
+//                 16 channels, double-buffered (the next chunk loaded during
+//                 the current chunk's nine taps).  The weights are stored in
+//                 MFMA operand order, so each wavefront reads its A operands
+//                 straight from global memory (cache-resident: every
+//                 workgroup reads the same few hundred KB), one vector load
+//                 per lane for four k-steps, the next step's loads in flight
+//                 during the current step's products; one barrier per
+//                 chunk.  The tower's output layer is the
 //                 constraint folded in: its 128 channels are the 8
 //                 interpolations' 16 stencil coefficients each (weights
 //                 W . nullspace, bias b . nullspace + constraint bias, formed
@@ -41,6 +42,7 @@
 #include <hip/hip_runtime.h>
 
 #include <type_traits>
+#include <utility>
 
 #include "fvm_terms.h"
 #include "jcfd_fft.h"
@@ -83,18 +85,41 @@ __device__ __forceinline__ float conv_acomp(const ConvAVec<KG>& v, int kk) {
   else return v;
 }
 
+// Calls f(integral_constant<int, S>) for S = B .. E-1 (a loop the compiler
+// sees fully unrolled, with compile-time step numbers).
+template <int B, int E, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+  if constexpr (B < E) {
+    f(std::integral_constant<int, B>{});
+    static_for<B + 1, E>(f);
+  }
+}
+
 // v_mfma_f32_16x16x4_f32 operand layout (checked on gfx950): A[i][k] in lane
 // i + 16 k, B[k][j] in lane j + 16 k, D[i][j] in lane j + 16 (i / 4),
 // register i % 4.  Here i = output channel, j = pixel, k = input channel.
-// Every accumulator sums its products tap by tap, input channels in order
-// within a tap.
+//
+// The input channels go in chunks of 4 KG (16 for 16 or more channels):
+// the halo tile of one chunk sits in one of two LDS buffers while the next
+// chunk's tile is loaded into registers, so only the first chunk's load is
+// exposed.  Each chunk runs its 9 taps (step s = 9 chunk + tap, weight
+// fragment tap GPT + chunk); every accumulator therefore sums its products
+// chunk by chunk, tap by tap within a chunk, channels in order within a
+// tap.  The steps are unrolled at compile time so that each weight wait
+// counts exactly the loads issued after it: a chunk's halo loads follow
+// the weight prefetch of its chunk's first step and are waited on two
+// steps later (loads and their waits share one counter, in issue order).
 template <int CINP, int COUT, bool RELU, bool FIRST>
 __global__ __launch_bounds__(kThreads) void li_conv3x3(LiConvArgs a) {
   static_assert(CINP % 4 == 0 && COUT % 16 == 0, "MFMA blocking");
   using F = ConvFrag<CINP, COUT>;
   constexpr int NCB = F::NCB, KG = F::KG, GPT = F::GPT, STEPS = F::STEPS;
+  constexpr int CH = 4 * KG;                  // channels per chunk
+  constexpr int CTOT = CH * kHaloPlane;       // halo words per chunk
+  constexpr int HPT = (CTOT + kThreads - 1) / kThreads;
+  constexpr int NBUF = GPT > 1 ? 2 : 1;
   using AV = ConvAVec<KG>;
-  __shared__ float sin[CINP * kHaloPitch];
+  __shared__ float shalo[NBUF * CH * kHaloPitch];
   const int n = a.n;
   const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
   const int tiles_x = n / kConvCols;
@@ -104,26 +129,41 @@ __global__ __launch_bounds__(kThreads) void li_conv3x3(LiConvArgs a) {
   const size_t inst = blockIdx.y;
   const float* in = a.in + inst * (FIRST ? 1 : CINP) * plane;
   const float* in_v = FIRST ? a.in_v + inst * plane : nullptr;
-  // fragment (step, cb) of this lane at wf[(step * NCB + cb) * 64]
+  // fragment (tap GPT + chunk, cb) of this lane at wf[(fragment * NCB + cb) * 64]
   const AV* wf = reinterpret_cast<const AV*>(a.w) + lane;
   auto load = [&](int s, AV (&w)[NCB]) {
     const int sc = s < STEPS ? s : STEPS - 1;  // unconditional: the last prefetch repeats
+    const int fr = (sc % 9) * GPT + sc / 9;
 #pragma unroll
-    for (int cb = 0; cb < NCB; ++cb) w[cb] = wf[(sc * NCB + cb) * 64];
+    for (int cb = 0; cb < NCB; ++cb) w[cb] = wf[(fr * NCB + cb) * 64];
   };
-  // the first step's weights go out first, then the halo tile (rows
-  // i0-1 .. i0+4, columns j0-1 .. j0+32, periodic)
+  // halo tile of chunk g (rows i0-1 .. i0+4, columns j0-1 .. j0+32,
+  // periodic) through registers
+  float hreg[HPT];
+  auto halo_load = [&](int g) {
+#pragma unroll
+    for (int k = 0; k < HPT; ++k) {
+      const int e0 = tid + k * kThreads;
+      const int e = (CTOT % kThreads == 0 || e0 < CTOT) ? e0 : CTOT - 1;
+      const int c = g * CH + e / kHaloPlane, r = (e % kHaloPlane) / kHaloW, cc = e % kHaloW;
+      const size_t gi = static_cast<size_t>((i0 - 1 + r) & (n - 1)) * n + ((j0 - 1 + cc) & (n - 1));
+      if constexpr (FIRST) hreg[k] = c == 0 ? in[gi] : (c == 1 ? in_v[gi] : 0.f);
+      else hreg[k] = in[c * plane + gi];
+    }
+  };
+  auto halo_store = [&](int buf) {
+    float* dst = shalo + buf * CH * kHaloPitch;
+#pragma unroll
+    for (int k = 0; k < HPT; ++k) {
+      const int e = tid + k * kThreads;
+      if (CTOT % kThreads == 0 || e < CTOT) dst[(e / kHaloPlane) * kHaloPitch + e % kHaloPlane] = hreg[k];
+    }
+  };
+  // the first step's weights go out first, then chunk 0's halo
   AV wa[NCB], wb[NCB];
   load(0, wa);
-  staged<CINP * kHaloPlane, 16, kThreads>(
-      tid,
-      [&](int e) {
-        const int c = e / kHaloPlane, r = (e % kHaloPlane) / kHaloW, cc = e % kHaloW;
-        const size_t g = static_cast<size_t>((i0 - 1 + r) & (n - 1)) * n + ((j0 - 1 + cc) & (n - 1));
-        if constexpr (FIRST) return c == 0 ? in[g] : (c == 1 ? in_v[g] : 0.f);
-        return in[c * plane + g];
-      },
-      [&](int e, float x) { sin[(e / kHaloPlane) * kHaloPitch + e % kHaloPlane] = x; });
+  halo_load(0);
+  halo_store(0);
   __syncthreads();
 
   floatx4 acc[NCB][2];
@@ -131,11 +171,11 @@ __global__ __launch_bounds__(kThreads) void li_conv3x3(LiConvArgs a) {
   for (int cb = 0; cb < NCB; ++cb) acc[cb][0] = acc[cb][1] = floatx4{0.f, 0.f, 0.f, 0.f};
   const int kq = lane / 16, lp = lane % 16;
   auto step = [&](int s, const AV (&w)[NCB]) {
-    const int tap = s / GPT, g = s % GPT;
-    const float* bin = sin + (wave + tap / 3) * kHaloW + tap % 3 + lp;
+    const int tap = s % 9, g = s / 9;
+    const float* bin = shalo + (g % NBUF) * CH * kHaloPitch + (wave + tap / 3) * kHaloW + tap % 3 + lp;
 #pragma unroll
     for (int kk = 0; kk < KG; ++kk) {
-      const int ci = (g * KG + kk) * 4 + kq;
+      const int ci = kk * 4 + kq;
       const float b0 = bin[ci * kHaloPitch];
       const float b1 = bin[ci * kHaloPitch + 16];
 #pragma unroll
@@ -146,14 +186,22 @@ __global__ __launch_bounds__(kThreads) void li_conv3x3(LiConvArgs a) {
       }
     }
   };
-  // two register sets in turn, so no copy waits for a load in flight
-#pragma unroll 1
-  for (int s = 0; s < STEPS; s += 2) {
-    load(s + 1, wb);
-    step(s, wa);
-    load(s + 2, wa);
-    if (STEPS % 2 == 0 || s + 1 < STEPS) step(s + 1, wb);
-  }
+  // step s computes with one register set while the other receives step
+  // s + 1's weights (the sets alternate, so no copy waits for a load)
+  auto run = [&](auto sc, AV (&cur)[NCB], AV (&nxt)[NCB]) {
+    constexpr int s = decltype(sc)::value;
+    load(s + 1, nxt);
+    if constexpr (s % 9 == 0 && s / 9 + 1 < GPT) halo_load(s / 9 + 1);
+    step(s, cur);
+    if constexpr (s % 9 == 8 && s / 9 + 1 < GPT) {
+      halo_store((s / 9 + 1) % NBUF);  // its buffer's last reader was chunk s/9 - 1
+      __syncthreads();
+    }
+  };
+  static_for<0, STEPS>([&](auto sc) {
+    if constexpr (decltype(sc)::value % 2 == 0) run(sc, wa, wb);
+    else run(sc, wb, wa);
+  });
 
   // out[co][i0 + wave][j0 + px] = act(acc + b[co])
   float* out = a.out + inst * COUT * plane + static_cast<size_t>(i0 + wave) * n + j0;
