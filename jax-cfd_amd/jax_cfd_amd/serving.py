@@ -250,6 +250,13 @@ class Pool:
     # initial fields being built on the server-wide stream)
     self.slot_buf = torch.zeros(capacity, dtype=torch.int32, device=family.device)
     self.slot_host = torch.zeros(capacity, dtype=torch.int32, pin_memory=True)
+    # pinned staging for finished states, one region per slot, allocated
+    # once with the pool, so the serving loop never page-locks host memory
+    # per result; collect() copies a result out into ordinary host memory
+    # before its slot is reused.
+    staging = [torch.empty(tuple(d.shape), dtype=d.dtype, pin_memory=True)
+               for d in self.state]
+    self.staging = [tuple(b[slot] for b in staging) for slot in range(capacity)]
     self.free: List[int] = list(range(capacity))[::-1]
     # slot -> [request, remaining, admitted, pinned host frames or None]
     self.active: Dict[int, List] = {}
@@ -342,7 +349,7 @@ class Pool:
       req, _, admitted, frames = self.active.pop(slot)
       with torch.cuda.stream(self.stream):
         data = self.family.read_slot(self.state, slot)
-        host = tuple(torch.empty(d.shape, dtype=d.dtype, pin_memory=True) for d in data)
+        host = self.staging[slot]
         for h, d in zip(host, data):
           h.copy_(d, non_blocking=True)
         done = torch.cuda.Event()
@@ -360,7 +367,8 @@ class Pool:
         keep.append(item)
         continue
       out.append(RolloutResult(
-          req.request_id, req.solver, req.n, req.steps, 'done', tuple(h.numpy() for h in host),
+          req.request_id, req.solver, req.n, req.steps, 'done',
+          tuple(h.numpy().copy() for h in host),
           req.arrival, admitted, now_fn(),
           tuple(f.numpy() for f in frames) if frames is not None else None))
       self.free.append(slot)
