@@ -1635,6 +1635,15 @@ void group_free_tables(jcfd_fvm_group* g) {
   g->cap_descs = g->cap_items = 0;
 }
 
+// First sizes of a group's device work tables and pinned upload buffer:
+// room for every slot of the default serving pools (960 instances, up to
+// ~41k work items per list), so a serving loop never regrows them -- a
+// regrow frees and reallocates device memory after a device-wide
+// synchronisation, which stalls every stream of the process.
+constexpr int kGroupMinDescs = 1024;
+constexpr int kGroupMinItems = 65536;
+constexpr size_t kGroupMinStaging = size_t(4) << 20;
+
 // Grow the device tables to hold n_desc descriptors and n_item items per
 // phase.  Tables in use by queued work are freed only after a device-wide
 // synchronisation; the graph is keyed on their addresses and so recaptured.
@@ -1642,8 +1651,8 @@ int group_reserve(jcfd_fvm_group* g, int n_desc, int n_item) {
   if (n_desc <= g->cap_descs && n_item <= g->cap_items) return JCFD_OK;
   int rc;
   if (g->descs != nullptr && (rc = check_hip(hipDeviceSynchronize(), "group regrow sync"))) return rc;
-  const int cd = std::max({n_desc, 2 * g->cap_descs, 16});
-  const int ci = std::max({n_item, 2 * g->cap_items, 256});
+  const int cd = std::max({n_desc, 2 * g->cap_descs, kGroupMinDescs});
+  const int ci = std::max({n_item, 2 * g->cap_items, kGroupMinItems});
   group_free_tables(g);
   int64_t unused = 0;
   if ((rc = device_alloc(&g->descs, static_cast<size_t>(cd), &unused)) ||
@@ -1667,7 +1676,7 @@ int group_upload(jcfd_fvm_group* g, const std::vector<char>& table, int n_desc, 
   if (table.size() > g->staging_bytes) {
     if (g->staging != nullptr) (void)hipHostFree(g->staging);
     g->staging = nullptr;
-    const size_t bytes = std::max(table.size(), 2 * g->staging_bytes);
+    const size_t bytes = std::max({table.size(), 2 * g->staging_bytes, kGroupMinStaging});
     if ((rc = check_hip(hipHostMalloc(reinterpret_cast<void**>(&g->staging), bytes, hipHostMallocDefault),
                         "pinned group staging")))
       return rc;
@@ -2261,9 +2270,9 @@ int ns_group_reserve(jcfd_ns2d_group* g, int n_desc, int n_item, int n_slot) {
   if (n_desc <= g->cap_descs && n_item <= g->cap_items && n_slot <= g->cap_slots) return JCFD_OK;
   int rc;
   if (g->descs != nullptr && (rc = check_hip(hipDeviceSynchronize(), "group regrow sync"))) return rc;
-  const int cd = std::max({n_desc, 2 * g->cap_descs, 16});
-  const int ci = std::max({n_item, 2 * g->cap_items, 256});
-  const int cs = std::max({n_slot, 2 * g->cap_slots, 16});
+  const int cd = std::max({n_desc, 2 * g->cap_descs, kGroupMinDescs});
+  const int ci = std::max({n_item, 2 * g->cap_items, kGroupMinItems});
+  const int cs = std::max({n_slot, 2 * g->cap_slots, kGroupMinDescs});
   ns_group_free_tables(g);
   int64_t unused = 0;
   if ((rc = device_alloc(&g->descs, static_cast<size_t>(cd) * JCFD_MAX_RK_STAGES, &unused)) ||
@@ -2298,7 +2307,7 @@ int ns_group_upload(jcfd_ns2d_group* g, const std::vector<char>& table, int n_de
   if (table.size() > g->staging_bytes) {
     if (g->staging != nullptr) (void)hipHostFree(g->staging);
     g->staging = nullptr;
-    const size_t bytes = std::max(table.size(), 2 * g->staging_bytes);
+    const size_t bytes = std::max({table.size(), 2 * g->staging_bytes, kGroupMinStaging});
     if ((rc = check_hip(hipHostMalloc(reinterpret_cast<void**>(&g->staging), bytes, hipHostMallocDefault),
                         "pinned group staging")))
       return rc;
