@@ -385,9 +385,10 @@ class PoolGroup:
   the pools share the group's stream and completion event.  Each slot still
   gets exactly the arithmetic of a lone instance."""
 
-  def __init__(self, family):
+  def __init__(self, family, stream: Optional[torch.cuda.Stream] = None):
     self.group = family.make_group()
-    self.stream = torch.cuda.Stream(device=family.device)  # default priority
+    # default priority
+    self.stream = stream if stream is not None else torch.cuda.Stream(device=family.device)
     self.pools: List[Pool] = []
     self.event: Optional[torch.cuda.Event] = None
 
@@ -439,6 +440,13 @@ class RolloutServer:
     self.families: Dict[Tuple[str, int], object] = {}
     self.reserved = 0
     self.ic_stream = torch.cuda.Stream(device=self.device)
+    # the streams of the groups and of the ungrouped default-size pools,
+    # taken together here rather than as pools appear between plan set-ups
+    self._streams = {s: torch.cuda.Stream(device=self.device) for s in self.group_solvers}
+    for s in FAMILIES:
+      for n in DEFAULT_CAPACITY:
+        if not (s in self.group_solvers and n <= FAMILIES[s].GROUP_MAX_N):
+          self._streams[(s, n)] = torch.cuda.Stream(device=self.device)
 
   def _family(self, solver, n):
     key = (solver, n)
@@ -467,9 +475,10 @@ class RolloutServer:
     grp = None
     if solver in self.group_solvers and n <= fam.GROUP_MAX_N:
       if solver not in self.groups:
-        self.groups[solver] = PoolGroup(fam)
+        self.groups[solver] = PoolGroup(fam, self._streams.get(solver))
       grp = self.groups[solver]
-    pool = Pool(fam, cap, tune=self.tune, stream=grp.stream if grp is not None else None)
+    pool = Pool(fam, cap, tune=self.tune,
+                stream=grp.stream if grp is not None else self._streams.get(key))
     pool.group = grp
     if grp is not None:
       grp.pools.append(pool)
